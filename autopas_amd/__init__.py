@@ -1,0 +1,15 @@
+"""autopas_amd -- MI355X-native engine for AutoPas's Lennard-Jones short-range path.
+
+The compute lives in ``libljgpu.so`` (HIP kernels for gfx950 behind the C ABI in include/ljgpu.h);
+this package only binds it.
+"""
+from .engine import (  # noqa: F401
+    KERNEL_CLUSTER_FP64,
+    KERNEL_CLUSTER_MIXED,
+    KERNEL_LC_FP64,
+    KERNEL_NAMES,
+    Engine,
+    EngineError,
+    generate_lattice,
+    load_library,
+)

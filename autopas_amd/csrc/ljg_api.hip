@@ -1,0 +1,493 @@
+// ljg_api.hip -- the C ABI (include/ljgpu.h): context lifetime, uploads/downloads in upload order, the
+// rebuild policy and the step loop.  All HIP errors are caught here and turned into status codes.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <new>
+
+#include "ljg_internal.h"
+
+struct ljg_ctx {
+  ljg::Ctx c;
+};
+
+namespace ljg {
+
+static ljg_status fail(Ctx &c, ljg_status st, const std::string &msg) {
+  c.err = msg;
+  return st;
+}
+
+#define LJG_API_BEGIN(ctxp)                  \
+  if (!(ctxp)) return LJG_ERR_ARG;           \
+  Ctx &c = (ctxp)->c;                        \
+  try {                                      \
+    LJG_HIP(hipSetDevice(c.device));
+#define LJG_API_END                                                                                  \
+  }                                                                                                  \
+  catch (const HipError &e) {                                                                        \
+    char buf[512];                                                                                   \
+    std::snprintf(buf, sizeof buf, "%s failed: %s (line %d)", e.what, hipGetErrorString(e.err), e.line); \
+    return fail(c, e.line < 0 ? LJG_ERR_CAPACITY : LJG_ERR_HIP, buf);                               \
+  }                                                                                                  \
+  catch (const std::bad_alloc &) {                                                                   \
+    return fail(c, LJG_ERR_HIP, "host allocation failed");                                          \
+  }
+
+__global__ void k_init_upload(int64_t n, StateView s, const int64_t *__restrict__ id_in, int32_t have_id) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s.fx[i] = 0.; s.fy[i] = 0.; s.fz[i] = 0.;
+  s.ofx[i] = 0.; s.ofy[i] = 0.; s.ofz[i] = 0.;
+  s.rx[i] = s.x[i]; s.ry[i] = s.y[i]; s.rz[i] = s.z[i];
+  s.orig[i] = (int32_t)i;
+  s.id[i] = have_id ? id_in[i] : i;
+}
+
+// out[orig[s]] = in[s] for the real slots (storage order -> upload order).
+__global__ void k_to_upload_order(int64_t nslots, const int32_t *__restrict__ orig, const double *__restrict__ in,
+                                  double *__restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int32_t o = orig[s];
+  if (o >= 0) out[o] = in[s];
+}
+
+// out[s] = in[orig[s]] (upload order -> storage order).
+__global__ void k_from_upload_order(int64_t nslots, const int32_t *__restrict__ orig, const double *__restrict__ in,
+                                    double *__restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int32_t o = orig[s];
+  if (o >= 0) out[s] = in[o];
+}
+
+__global__ void k_accumulate_hits(int64_t nitems, const int64_t *__restrict__ ph, int64_t *__restrict__ acc) {
+  __shared__ int64_t sh[1024];
+  int64_t a = 0;
+  for (int64_t i = threadIdx.x; i < nitems; i += 1024) a += ph[i];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[0] += sh[0];
+}
+
+__global__ void k_zero_i64(int64_t n, int64_t *p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_i32(int64_t n, int32_t *p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+static bool is_cluster_kernel(int k) { return k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_MIXED; }
+
+static void rebuild_for(Ctx &c, int kernel) {
+  wrap_positions(c);
+  if (is_cluster_kernel(kernel)) vcl_build(c);
+  else lc_bin(c);
+  md_set_rebuild_reference(c);
+  c.steps_since_rebuild = 0;
+}
+
+static bool layout_matches(const Ctx &c, int kernel) {
+  return is_cluster_kernel(kernel) ? c.layout == Layout::Clusters : c.layout == Layout::Cells;
+}
+
+static int64_t partial_items(const Ctx &c, int kernel) {
+  return is_cluster_kernel(kernel) ? blocks_for(c.nclusters, 4) : blocks_for(c.n, 256);
+}
+
+static void run_force(Ctx &c, int kernel, bool globals) {
+  if (kernel == LJG_KERNEL_LC_FP64) lc_force(c, globals);
+  else cluster_force(c, kernel == LJG_KERNEL_CLUSTER_MIXED, globals);
+}
+
+static int64_t candidate_evals(const Ctx &c, int kernel) {
+  if (is_cluster_kernel(kernel)) return c.nentries * (int64_t)c.M * c.M;
+  return -1;
+}
+
+}  // namespace ljg
+
+using namespace ljg;
+
+extern "C" {
+
+ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
+  if (!p || !out) return LJG_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LJG_ERR_NODEVICE;
+  if (p->device < 0 || p->device >= ndev) return LJG_ERR_ARG;
+  const double ilen = p->cutoff + p->skin;
+  if (!(p->cutoff > 0.) || !(p->skin >= 0.)) return LJG_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (!(p->box_max[d] - p->box_min[d] >= ilen)) return LJG_ERR_ARG;  // LogicHandler.h:967-983
+  if (p->cluster_size != 1 && p->cluster_size != 2 && p->cluster_size != 4 && p->cluster_size != 8)
+    return LJG_ERR_UNSUPPORTED;
+  if (p->rebuild_frequency < 1) return LJG_ERR_ARG;
+  ljg_ctx *h = new (std::nothrow) ljg_ctx;
+  if (!h) return LJG_ERR_HIP;
+  Ctx &c = h->c;
+  c.prm = *p;
+  c.device = p->device;
+  c.M = p->cluster_size;
+  try {
+    LJG_HIP(hipSetDevice(c.device));
+    LJG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    LJG_HIP(hipEventCreate(&c.ev0));
+    LJG_HIP(hipEventCreate(&c.ev1));
+    LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+    c.flag.ensure(4);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.flag.p);
+    LJG_HIP(hipGetLastError());
+  } catch (const HipError &e) {
+    delete h;
+    return LJG_ERR_HIP;
+  }
+  const double eps1 = 1., sig1 = 1., mass1 = 1.;
+  const ljg_status st = ljg_set_types(h, 1, &eps1, &sig1, &mass1);
+  if (st != LJG_OK) {
+    ljg_destroy(h);
+    return st;
+  }
+  *out = h;
+  return LJG_OK;
+}
+
+ljg_status ljg_destroy(ljg_ctx *h) {
+  if (!h) return LJG_ERR_ARG;
+  Ctx &c = h->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  for (auto &s : c.st) s.release();
+  c.cell_of.release(); c.cell_count.release(); c.cell_start.release(); c.rank.release(); c.perm.release();
+  c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
+  c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
+  c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_off.release(); c.list.release();
+  c.pos32.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
+  if (c.host_flag) (void)hipHostFree(c.host_flag);
+  if (c.ev0) (void)hipEventDestroy(c.ev0);
+  if (c.ev1) (void)hipEventDestroy(c.ev1);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return LJG_OK;
+}
+
+const char *ljg_last_error(const ljg_ctx *h) { return h ? h->c.err.c_str() : "null context"; }
+
+ljg_status ljg_device_name(const ljg_ctx *h, char *buf, int32_t buflen) {
+  if (!h || !buf || buflen <= 0) return LJG_ERR_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, h->c.device) != hipSuccess) return LJG_ERR_HIP;
+  std::snprintf(buf, (size_t)buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return LJG_OK;
+}
+
+ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const double *sig, const double *mass) {
+  if (!h || ntypes < 1 || ntypes > kMaxTypes || !eps || !sig) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  LJParams &lj = c.lj;
+  lj.ntypes = ntypes;
+  const double rc2 = c.prm.cutoff * c.prm.cutoff;
+  lj.rc2 = rc2;
+  // ParticlePropertiesLibrary.h:443-473 and calcShift6 :575-582
+  for (int a = 0; a < ntypes; ++a) {
+    for (int b = 0; b < ntypes; ++b) {
+      const int k = ntypes * a + b;
+      const double e24 = 24 * std::sqrt(eps[a] * eps[b]);
+      const double s = (sig[a] + sig[b]) / 2.0;
+      const double s2 = s * s;
+      lj.eps24[k] = e24;
+      lj.sig2[k] = s2;
+      if (c.prm.apply_shift) {
+        const double q2 = s2 / rc2;
+        const double q6 = q2 * q2 * q2;
+        lj.shift6[k] = e24 * (q6 - q6 * q6);
+      } else {
+        lj.shift6[k] = 0.;
+      }
+    }
+  }
+  for (int t = 0; t < kMaxTypes; ++t) {
+    lj.mass[t] = (mass && t < ntypes) ? mass[t] : 1.;
+    lj.inv2m[t] = 1. / (2. * lj.mass[t]);
+  }
+  std::vector<double> host(3 * ntypes * ntypes);
+  for (int k = 0; k < ntypes * ntypes; ++k) {
+    host[3 * k] = lj.eps24[k];
+    host[3 * k + 1] = lj.sig2[k];
+    host[3 * k + 2] = lj.shift6[k];
+  }
+  c.tab.ensure(host.size());
+  LJG_HIP(hipMemcpyAsync(c.tab.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.types_set = true;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                      const double *vy, const double *vz, const int32_t *type, const int64_t *id) {
+  if (!h || n <= 0 || n > (int64_t)INT32_MAX / 2 || !x || !y || !z) return LJG_ERR_ARG;
+  if (type) {
+    for (int64_t i = 0; i < n; ++i)
+      if (type[i] < 0 || type[i] >= h->c.lj.ntypes) return LJG_ERR_ARG;
+  }
+  LJG_API_BEGIN(h)
+  c.cur = 0;
+  c.st[0].ensure((size_t)n);
+  c.n = n;
+  c.nslots = n;
+  State &s = c.st[0];
+  const size_t bytes = (size_t)n * sizeof(double);
+  LJG_HIP(hipMemcpyAsync(s.x.p, x, bytes, hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipMemcpyAsync(s.y.p, y, bytes, hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipMemcpyAsync(s.z.p, z, bytes, hipMemcpyHostToDevice, c.stream));
+  if (vx && vy && vz) {
+    LJG_HIP(hipMemcpyAsync(s.vx.p, vx, bytes, hipMemcpyHostToDevice, c.stream));
+    LJG_HIP(hipMemcpyAsync(s.vy.p, vy, bytes, hipMemcpyHostToDevice, c.stream));
+    LJG_HIP(hipMemcpyAsync(s.vz.p, vz, bytes, hipMemcpyHostToDevice, c.stream));
+  } else {
+    LJG_HIP(hipMemsetAsync(s.vx.p, 0, bytes, c.stream));
+    LJG_HIP(hipMemsetAsync(s.vy.p, 0, bytes, c.stream));
+    LJG_HIP(hipMemsetAsync(s.vz.p, 0, bytes, c.stream));
+  }
+  if (type) LJG_HIP(hipMemcpyAsync(s.type.p, type, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  else LJG_HIP(hipMemsetAsync(s.type.p, 0, (size_t)n * sizeof(int32_t), c.stream));
+  int64_t *did = nullptr;
+  if (id) {
+    c.scan_tmp.ensure((size_t)n);
+    did = c.scan_tmp.p;
+    LJG_HIP(hipMemcpyAsync(did, id, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+  }
+  hipLaunchKernelGGL(k_init_upload, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, view(s), did,
+                     (int32_t)(id != nullptr));
+  LJG_HIP(hipGetLastError());
+  wrap_positions(c);
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.layout = Layout::Upload;
+  c.have_forces = false;
+  c.steps_since_rebuild = 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_forces(ljg_ctx *h, const double *fx, const double *fy, const double *fz) {
+  if (!h || !fx || !fy || !fz) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  State &s = c.st[c.cur];
+  c.part_e.ensure((size_t)c.n);
+  const double *src[3] = {fx, fy, fz};
+  double *dst[3] = {s.fx.p, s.fy.p, s.fz.p};
+  for (int d = 0; d < 3; ++d) {
+    LJG_HIP(hipMemcpyAsync(c.part_e.p, src[d], (size_t)c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_from_upload_order, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                       s.orig.p, c.part_e.p, dst[d]);
+    LJG_HIP(hipGetLastError());
+  }
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_bin(ljg_ctx *h) {
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  rebuild_for(c, LJG_KERNEL_LC_FP64);
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_build_pairs(ljg_ctx *h) {
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  rebuild_for(c, LJG_KERNEL_CLUSTER_MIXED);
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globals *out) {
+  if (!h || kernel < 0 || kernel > 2) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (!layout_matches(c, kernel)) rebuild_for(c, kernel);
+  run_force(c, kernel, calc_globals != 0);
+  ljg_globals g{};
+  reduce_partials(c, partial_items(c, kernel), &g, calc_globals != 0);
+  g.candidate_evals = candidate_evals(c, kernel);
+  if (!calc_globals) {
+    g.potential_energy = 0.;
+    g.virial = 0.;
+  }
+  c.have_forces = true;
+  if (out) *out = g;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
+  if (!h) return LJG_ERR_ARG;
+  h->c.timing = enabled != 0;
+  return LJG_OK;
+}
+
+ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out) {
+  if (!h || nsteps < 0 || kernel < 0 || kernel > 2 || !(dt > 0.)) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  const auto t0 = std::chrono::steady_clock::now();
+  ljg_run_stats rs{};
+  const bool timing = c.timing;
+  std::vector<hipEvent_t> fev, rev;  // pairs (start, stop) per force launch / rebuild
+  auto new_event = [](std::vector<hipEvent_t> &v) {
+    hipEvent_t e;
+    LJG_HIP(hipEventCreate(&e));
+    v.push_back(e);
+    return e;
+  };
+  c.red_hits.ensure(2);
+  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.red_hits.p);
+  LJG_HIP(hipGetLastError());
+  const bool saved_timing = c.timing;
+  c.timing = false;  // per-launch events are recorded here instead
+  if (!layout_matches(c, kernel)) {
+    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    rebuild_for(c, kernel);
+    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    ++rs.rebuilds;
+  }
+  for (int32_t s = 0; s < nsteps; ++s) {
+    // kick of the previous step (if any) fused with this step's drift
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)1, c.flag.p);
+    md_kick_drift(c, s > 0, true, dt);
+    bool rebuild = c.steps_since_rebuild >= c.prm.rebuild_frequency;
+    if (!rebuild && c.prm.dynamic_rebuild) {
+      LJG_HIP(hipMemcpyAsync(c.host_flag, c.flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+      LJG_HIP(hipStreamSynchronize(c.stream));
+      rebuild = c.host_flag[0] != 0;
+    }
+    if (rebuild) {
+      if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+      rebuild_for(c, kernel);
+      if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+      ++rs.rebuilds;
+    }
+    if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+    run_force(c, kernel, false);
+    if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+    hipLaunchKernelGGL(k_accumulate_hits, dim3(1), dim3(1024), 0, c.stream, partial_items(c, kernel), c.part_hits.p,
+                       c.red_hits.p);
+    LJG_HIP(hipGetLastError());
+    c.steps_since_rebuild++;
+  }
+  if (nsteps > 0) md_kick_drift(c, true, false, dt);  // final velocity update
+  c.timing = saved_timing;
+  int64_t hits2 = 0;
+  LJG_HIP(hipMemcpyAsync(&hits2, c.red_hits.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  rs.steps = nsteps;
+  rs.pair_interactions = hits2 / 2;
+  for (size_t k = 0; k + 1 < fev.size(); k += 2) {
+    float ms = 0.f;
+    LJG_HIP(hipEventElapsedTime(&ms, fev[k], fev[k + 1]));
+    rs.force_ms += ms;
+    rs.force_launches++;
+  }
+  for (size_t k = 0; k + 1 < rev.size(); k += 2) {
+    float ms = 0.f;
+    LJG_HIP(hipEventElapsedTime(&ms, rev[k], rev[k + 1]));
+    rs.rebuild_ms += ms;
+  }
+  for (auto e : fev) (void)hipEventDestroy(e);
+  for (auto e : rev) (void)hipEventDestroy(e);
+  c.have_forces = true;
+  rs.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = rs;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_download(ljg_ctx *h, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *fx,
+                        double *fy, double *fz) {
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  State &s = c.st[c.cur];
+  double *outs[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
+  const double *ins[9] = {s.x.p, s.y.p, s.z.p, s.vx.p, s.vy.p, s.vz.p, s.fx.p, s.fy.p, s.fz.p};
+  c.part_v.ensure((size_t)c.n);
+  for (int k = 0; k < 9; ++k) {
+    if (!outs[k]) continue;
+    hipLaunchKernelGGL(k_to_upload_order, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.orig.p,
+                       ins[k], c.part_v.p);
+    LJG_HIP(hipGetLastError());
+    LJG_HIP(hipMemcpyAsync(outs[k], c.part_v.p, (size_t)c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    LJG_HIP(hipStreamSynchronize(c.stream));
+  }
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_get_cells(ljg_ctx *h, int64_t *cell_of_particle, int64_t nh[3]) {
+  LJG_API_BEGIN(h)
+  if (c.layout != Layout::Cells) return fail(c, LJG_ERR_STATE, "particles are not binned into cells");
+  if (cell_of_particle)
+    LJG_HIP(hipMemcpy(cell_of_particle, c.cell_of_particle.p, (size_t)c.n * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (nh)
+    for (int d = 0; d < 3; ++d) nh[d] = c.cg.nh[d];
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_cluster_sizes(ljg_ctx *h, int64_t *num_slots, int64_t *num_clusters, int64_t *num_entries,
+                             int64_t towers_per_dim[2], double side[2], int64_t *towers_per_il) {
+  LJG_API_BEGIN(h)
+  if (c.layout != Layout::Clusters) return fail(c, LJG_ERR_STATE, "no cluster structure built");
+  if (num_slots) *num_slots = c.nslots;
+  if (num_clusters) *num_clusters = c.nclusters;
+  if (num_entries) *num_entries = c.nentries;
+  if (towers_per_dim) { towers_per_dim[0] = c.tg.nt[0]; towers_per_dim[1] = c.tg.nt[1]; }
+  if (side) { side[0] = c.tg.side[0]; side[1] = c.tg.side[1]; }
+  if (towers_per_il) *towers_per_il = c.tg.R;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_get_clusters(ljg_ctx *h, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
+                            int32_t *list_j, int32_t *list_shift, double *bbox) {
+  LJG_API_BEGIN(h)
+  if (c.layout != Layout::Clusters) return fail(c, LJG_ERR_STATE, "no cluster structure built");
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  if (slot_particle) {
+    std::vector<int32_t> o((size_t)c.nslots);
+    LJG_HIP(hipMemcpy(o.data(), c.st[c.cur].orig.p, o.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < o.size(); ++k) slot_particle[k] = o[k];
+  }
+  if (cluster_tower)
+    LJG_HIP(hipMemcpy(cluster_tower, c.cluster_tower.p, (size_t)c.nclusters * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (list_offsets)
+    LJG_HIP(hipMemcpy(list_offsets, c.list_off.p, (size_t)(c.nclusters + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (list_j || list_shift) {
+    std::vector<int2> l((size_t)c.nentries);
+    if (!l.empty()) LJG_HIP(hipMemcpy(l.data(), c.list.p, l.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < l.size(); ++k) {
+      if (list_j) list_j[k] = l[k].x;
+      if (list_shift) list_shift[k] = l[k].y;
+    }
+  }
+  if (bbox) LJG_HIP(hipMemcpy(bbox, c.bbox.p, (size_t)c.nclusters * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  return LJG_OK;
+  LJG_API_END
+}
+
+}  // extern "C"

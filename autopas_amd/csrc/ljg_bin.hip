@@ -1,0 +1,145 @@
+// ljg_bin.hip -- LinkedCells binning on the device: position -> CellBlock3D cell (bit-exact restatement
+// of CellBlock3D::get3DIndexOfPosition), counting sort (histogram, exclusive scan, scatter), a per-cell
+// stable order by particle id, and one gather pass that writes the whole particle state in cell order
+// (coalesced SoA).  Compiled with -ffp-contract=off: the index arithmetic must round exactly as the
+// reference's FP64 expression.
+#include <cmath>
+#include <utility>
+
+#include "ljg_internal.h"
+
+namespace ljg {
+
+// CellBlock3D.h:320-349 and threeToOneD (ThreeDimensionalMapping.h:29-31).
+__device__ inline int32_t cell_index_1d(const CellGeom &g, double px, double py, double pz) {
+  const double p[3] = {px, py, pz};
+  int32_t c3[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long value = (long)floor((p[d] - g.box_min[d]) * g.recip[d]) + g.cpil;
+    const long nonneg = value > 0 ? value : 0;
+    const long upper = g.nh[d] - 1;
+    long c = nonneg < upper ? nonneg : upper;
+    if (p[d] >= g.box_max[d]) {
+      const long first_upper_halo = g.nh[d] - g.cpil;
+      c = c > first_upper_halo ? c : first_upper_halo;
+    } else if (p[d] < g.box_min[d] && c == g.cpil) {
+      --c;
+    } else if (p[d] < g.box_max[d] && c == g.nh[d] - g.cpil) {
+      --c;
+    }
+    c3[d] = (int32_t)c;
+  }
+  return (c3[2] * g.nh[1] + c3[1]) * g.nh[0] + c3[0];
+}
+
+__global__ void k_cell_hist(int64_t nslots, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const int32_t *__restrict__ orig, CellGeom g,
+                            int32_t *__restrict__ cell_of, int32_t *__restrict__ rank,
+                            int32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  if (orig[i] < 0) {
+    cell_of[i] = -1;
+    return;
+  }
+  const int32_t c = cell_index_1d(g, x[i], y[i], z[i]);
+  cell_of[i] = c;
+  rank[i] = atomicAdd(&count[c], 1);
+}
+
+__global__ void k_cell_scatter(int64_t nslots, const int32_t *__restrict__ cell_of, const int32_t *__restrict__ rank,
+                               const int64_t *__restrict__ start, int32_t *__restrict__ items) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const int32_t c = cell_of[i];
+  if (c < 0) return;
+  items[start[c] + rank[i]] = (int32_t)i;
+}
+
+// The atomic histogram hands out ranks in arbitrary order; sort every cell's members by particle id so
+// the layout (and with it every floating-point summation order downstream) is reproducible.
+__global__ void k_cell_sort_by_id(int64_t ncells, const int64_t *__restrict__ start, int32_t *__restrict__ items,
+                                  const int64_t *__restrict__ id) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int64_t b = start[c], e = start[c + 1];
+  for (int64_t k = b + 1; k < e; ++k) {
+    const int32_t v = items[k];
+    const int64_t key = id[v];
+    int64_t m = k - 1;
+    while (m >= b && id[items[m]] > key) {
+      items[m + 1] = items[m];
+      --m;
+    }
+    items[m + 1] = v;
+  }
+}
+
+// Cell of each particle in upload order (inspection).
+__global__ void k_cell_of_particle(int64_t nslots, const int32_t *__restrict__ orig, const int32_t *__restrict__ cell,
+                                   int64_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const int32_t o = orig[i];
+  if (o >= 0) out[o] = cell[i];
+}
+
+__global__ void k_gather_cell(int64_t n, const int32_t *__restrict__ src, const int32_t *__restrict__ cell_old,
+                              int32_t *__restrict__ cell_new) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) cell_new[s] = cell_old[src[s]];
+}
+
+void lc_bin(Ctx &c) {
+  // Geometry: CellBlock3D::rebuild with cellSizeFactor 1 (CellBlock3D.h:359-405).
+  CellGeom &g = c.cg;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  g.cpil = 1;
+  int64_t ncells = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.box_min[d] = c.prm.box_min[d];
+    g.box_max[d] = c.prm.box_max[d];
+    const double L = c.prm.box_max[d] - c.prm.box_min[d];
+    g.L[d] = L;
+    long cpd = (long)std::floor(L / (ilen * 1.0));
+    if (cpd < 1) cpd = 1;
+    g.nh[d] = (int32_t)(cpd + 2 * g.cpil);
+    g.cell_len[d] = L / (double)cpd;
+    g.recip[d] = (double)cpd / L;
+    ncells *= g.nh[d];
+  }
+  c.ncells = ncells;
+  const int64_t ns = c.nslots;
+  c.cell_of.ensure(ns);
+  c.rank.ensure(ns);
+  c.perm.ensure(ns);
+  c.cell_count.ensure(ncells + 1);
+  c.cell_start.ensure(ncells + 1);
+  StateView s = view(c.st[c.cur]);
+  LJG_HIP(hipMemsetAsync(c.cell_count.p, 0, (ncells + 1) * sizeof(int32_t), c.stream));
+  hipLaunchKernelGGL(k_cell_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x, s.y, s.z, s.orig, g,
+                     c.cell_of.p, c.rank.p, c.cell_count.p);
+  LJG_HIP(hipGetLastError());
+  exclusive_scan_i32_to_i64(c, c.cell_count.p, c.cell_start.p, ncells);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.cell_of.p, c.rank.p,
+                     c.cell_start.p, c.perm.p);
+  LJG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cell_sort_by_id, dim3(blocks_for(ncells, 128)), dim3(128), 0, c.stream, ncells,
+                     c.cell_start.p, c.perm.p, s.id);
+  LJG_HIP(hipGetLastError());
+  c.cell_of_particle.ensure(c.n);
+  hipLaunchKernelGGL(k_cell_of_particle, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.orig, c.cell_of.p,
+                     c.cell_of_particle.p);
+  LJG_HIP(hipGetLastError());
+  // Gather the state into cell order: the real particles fill slots [0, n) (padding slots are dropped).
+  permute_state(c, c.perm.p, c.n);
+  // cell of every new slot, for the force kernel (old slot's cell, gathered through the same map)
+  hipLaunchKernelGGL(k_gather_cell, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.p, c.cell_of.p,
+                     c.rank.p);
+  LJG_HIP(hipGetLastError());
+  std::swap(c.cell_of, c.rank);
+  c.layout = Layout::Cells;
+}
+
+}  // namespace ljg

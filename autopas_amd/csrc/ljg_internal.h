@@ -1,0 +1,196 @@
+// ljg_internal.h -- engine internals shared by the HIP translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ljgpu.h"
+
+namespace ljg {
+
+constexpr int kWave = 64;
+constexpr int kMaxTypes = 16;
+// Largest tower the LDS bitonic z-sort handles in one workgroup (z: 8 B, id: 8 B, slot: 4 B per entry).
+constexpr int kTowerSortCap = 4096;
+// Dummy padding slots sit here (far from every box), so that every pair with them fails r^2 <= rc^2.
+constexpr double kDummyPos = 1.0e7;
+
+struct HipError {
+  hipError_t err;
+  const char *what;
+  int line;
+};
+
+#define LJG_HIP(call)                                         \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) throw ::ljg::HipError{e_, #call, __LINE__}; \
+  } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t n) {
+    if (n <= cap) return;
+    if (p) LJG_HIP(hipFree(p));
+    p = nullptr;
+    size_t c = n + n / 4 + 64;
+    LJG_HIP(hipMalloc(&p, c * sizeof(T)));
+    cap = c;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Particle state in storage order ("slots").  Slots beyond the real particles of a cluster are padding
+// (orig == -1, position kDummyPos, zero velocity/force).
+struct State {
+  DevBuf<double> x, y, z, vx, vy, vz, fx, fy, fz, ofx, ofy, ofz, rx, ry, rz;
+  DevBuf<int32_t> type, orig;
+  DevBuf<int64_t> id;
+  void ensure(size_t n) {
+    for (auto *b : {&x, &y, &z, &vx, &vy, &vz, &fx, &fy, &fz, &ofx, &ofy, &ofz, &rx, &ry, &rz}) b->ensure(n);
+    type.ensure(n);
+    orig.ensure(n);
+    id.ensure(n);
+  }
+  void release() {
+    for (auto *b : {&x, &y, &z, &vx, &vy, &vz, &fx, &fy, &fz, &ofx, &ofy, &ofz, &rx, &ry, &rz}) b->release();
+    type.release();
+    orig.release();
+    id.release();
+  }
+};
+
+// Plain views of a State for kernel arguments.
+struct StateView {
+  double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *ofx, *ofy, *ofz, *rx, *ry, *rz;
+  int32_t *type, *orig;
+  int64_t *id;
+};
+
+inline StateView view(State &s) {
+  return StateView{s.x.p,  s.y.p,  s.z.p,  s.vx.p, s.vy.p, s.vz.p, s.fx.p,   s.fy.p,   s.fz.p,
+                   s.ofx.p, s.ofy.p, s.ofz.p, s.rx.p, s.ry.p, s.rz.p, s.type.p, s.orig.p, s.id.p};
+}
+
+// CellBlock3D geometry (CellBlock3D.h:359-405) in the form the device needs.
+struct CellGeom {
+  double box_min[3], box_max[3], recip[3], cell_len[3], L[3];
+  int32_t cpil;     // cells per interaction length (1 for csf >= 1)
+  int32_t nh[3];    // cells per dimension including halo layers
+};
+
+// Periodic tower grid (ClusterTowerBlock2D.h:140-168 without halo towers).
+struct TowerGeom {
+  double box_min[3], L[3], side[2], recip[2];
+  int32_t nt[2];
+  int32_t R;        // towers per interaction length
+  double ilen;
+};
+
+// LJ parameters (mixing tables are ntypes x ntypes, row-major).
+struct LJParams {
+  int32_t ntypes;
+  double rc2;
+  double eps24[kMaxTypes * kMaxTypes], sig2[kMaxTypes * kMaxTypes], shift6[kMaxTypes * kMaxTypes];
+  double inv2m[kMaxTypes];  // 1/(2m) per type
+  double mass[kMaxTypes];
+};
+
+enum class Layout { None, Upload, Cells, Clusters };
+
+struct Ctx {
+  ljg_params prm{};
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing = false;
+
+  int64_t n = 0;       // real particles
+  int64_t nslots = 0;  // storage slots in the current layout
+  Layout layout = Layout::None;
+  State st[2];
+  int cur = 0;
+  LJParams lj{};
+  bool types_set = false;
+
+  // LinkedCells
+  CellGeom cg{};
+  int64_t ncells = 0;
+  DevBuf<int32_t> cell_of;      // per slot
+  DevBuf<int32_t> cell_count;   // ncells (+1)
+  DevBuf<int64_t> cell_start;   // ncells + 1
+  DevBuf<int32_t> rank;         // per slot, rank inside cell/tower from the atomic histogram
+  DevBuf<int32_t> perm;         // gather map new slot -> old slot
+  DevBuf<int64_t> cell_of_particle;  // upload order, for inspection
+
+  // Clusters
+  TowerGeom tg{};
+  int32_t M = 4;
+  int64_t ntowers = 0, nclusters = 0, nentries = 0;
+  DevBuf<int32_t> tower_of;     // per old slot
+  DevBuf<int32_t> tower_count;  // ntowers
+  DevBuf<int64_t> tower_start;  // ntowers + 1 (particle offsets)
+  DevBuf<int32_t> tower_items;  // sorted old slots per tower
+  DevBuf<int32_t> ctow_count;   // clusters per tower
+  DevBuf<int64_t> ctow_start;   // ntowers + 1 (cluster offsets)
+  DevBuf<int32_t> cluster_tower;  // nclusters
+  DevBuf<double> bbox;          // 6 * nclusters
+  DevBuf<double4> c0;           // nclusters: cluster reference point (bbox centre at build)
+  DevBuf<int32_t> list_count;   // nclusters
+  DevBuf<int64_t> list_off;     // nclusters + 1
+  DevBuf<int2> list;            // entries (j cluster, shift id)
+  DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
+
+  DevBuf<double> tab;             // 3 * ntypes^2: (eps24, sig2, shift6) per type pair
+
+  // Reductions
+  DevBuf<double> part_e, part_v;  // per work item
+  DevBuf<int64_t> part_hits;
+  DevBuf<double> red_out;         // [0]=e [1]=v
+  DevBuf<int64_t> red_hits;       // [0]
+  DevBuf<int32_t> flag;           // [0] displacement, [1] capacity error
+  int32_t *host_flag = nullptr;   // pinned
+
+  // Scan scratch
+  DevBuf<int64_t> scan_tmp;
+
+  bool have_forces = false;  // st[cur].fx holds the force of the current positions
+  int64_t steps_since_rebuild = 0;
+  int64_t last_pairs = 0;
+  int64_t last_evals = 0;
+};
+
+// ---- launchers (defined in the .hip translation units) -------------------------------------------
+void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t n);  // out has n+1 entries
+
+void lc_bin(Ctx &c);                       // ljg_bin.hip
+void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
+void vcl_build(Ctx &c);                    // ljg_vcl.hip
+void vcl_pos32(Ctx &c);                    // ljg_vcl.hip
+void cluster_force(Ctx &c, bool mixed, bool globals);  // ljg_force.hip
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt);  // ljg_md.hip
+void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
+void wrap_positions(Ctx &c);               // ljg_md.hip
+void permute_state(Ctx &c, const int32_t *src, int64_t new_slots);  // ljg_md.hip
+void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
+
+// Shift id (sx+1) + 3(sy+1) + 9(sz+1), sx,sy,sz in {-1,0,1}.
+__host__ __device__ inline int shift_component(int sid, int d) {
+  return (d == 0 ? sid % 3 : d == 1 ? (sid / 3) % 3 : sid / 9) - 1;
+}
+
+inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+}  // namespace ljg

@@ -1,0 +1,286 @@
+// ljg_md.hip -- device-wide scan, state permutation, velocity-Verlet integrator, periodic wrap and the
+// deterministic reductions of per-work-item partial sums.
+#include <cmath>
+
+#include "ljg_internal.h"
+
+namespace ljg {
+
+// ---------------------------------------------------------------------------------------------------
+// Exclusive scan: out[i] = sum_{k<i} in[k] for i in [0, n]; out[n] is the total.  Three phases (block
+// scan, scan of block totals, add), recursive on the block totals.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <class TIn>
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(const TIn *__restrict__ in, int64_t n_in,
+                                                              int64_t n_out, int64_t *__restrict__ out,
+                                                              int64_t *__restrict__ block_tot) {
+  __shared__ int64_t warp_tot[kScanThreads / kWave];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n_in ? (int64_t)in[i] : 0;
+    s += v[k];
+  }
+  // inclusive scan of per-thread sums within the wave
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  int64_t incl = s;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int64_t t = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += t;
+  }
+  if (lane == kWave - 1) warp_tot[w] = incl;
+  __syncthreads();
+  int64_t wprefix = 0;
+  for (int k = 0; k < w; ++k) wprefix += warp_tot[k];
+  int64_t run = wprefix + incl - s;  // exclusive prefix of this thread inside the block
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n_out) out[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kScanThreads - 1) block_tot[blockIdx.x] = wprefix + incl;
+}
+
+__global__ void k_scan_add(int64_t *__restrict__ out, int64_t n_out, const int64_t *__restrict__ block_off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_out) out[i] += block_off[i / kScanTile];
+}
+
+template <class TIn>
+static void scan_impl(Ctx &c, const TIn *in, int64_t n_in, int64_t *out, int64_t n_out, int64_t *scratch,
+                      int64_t scratch_len) {
+  const int64_t nb = (n_out + kScanTile - 1) / kScanTile;
+  if (scratch_len < 2 * (nb + 1)) throw HipError{hipErrorInvalidValue, "scan scratch too small", __LINE__};
+  int64_t *tot = scratch, *tot_scan = scratch + (nb + 1);
+  hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, c.stream, in, n_in, n_out, out,
+                     tot);
+  LJG_HIP(hipGetLastError());
+  if (nb > 1) {
+    scan_impl<int64_t>(c, tot, nb, tot_scan, nb, scratch + 2 * (nb + 1), scratch_len - 2 * (nb + 1));
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n_out, 256)), dim3(256), 0, c.stream, out, n_out, tot_scan);
+    LJG_HIP(hipGetLastError());
+  }
+}
+
+void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t n) {
+  const int64_t n_out = n + 1;
+  int64_t need = 0, m = n_out;
+  do {
+    const int64_t nb = (m + kScanTile - 1) / kScanTile;
+    need += 2 * (nb + 1);
+    m = nb;
+  } while (m > 1);
+  c.scan_tmp.ensure((size_t)need + 8);
+  scan_impl<int32_t>(c, in, n, out, n_out, c.scan_tmp.p, (int64_t)c.scan_tmp.cap);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// State permutation: new slot s takes the particle of old slot src[s]; src[s] < 0 makes a padding slot.
+__global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateView o, StateView n) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnew) return;
+  const int32_t k = src[s];
+  if (k >= 0) {
+    n.x[s] = o.x[k];   n.y[s] = o.y[k];   n.z[s] = o.z[k];
+    n.vx[s] = o.vx[k]; n.vy[s] = o.vy[k]; n.vz[s] = o.vz[k];
+    n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k];
+    n.ofx[s] = o.ofx[k]; n.ofy[s] = o.ofy[k]; n.ofz[s] = o.ofz[k];
+    n.rx[s] = o.rx[k]; n.ry[s] = o.ry[k]; n.rz[s] = o.rz[k];
+    n.type[s] = o.type[k]; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
+  } else {
+    n.x[s] = kDummyPos; n.y[s] = kDummyPos; n.z[s] = kDummyPos;
+    n.vx[s] = 0.; n.vy[s] = 0.; n.vz[s] = 0.;
+    n.fx[s] = 0.; n.fy[s] = 0.; n.fz[s] = 0.;
+    n.ofx[s] = 0.; n.ofy[s] = 0.; n.ofz[s] = 0.;
+    n.rx[s] = kDummyPos; n.ry[s] = kDummyPos; n.rz[s] = kDummyPos;
+    n.type[s] = 0; n.orig[s] = -1; n.id[s] = -1;
+  }
+}
+
+void permute_state(Ctx &c, const int32_t *src, int64_t new_slots) {
+  const int other = 1 - c.cur;
+  c.st[other].ensure((size_t)new_slots);
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, src,
+                     view(c.st[c.cur]), view(c.st[other]));
+  LJG_HIP(hipGetLastError());
+  c.cur = other;
+  c.nslots = new_slots;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Velocity Verlet, TimeDiscretization.cpp:152-165 (kick: v += (F + oldF) * (dt/(2m))) followed by
+// TimeDiscretization.cpp:17-69 (drift: oldF = F; x += v*dt + F*(dt*dt/(2m))).  Fusing the kick of
+// step k with the drift of step k+1 keeps each particle's FP64 operation sequence unchanged.  The
+// drift also refreshes the FP32 cluster-relative positions and raises the displacement flag of
+// LogicHandler.h:1000-1015 when |x - x_rebuild|^2 >= (skin/2)^2.
+struct MdArgs {
+  double dt;
+  double half_skin2;
+  int32_t kick, drift, write_pos32, M;
+  double kick_fac[kMaxTypes];   // dt / (2m), as TimeDiscretization.cpp:160 forms it
+  double drift_fac[kMaxTypes];  // dt * dt / (2m), as TimeDiscretization.cpp:40 forms it
+};
+
+__global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double4 *__restrict__ c0,
+                             float4 *__restrict__ pos32, int32_t *__restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  if (s.orig[i] < 0) return;
+  const int t = s.type[i];
+  double fx = s.fx[i], fy = s.fy[i], fz = s.fz[i];
+  double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+  if (a.kick) {
+    const double fac = a.kick_fac[t];
+    vx += (fx + s.ofx[i]) * fac;
+    vy += (fy + s.ofy[i]) * fac;
+    vz += (fz + s.ofz[i]) * fac;
+    s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
+  }
+  if (!a.drift) return;
+  const double ff = a.drift_fac[t];
+  const double dx = vx * a.dt + fx * ff;
+  const double dy = vy * a.dt + fy * ff;
+  const double dz = vz * a.dt + fz * ff;
+  s.ofx[i] = fx; s.ofy[i] = fy; s.ofz[i] = fz;
+  const double x = s.x[i] + dx, y = s.y[i] + dy, z = s.z[i] + dz;
+  s.x[i] = x; s.y[i] = y; s.z[i] = z;
+  const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
+  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) flag[0] = 1;
+  if (a.write_pos32) {
+    const double4 r = c0[i / a.M];
+    pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
+  }
+}
+
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt) {
+  MdArgs a{};
+  a.dt = dt;
+  const double hs = c.prm.skin * 0.5;
+  a.half_skin2 = hs * hs;
+  a.kick = kick;
+  a.drift = drift;
+  a.write_pos32 = c.layout == Layout::Clusters;
+  a.M = c.M;
+  for (int t = 0; t < kMaxTypes; ++t) {
+    const double m = c.lj.mass[t] > 0. ? c.lj.mass[t] : 1.;
+    a.kick_fac[t] = dt / (2 * m);
+    a.drift_fac[t] = dt * dt / (2 * m);
+  }
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.flag.p);
+  LJG_HIP(hipGetLastError());
+}
+
+__global__ void k_set_rebuild_ref(int64_t nslots, StateView s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  s.rx[i] = s.x[i]; s.ry[i] = s.y[i]; s.rz[i] = s.z[i];
+}
+
+void md_set_rebuild_reference(Ctx &c) {
+  hipLaunchKernelGGL(k_set_rebuild_ref, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                     view(c.st[c.cur]));
+  LJG_HIP(hipGetLastError());
+}
+
+// Periodic wrap of particles that left the box, RegularGridDecomposition.cpp:563-587.
+struct WrapArgs {
+  double mn[3], mx[3], L[3], just_inside[3];
+};
+
+__global__ void k_wrap(int64_t nslots, StateView s, WrapArgs w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots || s.orig[i] < 0) return;
+  double *c[3] = {s.x, s.y, s.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double p = c[d][i];
+    if (p < w.mn[d]) {
+      const double q = p + w.L[d];
+      p = q < w.just_inside[d] ? q : w.just_inside[d];
+      c[d][i] = p;
+    } else if (p >= w.mx[d]) {
+      const double q = p - w.L[d];
+      p = q > w.mn[d] ? q : w.mn[d];
+      c[d][i] = p;
+    }
+  }
+}
+
+void wrap_positions(Ctx &c) {
+  WrapArgs w{};
+  for (int d = 0; d < 3; ++d) {
+    w.mn[d] = c.prm.box_min[d];
+    w.mx[d] = c.prm.box_max[d];
+    w.L[d] = c.prm.box_max[d] - c.prm.box_min[d];
+    w.just_inside[d] = std::nextafter(c.prm.box_max[d], c.prm.box_min[d]);
+  }
+  hipLaunchKernelGGL(k_wrap, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                     view(c.st[c.cur]), w);
+  LJG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Deterministic reduction of per-work-item partials: one workgroup, fixed assignment of items to
+// threads and a fixed tree, so the result does not depend on scheduling.
+__global__ __launch_bounds__(1024) void k_reduce(int64_t n, const double *__restrict__ e,
+                                                 const double *__restrict__ v, const int64_t *__restrict__ h,
+                                                 double *__restrict__ out, int64_t *__restrict__ out_h,
+                                                 int32_t globals) {
+  __shared__ double se[1024], sv[1024];
+  __shared__ int64_t sh[1024];
+  double ae = 0., av = 0.;
+  int64_t ah = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    if (globals) {
+      ae += e[i];
+      av += v[i];
+    }
+    ah += h[i];
+  }
+  se[threadIdx.x] = ae;
+  sv[threadIdx.x] = av;
+  sh[threadIdx.x] = ah;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      sv[threadIdx.x] += sv[threadIdx.x + s];
+      sh[threadIdx.x] += sh[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = se[0];
+    out[1] = sv[0];
+    out_h[0] = sh[0];
+  }
+}
+
+void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals) {
+  c.red_out.ensure(2);
+  c.red_hits.ensure(1);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c.stream, nitems, c.part_e.p, c.part_v.p, c.part_hits.p,
+                     c.red_out.p, c.red_hits.p, (int32_t)globals);
+  LJG_HIP(hipGetLastError());
+  double hv[2] = {0., 0.};
+  int64_t hh = 0;
+  LJG_HIP(hipMemcpyAsync(hv, c.red_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipMemcpyAsync(&hh, c.red_hits.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  // Full lists visit each unique pair from both sides: LJFunctor::endTraversal halves the sums and
+  // divides the energy by 6 (LJFunctor.h:661-685).
+  out->potential_energy = hv[0] * 0.5 / 6.;
+  out->virial = hv[1] * 0.5;
+  out->pair_count = hh / 2;
+}
+
+}  // namespace ljg

@@ -1,0 +1,341 @@
+// ljg_vcl.hip -- Verlet-cluster build on the device (periodic box, shift vectors instead of halo copies):
+//   tower of every particle (ClusterTowerBlock2D.h:220-245), counting sort into towers, per-tower z sort
+//   in LDS (ClusterTower.h:93; ties by particle id), clusters of M consecutive particles padded with
+//   dummies (ClusterTower.h:95-103), cluster bounding boxes (Cluster.h:135-149) and the cluster-pair
+//   lists (VerletClusterListsRebuilder.h:237-409, boxDistanceSquared ArrayMath.h:697-707).
+// Compiled with -ffp-contract=off so every FP64 comparison rounds exactly like the reference expression.
+#include <cmath>
+
+#include "ljg_internal.h"
+
+namespace ljg {
+
+__device__ inline int32_t tower_index(const TowerGeom &g, double px, double py) {
+  const double p[2] = {px, py};
+  int32_t t2[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    long t = (long)floor((p[d] - g.box_min[d]) * g.recip[d]);
+    if (t < 0) t = 0;
+    if (t > g.nt[d] - 1) t = g.nt[d] - 1;
+    t2[d] = (int32_t)t;
+  }
+  return t2[0] + t2[1] * g.nt[0];
+}
+
+__global__ void k_tower_hist(int64_t nslots, const double *__restrict__ x, const double *__restrict__ y,
+                             const int32_t *__restrict__ orig, TowerGeom g, int32_t *__restrict__ tower_of,
+                             int32_t *__restrict__ rank, int32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  if (orig[i] < 0) {
+    tower_of[i] = -1;
+    return;
+  }
+  const int32_t t = tower_index(g, x[i], y[i]);
+  tower_of[i] = t;
+  rank[i] = atomicAdd(&count[t], 1);
+}
+
+__global__ void k_tower_scatter(int64_t nslots, const int32_t *__restrict__ tower_of, const int32_t *__restrict__ rank,
+                                const int64_t *__restrict__ start, int32_t *__restrict__ items) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const int32_t t = tower_of[i];
+  if (t < 0) return;
+  items[start[t] + rank[i]] = (int32_t)i;
+}
+
+__global__ void k_tower_nclusters(int64_t ntowers, const int32_t *__restrict__ count, int32_t M,
+                                  int32_t *__restrict__ ccount) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntowers) ccount[t] = (count[t] + M - 1) / M;
+}
+
+// One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
+// are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.
+constexpr int kSortThreads = 1024;
+
+__device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, const int64_t *__restrict__ id) {
+  if (sb < 0) return sa >= 0;  // padding sorts last
+  if (sa < 0) return false;
+  if (za < zb) return true;
+  if (za > zb) return false;
+  return id[sa] < id[sb];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_tower_sort(
+    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
+    const double *__restrict__ z, const int64_t *__restrict__ id, const int64_t *__restrict__ cstart, int32_t M,
+    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, int32_t *__restrict__ err) {
+  __shared__ double sz[kTowerSortCap];
+  __shared__ int32_t ss[kTowerSortCap];
+  const int64_t t = blockIdx.x;
+  if (t >= ntowers) return;
+  const int64_t b = start[t];
+  const int n = (int)(start[t + 1] - b);
+  if (n > kTowerSortCap) {
+    if (threadIdx.x == 0) err[1] = 1;
+    return;
+  }
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int k = threadIdx.x; k < P; k += kSortThreads) {
+    if (k < n) {
+      const int32_t s = items[b + k];
+      ss[k] = s;
+      sz[k] = z[s];
+    } else {
+      ss[k] = -1;
+      sz[k] = 0.;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < P / 2; k += kSortThreads) {
+        const int lo = 2 * k - (k & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const double za = sz[lo], zb = sz[hi];
+        const int32_t sa = ss[lo], sb = ss[hi];
+        const bool swap = up ? zid_less(zb, sb, za, sa, id) : zid_less(za, sa, zb, sb, id);
+        if (swap) {
+          sz[lo] = zb; sz[hi] = za;
+          ss[lo] = sb; ss[hi] = sa;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t c0 = cstart[t], c1 = cstart[t + 1];
+  const int64_t nslot = (c1 - c0) * M;
+  for (int64_t k = threadIdx.x; k < nslot; k += kSortThreads) src[c0 * M + k] = k < n ? ss[k] : -1;
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += kSortThreads) cluster_tower[c] = (int32_t)t;
+}
+
+// Cluster bounding box (Cluster.h:135-149: xy min/max over the members, z from first and last) over the
+// real members, and the reference point c0 = box centre with the number of real members in c0.w.
+__global__ void k_cluster_bbox(int64_t nclusters, int32_t M, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z,
+                               const int32_t *__restrict__ orig, double *__restrict__ bbox,
+                               double4 *__restrict__ c0) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nclusters) return;
+  const int64_t s0 = c * M;
+  double xl = x[s0], xh = x[s0], yl = y[s0], yh = y[s0];
+  const double zl = z[s0];
+  double zh = zl;
+  int nreal = 1;
+  for (int m = 1; m < M; ++m) {
+    const int64_t s = s0 + m;
+    if (orig[s] < 0) break;
+    xl = fmin(xl, x[s]); xh = fmax(xh, x[s]);
+    yl = fmin(yl, y[s]); yh = fmax(yh, y[s]);
+    zh = z[s];
+    ++nreal;
+  }
+  double *bb = bbox + 6 * c;
+  bb[0] = xl; bb[1] = yl; bb[2] = zl; bb[3] = xh; bb[4] = yh; bb[5] = zh;
+  c0[c] = make_double4((xl + xh) * 0.5, (yl + yh) * 0.5, (zl + zh) * 0.5, (double)nreal);
+}
+
+__device__ inline double box_dist2(const double *a, const double *b, double shx, double shy, double shz) {
+  const double sh[3] = {shx, shy, shz};
+  double s = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double bmin = b[d] + sh[d], bmax = b[3 + d] + sh[d];
+    const double ab0 = a[d] - bmax;
+    const double ba0 = bmin - a[3 + d];
+    const double ab = ab0 > 0. ? ab0 : 0.;
+    const double ba = ba0 > 0. ? ba0 : 0.;
+    s += ab * ab;
+    s += ba * ba;
+  }
+  return s;
+}
+
+// Cluster-pair lists, one thread per i-cluster; FILL=false counts, FILL=true writes.  The traversal
+// order (oy, ox, sz, j) is the oracle's, so lists compare entry by entry.
+template <bool FILL>
+__global__ void k_cluster_lists(int64_t nclusters, TowerGeom g, const int32_t *__restrict__ cluster_tower,
+                                const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
+                                int32_t *__restrict__ count, const int64_t *__restrict__ off,
+                                int2 *__restrict__ list) {
+  const int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= nclusters) return;
+  const int32_t t = cluster_tower[ci];
+  const int tx = t % g.nt[0], ty = t / g.nt[0];
+  double bi[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
+  const double il2 = g.ilen * g.ilen;
+  // prefilter window in z, widened so it never drops a cluster the exact test accepts
+  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+  int64_t w = FILL ? off[ci] : 0;
+  int32_t cnt = 0;
+  for (int oy = -g.R; oy <= g.R; ++oy) {
+    const int aoy = oy < 0 ? -oy : oy;
+    const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+    for (int ox = -g.R; ox <= g.R; ++ox) {
+      const int aox = ox < 0 ? -ox : ox;
+      const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+      if (dtx * dtx + dty * dty > il2) continue;
+      int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+      while (jx < 0) { jx += g.nt[0]; --sx; }
+      while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+      while (jy < 0) { jy += g.nt[1]; --sy; }
+      while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+      const int64_t tj = jx + (int64_t)jy * g.nt[0];
+      const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+      for (int sz = -1; sz <= 1; ++sz) {
+        const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
+        // first cluster whose zhi + shz reaches zlo_i - window (zhi is non-decreasing in the tower)
+        int64_t lo = cb, hi = ce;
+        const double zmin = bi[2] - zwin;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (bbox[6 * mid + 5] + shz < zmin) lo = mid + 1;
+          else hi = mid;
+        }
+        const double zmax = bi[5] + zwin;
+        for (int64_t cj = lo; cj < ce; ++cj) {
+          const double *bj = bbox + 6 * cj;
+          if (bj[2] + shz > zmax) break;
+          if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+            if (FILL) list[w++] = make_int2((int32_t)cj, (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (!FILL) count[ci] = cnt;
+}
+
+// FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots are
+// parked 3e4 away so that no real particle reaches them.
+__global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
+                        const double *__restrict__ z, const int32_t *__restrict__ type,
+                        const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  if (orig[s] < 0) {
+    pos32[s] = make_float4(3.0e4f, 3.0e4f, 3.0e4f, __int_as_float(0));
+    return;
+  }
+  const double4 r = c0[s / M];
+  pos32[s] = make_float4((float)(x[s] - r.x), (float)(y[s] - r.y), (float)(z[s] - r.z), __int_as_float(type[s]));
+}
+
+void vcl_pos32(Ctx &c) {
+  c.pos32.ensure(c.nslots);
+  State &s = c.st[c.cur];
+  hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
+                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p);
+  LJG_HIP(hipGetLastError());
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, p, v);
+  LJG_HIP(hipGetLastError());
+}
+
+void vcl_build(Ctx &c) {
+  const int M = c.M;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  // Tower grid: ClusterTowerBlock2D::estimateOptimalGridSideLength (ClusterTowerBlock2D.h:140-168) with
+  // the owned towers only (periodic images come from shift vectors) and resize (:89-114).
+  TowerGeom &g = c.tg;
+  double L[3];
+  for (int d = 0; d < 3; ++d) {
+    L[d] = c.prm.box_max[d] - c.prm.box_min[d];
+    g.box_min[d] = c.prm.box_min[d];
+    g.L[d] = L[d];
+  }
+  const double density = (double)c.n / (L[0] * L[1] * L[2]);
+  const double side0 = std::cbrt((double)M / density);
+  for (int d = 0; d < 2; ++d) {
+    long t = (long)std::ceil(L[d] / side0);
+    if (t < 1) t = 1;
+    g.nt[d] = (int32_t)t;
+    g.side[d] = L[d] / (double)t;
+    g.recip[d] = 1. / g.side[d];
+  }
+  const long r0 = (long)std::ceil(ilen / g.side[0]), r1 = (long)std::ceil(ilen / g.side[1]);
+  g.R = (int32_t)(r0 > r1 ? r0 : r1);
+  g.ilen = ilen;
+  const int64_t ntow = (int64_t)g.nt[0] * g.nt[1];
+  c.ntowers = ntow;
+
+  const int64_t ns = c.nslots;
+  State &s = c.st[c.cur];
+  c.tower_of.ensure(ns);
+  c.rank.ensure(ns);
+  c.tower_count.ensure(ntow + 1);
+  c.tower_start.ensure(ntow + 1);
+  c.tower_items.ensure(c.n + 1);
+  c.ctow_count.ensure(ntow + 1);
+  c.ctow_start.ensure(ntow + 1);
+  c.flag.ensure(4);
+  fill_i32(c, c.tower_count.p, ntow + 1, 0);
+  fill_i32(c, c.flag.p + 1, 1, 0);
+  hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.orig.p, g,
+                     c.tower_of.p, c.rank.p, c.tower_count.p);
+  LJG_HIP(hipGetLastError());
+  exclusive_scan_i32_to_i64(c, c.tower_count.p, c.tower_start.p, ntow);
+  hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.tower_of.p, c.rank.p,
+                     c.tower_start.p, c.tower_items.p);
+  LJG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_tower_nclusters, dim3(blocks_for(ntow, 256)), dim3(256), 0, c.stream, ntow, c.tower_count.p, M,
+                     c.ctow_count.p);
+  LJG_HIP(hipGetLastError());
+  exclusive_scan_i32_to_i64(c, c.ctow_count.p, c.ctow_start.p, ntow);
+  int64_t ncl = 0;
+  LJG_HIP(hipMemcpyAsync(&ncl, c.ctow_start.p + ntow, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.nclusters = ncl;
+  const int64_t new_slots = ncl * M;
+  c.perm.ensure(new_slots);
+  c.cluster_tower.ensure(ncl);
+  hipLaunchKernelGGL(k_tower_sort, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow, c.tower_start.p,
+                     c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p, c.flag.p);
+  LJG_HIP(hipGetLastError());
+  int32_t hflag[2] = {0, 0};
+  LJG_HIP(hipMemcpyAsync(hflag, c.flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  if (hflag[1]) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
+  permute_state(c, c.perm.p, new_slots);
+  c.M = M;
+  State &t = c.st[c.cur];
+  c.bbox.ensure(6 * ncl);
+  c.c0.ensure(ncl);
+  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(ncl, 256)), dim3(256), 0, c.stream, ncl, M, t.x.p, t.y.p, t.z.p,
+                     t.orig.p, c.bbox.p, c.c0.p);
+  LJG_HIP(hipGetLastError());
+  c.list_count.ensure(ncl + 1);
+  c.list_off.ensure(ncl + 1);
+  hipLaunchKernelGGL(k_cluster_lists<false>, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
+                     c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.list_count.p, c.list_off.p, c.list.p);
+  LJG_HIP(hipGetLastError());
+  exclusive_scan_i32_to_i64(c, c.list_count.p, c.list_off.p, ncl);
+  int64_t nent = 0;
+  LJG_HIP(hipMemcpyAsync(&nent, c.list_off.p + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.nentries = nent;
+  c.list.ensure(nent + 1);
+  hipLaunchKernelGGL(k_cluster_lists<true>, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
+                     c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.list_count.p, c.list_off.p, c.list.p);
+  LJG_HIP(hipGetLastError());
+  c.layout = Layout::Clusters;
+  vcl_pos32(c);
+}
+
+}  // namespace ljg

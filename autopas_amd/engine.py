@@ -1,0 +1,271 @@
+"""ctypes binding of the engine's C ABI (include/ljgpu.h).
+
+This is the thin host layer the tests and ``bench.py`` drive; it owns no compute.  Loading fails loudly
+when ``libljgpu.so`` has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libljgpu.so")
+
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_STATE = 3
+ERR_CAPACITY = 4
+ERR_UNSUPPORTED = 5
+ERR_NODEVICE = 6
+
+KERNEL_LC_FP64 = 0
+KERNEL_CLUSTER_FP64 = 1
+KERNEL_CLUSTER_MIXED = 2
+KERNEL_NAMES = {
+    KERNEL_LC_FP64: "lc_fp64",
+    KERNEL_CLUSTER_FP64: "cluster_fp64",
+    KERNEL_CLUSTER_MIXED: "cluster_fp32pair_fp64acc",
+}
+
+_d = ctypes.c_double
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("box_min", _d * 3),
+        ("box_max", _d * 3),
+        ("cutoff", _d),
+        ("skin", _d),
+        ("cluster_size", _i32),
+        ("rebuild_frequency", _i32),
+        ("dynamic_rebuild", _i32),
+        ("apply_shift", _i32),
+        ("device", _i32),
+        ("reserved", _i32 * 7),
+    ]
+
+
+class Globals(ctypes.Structure):
+    _fields_ = [
+        ("potential_energy", _d),
+        ("virial", _d),
+        ("pair_count", _i64),
+        ("candidate_evals", _i64),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", _i64),
+        ("rebuilds", _i64),
+        ("pair_interactions", _i64),
+        ("force_ms", _d),
+        ("force_launches", _i64),
+        ("rebuild_ms", _d),
+        ("wall_ms", _d),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ljg_create": (_i32, [ctypes.POINTER(Params), ctypes.POINTER(_p)]),
+    "ljg_destroy": (_i32, [_p]),
+    "ljg_last_error": (ctypes.c_char_p, [_p]),
+    "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
+    "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
+    "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "ljg_set_forces": (_i32, [_p, _p, _p, _p]),
+    "ljg_bin": (_i32, [_p]),
+    "ljg_build_pairs": (_i32, [_p]),
+    "ljg_force": (_i32, [_p, _i32, _i32, ctypes.POINTER(Globals)]),
+    "ljg_step": (_i32, [_p, _i32, _d, _i32, ctypes.POINTER(RunStats)]),
+    "ljg_set_timing": (_i32, [_p, _i32]),
+    "ljg_download": (_i32, [_p] + [_p] * 9),
+    "ljg_get_cells": (_i32, [_p, _p, _p]),
+    "ljg_cluster_sizes": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
+    "ljg_get_clusters": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
+    "ljg_generate_lattice": (_i32, [ctypes.c_uint64, _i64, _i64, _i64, _i64, _d, _p, _d, _d, _i32] + [_p] * 7),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libljgpu.so (once).  Raises if it is missing: the engine has no fallback path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP engine first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+def generate_lattice(seed, n, dims, spacing, box_min=(0.0, 0.0, 0.0), jitter=0.0, temperature=0.0, ntypes=1):
+    """Seeded jittered simple-cubic lattice (host-side generator in the engine library)."""
+    lib = load_library()
+    x, y, z, vx, vy, vz = (np.empty(n) for _ in range(6))
+    t = np.empty(n, dtype=np.int32)
+    bm = np.asarray(box_min, dtype=np.float64)
+    st = lib.ljg_generate_lattice(
+        seed, n, dims[0], dims[1], dims[2], spacing, _ptr(bm), jitter, temperature, ntypes,
+        _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(t),
+    )
+    if st != OK:
+        raise EngineError(st, "ljg_generate_lattice")
+    return x, y, z, vx, vy, vz, t
+
+
+class Engine:
+    """One periodic box on one GPU."""
+
+    def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
+                 dynamic_rebuild=True, apply_shift=True, device=0):
+        self.lib = load_library()
+        p = Params()
+        p.box_min[:] = list(box_min)
+        p.box_max[:] = list(box_max)
+        p.cutoff = cutoff
+        p.skin = skin
+        p.cluster_size = cluster_size
+        p.rebuild_frequency = rebuild_frequency
+        p.dynamic_rebuild = int(bool(dynamic_rebuild))
+        p.apply_shift = int(bool(apply_shift))
+        p.device = device
+        self.params = p
+        h = _p()
+        st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))
+        if st != OK:
+            raise EngineError(st, "ljg_create")
+        self.h = h
+        self.n = 0
+
+    def _check(self, st, what):
+        if st != OK:
+            msg = self.lib.ljg_last_error(self.h)
+            raise EngineError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ljg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        self._check(self.lib.ljg_device_name(self.h, buf, 256), "ljg_device_name")
+        return buf.value.decode()
+
+    def set_types(self, epsilon, sigma, mass=None):
+        e = np.ascontiguousarray(epsilon, dtype=np.float64)
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        self._check(self.lib.ljg_set_types(self.h, len(e), _ptr(e), _ptr(s), _ptr(m)), "ljg_set_types")
+
+    def upload(self, x, y, z, vx=None, vy=None, vz=None, type=None, id=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (x, y, z, vx, vy, vz)]
+        t = None if type is None else np.ascontiguousarray(type, dtype=np.int32)
+        i = None if id is None else np.ascontiguousarray(id, dtype=np.int64)
+        n = len(arrs[0])
+        self._keep = arrs + [t, i]
+        self._check(
+            self.lib.ljg_upload(self.h, n, *[_ptr(a) for a in arrs], _ptr(t), _ptr(i)), "ljg_upload"
+        )
+        self.n = n
+
+    def set_forces(self, fx, fy, fz):
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz)]
+        self._check(self.lib.ljg_set_forces(self.h, *[_ptr(a) for a in f]), "ljg_set_forces")
+
+    def bin(self):
+        self._check(self.lib.ljg_bin(self.h), "ljg_bin")
+
+    def build_pairs(self):
+        self._check(self.lib.ljg_build_pairs(self.h), "ljg_build_pairs")
+
+    def force(self, kernel=KERNEL_CLUSTER_MIXED, globals=True):
+        g = Globals()
+        self._check(self.lib.ljg_force(self.h, kernel, int(bool(globals)), ctypes.byref(g)), "ljg_force")
+        return g
+
+    def set_timing(self, on=True):
+        self._check(self.lib.ljg_set_timing(self.h, int(bool(on))), "ljg_set_timing")
+
+    def step(self, nsteps, dt, kernel=KERNEL_CLUSTER_MIXED):
+        rs = RunStats()
+        self._check(self.lib.ljg_step(self.h, nsteps, dt, kernel, ctypes.byref(rs)), "ljg_step")
+        return rs
+
+    def download(self):
+        out = [np.empty(self.n) for _ in range(9)]
+        self._check(self.lib.ljg_download(self.h, *[_ptr(a) for a in out]), "ljg_download")
+        keys = ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz")
+        return dict(zip(keys, out))
+
+    def cells(self):
+        c = np.empty(self.n, dtype=np.int64)
+        nh = np.empty(3, dtype=np.int64)
+        self._check(self.lib.ljg_get_cells(self.h, _ptr(c), _ptr(nh)), "ljg_get_cells")
+        return c, nh
+
+    def clusters(self):
+        ns, nc, ne = (np.zeros(1, dtype=np.int64) for _ in range(3))
+        tpd = np.zeros(2, dtype=np.int64)
+        side = np.zeros(2)
+        r = np.zeros(1, dtype=np.int64)
+        self._check(
+            self.lib.ljg_cluster_sizes(self.h, _ptr(ns), _ptr(nc), _ptr(ne), _ptr(tpd), _ptr(side), _ptr(r)),
+            "ljg_cluster_sizes",
+        )
+        ns, nc, ne = int(ns[0]), int(nc[0]), int(ne[0])
+        out = dict(
+            slot_particle=np.empty(ns, dtype=np.int64),
+            cluster_tower=np.empty(nc, dtype=np.int32),
+            list_offsets=np.empty(nc + 1, dtype=np.int64),
+            list_j=np.empty(max(ne, 1), dtype=np.int32),
+            list_shift=np.empty(max(ne, 1), dtype=np.int32),
+            bbox=np.empty(6 * nc),
+        )
+        self._check(
+            self.lib.ljg_get_clusters(
+                self.h, *[_ptr(out[k]) for k in ("slot_particle", "cluster_tower", "list_offsets", "list_j",
+                                                   "list_shift", "bbox")]
+            ),
+            "ljg_get_clusters",
+        )
+        out["list_j"] = out["list_j"][:ne]
+        out["list_shift"] = out["list_shift"][:ne]
+        out.update(towers_per_dim=tpd, side=side, towers_per_il=int(r[0]))
+        return out

@@ -1,0 +1,178 @@
+"""Benchmark: LJ pair interactions per second on one MI355X, uniform bulk liquid.
+
+Workload (the bulk-liquid configuration): 1,000,000 particles on a 100^3 simple-cubic lattice at reduced
+density 0.75 (a = 1.1006, periodic box 110.06), seeded uniform jitter +-0.05 sigma, Maxwell-Boltzmann
+velocities at T = 0.9 (zero net momentum), LJ eps = sigma = 1, rc = 2.5, skin 0.3, velocity Verlet with
+dt = 0.002, Verlet-cluster lists rebuilt every 10 steps or once a particle moved skin/2.
+
+A step is one full MD step on the device: drift, (rebuild), force, kick.  ``value`` is the number of
+unique pairs with r <= rc summed over the timed steps divided by the timed wall time, for the
+FP32-pair / FP64-accumulate cluster kernel; the plain FP64 linked-cells kernel is timed on the same
+system and reported beside it.  Inputs are resident in HBM before timing starts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LJ pair-interactions/s (1 GPU) vs plain FP64 kernel; p50/p99 job latency"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
+FLOP_PER_PAIR = 26        # LJFunctor.h:722-789: 8 (distance) + 18 (Newton-3 kernel call)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n-side", type=int, default=100)
+    p.add_argument("--cluster-size", type=int, default=0, help="0 = pick by a short tuning pass")
+    p.add_argument("--plain-steps", type=int, default=200)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
+    return p.parse_args()
+
+
+def make_system(n_side, seed=2):
+    import autopas_amd as ap
+
+    rho = 0.75
+    a = rho ** (-1.0 / 3.0)
+    L = n_side * a
+    n = n_side ** 3
+    x, y, z, vx, vy, vz, _ = ap.generate_lattice(seed, n, (n_side,) * 3, a, (0.0, 0.0, 0.0), 0.05, 0.9)
+    return dict(x=x, y=y, z=z, vx=vx, vy=vy, vz=vz, L=L, n=n)
+
+
+def run(sysd, kernel, cluster_size, steps, warmup, timing=True):
+    import autopas_amd as ap
+
+    L = sysd["L"]
+    e = ap.Engine((0.0, 0.0, 0.0), (L, L, L), 2.5, 0.3, cluster_size=cluster_size, rebuild_frequency=10,
+                  dynamic_rebuild=True, apply_shift=True)
+    try:
+        e.upload(sysd["x"], sysd["y"], sysd["z"], sysd["vx"], sysd["vy"], sysd["vz"])
+        if warmup:
+            e.step(warmup, 0.002, kernel)
+        e.set_timing(timing)
+        t0 = time.perf_counter()
+        rs = e.step(steps, 0.002, kernel)
+        t1 = time.perf_counter()
+        name = e.device_name()
+    finally:
+        e.close()
+    wall = t1 - t0
+    return dict(wall_s=wall, pairs=rs.pair_interactions, steps=rs.steps, rebuilds=rs.rebuilds,
+                force_ms=rs.force_ms, force_launches=rs.force_launches, rebuild_ms=rs.rebuild_ms,
+                pairs_per_s=rs.pair_interactions / wall, device=name)
+
+
+def cpu_baseline(seconds):
+    """FP64 LinkedCells + C08 + Newton-3 traversal (md-flexible's CPU path, restated in oracle/) on the
+    host cores, on a 32^3-particle sample of the same liquid (same density, seed and jitter)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+
+    s = make_system(32)
+    L = s["L"]
+    threads = min(16, os.cpu_count() or 1)
+    probe = oracle.c08_forces_halo(s["x"], s["y"], s["z"], 2.5, 0.3, (0, 0, 0), (L, L, L), threads=threads,
+                                   repeats=1)
+    reps = max(1, int(seconds / max(probe["traverse_seconds"], 1e-6)))
+    r = oracle.c08_forces_halo(s["x"], s["y"], s["z"], 2.5, 0.3, (0, 0, 0), (L, L, L), threads=threads,
+                               repeats=reps)
+    return dict(value=r["pairs"] / r["traverse_seconds"], unit="pair-interactions/s", cores=threads,
+                kind="port",
+                sample=f"32768-particle box (32^3 lattice, rho 0.75, seed 2), {reps} C08 traversals, "
+                       f"{r['pairs']} pairs each, {r['traverse_seconds'] * 1e3:.2f} ms per traversal")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU is out of scope)",
+              file=sys.stderr)
+        sys.exit(2)
+    import autopas_amd as ap
+
+    s = make_system(args.n_side)
+    M = args.cluster_size
+    tuning = {}
+    if M == 0:
+        for m in (2, 4, 8):
+            r = run(s, ap.KERNEL_CLUSTER_MIXED, m, 30, 5, timing=False)
+            tuning[m] = r["pairs_per_s"]
+        M = max(tuning, key=tuning.get)
+    mixed = run(s, ap.KERNEL_CLUSTER_MIXED, M, args.steps, args.warmup)
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "pairs_per_s": mixed["pairs_per_s"], "cluster_size": M}))
+        return
+    plain = run(s, ap.KERNEL_LC_FP64, M, args.plain_steps, min(args.warmup, 10))
+    cl64 = run(s, ap.KERNEL_CLUSTER_FP64, M, args.plain_steps, min(args.warmup, 10))
+
+    force_avg_s = mixed["force_ms"] / max(mixed["force_launches"], 1) * 1e-3
+    pairs_per_launch = mixed["pairs"] / max(mixed["steps"], 1)
+    achieved = pairs_per_launch * FLOP_PER_PAIR / force_avg_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": mixed["pairs_per_s"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["wall_s"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-pair/fp64-accumulate",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {
+            "workload": "uniform bulk LJ liquid, 1e6 particles (100^3 lattice, rho 0.75, T 0.9), rc 2.5, "
+                        "skin 0.3, dt 0.002, Verlet-cluster lists rebuilt every 10 steps",
+            "n_particles": s["n"],
+            "cluster_size": M,
+            "tuning_pairs_per_s": tuning,
+        },
+        "plain_fp64": {
+            "kernel": "linked cells, 27-cell full shell, FP64",
+            "value": plain["pairs_per_s"],
+            "ms_per_step": plain["wall_s"] / plain["steps"] * 1e3,
+            "force_ms_per_launch": plain["force_ms"] / max(plain["force_launches"], 1),
+        },
+        "cluster_fp64": {
+            "value": cl64["pairs_per_s"],
+            "ms_per_step": cl64["wall_s"] / cl64["steps"] * 1e3,
+            "force_ms_per_launch": cl64["force_ms"] / max(cl64["force_launches"], 1),
+        },
+        "speedup_vs_plain_fp64": mixed["pairs_per_s"] / plain["pairs_per_s"],
+        "ns_per_day_reduced_units": (args.steps / mixed["wall_s"]) * 0.002 * 86400,
+        "force_ms_per_launch": force_avg_s * 1e3,
+        "rebuilds": mixed["rebuilds"],
+        "rebuild_ms_total": mixed["rebuild_ms"],
+        "device": mixed["device"],
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": f"k_force_cluster<M={M}, FP32>",
+            "flop_per_pair": FLOP_PER_PAIR,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+/*
+ * ljgpu.h -- C ABI of the MI355X (gfx950) engine for AutoPas's Lennard-Jones short-range path.
+ *
+ * The engine owns device memory; host arrays passed in are borrowed for the duration of the call.
+ * Every entry point returns an ljg_status; no C++ exception crosses this boundary.  One context is
+ * one periodic box on one HIP stream and is not thread-safe; separate contexts are independent.
+ *
+ * In AutoPas the path this replaces is reached through C++ templates rather than an FFI:
+ *   AutoPas<Particle_T>::computeInteractions(Functor*)    src/autopas/AutoPasImpl.h:116-135
+ *     -> LogicHandler::computeInteractions                src/autopas/LogicHandler.h:1066-1143
+ *       -> container->rebuildNeighborLists / computeInteractions(traversal)
+ * The entry points below are the calls a C++ container + traversal plugged in under that API (see
+ * INTEGRATION.md) makes; each one cites the reference interface it stands in for.
+ */
+#ifndef LJGPU_H
+#define LJGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ljg_ctx ljg_ctx;
+typedef int ljg_status;
+
+#define LJG_OK 0
+#define LJG_ERR_ARG 1         /* invalid argument (null pointer, bad size, box smaller than ilen ...) */
+#define LJG_ERR_HIP 2         /* a HIP runtime call failed; ljg_last_error() has the text */
+#define LJG_ERR_STATE 3       /* call out of order (e.g. force before upload) */
+#define LJG_ERR_CAPACITY 4    /* a structural limit was exceeded (e.g. a tower longer than the sort tile) */
+#define LJG_ERR_UNSUPPORTED 5 /* configuration not provided by this engine */
+#define LJG_ERR_NODEVICE 6    /* no HIP device available */
+
+/* Force kernels.  The plain FP64 kernel is the linked-cells full-shell kernel; the cluster kernels run
+ * on the Verlet-cluster lists. */
+#define LJG_KERNEL_LC_FP64 0      /* LinkedCells, 27-cell full shell, FP64 everything */
+#define LJG_KERNEL_CLUSTER_FP64 1 /* cluster-pair tiles, FP64 pair term and sums */
+#define LJG_KERNEL_CLUSTER_MIXED 2 /* cluster-pair tiles, FP32 pair term, FP64 per-particle sums */
+
+typedef struct {
+  double box_min[3];
+  double box_max[3];
+  double cutoff;          /* rc; pairs with r^2 <= rc^2 interact (LJFunctor.h:300) */
+  double skin;            /* Verlet skin; interaction length ilen = rc + skin (LogicHandler.h:95) */
+  int32_t cluster_size;   /* 1, 2, 4 or 8 (VerletClusterLists clusterSize) */
+  int32_t rebuild_frequency; /* steps between list rebuilds (AutoPas rebuildFrequency) */
+  int32_t dynamic_rebuild;   /* also rebuild once any particle moved >= skin/2 (LogicHandler.h:1000-1015) */
+  int32_t apply_shift;    /* truncated-shifted potential energy (LJFunctor applyShift) */
+  int32_t device;         /* HIP device ordinal */
+  int32_t reserved[7];
+} ljg_params;
+
+typedef struct {
+  double potential_energy; /* LJFunctor::getPotentialEnergy semantics (LJFunctor.h:691-705) */
+  double virial;           /* LJFunctor::getVirial: x+y+z of sum r.F over pairs (LJFunctor.h:709-720) */
+  int64_t pair_count;      /* unique pairs with r^2 <= rc^2 (each unordered periodic pair once) */
+  int64_t candidate_evals; /* pair evaluations the kernel issued (incl. masked-out ones) */
+} ljg_globals;
+
+typedef struct {
+  int64_t steps;
+  int64_t rebuilds;
+  int64_t pair_interactions; /* sum over steps of the unique pairs within rc */
+  double force_ms;           /* summed HIP-event time of the force kernel launches (when timing is on) */
+  int64_t force_launches;
+  double rebuild_ms;         /* summed HIP-event time of rebuilds (bin/sort/list) (when timing is on) */
+  double wall_ms;            /* host wall time of the call */
+} ljg_run_stats;
+
+/* Create/destroy a context.  Replaces AutoPas::init() for the container this engine provides
+ * (AutoPasImpl.h:55-113; LogicHandler.h:58-100, checkMinimalSize :967-983). */
+ljg_status ljg_create(const ljg_params *params, ljg_ctx **out);
+ljg_status ljg_destroy(ljg_ctx *ctx);
+const char *ljg_last_error(const ljg_ctx *ctx);
+/* Human-readable name of the device the context runs on (for logs). */
+ljg_status ljg_device_name(const ljg_ctx *ctx, char *buf, int32_t buflen);
+
+/* Particle types: ParticlePropertiesLibrary::addSiteType + calculateMixingCoefficients
+ * (ParticlePropertiesLibrary.h:443-489): Lorentz-Berthelot mixing is done here, in FP64. */
+ljg_status ljg_set_types(ljg_ctx *ctx, int32_t ntypes, const double *epsilon, const double *sigma,
+                         const double *mass);
+
+/* Upload owned particles (AutoPas::addParticle, AutoPasDecl.h:172).  vx/vy/vz/type/id may be NULL
+ * (zero / type 0 / index).  Positions outside the periodic box are wrapped like md-flexible's
+ * RegularGridDecomposition.cpp:563-587.  Forces start at zero. */
+ljg_status ljg_upload(ljg_ctx *ctx, int64_t n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const int32_t *type, const int64_t *id);
+/* Set the current force (e.g. F0 restored from a checkpoint, MDFlexConfig.cpp:173). Upload order. */
+ljg_status ljg_set_forces(ljg_ctx *ctx, const double *fx, const double *fy, const double *fz);
+
+/* LinkedCells binning (counting sort into CellBlock3D cells; CellBlock3D.h:320-425, LinkedCells.h:147-203).
+ * Leaves the particle state in cell order.  Used by the plain FP64 kernel. */
+ljg_status ljg_bin(ljg_ctx *ctx);
+/* Verlet-cluster build (towers, z-sort, clusters, cluster-pair lists): VerletClusterLists::
+ * rebuildNeighborLists (VerletClusterLists.h:779-800, VerletClusterListsRebuilder.h:67-409).
+ * Leaves the particle state in cluster order. */
+ljg_status ljg_build_pairs(ljg_ctx *ctx);
+
+/* One force evaluation on the current positions with the given kernel: container->computeInteractions
+ * (traversal) + LJFunctor::endTraversal (LJFunctor.h:661-685).  Builds the structure the kernel needs
+ * if it is missing.  out may be NULL. */
+ljg_status ljg_force(ljg_ctx *ctx, int32_t kernel, int32_t calc_globals, ljg_globals *out);
+
+/* nsteps velocity-Verlet steps (TimeDiscretization.cpp:17-69, :152-165) with the rebuild policy of
+ * LogicHandler.h:987-1015, all on the device.  out may be NULL. */
+ljg_status ljg_step(ljg_ctx *ctx, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out);
+
+/* Turn HIP-event timing of force kernels and rebuilds on/off (events on the context's stream). */
+ljg_status ljg_set_timing(ljg_ctx *ctx, int32_t enabled);
+
+/* Download state in upload order.  Any pointer may be NULL. */
+ljg_status ljg_download(ljg_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                        double *fx, double *fy, double *fz);
+
+/* ---- inspection (parity tests) -------------------------------------------------------------- */
+/* Cell of each particle (upload order) after ljg_bin: CellBlock3D::get1DIndexOfPosition of the
+ * position the particle had when it was binned. */
+ljg_status ljg_get_cells(ljg_ctx *ctx, int64_t *cell_of_particle, int64_t cells_per_dim_with_halo[3]);
+/* Sizes of the cluster structure after ljg_build_pairs. */
+ljg_status ljg_cluster_sizes(ljg_ctx *ctx, int64_t *num_slots, int64_t *num_clusters, int64_t *num_entries,
+                             int64_t towers_per_dim[2], double side[2], int64_t *towers_per_il);
+/* slot_particle[num_slots] (upload index, -1 for padding), cluster_tower[num_clusters],
+ * list_offsets[num_clusters+1], list_j[num_entries], list_shift[num_entries], bbox[6*num_clusters]
+ * (xlo,ylo,zlo,xhi,yhi,zhi).  Any pointer may be NULL. */
+ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
+                            int32_t *list_j, int32_t *list_shift, double *bbox);
+
+/* ---- synthetic systems (host-side, seeded Philox4x32-10) ------------------------------------- */
+/* Simple-cubic lattice of nx*ny*nz sites, spacing a, positions box_min + (i+1/2)a plus uniform jitter in
+ * [-jitter, jitter]; the first n sites are used (n <= nx*ny*nz).  Velocities Maxwell-Boltzmann at
+ * temperature T for mass 1, net momentum removed, rescaled to T exactly with 3n-3 degrees of freedom.
+ * Types are assigned uniformly in [0, ntypes) if ntypes > 1 (type may be NULL otherwise). */
+ljg_status ljg_generate_lattice(uint64_t seed, int64_t n, int64_t nx, int64_t ny, int64_t nz, double a,
+                                const double box_min[3], double jitter, double temperature, int32_t ntypes,
+                                double *x, double *y, double *z, double *vx, double *vy, double *vz, int32_t *type);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,252 @@
+"""GPU parity: the HIP engine (through its C ABI) against the CPU oracle on the same seeded inputs.
+
+Tolerances (stated per test):
+  * cell assignment, tower/cluster layout, cluster-pair lists: bit-exact;
+  * FP64 kernels: forces within 1e-10 relative (max |dF| / max |F|), energy/virial 1e-10, pair count exact;
+  * FP32-pair / FP64-accumulate kernel: force RMS relative error <= 1e-5 (sqrt(sum|dF|^2 / sum|F|^2)),
+    energy/virial 1e-5 relative, pair count within 1e-4 relative (pairs at r ~ rc may flip in FP32).
+"""
+import numpy as np
+import pytest
+
+import autopas_amd as ap
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+RC = 2.5
+SKIN = 0.3
+
+
+def lattice_box(n_side, density, seed, jitter=0.05, temperature=0.9, ntypes=1):
+    a = density ** (-1.0 / 3.0)
+    L = n_side * a
+    n = n_side ** 3
+    x, y, z, vx, vy, vz, t = ap.generate_lattice(seed, n, (n_side,) * 3, a, (0.0, 0.0, 0.0), jitter, temperature,
+                                                 ntypes)
+    return dict(x=x, y=y, z=z, vx=vx, vy=vy, vz=vz, type=t, L=L, n=n)
+
+
+def rel_max(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def rms_rel(f, fr):
+    num = sum(np.sum((f[k] - fr[k]) ** 2) for k in ("fx", "fy", "fz"))
+    den = sum(np.sum(fr[k] ** 2) for k in ("fx", "fy", "fz"))
+    return float(np.sqrt(num / den))
+
+
+def make_engine(L, M=4, rc=RC, skin=SKIN, **kw):
+    return ap.Engine((0.0, 0.0, 0.0), (L, L, L), rc, skin, cluster_size=M, **kw)
+
+
+def test_binning_bit_exact(oracle):
+    announce("binning bit-exact: 5000 random + boundary positions, box 20^3, ilen 2.8")
+    rng = np.random.default_rng(11)
+    L = 20.0
+    x, y, z = (rng.uniform(0, L, 5000) for _ in range(3))
+    # boundary cases of CellBlock3DTest: at min, just below max, exactly on cell faces
+    edge = np.array([0.0, np.nextafter(L, 0.0), L / 7 * 3, 2.8, 5.6, 1e-300, L - 1e-12])
+    x[: len(edge)] = edge
+    y[len(edge): 2 * len(edge)] = edge
+    z[2 * len(edge): 3 * len(edge)] = edge
+    with make_engine(L) as e:
+        e.upload(x, y, z)
+        e.bin()
+        cells, nh = e.cells()
+    wx, wy, wz = oracle.wrap(x, y, z, (0, 0, 0), (L, L, L))
+    cb = oracle.cellblock((0, 0, 0), (L, L, L), RC + SKIN)
+    ref = oracle.cell_index_batch(cb, wx, wy, wz)
+    assert list(nh) == list(cb.cells_per_dim_halo)
+    assert np.array_equal(cells, ref)
+
+
+@pytest.mark.parametrize("M", [1, 2, 4, 8])
+def test_cluster_lists_bit_exact(oracle, M):
+    announce(f"cluster build bit-exact M={M}: 10^3 lattice rho 0.75")
+    s = lattice_box(10, 0.75, seed=7)
+    L = s["L"]
+    ids = np.arange(s["n"], dtype=np.int64)[::-1].copy()  # ids differ from upload index
+    with make_engine(L, M=M) as e:
+        e.upload(s["x"], s["y"], s["z"], id=ids)
+        e.build_pairs()
+        g = e.clusters()
+    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), (L, L, L), RC + SKIN, M)
+    assert list(g["towers_per_dim"]) == list(r["towers_per_dim"])
+    assert g["towers_per_il"] == r["towers_per_il"]
+    assert np.array_equal(g["side"], r["side"])
+    for k in ("slot_particle", "cluster_tower", "list_offsets", "list_j", "list_shift", "bbox"):
+        assert np.array_equal(g[k], r[k]), k
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+@pytest.mark.parametrize("M", [4, 8])
+def test_forces_vs_direct_sum(oracle, kernel, M):
+    if kernel == ap.KERNEL_LC_FP64 and M == 8:
+        pytest.skip("cluster size does not apply to the linked-cells kernel")
+    announce(f"forces vs direct sum kernel={ap.KERNEL_NAMES[kernel]} M={M}: 8^3 lattice")
+    s = lattice_box(8, 0.75, seed=3)
+    L = s["L"]
+    with make_engine(L, M=M) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        g = e.force(kernel, globals=True)
+        f = e.download()
+    ref = oracle.direct_sum(f["x"], f["y"], f["z"], RC, box_len=(L, L, L))
+    if kernel == ap.KERNEL_CLUSTER_MIXED:
+        assert rms_rel(f, ref) <= 1e-5
+        assert abs(g.potential_energy - ref["epot"]) <= 1e-5 * abs(ref["epot"])
+        assert abs(g.virial - ref["virial"]) <= 1e-5 * abs(ref["virial"])
+        assert abs(g.pair_count - ref["pairs"]) <= 1e-4 * ref["pairs"]
+    else:
+        for k in ("fx", "fy", "fz"):
+            assert rel_max(f[k], ref[k]) <= 1e-10, k
+        assert abs(g.potential_energy - ref["epot"]) <= 1e-10 * abs(ref["epot"])
+        assert abs(g.virial - ref["virial"]) <= 1e-10 * abs(ref["virial"])
+        assert g.pair_count == ref["pairs"]
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+def test_small_box_multiple_images(oracle, kernel):
+    # L = 4.31 < 2 * (rc + skin): a particle sees several images of the same neighbour.
+    announce(f"small box images kernel={ap.KERNEL_NAMES[kernel]}: 4^3 lattice rho 0.75, L=5.29")
+    s = lattice_box(4, 0.43, seed=5, jitter=0.1)
+    L = s["L"]
+    assert L < 2 * (RC + SKIN)
+    with make_engine(L, M=4) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        g = e.force(kernel, globals=True)
+        f = e.download()
+    ref = oracle.direct_sum(f["x"], f["y"], f["z"], RC, box_len=(L, L, L))
+    tol = 1e-5 if kernel == ap.KERNEL_CLUSTER_MIXED else 1e-10
+    assert rms_rel(f, ref) <= tol
+    if kernel != ap.KERNEL_CLUSTER_MIXED:
+        assert g.pair_count == ref["pairs"]
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+def test_four_type_mixing(oracle, kernel):
+    announce(f"4-type LB mixing kernel={ap.KERNEL_NAMES[kernel]}: rc 3, 8^3 lattice")
+    eps = [0.8, 1.0, 1.1, 1.2]
+    sig = [0.9, 1.0, 1.05, 1.1]
+    s = lattice_box(8, 0.7, seed=9, ntypes=4)
+    L = s["L"]
+    with make_engine(L, M=4, rc=3.0) as e:
+        e.set_types(eps, sig, [1.0] * 4)
+        e.upload(s["x"], s["y"], s["z"], type=s["type"])
+        g = e.force(kernel, globals=True)
+        f = e.download()
+    ref = oracle.direct_sum(f["x"], f["y"], f["z"], 3.0, box_len=(L, L, L), types=s["type"], eps=eps, sig=sig)
+    tol = 1e-5 if kernel == ap.KERNEL_CLUSTER_MIXED else 1e-10
+    assert rms_rel(f, ref) <= tol
+    assert abs(g.potential_energy - ref["epot"]) <= tol * abs(ref["epot"])
+
+
+@pytest.mark.parametrize("mixing", [False, True])
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+def test_known_answer_two_particles(kernel, mixing):
+    # LJFunctorTestNoGlobals.h:27-28: p1 = (0,0,0), p2 = (0.1,0.2,0.3), rc = 1, eps = sigma = 1
+    # (mixing: p2 has eps = sigma = 2).
+    announce(f"known answer two particles kernel={ap.KERNEL_NAMES[kernel]} mixing={mixing}")
+    expected = np.array([-4547248.8989645941, -9094497.7979291882, -13641746.696893783])
+    expected_mix = np.array([-835415983.7676939964294, -1670831967.5353879928588, -2506247951.3030819892883])
+    e = ap.Engine((-1.0, -1.0, -1.0), (4.0, 4.0, 4.0), 1.0, 0.3, cluster_size=2, apply_shift=False)
+    with e:
+        if mixing:
+            e.set_types([1.0, 2.0], [1.0, 2.0], [1.0, 1.0])
+        e.upload(np.array([0.0, 0.1]), np.array([0.0, 0.2]), np.array([0.0, 0.3]),
+                 type=np.array([0, 1 if mixing else 0], dtype=np.int32))
+        e.force(kernel, globals=False)
+        f = e.download()
+    exp = expected_mix if mixing else expected
+    got1 = np.array([f["fx"][0], f["fy"][0], f["fz"][0]])
+    got2 = np.array([f["fx"][1], f["fy"][1], f["fz"][1]])
+    if kernel == ap.KERNEL_CLUSTER_MIXED:
+        assert np.allclose(got1, exp, rtol=1e-5, atol=0)
+        assert np.allclose(got2, -exp, rtol=1e-5, atol=0)
+    else:
+        assert np.allclose(got1, exp, rtol=0, atol=1e-7 * max(1.0, np.abs(exp).max() / 1e7))
+        assert np.allclose(got2, -exp, rtol=0, atol=1e-7 * max(1.0, np.abs(exp).max() / 1e7))
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+@pytest.mark.parametrize("spacing,cutoff,fexp", [(1.0, 1.1, 24.0), (2 ** (1 / 6), 1.3, 0.0)])
+def test_force_calculation_grid(kernel, spacing, cutoff, fexp):
+    # ForceCalculationTest.cpp:43-86: 2x2x1 grid (offset spacing/2... here spacing/2), eps24 = 24, sigma 1.
+    announce(f"2x2x1 grid kernel={ap.KERNEL_NAMES[kernel]} spacing={spacing:.4f} rc={cutoff}")
+    L = 3.0 * max(spacing, 1.0) + cutoff
+    e = ap.Engine((0.0, 0.0, 0.0), (L, L, L), cutoff, 0.2, cluster_size=2, apply_shift=False)
+    pos = np.array([[0.5, 0.5, 0.5], [0.5 + spacing, 0.5, 0.5], [0.5, 0.5 + spacing, 0.5],
+                    [0.5 + spacing, 0.5 + spacing, 0.5]])
+    with e:
+        e.upload(pos[:, 0], pos[:, 1], pos[:, 2])
+        e.force(kernel, globals=False)
+        f = e.download()
+    exp = np.array([[-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 1, 0]]) * fexp
+    got = np.stack([f["fx"], f["fy"], f["fz"]], axis=1)
+    tol = 1e-13 if kernel != ap.KERNEL_CLUSTER_MIXED else 1e-4
+    assert np.allclose(got, exp, rtol=0, atol=tol)
+
+
+def test_md_trajectory_lc_fp64(oracle):
+    announce("md trajectory LC FP64: 6^3 lattice, 40 steps dt 0.002, rebuild 10")
+    s = lattice_box(6, 0.75, seed=4, temperature=0.9)
+    L = s["L"]
+    dt = 0.002
+    with make_engine(L, M=4, rebuild_frequency=10) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        e.step(40, dt, ap.KERNEL_LC_FP64)
+        g = e.download()
+    x, y, z = s["x"].copy(), s["y"].copy(), s["z"].copy()
+    vx, vy, vz = s["vx"].copy(), s["vy"].copy(), s["vz"].copy()
+    fx, fy, fz = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    m = np.array([1.0])
+    for _ in range(40):
+        ofx, ofy, ofz = oracle.verlet_positions(x, y, z, vx, vy, vz, fx, fy, fz, m, None, dt)
+        x, y, z = oracle.wrap(x, y, z, (0, 0, 0), (L, L, L))
+        r = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L))
+        fx, fy, fz = r["fx"], r["fy"], r["fz"]
+        oracle.verlet_velocities(vx, vy, vz, fx, fy, fz, ofx, ofy, ofz, m, None, dt)
+    # the GPU wraps only at rebuilds; compare minimum-image displacement
+    d = np.stack([g["x"] - x, g["y"] - y, g["z"] - z])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-9
+    assert np.max(np.abs(g["vx"] - vx)) < 1e-8
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_MIXED])
+def test_deterministic(kernel):
+    announce(f"determinism kernel={ap.KERNEL_NAMES[kernel]}: two identical runs")
+    s = lattice_box(8, 0.75, seed=12)
+    outs = []
+    for _ in range(2):
+        with make_engine(s["L"], M=4) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            e.step(25, 0.002, kernel)
+            outs.append(e.download())
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_mixed_precision_accuracy_c2_subsample(oracle):
+    # 32^3 particles at rho 0.75 (seed 2): the subsample box on which the FP32-pair kernel's force error is
+    # quoted against FP64.
+    announce("mixed precision accuracy: 32^3 lattice rho 0.75 seed 2")
+    s = lattice_box(32, 0.75, seed=2, temperature=0.9)
+    L = s["L"]
+    res = {}
+    for kernel in (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64):
+        with make_engine(L, M=4) as e:
+            e.upload(s["x"], s["y"], s["z"])
+            g = e.force(kernel, globals=True)
+            res[kernel] = (e.download(), g)
+    f = res[ap.KERNEL_CLUSTER_MIXED][0]
+    ref = oracle.lc_forces(f["x"], f["y"], f["z"], RC, SKIN, (0, 0, 0), (L, L, L), threads=8)
+    assert rms_rel(res[ap.KERNEL_CLUSTER_FP64][0], ref) <= 1e-12
+    assert rms_rel(res[ap.KERNEL_LC_FP64][0], ref) <= 1e-12
+    err = rms_rel(f, ref)
+    print(f"force RMS relative error FP32-pair/FP64-acc vs FP64: {err:.3e}")
+    assert err <= 1e-5
+    assert res[ap.KERNEL_LC_FP64][1].pair_count == ref["pairs"]
+    assert res[ap.KERNEL_CLUSTER_FP64][1].pair_count == ref["pairs"]
