@@ -62,19 +62,6 @@ __global__ void k_from_upload_order(int64_t nslots, const int32_t *__restrict__ 
   if (o >= 0) out[s] = in[o];
 }
 
-__global__ void k_accumulate_hits(int64_t nitems, const int64_t *__restrict__ ph, int64_t *__restrict__ acc) {
-  __shared__ int64_t sh[1024];
-  int64_t a = 0;
-  for (int64_t i = threadIdx.x; i < nitems; i += 1024) a += ph[i];
-  sh[threadIdx.x] = a;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) acc[0] += sh[0];
-}
-
 __global__ void k_zero_i64(int64_t n, int64_t *p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -99,9 +86,7 @@ static bool layout_matches(const Ctx &c, int kernel) {
   return is_cluster_kernel(kernel) ? c.layout == Layout::Clusters : c.layout == Layout::Cells;
 }
 
-static int64_t partial_items(const Ctx &c, int kernel) {
-  return is_cluster_kernel(kernel) ? blocks_for(c.nclusters, 4) : blocks_for(c.n, 256);
-}
+static int64_t partial_items(const Ctx &c, int /*kernel*/) { return c.part_items; }
 
 static void run_force(Ctx &c, int kernel, bool globals) {
   if (kernel == LJG_KERNEL_LC_FP64) lc_force(c, globals);
@@ -132,6 +117,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->cluster_size != 1 && p->cluster_size != 2 && p->cluster_size != 4 && p->cluster_size != 8)
     return LJG_ERR_UNSUPPORTED;
   if (p->rebuild_frequency < 1) return LJG_ERR_ARG;
+  if (p->tile_groups < 0 || p->tile_groups > 64) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;
@@ -170,7 +156,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of.release(); c.cell_count.release(); c.cell_start.release(); c.rank.release(); c.perm.release();
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
-  c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_off.release(); c.list.release();
+  c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
   c.pos32.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
@@ -367,15 +353,17 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
     ++rs.rebuilds;
   }
+  c.hits_acc = reinterpret_cast<unsigned long long *>(c.red_hits.p);
   for (int32_t s = 0; s < nsteps; ++s) {
-    // kick of the previous step (if any) fused with this step's drift
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)1, c.flag.p);
-    md_kick_drift(c, s > 0, true, dt);
+    // kick of the previous step (if any) fused with this step's drift; a particle that moved skin/2
+    // since the last rebuild stamps flag[0] with this step's number
+    const int32_t stamp = ++c.step_stamp;
+    md_kick_drift(c, s > 0, true, dt, stamp);
     bool rebuild = c.steps_since_rebuild >= c.prm.rebuild_frequency;
     if (!rebuild && c.prm.dynamic_rebuild) {
       LJG_HIP(hipMemcpyAsync(c.host_flag, c.flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
       LJG_HIP(hipStreamSynchronize(c.stream));
-      rebuild = c.host_flag[0] != 0;
+      rebuild = c.host_flag[0] == stamp;
     }
     if (rebuild) {
       if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
@@ -386,12 +374,10 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
     run_force(c, kernel, false);
     if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
-    hipLaunchKernelGGL(k_accumulate_hits, dim3(1), dim3(1024), 0, c.stream, partial_items(c, kernel), c.part_hits.p,
-                       c.red_hits.p);
-    LJG_HIP(hipGetLastError());
     c.steps_since_rebuild++;
   }
-  if (nsteps > 0) md_kick_drift(c, true, false, dt);  // final velocity update
+  c.hits_acc = nullptr;
+  if (nsteps > 0) md_kick_drift(c, true, false, dt, 0);  // final velocity update
   c.timing = saved_timing;
   int64_t hits2 = 0;
   LJG_HIP(hipMemcpyAsync(&hits2, c.red_hits.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
@@ -475,14 +461,24 @@ ljg_status ljg_get_clusters(ljg_ctx *h, int64_t *slot_particle, int32_t *cluster
   }
   if (cluster_tower)
     LJG_HIP(hipMemcpy(cluster_tower, c.cluster_tower.p, (size_t)c.nclusters * sizeof(int32_t), hipMemcpyDeviceToHost));
-  if (list_offsets)
-    LJG_HIP(hipMemcpy(list_offsets, c.list_off.p, (size_t)(c.nclusters + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  // lists live in fixed-capacity rows on the device; hand them out compacted
+  std::vector<int32_t> cnt((size_t)c.nclusters);
+  if (!cnt.empty())
+    LJG_HIP(hipMemcpy(cnt.data(), c.list_count.p, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (list_offsets) {
+    list_offsets[0] = 0;
+    for (size_t k = 0; k < cnt.size(); ++k) list_offsets[k + 1] = list_offsets[k] + cnt[k];
+  }
   if (list_j || list_shift) {
-    std::vector<int2> l((size_t)c.nentries);
-    if (!l.empty()) LJG_HIP(hipMemcpy(l.data(), c.list.p, l.size() * sizeof(int2), hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < l.size(); ++k) {
-      if (list_j) list_j[k] = l[k].x;
-      if (list_shift) list_shift[k] = l[k].y;
+    std::vector<int4> l((size_t)c.nclusters * c.list_row);
+    if (!l.empty()) LJG_HIP(hipMemcpy(l.data(), c.list.p, l.size() * sizeof(int4), hipMemcpyDeviceToHost));
+    size_t w = 0;
+    for (size_t ci = 0; ci < cnt.size(); ++ci) {
+      for (int32_t k = 0; k < cnt[ci]; ++k, ++w) {
+        const int4 &en = l[ci * c.list_row + k];
+        if (list_j) list_j[w] = en.x & kClusterMask;
+        if (list_shift) list_shift[w] = (int32_t)((uint32_t)en.x >> kShiftBits);
+      }
     }
   }
   if (bbox) LJG_HIP(hipMemcpy(bbox, c.bbox.p, (size_t)c.nclusters * 6 * sizeof(double), hipMemcpyDeviceToHost));

@@ -38,14 +38,13 @@ __global__ void k_cell_hist(int64_t nslots, const double *__restrict__ x, const 
                             int32_t *__restrict__ cell_of, int32_t *__restrict__ rank,
                             int32_t *__restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  if (orig[i] < 0) {
-    cell_of[i] = -1;
-    return;
+  const bool active = i < nslots && orig[i] >= 0;
+  const int32_t c = active ? cell_index_1d(g, x[i], y[i], z[i]) : -1;
+  const int32_t r = wave_aggregated_rank(c, active, count);
+  if (i < nslots) {
+    cell_of[i] = c;
+    rank[i] = r;
   }
-  const int32_t c = cell_index_1d(g, x[i], y[i], z[i]);
-  cell_of[i] = c;
-  rank[i] = atomicAdd(&count[c], 1);
 }
 
 __global__ void k_cell_scatter(int64_t nslots, const int32_t *__restrict__ cell_of, const int32_t *__restrict__ rank,

@@ -33,8 +33,10 @@ constexpr int kFThreads = 256;
 constexpr int kWavesPerBlock = kFThreads / kWave;
 
 // ---- block-level deterministic partials ------------------------------------------------------------
+// Per-block partial sums (fixed tree, so reproducible); the block's pair count is also added to *acc
+// when acc is set (integer addition, so the order of the atomics does not matter).
 __device__ inline void block_partials(double e, double v, int64_t h, double *pe, double *pv, int64_t *ph,
-                                      bool globals) {
+                                      bool globals, unsigned long long *acc) {
   __shared__ double se[kFThreads], sv[kFThreads];
   __shared__ int64_t sh[kFThreads];
   se[threadIdx.x] = globals ? e : 0.;
@@ -53,6 +55,7 @@ __device__ inline void block_partials(double e, double v, int64_t h, double *pe,
     pe[blockIdx.x] = se[0];
     pv[blockIdx.x] = sv[0];
     ph[blockIdx.x] = sh[0];
+    if (acc) atomicAdd(acc, (unsigned long long)sh[0]);
   }
 }
 
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_lc_fp64(
     const int32_t *__restrict__ type, const int32_t *__restrict__ cell_of, const int64_t *__restrict__ cell_start,
     CellGeom g, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
-    int32_t globals) {
+    int32_t globals, unsigned long long *__restrict__ acc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ax = 0., ay = 0., az = 0., ae = 0., av = 0.;
   int64_t hits = 0;
@@ -122,20 +125,25 @@ __global__ __launch_bounds__(kFThreads) void k_force_lc_fp64(
     fy[i] = ay;
     fz[i] = az;
   }
-  block_partials(ae, av, hits, pe, pv, ph, globals != 0);
+  block_partials(ae, av, hits, pe, pv, ph, globals != 0, acc);
 }
 
 // ---- cluster-pair tiles -------------------------------------------------------------------------------
-// Wave reduce-scatter: v[NV] per lane (NV a power of two <= 64).  After the call lane l holds, in v[0],
-// the wave total of value index vidx<NV>(l).
-template <int NV>
-__device__ inline void wave_reduce_scatter(double (&v)[NV], int lane) {
-  constexpr int K = NV == 1 ? 0 : NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
+
+// Reduce-scatter of v[NV] over a group of LG consecutive lanes (LG, NV powers of two).  Each butterfly
+// step halves the values a lane carries; once a lane carries one value the remaining steps add pairs.
+// Afterwards lane l holds the group totals of values [q*R, q*R+R) in v[0..R-1], R = NV >> min(log LG,
+// log NV), q = group_value_slot<NV, LG>(l).
+template <int NV, int LG>
+__device__ inline void group_reduce_scatter(double (&v)[NV], int l) {
+  constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int step = 32 >> k;
+    const int step = LG >> (k + 1);
     const int half = NV >> (k + 1);
-    const bool upper = (lane & step) != 0;
+    const bool upper = (l & step) != 0;
 #pragma unroll
     for (int q = 0; q < half; ++q) {
       const double send = upper ? v[q] : v[q + half];
@@ -144,123 +152,109 @@ __device__ inline void wave_reduce_scatter(double (&v)[NV], int lane) {
     }
   }
 #pragma unroll
-  for (int step = 32 >> K; step >= 1; step >>= 1) v[0] += __shfl_xor(v[0], step, kWave);
+  for (int step = LG >> (K + 1); step >= 1; step >>= 1) v[0] += __shfl_xor(v[0], step, kWave);
 }
 
-template <int NV>
-__device__ inline int vidx(int lane) {
-  constexpr int K = NV == 1 ? 0 : NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+template <int NV, int LG>
+__device__ inline int group_value_slot(int l) {
+  constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
   int q = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) q |= ((lane >> (5 - k)) & 1) << (K - 1 - k);
+  for (int k = 0; k < K; ++k) q |= ((l >> (KL - 1 - k)) & 1) << (K - 1 - k);
   return q;
 }
 
-constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
-
-template <int M, bool FP32, bool MIX, bool GLOB>
-__global__ __launch_bounds__(kFThreads) void k_force_cluster(
-    int64_t nclusters, const float4 *__restrict__ pos32, const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, const int32_t *__restrict__ type, const double4 *__restrict__ c0,
-    const int64_t *__restrict__ loff, const int2 *__restrict__ list, ForceArgs a, const double *__restrict__ tab64,
-    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe,
-    double *__restrict__ pv, int64_t *__restrict__ ph) {
-  using real = typename std::conditional<FP32, float, double>::type;
-  constexpr int NVAL = 3 * M + (GLOB ? 2 : 0);
+// FP32-pair / FP64-accumulate cluster kernel.  G i-clusters per wave, LG = 64/G lanes per i-cluster; the
+// lanes of a group stream the flat j sequence of their i-cluster's list (entry e, member b) LG at a time
+// and evaluate each j against the M i-particles.  Loads of the next j are issued before the pair math
+// of the current one.  Padding j-members (and lanes past the end of the list) sit at +3e30 and padding
+// i-members are moved to -3e30, so their r^2 overflows to +inf; the single unsigned compare of the bit
+// pattern, (bits(r^2) - 1) < bits(rc^2), is 0 < r^2 <= rc^2 and rejects them and the i == j self pair.
+template <int M, int G, bool MIX, bool GLOB>
+__global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
+    int64_t nclusters, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
+    const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
+    const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+  constexpr int LG = kWave / G;
+  constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
-  __shared__ real stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
-  if (MIX) {
-    for (int k = threadIdx.x; k < 3 * a.ntypes * a.ntypes; k += kFThreads) stab[k] = (real)tab64[k];
+  __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
+  if constexpr (MIX) {
+    for (int k = threadIdx.x; k < 3 * ntypes * ntypes; k += kFThreads) stab[k] = (float)tab64[k];
     __syncthreads();
   }
   const int lane = threadIdx.x & (kWave - 1);
-  const int ic = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave));
-  double e_lane = 0., v_lane = 0.;
-  int64_t hits = 0;
+  const int grp = lane / LG, l = lane % LG;
+  int ic = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
+  if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
+  const uint32_t rc2_bits = __float_as_uint(rc2);
+  double e_out = 0., v_out = 0.;
+  int64_t h_out = 0;
   if (ic < nclusters) {
-    const double4 ci = c0[ic];
-    const int nreal = (int)ci.w;
-    real xi[M], yi[M], zi[M];
+    float xi[M], yi[M], zi[M];
     int ti[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int64_t s = (int64_t)ic * M + q;
-      if constexpr (FP32) {
-        const float4 p = pos32[s];
-        xi[q] = (real)p.x; yi[q] = (real)p.y; zi[q] = (real)p.z;
-        ti[q] = __float_as_int(p.w);
-      } else {
-        xi[q] = (real)x[s]; yi[q] = (real)y[s]; zi[q] = (real)z[s];
-        ti[q] = type[s];
-      }
+      const float4 p = pos32[(int64_t)ic * M + q];
+      const bool pad = p.x == kPadPos32;
+      xi[q] = pad ? -kPadPos32 : p.x;
+      yi[q] = pad ? -kPadPos32 : p.y;
+      zi[q] = pad ? -kPadPos32 : p.z;
+      ti[q] = __float_as_int(p.w);
     }
-    real ax[M], ay[M], az[M], ae[M], av[M];
+    uint32_t real = 0;  // bit q set for the i-cluster's real (non-padding) members
 #pragma unroll
-    for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = (real)0; }
-    const int64_t beg = loff[ic];
-    const int nj = (int)(loff[ic + 1] - beg) * M;
-    const real rc2 = (real)a.rc2;
-    for (int base = 0; base < nj; base += kWave) {
-      const int k = base + lane;
-      const bool valid = k < nj;
-      const int kk = valid ? k : nj - 1;
-      const int2 ent = list[beg + kk / M];
-      const int sid = ent.y;
-      const double shx = (double)(sid % 3 - 1) * a.L[0];
-      const double shy = (double)((sid / 3) % 3 - 1) * a.L[1];
-      const double shz = (double)(sid / 9 - 1) * a.L[2];
-      const int64_t sj = (int64_t)ent.x * M + (kk & (M - 1));
-      real xj, yj, zj;
-      int tj;
-      if constexpr (FP32) {
-        const double4 cj = c0[ent.x];
-        const float4 p = pos32[sj];
-        xj = p.x + (float)((cj.x + shx) - ci.x);
-        yj = p.y + (float)((cj.y + shy) - ci.y);
-        zj = p.z + (float)((cj.z + shz) - ci.z);
-        tj = __float_as_int(p.w);
-      } else {
-        xj = x[sj] + shx;
-        yj = y[sj] + shy;
-        zj = z[sj] + shz;
-        tj = type[sj];
+    for (int q = 0; q < M; ++q) real |= (xi[q] == -kPadPos32 ? 0u : 1u) << q;
+    float ax[M], ay[M], az[M], ae[M], av[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
+    int hits = 0;
+    const int64_t beg = (int64_t)ic * row;
+    const int nj = lcount[ic] * M;  // >= M: every list holds its own cluster
+    // prefetch the first j of this lane (an empty list skips the loop and never loads)
+    const int kk = l < nj ? l : nj - 1;
+    int4 e = make_int4(0, 0, 0, 0);
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (nj > 0) {
+      e = list[beg + (uint32_t)kk / M];
+      p = pos32[(int64_t)(e.x & kClusterMask) * M + (kk & (M - 1))];
+    }
+    for (int base = 0; base < nj; base += LG) {
+      const bool valid = base + l < nj;
+      const float xj = valid ? p.x + __int_as_float(e.y) : kPadPos32;
+      const float yj = p.y + __int_as_float(e.z);
+      const float zj = p.z + __int_as_float(e.w);
+      const int tj = __float_as_int(p.w);
+      // issue the next loads before the pair math
+      {
+        const int kn = base + LG + l;
+        const int kc = kn < nj ? kn : nj - 1;
+        e = list[beg + (uint32_t)kc / M];
+        p = pos32[(int64_t)(e.x & kClusterMask) * M + (kc & (M - 1))];
       }
 #pragma unroll
       for (int q = 0; q < M; ++q) {
-        if (q < nreal) {
-          const real drx = xi[q] - xj, dry = yi[q] - yj, drz = zi[q] - zj;
-          const real dr2 = drx * drx + dry * dry + drz * drz;
-          real eps24, sig2, shift6;
-          if constexpr (MIX) {
-            const int t = 3 * (ti[q] * a.ntypes + tj);
-            eps24 = stab[t]; sig2 = stab[t + 1]; shift6 = stab[t + 2];
-          } else {
-            eps24 = (real)a.eps24; sig2 = (real)a.sig2; shift6 = (real)a.shift6;
-          }
-          const bool mask = valid && dr2 <= rc2 && dr2 > (real)0;
-          real fac, lj12m6;
-          if constexpr (FP32) {
-            const float inv = __builtin_amdgcn_rcpf(dr2);
-            const float lj2 = sig2 * inv;
-            const float lj6 = lj2 * lj2 * lj2;
-            lj12m6 = lj6 * lj6 - lj6;
-            fac = mask ? eps24 * (lj6 * lj6 + lj12m6) * inv : 0.f;
-          } else {
-            const double invdr2 = 1. / dr2;
-            const double lj2 = sig2 * invdr2;
-            const double lj6 = lj2 * lj2 * lj2;
-            const double lj12 = lj6 * lj6;
-            lj12m6 = lj12 - lj6;
-            fac = mask ? eps24 * (lj12 + lj12m6) * invdr2 : 0.;
-          }
-          ax[q] += drx * fac;
-          ay[q] += dry * fac;
-          az[q] += drz * fac;
-          if (GLOB) {
-            ae[q] += mask ? eps24 * lj12m6 + shift6 : (real)0;
-            av[q] += dr2 * fac;
-          }
-          hits += __popcll(__ballot(mask));
+        const float dx = xi[q] - xj, dy = yi[q] - yj, dz = zi[q] - zj;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+        float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
+        if constexpr (MIX) {
+          const int t = 3 * (ti[q] * ntypes + tj);
+          eps24 = stab[t]; sig2 = stab[t + 1]; shift6 = stab[t + 2];
+        }
+        const float inv = __builtin_amdgcn_rcpf(r2);
+        const float lj2 = sig2 * inv;
+        const float lj6 = lj2 * lj2 * lj2;
+        const float fv = eps24 * lj6 * __builtin_fmaf(2.f, lj6, -1.f) * inv;  // eps24 (lj12 + lj12m6) / r^2
+        const float fac = m ? fv : 0.f;
+        ax[q] = __builtin_fmaf(dx, fac, ax[q]);
+        ay[q] = __builtin_fmaf(dy, fac, ay[q]);
+        az[q] = __builtin_fmaf(dz, fac, az[q]);
+        hits += m ? 1 : 0;
+        if constexpr (GLOB) {
+          ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
+          av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);  // r2 is +inf for padding
         }
       }
     }
@@ -273,32 +267,152 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster(
       v[M + q] = (double)ay[q];
       v[2 * M + q] = (double)az[q];
     }
-    if (GLOB) {
+    v[3 * M] = (double)hits;
+    if constexpr (GLOB) {
       double se = 0., sv = 0.;
 #pragma unroll
       for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
-      v[3 * M] = se;
-      v[3 * M + 1] = sv;
+      v[3 * M + 1] = se;
+      v[3 * M + 2] = sv;
     }
-    wave_reduce_scatter<NV>(v, lane);
-    const int q = vidx<NV>(lane);
-    constexpr int group = 64 / NV;
-    if ((lane & (group - 1)) == 0) {
+    group_reduce_scatter<NV, LG>(v, l);
+    constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
+    constexpr int R = NV >> K;
+    const int q0 = group_value_slot<NV, LG>(l) * R;
+    if ((l & ((1 << (KL - K)) - 1)) == 0) {
       const int64_t s0 = (int64_t)ic * M;
-      if (q < M) {
-        if (q < nreal) fx[s0 + q] = v[0];
-      } else if (q < 2 * M) {
-        if (q - M < nreal) fy[s0 + q - M] = v[0];
-      } else if (q < 3 * M) {
-        if (q - 2 * M < nreal) fz[s0 + q - 2 * M] = v[0];
-      } else if (GLOB && q == 3 * M) {
-        e_lane = v[0];
-      } else if (GLOB && q == 3 * M + 1) {
-        v_lane = v[0];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = q0 + r;
+        const double val = v[r];
+        // padding members are never stored
+        if (q < M) {
+          if ((real >> q) & 1u) fx[s0 + q] = val;
+        } else if (q < 2 * M) {
+          if ((real >> (q - M)) & 1u) fy[s0 + q - M] = val;
+        } else if (q < 3 * M) {
+          if ((real >> (q - 2 * M)) & 1u) fz[s0 + q - 2 * M] = val;
+        } else if (q == 3 * M) {
+          h_out = (int64_t)val;
+        } else if (GLOB && q == 3 * M + 1) {
+          e_out = val;
+        } else if (GLOB && q == 3 * M + 2) {
+          v_out = val;
+        }
       }
     }
   }
-  block_partials(e_lane, v_lane, lane == 0 ? hits : 0, pe, pv, ph, GLOB);
+  block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
+}
+
+// FP64 cluster kernel: same tiles (one i-cluster per wave), absolute FP64 positions with the periodic
+// shift of the entry, pair term exactly as LJFunctor.h:288-320.
+template <int M, bool MIX, bool GLOB>
+__global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
+    int64_t nclusters, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const int32_t *__restrict__ type, const int32_t *__restrict__ orig, int32_t row,
+    const int32_t *__restrict__ lcount, const int4 *__restrict__ list, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+  constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
+  constexpr int NV = pow2_at_least(NVAL);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int ic = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave));
+  double e_out = 0., v_out = 0.;
+  int64_t h_out = 0;
+  if (ic < nclusters) {
+    double xi[M], yi[M], zi[M];
+    int ti[M];
+    uint32_t real = 0;  // bit q set for real (non-padding) members
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int64_t s = (int64_t)ic * M + q;
+      xi[q] = x[s]; yi[q] = y[s]; zi[q] = z[s];
+      ti[q] = type[s];
+      real |= (orig[s] >= 0 ? 1u : 0u) << q;
+    }
+    double ax[M], ay[M], az[M], ae[M], av[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.; }
+    int hits = 0;
+    const int64_t beg = (int64_t)ic * row;
+    const int nj = lcount[ic] * M;
+    for (int base = 0; base < nj; base += kWave) {
+      const int k = base + lane;
+      const bool valid = k < nj;
+      const int kk = valid ? k : nj - 1;
+      const int4 e = list[beg + (uint32_t)kk / M];
+      const int sid = (int)((uint32_t)e.x >> kShiftBits);
+      const int64_t sj = (int64_t)(e.x & kClusterMask) * M + (kk & (M - 1));
+      const double xj = x[sj] + shift_component(sid, 0) * a.L[0];
+      const double yj = y[sj] + shift_component(sid, 1) * a.L[1];
+      const double zj = z[sj] + shift_component(sid, 2) * a.L[2];
+      const int tj = type[sj];
+      const bool rj = valid && orig[sj] >= 0;
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        const double drx = xi[q] - xj, dry = yi[q] - yj, drz = zi[q] - zj;
+        const double dr2 = drx * drx + dry * dry + drz * drz;
+        double eps24 = a.eps24, sig2 = a.sig2, shift6 = a.shift6;
+        if constexpr (MIX) {
+          const int t = 3 * (ti[q] * a.ntypes + tj);
+          eps24 = tab[t]; sig2 = tab[t + 1]; shift6 = tab[t + 2];
+        }
+        const bool m = rj && ((real >> q) & 1u) && dr2 <= a.rc2 && dr2 > 0.;
+        const double invdr2 = 1. / dr2;
+        const double lj2 = sig2 * invdr2;
+        const double lj6 = lj2 * lj2 * lj2;
+        const double lj12 = lj6 * lj6;
+        const double lj12m6 = lj12 - lj6;
+        const double fac = m ? eps24 * (lj12 + lj12m6) * invdr2 : 0.;
+        ax[q] += drx * fac;
+        ay[q] += dry * fac;
+        az[q] += drz * fac;
+        hits += m ? 1 : 0;
+        if constexpr (GLOB) {
+          ae[q] += m ? eps24 * lj12m6 + shift6 : 0.;
+          av[q] += drx * (drx * fac) + dry * (dry * fac) + drz * (drz * fac);
+        }
+      }
+    }
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      v[q] = ax[q];
+      v[M + q] = ay[q];
+      v[2 * M + q] = az[q];
+    }
+    v[3 * M] = (double)hits;
+    if constexpr (GLOB) {
+      double se = 0., sv = 0.;
+#pragma unroll
+      for (int q = 0; q < M; ++q) { se += ae[q]; sv += av[q]; }
+      v[3 * M + 1] = se;
+      v[3 * M + 2] = sv;
+    }
+    group_reduce_scatter<NV, kWave>(v, lane);
+    const int q = group_value_slot<NV, kWave>(lane);
+    constexpr int KV = ilog2(NV);
+    if ((lane & ((1 << (6 - KV)) - 1)) == 0) {
+      const int64_t s0 = (int64_t)ic * M;
+      if (q < M) {
+        if ((real >> q) & 1u) fx[s0 + q] = v[0];
+      } else if (q < 2 * M) {
+        if ((real >> (q - M)) & 1u) fy[s0 + q - M] = v[0];
+      } else if (q < 3 * M) {
+        if ((real >> (q - 2 * M)) & 1u) fz[s0 + q - 2 * M] = v[0];
+      } else if (q == 3 * M) {
+        h_out = (int64_t)v[0];
+      } else if (GLOB && q == 3 * M + 1) {
+        e_out = v[0];
+      } else if (GLOB && q == 3 * M + 2) {
+        v_out = v[0];
+      }
+    }
+  }
+  block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
 }
 
 static ForceArgs make_args(const Ctx &c) {
@@ -320,27 +434,35 @@ void lc_force(Ctx &c, bool globals) {
   c.part_e.ensure(nb);
   c.part_v.ensure(nb);
   c.part_hits.ensure(nb);
+  c.part_items = nb;
   State &s = c.st[c.cur];
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
   if (mix)
     hipLaunchKernelGGL(k_force_lc_fp64<true>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
                        s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals);
+                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc);
   else
     hipLaunchKernelGGL(k_force_lc_fp64<false>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
                        s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals);
+                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc);
   LJG_HIP(hipGetLastError());
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }
 
-template <int M, bool FP32>
-static void launch_cluster(Ctx &c, bool mix, bool glob, int nb, const ForceArgs &a, const double *tab) {
-  State &s = c.st[c.cur];
+template <int M, int G>
+static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
+  constexpr int per_block = kWavesPerBlock * G;
+  const int nb = blocks_for(c.nclusters, per_block);
+  c.part_e.ensure(nb);
+  c.part_v.ensure(nb);
+  c.part_hits.ensure(nb);
+  c.part_items = nb;
+  const float rc2 = (float)a.rc2;
 #define LJG_LAUNCH(MIXV, GLOBV)                                                                                    \
-  hipLaunchKernelGGL((k_force_cluster<M, FP32, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream, c.nclusters, \
-                     c.pos32.p, s.x.p, s.y.p, s.z.p, s.type.p, c.c0.p, c.list_off.p, c.list.p, a, tab, s.fx.p,     \
-                     s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p)
+  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,          \
+                     c.nclusters, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,           \
+                     (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
+                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc)
   if (mix) {
     if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
   } else {
@@ -349,21 +471,53 @@ static void launch_cluster(Ctx &c, bool mix, bool glob, int nb, const ForceArgs 
 #undef LJG_LAUNCH
 }
 
-void cluster_force(Ctx &c, bool mixed, bool globals) {
-  const ForceArgs a = make_args(c);
-  const bool mix = c.lj.ntypes > 1;
-  const double *tab = mix ? c.tab.p : nullptr;
+template <int M>
+static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
   const int nb = blocks_for(c.nclusters, kWavesPerBlock);
   c.part_e.ensure(nb);
   c.part_v.ensure(nb);
   c.part_hits.ensure(nb);
+  c.part_items = nb;
+  State &s = c.st[c.cur];
+#define LJG_LAUNCH(MIXV, GLOBV)                                                                                   \
+  hipLaunchKernelGGL((k_force_cluster_fp64<M, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream, c.nclusters, \
+                     s.x.p, s.y.p, s.z.p, s.type.p, s.orig.p, c.list_row, c.list_count.p, c.list.p, a, tab, s.fx.p,\
+                     s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc)
+  if (mix) {
+    if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
+  } else {
+    if (glob) LJG_LAUNCH(false, true); else LJG_LAUNCH(false, false);
+  }
+#undef LJG_LAUNCH
+}
+
+// i-clusters per wave for the FP32 kernel: the tuned default per cluster size, or the caller's choice.
+int default_groups(int M) { return M >= 8 ? 1 : M == 4 ? 2 : M == 2 ? 4 : 16; }
+
+void cluster_force(Ctx &c, bool mixed, bool globals) {
+  const ForceArgs a = make_args(c);
+  const bool mix = c.lj.ntypes > 1;
+  const double *tab = mix ? c.tab.p : nullptr;
+  const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_groups(c.M);
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
-  switch (c.M) {
-    case 1: mixed ? launch_cluster<1, true>(c, mix, globals, nb, a, tab) : launch_cluster<1, false>(c, mix, globals, nb, a, tab); break;
-    case 2: mixed ? launch_cluster<2, true>(c, mix, globals, nb, a, tab) : launch_cluster<2, false>(c, mix, globals, nb, a, tab); break;
-    case 4: mixed ? launch_cluster<4, true>(c, mix, globals, nb, a, tab) : launch_cluster<4, false>(c, mix, globals, nb, a, tab); break;
-    case 8: mixed ? launch_cluster<8, true>(c, mix, globals, nb, a, tab) : launch_cluster<8, false>(c, mix, globals, nb, a, tab); break;
-    default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", __LINE__};
+  if (!mixed) {
+    switch (c.M) {
+      case 1: launch_fp64<1>(c, mix, globals, a, tab); break;
+      case 2: launch_fp64<2>(c, mix, globals, a, tab); break;
+      case 4: launch_fp64<4>(c, mix, globals, a, tab); break;
+      case 8: launch_fp64<8>(c, mix, globals, a, tab); break;
+      default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", __LINE__};
+    }
+  } else {
+#define LJG_CASE(MV, GV) \
+  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab); goto launched; }
+    LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4)
+    LJG_CASE(4, 1) LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8)
+    LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)
+    LJG_CASE(1, 4) LJG_CASE(1, 8) LJG_CASE(1, 16) LJG_CASE(1, 32) LJG_CASE(1, 64)
+#undef LJG_CASE
+    throw HipError{hipErrorInvalidValue, "unsupported (cluster size, i-clusters per wave) combination", -1};
+  launched:;
   }
   LJG_HIP(hipGetLastError());
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));

@@ -18,6 +18,9 @@ constexpr int kMaxTypes = 16;
 constexpr int kTowerSortCap = 4096;
 // Dummy padding slots sit here (far from every box), so that every pair with them fails r^2 <= rc^2.
 constexpr double kDummyPos = 1.0e7;
+// FP32 cluster-relative position of padding slots; the FP32 force kernel moves padding i-members to
+// -kPadPos32 so that padding never meets padding at a finite distance.
+constexpr float kPadPos32 = 3.0e30f;
 
 struct HipError {
   hipError_t err;
@@ -148,9 +151,13 @@ struct Ctx {
   DevBuf<int32_t> cluster_tower;  // nclusters
   DevBuf<double> bbox;          // 6 * nclusters
   DevBuf<double4> c0;           // nclusters: cluster reference point (bbox centre at build)
-  DevBuf<int32_t> list_count;   // nclusters
-  DevBuf<int64_t> list_off;     // nclusters + 1
-  DevBuf<int2> list;            // entries (j cluster, shift id)
+  DevBuf<int32_t> list_count;   // entries per i-cluster
+  DevBuf<unsigned long long> list_stats;  // [0] total entries, [1] longest list
+  int32_t list_row = 0;         // row capacity of the current lists: cluster ci owns [ci*row, ci*row + count)
+  int32_t list_cap = 0;         // row capacity to start the next build with
+  // List entry: x = j cluster | shift id << 27; y,z,w = FP32 bits of c0_j + shift - c0_i (the offset
+  // that turns j's cluster-relative FP32 position into i's frame, formed in FP64 and rounded once).
+  DevBuf<int4> list;
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
 
   DevBuf<double> tab;             // 3 * ntypes^2: (eps24, sig2, shift6) per type pair
@@ -158,6 +165,8 @@ struct Ctx {
   // Reductions
   DevBuf<double> part_e, part_v;  // per work item
   DevBuf<int64_t> part_hits;
+  int64_t part_items = 0;         // partial-sum slots written by the last force launch
+  unsigned long long *hits_acc = nullptr;  // when set, force kernels add their pair counts here
   DevBuf<double> red_out;         // [0]=e [1]=v
   DevBuf<int64_t> red_hits;       // [0]
   DevBuf<int32_t> flag;           // [0] displacement, [1] capacity error
@@ -168,6 +177,7 @@ struct Ctx {
 
   bool have_forces = false;  // st[cur].fx holds the force of the current positions
   int64_t steps_since_rebuild = 0;
+  int32_t step_stamp = 0;    // running step number written into flag[0] by the displacement check
   int64_t last_pairs = 0;
   int64_t last_evals = 0;
 };
@@ -180,11 +190,14 @@ void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
 void vcl_pos32(Ctx &c);                    // ljg_vcl.hip
 void cluster_force(Ctx &c, bool mixed, bool globals);  // ljg_force.hip
-void md_kick_drift(Ctx &c, bool kick, bool drift, double dt);  // ljg_md.hip
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
 void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip
 void permute_state(Ctx &c, const int32_t *src, int64_t new_slots);  // ljg_md.hip
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
+
+constexpr int kShiftBits = 27;
+constexpr int32_t kClusterMask = (1 << kShiftBits) - 1;
 
 // Shift id (sx+1) + 3(sy+1) + 9(sz+1), sx,sy,sz in {-1,0,1}.
 __host__ __device__ inline int shift_component(int sid, int d) {
@@ -192,5 +205,25 @@ __host__ __device__ inline int shift_component(int sid, int d) {
 }
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// Histogram rank with one atomic per run of equal keys inside the wavefront.  Particles arrive in the
+// previous build's order, so neighbouring lanes mostly share a cell/tower and the atomics drop ~30x.
+// Every lane of the wave must call this (inactive lanes pass active = false); returns the slot inside
+// the bin for active lanes.
+__device__ inline int32_t wave_aggregated_rank(int32_t key, bool active, int32_t *count) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int32_t k = active ? key : -1;
+  const int32_t prev = __shfl_up(k, 1, kWave);
+  const bool head = lane == 0 || k != prev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+  const int leader = 63 - __clzll((long long)(heads & upto));
+  const unsigned long long after = heads & ~upto;
+  const int next = after ? __ffsll((long long)after) - 1 : kWave;
+  int32_t base = 0;
+  if (lane == leader && k >= 0) base = atomicAdd(&count[k], next - leader);
+  base = __shfl(base, leader, kWave);
+  return base + (lane - leader);
+}
 
 }  // namespace ljg

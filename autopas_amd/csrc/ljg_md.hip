@@ -84,6 +84,7 @@ void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t 
 
 // ---------------------------------------------------------------------------------------------------
 // State permutation: new slot s takes the particle of old slot src[s]; src[s] < 0 makes a padding slot.
+// The rebuild reference positions are not carried: every rebuild resets them from the new positions.
 __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateView o, StateView n) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnew) return;
@@ -93,7 +94,6 @@ __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateVi
     n.vx[s] = o.vx[k]; n.vy[s] = o.vy[k]; n.vz[s] = o.vz[k];
     n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k];
     n.ofx[s] = o.ofx[k]; n.ofy[s] = o.ofy[k]; n.ofz[s] = o.ofz[k];
-    n.rx[s] = o.rx[k]; n.ry[s] = o.ry[k]; n.rz[s] = o.rz[k];
     n.type[s] = o.type[k]; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
   } else {
     n.x[s] = kDummyPos; n.y[s] = kDummyPos; n.z[s] = kDummyPos;
@@ -124,7 +124,7 @@ void permute_state(Ctx &c, const int32_t *src, int64_t new_slots) {
 struct MdArgs {
   double dt;
   double half_skin2;
-  int32_t kick, drift, write_pos32, M;
+  int32_t kick, drift, write_pos32, M, stamp;
   double kick_fac[kMaxTypes];   // dt / (2m), as TimeDiscretization.cpp:160 forms it
   double drift_fac[kMaxTypes];  // dt * dt / (2m), as TimeDiscretization.cpp:40 forms it
 };
@@ -153,16 +153,17 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
   const double x = s.x[i] + dx, y = s.y[i] + dy, z = s.z[i] + dz;
   s.x[i] = x; s.y[i] = y; s.z[i] = z;
   const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
-  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) flag[0] = 1;
+  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) flag[0] = a.stamp;
   if (a.write_pos32) {
-    const double4 r = c0[i / a.M];
+    const double4 r = c0[i / a.M];  // padding slots returned above and keep their sentinel positions
     pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
   }
 }
 
-void md_kick_drift(Ctx &c, bool kick, bool drift, double dt) {
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   MdArgs a{};
   a.dt = dt;
+  a.stamp = stamp;
   const double hs = c.prm.skin * 0.5;
   a.half_skin2 = hs * hs;
   a.kick = kick;

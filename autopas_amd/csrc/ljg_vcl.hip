@@ -27,14 +27,13 @@ __global__ void k_tower_hist(int64_t nslots, const double *__restrict__ x, const
                              const int32_t *__restrict__ orig, TowerGeom g, int32_t *__restrict__ tower_of,
                              int32_t *__restrict__ rank, int32_t *__restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  if (orig[i] < 0) {
-    tower_of[i] = -1;
-    return;
+  const bool active = i < nslots && orig[i] >= 0;
+  const int32_t t = active ? tower_index(g, x[i], y[i]) : -1;
+  const int32_t r = wave_aggregated_rank(t, active, count);
+  if (i < nslots) {
+    tower_of[i] = t;
+    rank[i] = r;
   }
-  const int32_t t = tower_index(g, x[i], y[i]);
-  tower_of[i] = t;
-  rank[i] = atomicAdd(&count[t], 1);
 }
 
 __global__ void k_tower_scatter(int64_t nslots, const int32_t *__restrict__ tower_of, const int32_t *__restrict__ rank,
@@ -156,73 +155,104 @@ __device__ inline double box_dist2(const double *a, const double *b, double shx,
   return s;
 }
 
-// Cluster-pair lists, one thread per i-cluster; FILL=false counts, FILL=true writes.  The traversal
-// order (oy, ox, sz, j) is the oracle's, so lists compare entry by entry.
-template <bool FILL>
-__global__ void k_cluster_lists(int64_t nclusters, TowerGeom g, const int32_t *__restrict__ cluster_tower,
-                                const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
-                                int32_t *__restrict__ count, const int64_t *__restrict__ off,
-                                int2 *__restrict__ list) {
+// Cluster-pair lists, one thread per i-cluster, written into a fixed-capacity row per i-cluster
+// (entries [ci*cap, ci*cap + count[ci])).  The traversal order (oy, ox, sz, j) is the oracle's, so the
+// lists compare entry by entry.  The first j-cluster of a tower whose z-range can reach the i-cluster is
+// found by interpolating on the tower's monotone cluster z-extents and walking to the exact boundary
+// (z-sorted towers of near-uniform density need one or two probes).  stats[0] += entries (per block),
+// stats[1] = max(count) so the host can grow cap and rebuild if a row overflowed.
+__global__ __launch_bounds__(128) void k_cluster_lists(int64_t nclusters, TowerGeom g,
+                                                       const int32_t *__restrict__ cluster_tower,
+                                                       const int64_t *__restrict__ cstart,
+                                                       const double *__restrict__ bbox,
+                                                       const double4 *__restrict__ c0, int32_t cap,
+                                                       int32_t *__restrict__ count, int4 *__restrict__ list,
+                                                       unsigned long long *__restrict__ stats) {
+  __shared__ unsigned long long block_total;
+  if (threadIdx.x == 0) block_total = 0;
+  __syncthreads();
   const int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= nclusters) return;
-  const int32_t t = cluster_tower[ci];
-  const int tx = t % g.nt[0], ty = t / g.nt[0];
-  double bi[6];
+  if (ci < nclusters) {
+    const int32_t t = cluster_tower[ci];
+    const int tx = t % g.nt[0], ty = t / g.nt[0];
+    double bi[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
-  const double il2 = g.ilen * g.ilen;
-  // prefilter window in z, widened so it never drops a cluster the exact test accepts
-  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
-  int64_t w = FILL ? off[ci] : 0;
-  int32_t cnt = 0;
-  for (int oy = -g.R; oy <= g.R; ++oy) {
-    const int aoy = oy < 0 ? -oy : oy;
-    const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
-    for (int ox = -g.R; ox <= g.R; ++ox) {
-      const int aox = ox < 0 ? -ox : ox;
-      const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
-      if (dtx * dtx + dty * dty > il2) continue;
-      int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
-      while (jx < 0) { jx += g.nt[0]; --sx; }
-      while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
-      while (jy < 0) { jy += g.nt[1]; --sy; }
-      while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
-      const int64_t tj = jx + (int64_t)jy * g.nt[0];
-      const int64_t cb = cstart[tj], ce = cstart[tj + 1];
-      for (int sz = -1; sz <= 1; ++sz) {
-        const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
-        // first cluster whose zhi + shz reaches zlo_i - window (zhi is non-decreasing in the tower)
-        int64_t lo = cb, hi = ce;
-        const double zmin = bi[2] - zwin;
-        while (lo < hi) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (bbox[6 * mid + 5] + shz < zmin) lo = mid + 1;
-          else hi = mid;
-        }
-        const double zmax = bi[5] + zwin;
-        for (int64_t cj = lo; cj < ce; ++cj) {
-          const double *bj = bbox + 6 * cj;
-          if (bj[2] + shz > zmax) break;
-          if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
-            if (FILL) list[w++] = make_int2((int32_t)cj, (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
-            ++cnt;
+    for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
+    const double4 ai = c0[ci];
+    const double il2 = g.ilen * g.ilen;
+    // z window, widened so that it never drops a cluster the exact test accepts
+    const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+    const int64_t w0 = ci * (int64_t)cap;
+    int32_t cnt = 0;
+    for (int oy = -g.R; oy <= g.R; ++oy) {
+      const int aoy = oy < 0 ? -oy : oy;
+      const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+      for (int ox = -g.R; ox <= g.R; ++ox) {
+        const int aox = ox < 0 ? -ox : ox;
+        const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+        if (dtx * dtx + dty * dty > il2) continue;
+        int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+        while (jx < 0) { jx += g.nt[0]; --sx; }
+        while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+        while (jy < 0) { jy += g.nt[1]; --sy; }
+        while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+        const int64_t tj = jx + (int64_t)jy * g.nt[0];
+        const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+        if (cb == ce) continue;
+        const double zf = bbox[6 * cb + 5], zl = bbox[6 * (ce - 1) + 5];
+        for (int sz = -1; sz <= 1; ++sz) {
+          const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
+          const double zmin = bi[2] - zwin - shz;  // in the j tower's own frame
+          const double zmax = bi[5] + zwin - shz;
+          if (zl < zmin) continue;
+          int64_t lo;
+          if (zf >= zmin) {
+            lo = cb;
+          } else {
+            // lower bound of zhi >= zmin: interpolate, then walk (zhi is non-decreasing)
+            const double frac = (zmin - zf) / (zl - zf);
+            lo = cb + (int64_t)(frac * (double)(ce - 1 - cb));
+            lo = lo < cb ? cb : (lo > ce - 1 ? ce - 1 : lo);
+            while (lo > cb && bbox[6 * (lo - 1) + 5] >= zmin) --lo;
+            while (lo < ce && bbox[6 * lo + 5] < zmin) ++lo;
+          }
+          for (int64_t cj = lo; cj < ce; ++cj) {
+            const double *bj = bbox + 6 * cj;
+            if (bj[2] > zmax) break;
+            if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+              if (cnt < cap) {
+                const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                const double4 b = c0[cj];
+                const float dx = (float)((b.x + shx) - ai.x);
+                const float dy = (float)((b.y + shy) - ai.y);
+                const float dz = (float)((b.z + shz) - ai.z);
+                list[w0 + cnt] = make_int4((int32_t)cj | (sid << kShiftBits), __float_as_int(dx),
+                                           __float_as_int(dy), __float_as_int(dz));
+              }
+              ++cnt;
+            }
           }
         }
       }
     }
+    count[ci] = cnt;
+    atomicAdd(&block_total, (unsigned long long)cnt);
+    atomicMax(&stats[1], (unsigned long long)cnt);
   }
-  if (!FILL) count[ci] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&stats[0], block_total);
 }
 
-// FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots are
-// parked 3e4 away so that no real particle reaches them.
+// FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots sit at
+// the sentinel +3e30 in every coordinate: any distance to them squares to +inf in FP32, which fails the
+// cutoff test, while the difference itself stays finite (so masked terms contribute exact zeros).
 __global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
                         const double *__restrict__ z, const int32_t *__restrict__ type,
                         const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
   if (orig[s] < 0) {
-    pos32[s] = make_float4(3.0e4f, 3.0e4f, 3.0e4f, __int_as_float(0));
+    pos32[s] = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
     return;
   }
   const double4 r = c0[s / M];
@@ -235,6 +265,11 @@ void vcl_pos32(Ctx &c) {
   hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
                      s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p);
   LJG_HIP(hipGetLastError());
+}
+
+__global__ void k_zero_u64(int64_t n, unsigned long long *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
 }
 
 __global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {
@@ -321,19 +356,30 @@ void vcl_build(Ctx &c) {
                      t.orig.p, c.bbox.p, c.c0.p);
   LJG_HIP(hipGetLastError());
   c.list_count.ensure(ncl + 1);
-  c.list_off.ensure(ncl + 1);
-  hipLaunchKernelGGL(k_cluster_lists<false>, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
-                     c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.list_count.p, c.list_off.p, c.list.p);
-  LJG_HIP(hipGetLastError());
-  exclusive_scan_i32_to_i64(c, c.list_count.p, c.list_off.p, ncl);
-  int64_t nent = 0;
-  LJG_HIP(hipMemcpyAsync(&nent, c.list_off.p + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
-  c.nentries = nent;
-  c.list.ensure(nent + 1);
-  hipLaunchKernelGGL(k_cluster_lists<true>, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
-                     c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.list_count.p, c.list_off.p, c.list.p);
-  LJG_HIP(hipGetLastError());
+  c.list_stats.ensure(2);
+  int32_t cap = c.list_cap > 0 ? c.list_cap : 64;
+  for (;;) {
+    c.list.ensure((size_t)ncl * cap + 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.list_stats.p);
+    LJG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cluster_lists, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
+                       c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p,
+                       c.list_stats.p);
+    LJG_HIP(hipGetLastError());
+    unsigned long long st[2] = {0, 0};
+    LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
+    LJG_HIP(hipStreamSynchronize(c.stream));
+    c.nentries = (int64_t)st[0];
+    const int32_t maxc = (int32_t)st[1];
+    if (maxc <= cap) {
+      // keep the row capacity for the next build, with headroom for fluctuations
+      const int32_t want = maxc + maxc / 8 + 8;
+      c.list_cap = want > cap ? want : (cap > 2 * want ? want : cap);
+      break;
+    }
+    cap = maxc + maxc / 8 + 8;
+  }
+  c.list_row = cap;
   c.layout = Layout::Clusters;
   vcl_pos32(c);
 }

@@ -45,7 +45,8 @@ class Params(ctypes.Structure):
         ("dynamic_rebuild", _i32),
         ("apply_shift", _i32),
         ("device", _i32),
-        ("reserved", _i32 * 7),
+        ("tile_groups", _i32),
+        ("reserved", _i32 * 6),
     ]
 
 
@@ -141,7 +142,7 @@ class Engine:
     """One periodic box on one GPU."""
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
-                 dynamic_rebuild=True, apply_shift=True, device=0):
+                 dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -153,6 +154,7 @@ class Engine:
         p.dynamic_rebuild = int(bool(dynamic_rebuild))
         p.apply_shift = int(bool(apply_shift))
         p.device = device
+        p.tile_groups = tile_groups
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))

@@ -48,7 +48,8 @@ typedef struct {
   int32_t dynamic_rebuild;   /* also rebuild once any particle moved >= skin/2 (LogicHandler.h:1000-1015) */
   int32_t apply_shift;    /* truncated-shifted potential energy (LJFunctor applyShift) */
   int32_t device;         /* HIP device ordinal */
-  int32_t reserved[7];
+  int32_t tile_groups;    /* i-clusters per wavefront in the FP32 cluster kernel; 0 = default per cluster size */
+  int32_t reserved[6];
 } ljg_params;
 
 typedef struct {
