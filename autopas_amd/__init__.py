@@ -7,6 +7,7 @@ from .engine import (  # noqa: F401
     KERNEL_CLUSTER_FP64,
     KERNEL_CLUSTER_MIXED,
     KERNEL_LC_FP64,
+    KERNEL_VERLET_MIXED,
     KERNEL_NAMES,
     Engine,
     EngineError,

@@ -33,6 +33,7 @@ SOURCES = {
     "ljg_vcl.hip": ["-ffp-contract=off"],
     "ljg_md.hip": ["-ffp-contract=off"],
     "ljg_force.hip": [],
+    "ljg_verlet.hip": ["-ffp-contract=off"],
     "ljg_generate.cpp": ["-ffp-contract=off"],
 }
 

@@ -76,25 +76,42 @@ static bool is_cluster_kernel(int k) { return k == LJG_KERNEL_CLUSTER_FP64 || k 
 
 static void rebuild_for(Ctx &c, int kernel) {
   wrap_positions(c);
-  if (is_cluster_kernel(kernel)) vcl_build(c);
-  else lc_bin(c);
+  if (is_cluster_kernel(kernel)) {
+    vcl_build(c);
+  } else {
+    lc_bin(c);
+    if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_build(c);
+  }
   md_set_rebuild_reference(c);
   c.steps_since_rebuild = 0;
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {
-  return is_cluster_kernel(kernel) ? c.layout == Layout::Clusters : c.layout == Layout::Cells;
+  if (is_cluster_kernel(kernel)) return c.layout == Layout::Clusters;
+  if (kernel == LJG_KERNEL_VERLET_MIXED) return c.layout == Layout::Verlet;
+  return c.layout == Layout::Cells;
+}
+
+// The Verlet-list kernel relies on minimum-image fixed-point differences and distinct neighbour cells.
+static bool kernel_supported(const Ctx &c, int kernel) {
+  if (kernel != LJG_KERNEL_VERLET_MIXED) return true;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  for (int d = 0; d < 3; ++d)
+    if (!(c.prm.box_max[d] - c.prm.box_min[d] >= 3.0 * ilen)) return false;
+  return true;
 }
 
 static int64_t partial_items(const Ctx &c, int /*kernel*/) { return c.part_items; }
 
 static void run_force(Ctx &c, int kernel, bool globals) {
   if (kernel == LJG_KERNEL_LC_FP64) lc_force(c, globals);
+  else if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_force(c, globals);
   else cluster_force(c, kernel == LJG_KERNEL_CLUSTER_MIXED, globals);
 }
 
 static int64_t candidate_evals(const Ctx &c, int kernel) {
   if (is_cluster_kernel(kernel)) return c.nentries * (int64_t)c.M * c.M;
+  if (kernel == LJG_KERNEL_VERLET_MIXED) return c.nentries;
   return -1;
 }
 
@@ -157,7 +174,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.pos32.release(); c.posfx.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -304,9 +321,10 @@ ljg_status ljg_build_pairs(ljg_ctx *h) {
 }
 
 ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globals *out) {
-  if (!h || kernel < 0 || kernel > 2) return LJG_ERR_ARG;
+  if (!h || kernel < 0 || kernel > 3) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (!kernel_supported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, "box shorter than 3 interaction lengths");
   if (!layout_matches(c, kernel)) rebuild_for(c, kernel);
   run_force(c, kernel, calc_globals != 0);
   ljg_globals g{};
@@ -329,9 +347,10 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 }
 
 ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out) {
-  if (!h || nsteps < 0 || kernel < 0 || kernel > 2 || !(dt > 0.)) return LJG_ERR_ARG;
+  if (!h || nsteps < 0 || kernel < 0 || kernel > 3 || !(dt > 0.)) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (!kernel_supported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, "box shorter than 3 interaction lengths");
   const auto t0 = std::chrono::steady_clock::now();
   ljg_run_stats rs{};
   const bool timing = c.timing;
@@ -367,7 +386,9 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     }
     if (rebuild) {
       if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+      c.carry_force = false;  // the force of this step is computed right after the rebuild
       rebuild_for(c, kernel);
+      c.carry_force = true;
       if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
       ++rs.rebuilds;
     }
@@ -482,6 +503,31 @@ ljg_status ljg_get_clusters(ljg_ctx *h, int64_t *slot_particle, int32_t *cluster
     }
   }
   if (bbox) LJG_HIP(hipMemcpy(bbox, c.bbox.p, (size_t)c.nclusters * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_get_verlet_lists(ljg_ctx *h, int64_t *offsets, int32_t *neighbors) {
+  if (!h || !offsets) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.layout != Layout::Verlet) return fail(c, LJG_ERR_STATE, "no Verlet lists built");
+  const int64_t n = c.n;
+  std::vector<int32_t> cnt((size_t)n), orig((size_t)n);
+  LJG_HIP(hipMemcpy(cnt.data(), c.vcount.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  LJG_HIP(hipMemcpy(orig.data(), c.st[c.cur].orig.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int64_t> slot_of((size_t)n);
+  for (int64_t s = 0; s < n; ++s) slot_of[orig[s]] = s;
+  offsets[0] = 0;
+  for (int64_t p = 0; p < n; ++p) offsets[p + 1] = offsets[p] + cnt[slot_of[p]];
+  if (neighbors) {
+    const int64_t nw = (n + kWave - 1) / kWave;
+    std::vector<int32_t> l((size_t)(nw * c.vrow * kWave));
+    LJG_HIP(hipMemcpy(l.data(), c.vlist.p, l.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < n; ++p) {
+      const int64_t s = slot_of[p], w = s / kWave, lane = s % kWave;
+      for (int32_t k = 0; k < cnt[s]; ++k) neighbors[offsets[p] + k] = orig[l[(w * c.vrow + k) * kWave + lane]];
+    }
+  }
   return LJG_OK;
   LJG_API_END
 }

@@ -246,7 +246,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         const float inv = __builtin_amdgcn_rcpf(r2);
         const float lj2 = sig2 * inv;
         const float lj6 = lj2 * lj2 * lj2;
-        const float fv = eps24 * lj6 * __builtin_fmaf(2.f, lj6, -1.f) * inv;  // eps24 (lj12 + lj12m6) / r^2
+        // eps24 (lj12 + lj12m6) / r^2 = (lj6 / r^2) (2 eps24 lj6 - eps24)
+        const float fv = (lj6 * inv) * __builtin_fmaf(eps24 + eps24, lj6, -eps24);
         const float fac = m ? fv : 0.f;
         ax[q] = __builtin_fmaf(dx, fac, ax[q]);
         ay[q] = __builtin_fmaf(dy, fac, ay[q]);

@@ -101,6 +101,13 @@ struct TowerGeom {
   double ilen;
 };
 
+// 32-bit fixed-point coordinates for the Verlet-list kernels: X = round((x - min) * inv_scale) mod 2^32.
+struct FixGeom {
+  double min[3], scale[3], inv_scale[3];  // scale = L / 2^32
+  float scale32[3];
+  int32_t cubic;
+};
+
 // LJ parameters (mixing tables are ntypes x ntypes, row-major).
 struct LJParams {
   int32_t ntypes;
@@ -110,7 +117,7 @@ struct LJParams {
   double mass[kMaxTypes];
 };
 
-enum class Layout { None, Upload, Cells, Clusters };
+enum class Layout { None, Upload, Cells, Clusters, Verlet };
 
 struct Ctx {
   ljg_params prm{};
@@ -160,6 +167,13 @@ struct Ctx {
   DevBuf<int4> list;
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
 
+  // Verlet lists on the linked cells (fixed-point positions, transposed rows per wavefront)
+  FixGeom fg{};
+  DevBuf<int4> posfx;            // per slot: fixed-point x, y, z and the type
+  DevBuf<int32_t> vcount;        // neighbours per particle
+  DevBuf<int32_t> vlist;         // rows of vrow entries, transposed per wave of 64 particles
+  int32_t vrow = 0, vcap = 0;
+
   DevBuf<double> tab;             // 3 * ntypes^2: (eps24, sig2, shift6) per type pair
 
   // Reductions
@@ -178,6 +192,7 @@ struct Ctx {
   bool have_forces = false;  // st[cur].fx holds the force of the current positions
   int64_t steps_since_rebuild = 0;
   int32_t step_stamp = 0;    // running step number written into flag[0] by the displacement check
+  bool carry_force = true;   // permutes keep the current force (false inside the step loop)
   int64_t last_pairs = 0;
   int64_t last_evals = 0;
 };
@@ -190,6 +205,9 @@ void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
 void vcl_pos32(Ctx &c);                    // ljg_vcl.hip
 void cluster_force(Ctx &c, bool mixed, bool globals);  // ljg_force.hip
+void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
+void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
+void verlet_force(Ctx &c, bool globals);   // ljg_verlet.hip
 void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
 void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip

@@ -85,14 +85,16 @@ void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t 
 // ---------------------------------------------------------------------------------------------------
 // State permutation: new slot s takes the particle of old slot src[s]; src[s] < 0 makes a padding slot.
 // The rebuild reference positions are not carried: every rebuild resets them from the new positions.
-__global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateView o, StateView n) {
+// Inside the step loop the force is recomputed right after a rebuild, so it is not carried either.
+__global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateView o, StateView n,
+                          int32_t carry_force) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnew) return;
   const int32_t k = src[s];
   if (k >= 0) {
     n.x[s] = o.x[k];   n.y[s] = o.y[k];   n.z[s] = o.z[k];
     n.vx[s] = o.vx[k]; n.vy[s] = o.vy[k]; n.vz[s] = o.vz[k];
-    n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k];
+    if (carry_force) { n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k]; }
     n.ofx[s] = o.ofx[k]; n.ofy[s] = o.ofy[k]; n.ofz[s] = o.ofz[k];
     n.type[s] = o.type[k]; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
   } else {
@@ -109,7 +111,7 @@ void permute_state(Ctx &c, const int32_t *src, int64_t new_slots) {
   const int other = 1 - c.cur;
   c.st[other].ensure((size_t)new_slots);
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, src,
-                     view(c.st[c.cur]), view(c.st[other]));
+                     view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force);
   LJG_HIP(hipGetLastError());
   c.cur = other;
   c.nslots = new_slots;
@@ -124,13 +126,14 @@ void permute_state(Ctx &c, const int32_t *src, int64_t new_slots) {
 struct MdArgs {
   double dt;
   double half_skin2;
-  int32_t kick, drift, write_pos32, M, stamp;
+  int32_t kick, drift, write_pos32, write_fixed, M, stamp;
+  FixGeom fg;
   double kick_fac[kMaxTypes];   // dt / (2m), as TimeDiscretization.cpp:160 forms it
   double drift_fac[kMaxTypes];  // dt * dt / (2m), as TimeDiscretization.cpp:40 forms it
 };
 
 __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double4 *__restrict__ c0,
-                             float4 *__restrict__ pos32, int32_t *__restrict__ flag) {
+                             float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
   if (s.orig[i] < 0) return;
@@ -158,6 +161,11 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
     const double4 r = c0[i / a.M];  // padding slots returned above and keep their sentinel positions
     pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
   }
+  if (a.write_fixed) {
+    posfx[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - a.fg.min[0]) * a.fg.inv_scale[0]),
+                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - a.fg.min[1]) * a.fg.inv_scale[1]),
+                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - a.fg.min[2]) * a.fg.inv_scale[2]), t);
+  }
 }
 
 void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
@@ -169,6 +177,8 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   a.kick = kick;
   a.drift = drift;
   a.write_pos32 = c.layout == Layout::Clusters;
+  a.write_fixed = c.layout == Layout::Verlet;
+  a.fg = c.fg;
   a.M = c.M;
   for (int t = 0; t < kMaxTypes; ++t) {
     const double m = c.lj.mass[t] > 0. ? c.lj.mass[t] : 1.;
@@ -176,7 +186,7 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
     a.drift_fac[t] = dt * dt / (2 * m);
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.flag.p);
+                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p);
   LJG_HIP(hipGetLastError());
 }
 

@@ -53,7 +53,7 @@ __global__ void k_tower_nclusters(int64_t ntowers, const int32_t *__restrict__ c
 
 // One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
 // are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.
-constexpr int kSortThreads = 1024;
+constexpr int kSortThreads = 256;
 
 __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, const int64_t *__restrict__ id) {
   if (sb < 0) return sa >= 0;  // padding sorts last
@@ -155,33 +155,39 @@ __device__ inline double box_dist2(const double *a, const double *b, double shx,
   return s;
 }
 
-// Cluster-pair lists, one thread per i-cluster, written into a fixed-capacity row per i-cluster
-// (entries [ci*cap, ci*cap + count[ci])).  The traversal order (oy, ox, sz, j) is the oracle's, so the
-// lists compare entry by entry.  The first j-cluster of a tower whose z-range can reach the i-cluster is
-// found by interpolating on the tower's monotone cluster z-extents and walking to the exact boundary
-// (z-sorted towers of near-uniform density need one or two probes).  stats[0] += entries (per block),
+// Cluster-pair lists, one workgroup (one wavefront) per i-tower: each neighbour tower's cluster boxes and
+// reference points are staged in LDS once and shared by all i-clusters of the tower (one lane each).
+// Rows have a fixed capacity (entries [ci*cap, ci*cap + count[ci])); the traversal order (oy, ox, sz, j)
+// is the oracle's, so lists compare entry by entry.  The first j-cluster whose z-range can reach the
+// i-cluster is a binary search on the monotone upper z-extents held in LDS.  stats[0] += entries,
 // stats[1] = max(count) so the host can grow cap and rebuild if a row overflowed.
-__global__ __launch_bounds__(128) void k_cluster_lists(int64_t nclusters, TowerGeom g,
-                                                       const int32_t *__restrict__ cluster_tower,
-                                                       const int64_t *__restrict__ cstart,
-                                                       const double *__restrict__ bbox,
-                                                       const double4 *__restrict__ c0, int32_t cap,
-                                                       int32_t *__restrict__ count, int4 *__restrict__ list,
-                                                       unsigned long long *__restrict__ stats) {
-  __shared__ unsigned long long block_total;
-  if (threadIdx.x == 0) block_total = 0;
-  __syncthreads();
-  const int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci < nclusters) {
-    const int32_t t = cluster_tower[ci];
-    const int tx = t % g.nt[0], ty = t / g.nt[0];
-    double bi[6];
+constexpr int kListThreads = 64;
+constexpr int kListLds = 128;  // j-clusters staged per chunk
+
+__global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
+    int64_t ntowers, TowerGeom g, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
+    const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
+    unsigned long long *__restrict__ stats) {
+  __shared__ double sb[kListLds * 6];
+  __shared__ double4 sc[kListLds];
+  const int64_t t = blockIdx.x;
+  if (t >= ntowers) return;
+  const int tx = (int)(t % g.nt[0]), ty = (int)(t / g.nt[0]);
+  const int64_t ib = cstart[t], ie = cstart[t + 1];
+  const double il2 = g.ilen * g.ilen;
+  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+  unsigned long long tot = 0;
+  int32_t mx = 0;
+  for (int64_t base = ib; base < ie; base += kListThreads) {
+    const int64_t ci = base + threadIdx.x;
+    const bool act = ci < ie;
+    double bi[6] = {0., 0., 0., 0., 0., 0.};
+    double4 ai = make_double4(0., 0., 0., 0.);
+    if (act) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
-    const double4 ai = c0[ci];
-    const double il2 = g.ilen * g.ilen;
-    // z window, widened so that it never drops a cluster the exact test accepts
-    const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+      for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
+      ai = c0[ci];
+    }
     const int64_t w0 = ci * (int64_t)cap;
     int32_t cnt = 0;
     for (int oy = -g.R; oy <= g.R; ++oy) {
@@ -199,48 +205,64 @@ __global__ __launch_bounds__(128) void k_cluster_lists(int64_t nclusters, TowerG
         const int64_t tj = jx + (int64_t)jy * g.nt[0];
         const int64_t cb = cstart[tj], ce = cstart[tj + 1];
         if (cb == ce) continue;
-        const double zf = bbox[6 * cb + 5], zl = bbox[6 * (ce - 1) + 5];
+        const bool single = ce - cb <= kListLds;
         for (int sz = -1; sz <= 1; ++sz) {
           const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
-          const double zmin = bi[2] - zwin - shz;  // in the j tower's own frame
+          const double zmin = bi[2] - zwin - shz;  // window in the j tower's own frame
           const double zmax = bi[5] + zwin - shz;
-          if (zl < zmin) continue;
-          int64_t lo;
-          if (zf >= zmin) {
-            lo = cb;
-          } else {
-            // lower bound of zhi >= zmin: interpolate, then walk (zhi is non-decreasing)
-            const double frac = (zmin - zf) / (zl - zf);
-            lo = cb + (int64_t)(frac * (double)(ce - 1 - cb));
-            lo = lo < cb ? cb : (lo > ce - 1 ? ce - 1 : lo);
-            while (lo > cb && bbox[6 * (lo - 1) + 5] >= zmin) --lo;
-            while (lo < ce && bbox[6 * lo + 5] < zmin) ++lo;
-          }
-          for (int64_t cj = lo; cj < ce; ++cj) {
-            const double *bj = bbox + 6 * cj;
-            if (bj[2] > zmax) break;
-            if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
-              if (cnt < cap) {
-                const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-                const double4 b = c0[cj];
-                const float dx = (float)((b.x + shx) - ai.x);
-                const float dy = (float)((b.y + shy) - ai.y);
-                const float dz = (float)((b.z + shz) - ai.z);
-                list[w0 + cnt] = make_int4((int32_t)cj | (sid << kShiftBits), __float_as_int(dx),
-                                           __float_as_int(dy), __float_as_int(dz));
+          for (int64_t jb = cb; jb < ce; jb += kListLds) {
+            const int nj = (int)(ce - jb < kListLds ? ce - jb : kListLds);
+            if (!single || (sz == -1 && jb == cb)) {
+              __syncthreads();
+              for (int k = threadIdx.x; k < 6 * nj; k += kListThreads) sb[k] = bbox[6 * jb + k];
+              for (int k = threadIdx.x; k < nj; k += kListThreads) sc[k] = c0[jb + k];
+              __syncthreads();
+            }
+            if (!act) continue;
+            // lower bound of zhi >= zmin in the chunk
+            int lo = 0, hi = nj;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (sb[6 * mid + 5] < zmin) lo = mid + 1;
+              else hi = mid;
+            }
+            for (int k = lo; k < nj; ++k) {
+              const double *bj = sb + 6 * k;
+              if (bj[2] > zmax) break;
+              if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+                if (cnt < cap) {
+                  const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                  const double4 b = sc[k];
+                  const float dx = (float)((b.x + shx) - ai.x);
+                  const float dy = (float)((b.y + shy) - ai.y);
+                  const float dz = (float)((b.z + shz) - ai.z);
+                  list[w0 + cnt] = make_int4((int32_t)(jb + k) | (sid << kShiftBits), __float_as_int(dx),
+                                             __float_as_int(dy), __float_as_int(dz));
+                }
+                ++cnt;
               }
-              ++cnt;
             }
           }
         }
       }
     }
-    count[ci] = cnt;
-    atomicAdd(&block_total, (unsigned long long)cnt);
-    atomicMax(&stats[1], (unsigned long long)cnt);
+    if (act) {
+      count[ci] = cnt;
+      tot += (unsigned long long)cnt;
+      mx = cnt > mx ? cnt : mx;
+    }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(&stats[0], block_total);
+  // one atomic per wavefront for the totals
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    tot += __shfl_xor(tot, off, kWave);
+    const int32_t o = __shfl_xor(mx, off, kWave);
+    mx = o > mx ? o : mx;
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&stats[0], tot);
+    atomicMax(&stats[1], (unsigned long long)mx);
+  }
 }
 
 // FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots sit at
@@ -362,9 +384,8 @@ void vcl_build(Ctx &c) {
     c.list.ensure((size_t)ncl * cap + 1);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.list_stats.p);
     LJG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cluster_lists, dim3(blocks_for(ncl, 128)), dim3(128), 0, c.stream, ncl, g,
-                       c.cluster_tower.p, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p,
-                       c.list_stats.p);
+    hipLaunchKernelGGL(k_cluster_lists_tower, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow, g,
+                       c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, c.list_stats.p);
     LJG_HIP(hipGetLastError());
     unsigned long long st[2] = {0, 0};
     LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));

@@ -22,10 +22,12 @@ ERR_NODEVICE = 6
 KERNEL_LC_FP64 = 0
 KERNEL_CLUSTER_FP64 = 1
 KERNEL_CLUSTER_MIXED = 2
+KERNEL_VERLET_MIXED = 3
 KERNEL_NAMES = {
     KERNEL_LC_FP64: "lc_fp64",
     KERNEL_CLUSTER_FP64: "cluster_fp64",
     KERNEL_CLUSTER_MIXED: "cluster_fp32pair_fp64acc",
+    KERNEL_VERLET_MIXED: "verlet_fp32pair_fp64acc",
 }
 
 _d = ctypes.c_double
@@ -89,6 +91,7 @@ SIGNATURES = {
     "ljg_get_cells": (_i32, [_p, _p, _p]),
     "ljg_cluster_sizes": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
     "ljg_get_clusters": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
+    "ljg_get_verlet_lists": (_i32, [_p, _p, _p]),
     "ljg_generate_lattice": (_i32, [ctypes.c_uint64, _i64, _i64, _i64, _i64, _d, _p, _d, _d, _i32] + [_p] * 7),
 }
 
@@ -271,3 +274,11 @@ class Engine:
         out["list_shift"] = out["list_shift"][:ne]
         out.update(towers_per_dim=tpd, side=side, towers_per_il=int(r[0]))
         return out
+
+    def verlet_lists(self):
+        """Neighbour lists (upload indices) after a Verlet-list build: (offsets[n+1], neighbors)."""
+        off = np.empty(self.n + 1, dtype=np.int64)
+        self._check(self.lib.ljg_get_verlet_lists(self.h, _ptr(off), None), "ljg_get_verlet_lists")
+        nb = np.empty(max(int(off[-1]), 1), dtype=np.int32)
+        self._check(self.lib.ljg_get_verlet_lists(self.h, _ptr(off), _ptr(nb)), "ljg_get_verlet_lists")
+        return off, nb[: int(off[-1])]

@@ -37,6 +37,8 @@ typedef int ljg_status;
 #define LJG_KERNEL_LC_FP64 0      /* LinkedCells, 27-cell full shell, FP64 everything */
 #define LJG_KERNEL_CLUSTER_FP64 1 /* cluster-pair tiles, FP64 pair term and sums */
 #define LJG_KERNEL_CLUSTER_MIXED 2 /* cluster-pair tiles, FP32 pair term, FP64 per-particle sums */
+#define LJG_KERNEL_VERLET_MIXED 3  /* Verlet lists on linked cells (cluster size 1), fixed-point positions,
+                                      FP32 pair term, FP64 per-particle sums; boxes >= 3 (rc + skin) */
 
 typedef struct {
   double box_min[3];
@@ -126,6 +128,11 @@ ljg_status ljg_cluster_sizes(ljg_ctx *ctx, int64_t *num_slots, int64_t *num_clus
  * (xlo,ylo,zlo,xhi,yhi,zhi).  Any pointer may be NULL. */
 ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
                             int32_t *list_j, int32_t *list_shift, double *bbox);
+
+/* Neighbour lists after a Verlet-list build, in upload indices: neighbours of particle p are
+ * neighbors[offsets[p] .. offsets[p+1]).  offsets has n+1 entries; neighbors may be NULL to query sizes
+ * (offsets[n] is the total). */
+ljg_status ljg_get_verlet_lists(ljg_ctx *ctx, int64_t *offsets, int32_t *neighbors);
 
 /* ---- synthetic systems (host-side, seeded Philox4x32-10) ------------------------------------- */
 /* Simple-cubic lattice of nx*ny*nz sites, spacing a, positions box_min + (i+1/2)a plus uniform jitter in

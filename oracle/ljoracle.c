@@ -789,6 +789,45 @@ void ljo_vcl_free(ljo_vcl *v) {
 }
 
 /* ------------------------------------------------------------------------------------------------ */
+static uint32_t to_fixed(double x, double mn, double inv_scale) {
+  return (uint32_t)(uint64_t)llrint((x - mn) * inv_scale);
+}
+
+int64_t ljo_verlet_pairs_fixed(int64_t n, const double *x, const double *y, const double *z, const double box_min[3],
+                               const double box_max[3], double ilen, int64_t *off, int32_t *nb, int64_t cap) {
+  double scale[3], inv[3];
+  for (int d = 0; d < 3; ++d) {
+    const double L = box_max[d] - box_min[d];
+    scale[d] = L / 4294967296.0;
+    inv[d] = 4294967296.0 / L;
+  }
+  uint32_t *X = malloc((n ? n : 1) * 3 * sizeof(uint32_t));
+  for (int64_t i = 0; i < n; ++i) {
+    X[3 * i] = to_fixed(x[i], box_min[0], inv[0]);
+    X[3 * i + 1] = to_fixed(y[i], box_min[1], inv[1]);
+    X[3 * i + 2] = to_fixed(z[i], box_min[2], inv[2]);
+  }
+  const double il2 = ilen * ilen;
+  int64_t tot = 0;
+  off[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double dx = (double)(int32_t)(X[3 * i] - X[3 * j]) * scale[0];
+      const double dy = (double)(int32_t)(X[3 * i + 1] - X[3 * j + 1]) * scale[1];
+      const double dz = (double)(int32_t)(X[3 * i + 2] - X[3 * j + 2]) * scale[2];
+      if (dx * dx + dy * dy + dz * dz <= il2) {
+        if (tot < cap) nb[tot] = (int32_t)j;
+        ++tot;
+      }
+    }
+    off[i + 1] = tot;
+  }
+  free(X);
+  return tot <= cap ? tot : -tot;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
 /* TimeDiscretization.cpp:17-69: oldF = F; F = 0 (no global force); x += v*dt + F*(dt*dt/(2m)). */
 void ljo_verlet_positions(int64_t n, double *x, double *y, double *z, const double *vx, const double *vy,
                           const double *vz, double *fx, double *fy, double *fz, double *ofx, double *ofy,

@@ -113,6 +113,15 @@ void ljo_vcl_get(const ljo_vcl *v, int64_t *slot_particle, int32_t *cluster_towe
                  int32_t *list_j, int32_t *list_shift, double *bbox);
 void ljo_vcl_free(ljo_vcl *v);
 
+/* ---- Verlet lists (VerletLists / SoAFunctorVerlet neighbour sets, VerletLists.h) ----------------
+ * Pairs within ilen (inclusive) in a periodic box, evaluated on 32-bit fixed-point coordinates
+ * X = round((x - min) * 2^32 / L) mod 2^32 (the engine's representation): separation = wrapping int32
+ * difference times L / 2^32, r^2 summed in FP64 as (dx*dx + dy*dy) + dz*dz.  Brute force O(N^2).
+ * neighbors of particle i are written to nb[off[i] .. off[i+1]) in increasing j.  Returns the total,
+ * or -total if cap is too small (off is always complete). */
+int64_t ljo_verlet_pairs_fixed(int64_t n, const double *x, const double *y, const double *z, const double box_min[3],
+                               const double box_max[3], double ilen, int64_t *off, int32_t *nb, int64_t cap);
+
 /* ---- velocity Verlet: TimeDiscretization.cpp:17-69 (positions), :152-165 (velocities) ---------- */
 void ljo_verlet_positions(int64_t n, double *x, double *y, double *z, const double *vx, const double *vy,
                           const double *vz, double *fx, double *fy, double *fz, double *ofx, double *ofy,

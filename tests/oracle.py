@@ -74,6 +74,8 @@ def lib():
     L.ljo_vcl_get.argtypes = [_p] * 7
     L.ljo_vcl_free.restype = None
     L.ljo_vcl_free.argtypes = [_p]
+    L.ljo_verlet_pairs_fixed.restype = _i64
+    L.ljo_verlet_pairs_fixed.argtypes = [_i64, _p, _p, _p, _p, _p, _d, _p, _p, _i64]
     L.ljo_verlet_positions.restype = None
     L.ljo_verlet_positions.argtypes = [_i64] + [_p] * 14 + [_d]
     L.ljo_verlet_velocities.restype = None
@@ -221,3 +223,14 @@ def verlet_velocities(vx, vy, vz, fx, fy, fz, ofx, ofy, ofz, mass_of_type, types
     t = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
     lib().ljo_verlet_velocities(len(vx), P(vx), P(vy), P(vz), P(fx), P(fy), P(fz), P(ofx), P(ofy), P(ofz),
                                 P(f64(mass_of_type)), P(t), dt)
+
+
+def verlet_pairs_fixed(x, y, z, box_min, box_max, ilen):
+    n = len(x)
+    off = np.zeros(n + 1, dtype=np.int64)
+    cap = 200 * n
+    nb = np.zeros(max(cap, 1), dtype=np.int32)
+    tot = lib().ljo_verlet_pairs_fixed(n, P(f64(x)), P(f64(y)), P(f64(z)), P(f64(box_min)), P(f64(box_max)), ilen,
+                                       P(off), P(nb), cap)
+    assert tot >= 0
+    return off, nb[:tot]

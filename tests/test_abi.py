@@ -1,0 +1,78 @@
+"""CPU checks of the product library without a GPU: it loads, exports every entry point include/ljgpu.h
+declares, reports the missing device instead of falling back, and its host-side system generator."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import autopas_amd as ap
+from autopas_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "ljgpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:ljg_status|const char \*)\s*(ljg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ap.load_library()
+    names = declared_entry_points()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(engine.SIGNATURES), set(names) ^ set(engine.SIGNATURES)
+
+
+def test_no_device_is_reported_not_emulated():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(ap.EngineError) as ei:
+        ap.Engine((0, 0, 0), (10, 10, 10), 2.5, 0.3)
+    assert ei.value.status == engine.ERR_NODEVICE
+
+
+def test_generator_is_deterministic_and_thermalised():
+    n_side, a = 10, 1.1
+    out1 = ap.generate_lattice(42, n_side ** 3, (n_side,) * 3, a, (0, 0, 0), 0.05, 0.7, ntypes=4)
+    out2 = ap.generate_lattice(42, n_side ** 3, (n_side,) * 3, a, (0, 0, 0), 0.05, 0.7, ntypes=4)
+    for u, v in zip(out1, out2):
+        assert np.array_equal(u, v)
+    x, y, z, vx, vy, vz, t = out1
+    n = n_side ** 3
+    # jitter bounded by +-0.05 around (i + 1/2) a
+    ix = np.arange(n) % n_side
+    assert np.all(np.abs(x - (ix + 0.5) * a) <= 0.05)
+    # zero net momentum, exact temperature with 3n - 3 degrees of freedom
+    assert abs(vx.sum()) < 1e-10 and abs(vy.sum()) < 1e-10 and abs(vz.sum()) < 1e-10
+    assert (vx @ vx + vy @ vy + vz @ vz) / (3 * n - 3) == pytest.approx(0.7, rel=1e-12)
+    assert set(np.unique(t)) <= {0, 1, 2, 3} and len(np.unique(t)) == 4
+    other = ap.generate_lattice(43, n, (n_side,) * 3, a, (0, 0, 0), 0.05, 0.7)
+    assert not np.array_equal(other[0], x)
+
+
+def test_generator_rejects_bad_arguments():
+    with pytest.raises(ap.EngineError):
+        ap.generate_lattice(1, 10, (2, 2, 2), 1.0)  # 10 > 8 sites
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_params_struct_matches_header():
+    text = open(os.path.join(ROOT, "include", "ljgpu.h")).read()
+    body = text[text.index("typedef struct {"):text.index("} ljg_params;")]
+    fields = re.findall(r"(double|int32_t)\s+(\w+)(\[\d+\])?;", body)
+    assert [f[1] for f in fields] == [f[0] for f in engine.Params._fields_]
+    assert ctypes.sizeof(engine.Params) == 8 * 8 + 12 * 4  # 8 doubles, 12 int32

@@ -81,11 +81,30 @@ def test_cluster_lists_bit_exact(oracle, M):
         assert np.array_equal(g[k], r[k]), k
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+MIXED = (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED)
+ALL_KERNELS = [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED]
+
+
+def test_verlet_lists_exact(oracle):
+    announce("verlet lists exact: 10^3 lattice rho 0.75, ilen 2.8")
+    s = lattice_box(10, 0.75, seed=8)
+    L = s["L"]
+    with make_engine(L) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        e.force(ap.KERNEL_VERLET_MIXED, globals=False)
+        off, nb = e.verlet_lists()
+    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
+    roff, rnb = oracle.verlet_pairs_fixed(wx, wy, wz, (0, 0, 0), (L, L, L), RC + SKIN)
+    assert np.array_equal(off, roff)
+    for p in range(s["n"]):
+        assert np.array_equal(np.sort(nb[off[p]:off[p + 1]]), rnb[roff[p]:roff[p + 1]]), p
+
+
+@pytest.mark.parametrize("kernel", ALL_KERNELS)
 @pytest.mark.parametrize("M", [4, 8])
 def test_forces_vs_direct_sum(oracle, kernel, M):
-    if kernel == ap.KERNEL_LC_FP64 and M == 8:
-        pytest.skip("cluster size does not apply to the linked-cells kernel")
+    if kernel in (ap.KERNEL_LC_FP64, ap.KERNEL_VERLET_MIXED) and M == 8:
+        pytest.skip("cluster size does not apply to this kernel")
     announce(f"forces vs direct sum kernel={ap.KERNEL_NAMES[kernel]} M={M}: 8^3 lattice")
     s = lattice_box(8, 0.75, seed=3)
     L = s["L"]
@@ -94,7 +113,7 @@ def test_forces_vs_direct_sum(oracle, kernel, M):
         g = e.force(kernel, globals=True)
         f = e.download()
     ref = oracle.direct_sum(f["x"], f["y"], f["z"], RC, box_len=(L, L, L))
-    if kernel == ap.KERNEL_CLUSTER_MIXED:
+    if kernel in MIXED:
         assert rms_rel(f, ref) <= 1e-5
         assert abs(g.potential_energy - ref["epot"]) <= 1e-5 * abs(ref["epot"])
         assert abs(g.virial - ref["virial"]) <= 1e-5 * abs(ref["virial"])
@@ -105,6 +124,16 @@ def test_forces_vs_direct_sum(oracle, kernel, M):
         assert abs(g.potential_energy - ref["epot"]) <= 1e-10 * abs(ref["epot"])
         assert abs(g.virial - ref["virial"]) <= 1e-10 * abs(ref["virial"])
         assert g.pair_count == ref["pairs"]
+
+
+def test_verlet_rejects_small_box():
+    announce("verlet kernel on a box shorter than 3 interaction lengths: rejected")
+    s = lattice_box(4, 0.43, seed=5, jitter=0.1)
+    with make_engine(s["L"]) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        with pytest.raises(ap.EngineError) as ei:
+            e.force(ap.KERNEL_VERLET_MIXED)
+        assert ei.value.status == 5
 
 
 @pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
@@ -125,12 +154,12 @@ def test_small_box_multiple_images(oracle, kernel):
         assert g.pair_count == ref["pairs"]
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+@pytest.mark.parametrize("kernel", ALL_KERNELS)
 def test_four_type_mixing(oracle, kernel):
-    announce(f"4-type LB mixing kernel={ap.KERNEL_NAMES[kernel]}: rc 3, 8^3 lattice")
+    announce(f"4-type LB mixing kernel={ap.KERNEL_NAMES[kernel]}: rc 3, 10^3 lattice")
     eps = [0.8, 1.0, 1.1, 1.2]
     sig = [0.9, 1.0, 1.05, 1.1]
-    s = lattice_box(8, 0.7, seed=9, ntypes=4)
+    s = lattice_box(10, 0.7, seed=9, ntypes=4)
     L = s["L"]
     with make_engine(L, M=4, rc=3.0) as e:
         e.set_types(eps, sig, [1.0] * 4)
@@ -138,7 +167,7 @@ def test_four_type_mixing(oracle, kernel):
         g = e.force(kernel, globals=True)
         f = e.download()
     ref = oracle.direct_sum(f["x"], f["y"], f["z"], 3.0, box_len=(L, L, L), types=s["type"], eps=eps, sig=sig)
-    tol = 1e-5 if kernel == ap.KERNEL_CLUSTER_MIXED else 1e-10
+    tol = 1e-5 if kernel in MIXED else 1e-10
     assert rms_rel(f, ref) <= tol
     assert abs(g.potential_energy - ref["epot"]) <= tol * abs(ref["epot"])
 
@@ -215,7 +244,7 @@ def test_md_trajectory_lc_fp64(oracle):
     assert np.max(np.abs(g["vx"] - vx)) < 1e-8
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_MIXED])
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED])
 def test_deterministic(kernel):
     announce(f"determinism kernel={ap.KERNEL_NAMES[kernel]}: two identical runs")
     s = lattice_box(8, 0.75, seed=12)
@@ -236,7 +265,7 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     s = lattice_box(32, 0.75, seed=2, temperature=0.9)
     L = s["L"]
     res = {}
-    for kernel in (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64):
+    for kernel in (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64, ap.KERNEL_VERLET_MIXED):
         with make_engine(L, M=4) as e:
             e.upload(s["x"], s["y"], s["z"])
             g = e.force(kernel, globals=True)
@@ -246,7 +275,9 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     assert rms_rel(res[ap.KERNEL_CLUSTER_FP64][0], ref) <= 1e-12
     assert rms_rel(res[ap.KERNEL_LC_FP64][0], ref) <= 1e-12
     err = rms_rel(f, ref)
-    print(f"force RMS relative error FP32-pair/FP64-acc vs FP64: {err:.3e}")
+    errv = rms_rel(res[ap.KERNEL_VERLET_MIXED][0], ref)
+    print(f"force RMS relative error vs FP64: cluster FP32-pair {err:.3e}, verlet FP32-pair {errv:.3e}")
     assert err <= 1e-5
+    assert errv <= 1e-5
     assert res[ap.KERNEL_LC_FP64][1].pair_count == ref["pairs"]
     assert res[ap.KERNEL_CLUSTER_FP64][1].pair_count == ref["pairs"]
