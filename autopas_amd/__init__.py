@@ -5,6 +5,7 @@ this package only binds it.
 """
 from .engine import (  # noqa: F401
     KERNEL_CLUSTER_FP64,
+    KERNEL_CLUSTER_JROWS_MIXED,
     KERNEL_CLUSTER_MIXED,
     KERNEL_LC_FP64,
     KERNEL_VERLET_MIXED,

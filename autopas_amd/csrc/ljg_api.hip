@@ -28,7 +28,7 @@ static ljg_status fail(Ctx &c, ljg_status st, const std::string &msg) {
   catch (const HipError &e) {                                                                        \
     char buf[512];                                                                                   \
     std::snprintf(buf, sizeof buf, "%s failed: %s (line %d)", e.what, hipGetErrorString(e.err), e.line); \
-    return fail(c, e.line < 0 ? LJG_ERR_CAPACITY : LJG_ERR_HIP, buf);                               \
+    return fail(c, e.line == -2 ? LJG_ERR_UNSUPPORTED : e.line < 0 ? LJG_ERR_CAPACITY : LJG_ERR_HIP, buf); \
   }                                                                                                  \
   catch (const std::bad_alloc &) {                                                                   \
     return fail(c, LJG_ERR_HIP, "host allocation failed");                                          \
@@ -72,11 +72,14 @@ __global__ void k_zero_i32(int64_t n, int32_t *p) {
   if (i < n) p[i] = 0;
 }
 
-static bool is_cluster_kernel(int k) { return k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_MIXED; }
+static bool is_cluster_kernel(int k) {
+  return k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_MIXED || k == LJG_KERNEL_CLUSTER_JROWS_MIXED;
+}
 
 static void rebuild_for(Ctx &c, int kernel) {
   wrap_positions(c);
   if (is_cluster_kernel(kernel)) {
+    c.want_jrows = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
     vcl_build(c);
   } else {
     lc_bin(c);
@@ -87,18 +90,23 @@ static void rebuild_for(Ctx &c, int kernel) {
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {
-  if (is_cluster_kernel(kernel)) return c.layout == Layout::Clusters;
+  if (is_cluster_kernel(kernel))
+    return c.layout == Layout::Clusters && c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED);
   if (kernel == LJG_KERNEL_VERLET_MIXED) return c.layout == Layout::Verlet;
   return c.layout == Layout::Cells;
 }
 
-// The Verlet-list kernel relies on minimum-image fixed-point differences and distinct neighbour cells.
-static bool kernel_supported(const Ctx &c, int kernel) {
-  if (kernel != LJG_KERNEL_VERLET_MIXED) return true;
+// The fixed-point kernels rely on minimum-image differences: the Verlet-list kernel also on 27 distinct
+// neighbour cells (box >= 3 ilen), the j-row cluster kernel on a single image within ilen (box > 2 ilen).
+static const char *kernel_unsupported(const Ctx &c, int kernel) {
   const double ilen = c.prm.cutoff + c.prm.skin;
-  for (int d = 0; d < 3; ++d)
-    if (!(c.prm.box_max[d] - c.prm.box_min[d] >= 3.0 * ilen)) return false;
-  return true;
+  for (int d = 0; d < 3; ++d) {
+    const double L = c.prm.box_max[d] - c.prm.box_min[d];
+    if (kernel == LJG_KERNEL_VERLET_MIXED && !(L >= 3.0 * ilen)) return "box shorter than 3 interaction lengths";
+    if (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED && !(L > 2.0 * ilen))
+      return "box not longer than 2 interaction lengths";
+  }
+  return nullptr;
 }
 
 static int64_t partial_items(const Ctx &c, int /*kernel*/) { return c.part_items; }
@@ -106,10 +114,12 @@ static int64_t partial_items(const Ctx &c, int /*kernel*/) { return c.part_items
 static void run_force(Ctx &c, int kernel, bool globals) {
   if (kernel == LJG_KERNEL_LC_FP64) lc_force(c, globals);
   else if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_force(c, globals);
+  else if (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) cluster_jrows_force(c, globals);
   else cluster_force(c, kernel == LJG_KERNEL_CLUSTER_MIXED, globals);
 }
 
 static int64_t candidate_evals(const Ctx &c, int kernel) {
+  if (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) return c.nentries * (int64_t)c.M;
   if (is_cluster_kernel(kernel)) return c.nentries * (int64_t)c.M * c.M;
   if (kernel == LJG_KERNEL_VERLET_MIXED) return c.nentries;
   return -1;
@@ -174,8 +184,8 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.posfx.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
-  c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
+  c.pos32.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.red_out.release(); c.red_hits.release(); c.hits_slots.release(); c.flag.release(); c.scan_tmp.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -321,10 +331,10 @@ ljg_status ljg_build_pairs(ljg_ctx *h) {
 }
 
 ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globals *out) {
-  if (!h || kernel < 0 || kernel > 3) return LJG_ERR_ARG;
+  if (!h || kernel < 0 || kernel > LJG_KERNEL_LAST) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
-  if (!kernel_supported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, "box shorter than 3 interaction lengths");
+  if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
   if (!layout_matches(c, kernel)) rebuild_for(c, kernel);
   run_force(c, kernel, calc_globals != 0);
   ljg_globals g{};
@@ -347,10 +357,10 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 }
 
 ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out) {
-  if (!h || nsteps < 0 || kernel < 0 || kernel > 3 || !(dt > 0.)) return LJG_ERR_ARG;
+  if (!h || nsteps < 0 || kernel < 0 || kernel > LJG_KERNEL_LAST || !(dt > 0.)) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
-  if (!kernel_supported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, "box shorter than 3 interaction lengths");
+  if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
   const auto t0 = std::chrono::steady_clock::now();
   ljg_run_stats rs{};
   const bool timing = c.timing;
@@ -361,8 +371,9 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     v.push_back(e);
     return e;
   };
-  c.red_hits.ensure(2);
-  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.red_hits.p);
+  c.hits_slots.ensure(kAccSlots * kAccStride);
+  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kAccSlots * kAccStride), 0, c.stream, (int64_t)(kAccSlots * kAccStride),
+                     reinterpret_cast<int64_t *>(c.hits_slots.p));
   LJG_HIP(hipGetLastError());
   const bool saved_timing = c.timing;
   c.timing = false;  // per-launch events are recorded here instead
@@ -372,7 +383,7 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
     ++rs.rebuilds;
   }
-  c.hits_acc = reinterpret_cast<unsigned long long *>(c.red_hits.p);
+  c.hits_acc = c.hits_slots.p;
   for (int32_t s = 0; s < nsteps; ++s) {
     // kick of the previous step (if any) fused with this step's drift; a particle that moved skin/2
     // since the last rebuild stamps flag[0] with this step's number
@@ -400,9 +411,11 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
   c.hits_acc = nullptr;
   if (nsteps > 0) md_kick_drift(c, true, false, dt, 0);  // final velocity update
   c.timing = saved_timing;
-  int64_t hits2 = 0;
-  LJG_HIP(hipMemcpyAsync(&hits2, c.red_hits.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  unsigned long long slots[kAccSlots * kAccStride];
+  LJG_HIP(hipMemcpyAsync(slots, c.hits_slots.p, sizeof slots, hipMemcpyDeviceToHost, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
+  int64_t hits2 = 0;
+  for (int k = 0; k < kAccSlots; ++k) hits2 += (int64_t)slots[k * kAccStride];
   rs.steps = nsteps;
   rs.pair_interactions = hits2 / 2;
   for (size_t k = 0; k + 1 < fev.size(); k += 2) {
@@ -485,12 +498,25 @@ ljg_status ljg_get_clusters(ljg_ctx *h, int64_t *slot_particle, int32_t *cluster
   // lists live in fixed-capacity rows on the device; hand them out compacted
   std::vector<int32_t> cnt((size_t)c.nclusters);
   if (!cnt.empty())
-    LJG_HIP(hipMemcpy(cnt.data(), c.list_count.p, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    LJG_HIP(hipMemcpy(cnt.data(), c.jrows_valid ? c.jcount.p : c.list_count.p, cnt.size() * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
   if (list_offsets) {
     list_offsets[0] = 0;
     for (size_t k = 0; k < cnt.size(); ++k) list_offsets[k + 1] = list_offsets[k] + cnt[k];
   }
-  if (list_j || list_shift) {
+  if ((list_j || list_shift) && c.jrows_valid) {
+    // j-particle rows: upload index of each j, no shift (minimum image)
+    std::vector<int32_t> l((size_t)c.nclusters * c.list_row), o((size_t)c.nslots);
+    if (!l.empty()) LJG_HIP(hipMemcpy(l.data(), c.jlist.p, l.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    LJG_HIP(hipMemcpy(o.data(), c.st[c.cur].orig.p, o.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    size_t w = 0;
+    for (size_t ci = 0; ci < cnt.size(); ++ci) {
+      for (int32_t k = 0; k < cnt[ci]; ++k, ++w) {
+        if (list_j) list_j[w] = o[l[ci * c.list_row + k]];
+        if (list_shift) list_shift[w] = -1;
+      }
+    }
+  } else if (list_j || list_shift) {
     std::vector<int4> l((size_t)c.nclusters * c.list_row);
     if (!l.empty()) LJG_HIP(hipMemcpy(l.data(), c.list.p, l.size() * sizeof(int4), hipMemcpyDeviceToHost));
     size_t w = 0;

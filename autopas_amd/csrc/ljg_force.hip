@@ -33,8 +33,8 @@ constexpr int kFThreads = 256;
 constexpr int kWavesPerBlock = kFThreads / kWave;
 
 // ---- block-level deterministic partials ------------------------------------------------------------
-// Per-block partial sums (fixed tree, so reproducible); the block's pair count is also added to *acc
-// when acc is set (integer addition, so the order of the atomics does not matter).
+// Per-block partial sums (fixed tree, so reproducible); the block's pair count is also added to acc's
+// per-XCD lines when acc is set (integer addition, so the order of the atomics does not matter).
 __device__ inline void block_partials(double e, double v, int64_t h, double *pe, double *pv, int64_t *ph,
                                       bool globals, unsigned long long *acc) {
   __shared__ double se[kFThreads], sv[kFThreads];
@@ -55,7 +55,7 @@ __device__ inline void block_partials(double e, double v, int64_t h, double *pe,
     pe[blockIdx.x] = se[0];
     pv[blockIdx.x] = sv[0];
     ph[blockIdx.x] = sh[0];
-    if (acc) atomicAdd(acc, (unsigned long long)sh[0]);
+    if (acc) acc_add(acc, (unsigned long long)sh[0]);
   }
 }
 
@@ -306,6 +306,163 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
   block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
 }
 
+// i-clusters against j-particle rows on fixed-point positions.  G i-clusters per wave, LG = 64/G lanes per
+// i-cluster; the lanes of a group stream the j slots of their row (the union of the members' neighbour
+// lists, so no j is evaluated that is farther than rc + skin from every i-member) and evaluate each j
+// against the M i-members held in registers.
+//  * FAST (cubic boxes whose rows passed the frame check of the list build): positions enter as wrapping
+//    int32 differences to the cluster's first member, converted to FP32 once per particle, and
+//    x_q - x_j is one FP32 subtraction.  Everything runs in fixed-point units (r^2, sigma^2, rc^2 scaled
+//    by (2^32/L)^2) and the force sums are rescaled once; energy and virial are scale-free.  Padding
+//    i-members sit at -3e30 and lanes past the end of the row at +3e30, so their r^2 overflows to +inf
+//    and fails the bit-pattern cutoff test.
+//  * otherwise: every pair separation is the exact minimum-image int32 difference x_q - x_j, scaled per
+//    dimension; padding members and lanes past the end of the row are masked.
+template <int M, int G, bool FAST, bool MIX, bool GLOB>
+__global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
+    int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t row,
+    const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, float rc2, float eps24_1,
+    float sig2_1, float shift6_1, int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+  constexpr int LG = kWave / G;
+  constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
+  constexpr int NV = pow2_at_least(NVAL);
+  __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
+  if constexpr (MIX) {
+    const float s2u = FAST ? (float)(f.inv_scale[0] * f.inv_scale[0]) : 1.f;
+    for (int k = threadIdx.x; k < 3 * ntypes * ntypes; k += kFThreads)
+      stab[k] = (float)tab64[k] * (k % 3 == 1 ? s2u : 1.f);
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int grp = lane / LG, l = lane % LG;
+  int ic = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
+  if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
+  const uint32_t rc2_bits = __float_as_uint(rc2);
+  double e_out = 0., v_out = 0.;
+  int64_t h_out = 0;
+  if (ic < nclusters) {
+    const int nreal = (int)c0[ic].w;
+    const int4 ref = posfx[(int64_t)ic * M];  // the first member is always real
+    float xi[M], yi[M], zi[M];   // FAST: FP32 offsets from the first member (fixed-point units)
+    uint32_t ux[M], uy[M], uz[M];  // exact variant: fixed-point positions
+    int ti[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int4 p = posfx[(int64_t)ic * M + q];
+      const bool pad = q >= nreal;
+      ux[q] = (uint32_t)p.x;
+      uy[q] = (uint32_t)p.y;
+      uz[q] = (uint32_t)p.z;
+      xi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.x - (uint32_t)ref.x);
+      yi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.y - (uint32_t)ref.y);
+      zi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.z - (uint32_t)ref.z);
+      ti[q] = pad ? 0 : p.w;
+    }
+    float ax[M], ay[M], az[M], ae[M], av[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
+    int hits = 0;
+    const int32_t *rowp = jlist + (int64_t)ic * row;
+    const int nj = lcount[ic];  // >= nreal: every row holds the cluster's own members
+    int4 p = make_int4(0, 0, 0, 0);
+    if (nj > 0) p = posfx[rowp[l < nj ? l : nj - 1]];
+    for (int base = 0; base < nj; base += LG) {
+      const bool valid = base + l < nj;
+      const int4 pj = p;
+      {
+        const int kn = base + LG + l;
+        p = posfx[rowp[kn < nj ? kn : nj - 1]];
+      }
+      const int tj = pj.w;
+      float xj = 0.f, yj = 0.f, zj = 0.f;
+      if constexpr (FAST) {
+        xj = valid ? (float)(int32_t)((uint32_t)pj.x - (uint32_t)ref.x) : kPadPos32;
+        yj = (float)(int32_t)((uint32_t)pj.y - (uint32_t)ref.y);
+        zj = (float)(int32_t)((uint32_t)pj.z - (uint32_t)ref.z);
+      }
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        float dx, dy, dz;
+        if constexpr (FAST) {
+          dx = xi[q] - xj; dy = yi[q] - yj; dz = zi[q] - zj;
+        } else {
+          dx = (float)(int32_t)(ux[q] - (uint32_t)pj.x) * f.scale32[0];
+          dy = (float)(int32_t)(uy[q] - (uint32_t)pj.y) * f.scale32[1];
+          dz = (float)(int32_t)(uz[q] - (uint32_t)pj.z) * f.scale32[2];
+        }
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+        if constexpr (!FAST) m = m && valid && q < nreal;
+        float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
+        if constexpr (MIX) {
+          const int t = 3 * (ti[q] * ntypes + tj);
+          eps24 = stab[t]; sig2 = stab[t + 1]; shift6 = stab[t + 2];
+        }
+        const float inv = __builtin_amdgcn_rcpf(r2);
+        const float lj2 = sig2 * inv;
+        const float lj6 = lj2 * lj2 * lj2;
+        const float fv = (lj6 * inv) * __builtin_fmaf(eps24 + eps24, lj6, -eps24);
+        const float fac = m ? fv : 0.f;
+        ax[q] = __builtin_fmaf(dx, fac, ax[q]);
+        ay[q] = __builtin_fmaf(dy, fac, ay[q]);
+        az[q] = __builtin_fmaf(dz, fac, az[q]);
+        hits += m ? 1 : 0;
+        if constexpr (GLOB) {
+          ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
+          av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);
+        }
+      }
+    }
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      v[q] = (double)ax[q];
+      v[M + q] = (double)ay[q];
+      v[2 * M + q] = (double)az[q];
+    }
+    v[3 * M] = (double)hits;
+    if constexpr (GLOB) {
+      double se = 0., sv = 0.;
+#pragma unroll
+      for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
+      v[3 * M + 1] = se;
+      v[3 * M + 2] = sv;
+    }
+    group_reduce_scatter<NV, LG>(v, l);
+    constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
+    constexpr int R = NV >> K;
+    const int q0 = group_value_slot<NV, LG>(l) * R;
+    // FAST: the sums are in fixed-point units; F = sum(d_u * fac_u) * (2^32 / L)
+    const double fs = FAST ? f.inv_scale[0] : 1.0;
+    if ((l & ((1 << (KL - K)) - 1)) == 0) {
+      const int64_t s0 = (int64_t)ic * M;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = q0 + r;
+        const double val = v[r];
+        if (q < M) {
+          if (q < nreal) fx[s0 + q] = val * fs;
+        } else if (q < 2 * M) {
+          if (q - M < nreal) fy[s0 + q - M] = val * fs;
+        } else if (q < 3 * M) {
+          if (q - 2 * M < nreal) fz[s0 + q - 2 * M] = val * fs;
+        } else if (q == 3 * M) {
+          h_out = (int64_t)val;
+        } else if (GLOB && q == 3 * M + 1) {
+          e_out = val;
+        } else if (GLOB && q == 3 * M + 2) {
+          v_out = val;
+        }
+      }
+    }
+  }
+  block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
+}
+
 // FP64 cluster kernel: same tiles (one i-cluster per wave), absolute FP64 positions with the periodic
 // shift of the entry, pair term exactly as LJFunctor.h:288-320.
 template <int M, bool MIX, bool GLOB>
@@ -492,6 +649,40 @@ static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const d
 #undef LJG_LAUNCH
 }
 
+template <int M, int G>
+static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
+  constexpr int per_block = kWavesPerBlock * G;
+  const int nb = blocks_for(c.nclusters, per_block);
+  c.part_e.ensure(nb);
+  c.part_v.ensure(nb);
+  c.part_hits.ensure(nb);
+  c.part_items = nb;
+  const FixGeom &f = c.fg;
+  const bool fast = f.cubic && c.jrows_frame_ok;
+  const double u2 = fast ? f.inv_scale[0] * f.inv_scale[0] : 1.0;  // (2^32/L)^2 in the fast variant
+  const float rc2 = (float)(a.rc2 * u2);
+  const float sig2 = (float)(a.sig2 * u2);
+  State &s = c.st[c.cur];
+#define LJG_LAUNCH(CUB, MIXV, GLOBV)                                                                              \
+  hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
+                     c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, rc2,                 \
+                     (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,  \
+                     c.part_v.p, c.part_hits.p, c.hits_acc)
+#define LJG_LAUNCH2(CUB)                                                              \
+  if (mix) {                                                                          \
+    if (glob) LJG_LAUNCH(CUB, true, true); else LJG_LAUNCH(CUB, true, false);        \
+  } else {                                                                            \
+    if (glob) LJG_LAUNCH(CUB, false, true); else LJG_LAUNCH(CUB, false, false);      \
+  }
+  if (fast) {
+    LJG_LAUNCH2(true)
+  } else {
+    LJG_LAUNCH2(false)
+  }
+#undef LJG_LAUNCH2
+#undef LJG_LAUNCH
+}
+
 // i-clusters per wave for the FP32 kernel: the tuned default per cluster size, or the caller's choice.
 int default_groups(int M) { return M >= 8 ? 1 : M == 4 ? 2 : M == 2 ? 4 : 16; }
 
@@ -517,9 +708,31 @@ void cluster_force(Ctx &c, bool mixed, bool globals) {
     LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)
     LJG_CASE(1, 4) LJG_CASE(1, 8) LJG_CASE(1, 16) LJG_CASE(1, 32) LJG_CASE(1, 64)
 #undef LJG_CASE
-    throw HipError{hipErrorInvalidValue, "unsupported (cluster size, i-clusters per wave) combination", -1};
+    throw HipError{hipErrorInvalidValue, "unsupported (cluster size, i-clusters per wave) combination", -2};
   launched:;
   }
+  LJG_HIP(hipGetLastError());
+  if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
+}
+
+// i-clusters per wave for the j-row kernel (rows are ~1.5x longer than a member's own list).
+static int default_jrow_groups(int M) { return M >= 8 ? 4 : M == 4 ? 8 : M == 2 ? 16 : 32; }
+
+void cluster_jrows_force(Ctx &c, bool globals) {
+  const ForceArgs a = make_args(c);
+  const bool mix = c.lj.ntypes > 1;
+  const double *tab = mix ? c.tab.p : nullptr;
+  const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_jrow_groups(c.M);
+  if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
+#define LJG_CASE(MV, GV) \
+  if (c.M == MV && G == GV) { launch_jrows<MV, GV>(c, mix, globals, a, tab); goto launched; }
+  LJG_CASE(8, 2) LJG_CASE(8, 4) LJG_CASE(8, 8)
+  LJG_CASE(4, 4) LJG_CASE(4, 8) LJG_CASE(4, 16)
+  LJG_CASE(2, 8) LJG_CASE(2, 16)
+  LJG_CASE(1, 16) LJG_CASE(1, 32)
+#undef LJG_CASE
+  throw HipError{hipErrorInvalidValue, "unsupported (cluster size, i-clusters per wave) combination", -2};
+launched:
   LJG_HIP(hipGetLastError());
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }

@@ -170,6 +170,14 @@ struct Ctx {
   // Verlet lists on the linked cells (fixed-point positions, transposed rows per wavefront)
   FixGeom fg{};
   DevBuf<int4> posfx;            // per slot: fixed-point x, y, z and the type
+  DevBuf<int32_t> jcl;           // j-row build, first pass: per i-cluster row of j-cluster indices
+  int32_t jccap = 0;
+  DevBuf<int32_t> jlist;         // cluster layout: per i-cluster row of j slots (union of members' lists)
+  DevBuf<int32_t> jcount;        // j slots per i-cluster
+  int32_t jcap = 0;
+  bool want_jrows = false;       // the next cluster build emits j-particle rows instead of cluster entries
+  bool jrows_valid = false;      // the current cluster rows are j-particle rows
+  bool jrows_frame_ok = false;   // every row pair within ilen has the same separation in the first member's frame
   DevBuf<int32_t> vcount;        // neighbours per particle
   DevBuf<int32_t> vlist;         // rows of vrow entries, transposed per wave of 64 particles
   int32_t vrow = 0, vcap = 0;
@@ -180,7 +188,8 @@ struct Ctx {
   DevBuf<double> part_e, part_v;  // per work item
   DevBuf<int64_t> part_hits;
   int64_t part_items = 0;         // partial-sum slots written by the last force launch
-  unsigned long long *hits_acc = nullptr;  // when set, force kernels add their pair counts here
+  unsigned long long *hits_acc = nullptr;  // when set, force kernels add their pair counts here (kAccSlots lines)
+  DevBuf<unsigned long long> hits_slots;   // the step loop's pair-count lines
   DevBuf<double> red_out;         // [0]=e [1]=v
   DevBuf<int64_t> red_hits;       // [0]
   DevBuf<int32_t> flag;           // [0] displacement, [1] capacity error
@@ -208,11 +217,16 @@ void cluster_force(Ctx &c, bool mixed, bool globals);  // ljg_force.hip
 void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
 void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
 void verlet_force(Ctx &c, bool globals);   // ljg_verlet.hip
+void verlet_geometry(Ctx &c);              // ljg_verlet.hip: fixed-point scales of the box
+void verlet_fixed_slots(Ctx &c, int64_t nslots);  // ljg_verlet.hip: posfx for slots [0, nslots)
+void cluster_jrows_force(Ctx &c, bool globals);   // ljg_force.hip
 void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
 void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip
 void permute_state(Ctx &c, const int32_t *src, int64_t new_slots);  // ljg_md.hip
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
+// stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
+void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
 
 constexpr int kShiftBits = 27;
 constexpr int32_t kClusterMask = (1 << kShiftBits) - 1;
@@ -223,6 +237,15 @@ __host__ __device__ inline int shift_component(int sid, int d) {
 }
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// Same-address device atomics from many workgroups serialize (~12 ns each on MI355X); counters that
+// every workgroup of a kernel updates are spread over kAccSlots lines, one per XCD (workgroups are
+// dealt to the 8 XCDs round-robin), and summed by the host.
+constexpr int kAccSlots = 8;
+constexpr int kAccStride = 16;  // u64 per slot: one 128-byte line each
+__device__ inline void acc_add(unsigned long long *acc, unsigned long long v) {
+  atomicAdd(acc + (blockIdx.x & (kAccSlots - 1)) * kAccStride, v);
+}
 
 // Histogram rank with one atomic per run of equal keys inside the wavefront.  Particles arrive in the
 // previous build's order, so neighbouring lanes mostly share a cell/tower and the atomics drop ~30x.

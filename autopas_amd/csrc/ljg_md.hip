@@ -176,8 +176,8 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   a.half_skin2 = hs * hs;
   a.kick = kick;
   a.drift = drift;
-  a.write_pos32 = c.layout == Layout::Clusters;
-  a.write_fixed = c.layout == Layout::Verlet;
+  a.write_pos32 = c.layout == Layout::Clusters && !c.jrows_valid;
+  a.write_fixed = c.layout == Layout::Verlet || (c.layout == Layout::Clusters && c.jrows_valid);
   a.fg = c.fg;
   a.M = c.M;
   for (int t = 0; t < kMaxTypes; ++t) {
@@ -274,6 +274,43 @@ __global__ __launch_bounds__(1024) void k_reduce(int64_t n, const double *__rest
     out[1] = sv[0];
     out_h[0] = sh[0];
   }
+}
+
+constexpr int kStatThreads = 256;
+
+__global__ __launch_bounds__(kStatThreads) void k_count_stats(int64_t n, const int32_t *__restrict__ cnt,
+                                                              unsigned long long *__restrict__ stats) {
+  __shared__ unsigned long long ss[kStatThreads];
+  __shared__ int32_t sm[kStatThreads];
+  unsigned long long sum = 0;
+  int32_t mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kStatThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kStatThreads) {
+    const int32_t v = cnt[i];
+    sum += (unsigned long long)v;
+    mx = v > mx ? v : mx;
+  }
+  ss[threadIdx.x] = sum;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int k = kStatThreads / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      ss[threadIdx.x] += ss[threadIdx.x + k];
+      sm[threadIdx.x] = sm[threadIdx.x + k] > sm[threadIdx.x] ? sm[threadIdx.x + k] : sm[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&stats[0], ss[0]);
+    atomicMax(&stats[1], (unsigned long long)sm[0]);
+  }
+}
+
+void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats) {
+  if (n <= 0) return;
+  int nb = blocks_for(n, kStatThreads * 16);
+  nb = nb < 256 ? nb : 256;
+  hipLaunchKernelGGL(k_count_stats, dim3(nb), dim3(kStatThreads), 0, c.stream, n, cnt, stats);
+  LJG_HIP(hipGetLastError());
 }
 
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals) {

@@ -159,15 +159,49 @@ __device__ inline double box_dist2(const double *a, const double *b, double shx,
 // reference points are staged in LDS once and shared by all i-clusters of the tower (one lane each).
 // Rows have a fixed capacity (entries [ci*cap, ci*cap + count[ci])); the traversal order (oy, ox, sz, j)
 // is the oracle's, so lists compare entry by entry.  The first j-cluster whose z-range can reach the
-// i-cluster is a binary search on the monotone upper z-extents held in LDS.  stats[0] += entries,
-// stats[1] = max(count) so the host can grow cap and rebuild if a row overflowed.
+// i-cluster is a binary search on the monotone upper z-extents held in LDS.  count[ci] is the row's full
+// length even when it exceeds cap, so the host can grow cap and rebuild if a row overflowed.
 constexpr int kListThreads = 64;
 constexpr int kListLds = 128;  // j-clusters staged per chunk
 
+// JCL = true (the j-particle row build, first pass): entries are the j-cluster index only (int32), and a
+// j-cluster that passes the box test under several images (small boxes, long clusters) is listed only at
+// the first of them in traversal order, since the expansion pass tests minimum-image separations.
+constexpr int kMaxM = 8;
+
+__device__ inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Whether the j-cluster with box bj passed the box test at a visit before (oy, ox, sz) in the traversal
+// order of k_cluster_lists_tower.  xy_repeats: the +-R tower window covers some tower twice.
+__device__ inline bool passed_earlier(const double *bi, const double *bj, const TowerGeom &g, int tx, int ty,
+                                      int oy, int ox, int sz, bool xy_repeats, double il2) {
+  const double shx = floor_div(tx + ox, g.nt[0]) * g.L[0], shy = floor_div(ty + oy, g.nt[1]) * g.L[1];
+  for (int s2 = -1; s2 < sz; ++s2)
+    if (box_dist2(bi, bj, shx, shy, s2 * g.L[2]) <= il2) return true;
+  if (!xy_repeats) return false;
+  for (int oy2 = -g.R; oy2 <= oy; ++oy2) {
+    if ((oy2 - oy) % g.nt[1] != 0) continue;
+    const int aoy = oy2 < 0 ? -oy2 : oy2;
+    const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+    for (int ox2 = -g.R; ox2 <= g.R; ++ox2) {
+      if (oy2 == oy && ox2 >= ox) break;
+      if ((ox2 - ox) % g.nt[0] != 0) continue;
+      const int aox = ox2 < 0 ? -ox2 : ox2;
+      const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+      if (dtx * dtx + dty * dty > il2) continue;
+      const double sx2 = floor_div(tx + ox2, g.nt[0]) * g.L[0], sy2 = floor_div(ty + oy2, g.nt[1]) * g.L[1];
+      for (int s2 = -1; s2 <= 1; ++s2)
+        if (box_dist2(bi, bj, sx2, sy2, s2 * g.L[2]) <= il2) return true;
+    }
+  }
+  return false;
+}
+
+template <bool JCL>
 __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
     int64_t ntowers, TowerGeom g, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
     const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
-    unsigned long long *__restrict__ stats) {
+    int32_t *__restrict__ jcl) {
   __shared__ double sb[kListLds * 6];
   __shared__ double4 sc[kListLds];
   const int64_t t = blockIdx.x;
@@ -176,8 +210,7 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
   const int64_t ib = cstart[t], ie = cstart[t + 1];
   const double il2 = g.ilen * g.ilen;
   const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
-  unsigned long long tot = 0;
-  int32_t mx = 0;
+  const bool xy_repeats = g.nt[0] < 2 * g.R + 1 || g.nt[1] < 2 * g.R + 1;
   for (int64_t base = ib; base < ie; base += kListThreads) {
     const int64_t ci = base + threadIdx.x;
     const bool act = ci < ie;
@@ -230,39 +263,105 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
               const double *bj = sb + 6 * k;
               if (bj[2] > zmax) break;
               if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
-                if (cnt < cap) {
-                  const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-                  const double4 b = sc[k];
-                  const float dx = (float)((b.x + shx) - ai.x);
-                  const float dy = (float)((b.y + shy) - ai.y);
-                  const float dz = (float)((b.z + shz) - ai.z);
-                  list[w0 + cnt] = make_int4((int32_t)(jb + k) | (sid << kShiftBits), __float_as_int(dx),
-                                             __float_as_int(dy), __float_as_int(dz));
+                if constexpr (JCL) {
+                  if (!passed_earlier(bi, bj, g, tx, ty, oy, ox, sz, xy_repeats, il2)) {
+                    if (cnt < cap) jcl[w0 + cnt] = (int32_t)(jb + k);
+                    ++cnt;
+                  }
+                } else {
+                  if (cnt < cap) {
+                    const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                    const double4 b = sc[k];
+                    const float dx = (float)((b.x + shx) - ai.x);
+                    const float dy = (float)((b.y + shy) - ai.y);
+                    const float dz = (float)((b.z + shz) - ai.z);
+                    list[w0 + cnt] = make_int4((int32_t)(jb + k) | (sid << kShiftBits), __float_as_int(dx),
+                                               __float_as_int(dy), __float_as_int(dz));
+                  }
+                  ++cnt;
                 }
-                ++cnt;
               }
             }
           }
         }
       }
     }
-    if (act) {
-      count[ci] = cnt;
-      tot += (unsigned long long)cnt;
-      mx = cnt > mx ? cnt : mx;
-    }
+    if (act) count[ci] = cnt;
   }
-  // one atomic per wavefront for the totals
+}
+
+// Second pass of the j-particle row build: one wavefront per i-cluster streams the members of its listed
+// j-clusters (64 at a time) and keeps the real j within ilen of at least one real i-member, by the FP64
+// length of the exact minimum-image fixed-point separation (the arithmetic of the Verlet-list build; it
+// needs L > 2 ilen).  Survivors are compacted in stream order with a ballot and a prefix popcount, so the
+// row is the union of the members' Verlet lists (members included), each j once.
+// stats[2] is set when, for some member/j pair within ilen, the member's or j's minimum-image offset
+// from the cluster's first member exceeds L/2 - ilen, or the pair separation is not the difference of
+// the two offsets (boxes not much larger than ilen plus the cluster extent).  The margin ilen covers the
+// motion allowed between rebuilds (skin/2 per particle).  The force kernel then uses exact per-pair
+// differences.
+constexpr int kExpandThreads = 256;
+
+template <int M>
+__global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
+    int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t ccap,
+    const int32_t *__restrict__ ccount, const int32_t *__restrict__ jcl, FixGeom f, double ilen, int32_t cap,
+    int32_t *__restrict__ count, int32_t *__restrict__ jrows, unsigned long long *__restrict__ stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t ic = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kExpandThreads / kWave) + threadIdx.x / kWave));
+  if (ic >= nclusters) return;
+  const double il2 = ilen * ilen;
+  int64_t flim[3];
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    tot += __shfl_xor(tot, off, kWave);
-    const int32_t o = __shfl_xor(mx, off, kWave);
-    mx = o > mx ? o : mx;
+  for (int d = 0; d < 3; ++d) flim[d] = (int64_t)(2147483648.0 - ilen * f.inv_scale[d]);
+  const int nri = (int)c0[ic].w;
+  int4 xi[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) xi[q] = posfx[ic * M + q];
+  const int nflat = ccount[ic] * M;
+  const int32_t *crow = jcl + ic * (int64_t)ccap;
+  int32_t *jrow = jrows + ic * (int64_t)cap;
+  int32_t cnt = 0;
+  bool frame_bad = false;
+  for (int base = 0; base < nflat; base += kWave) {
+    const int k = base + lane;
+    bool near = false;
+    int32_t js = 0;
+    if (k < nflat) {
+      js = crow[k / M] * M + (k % M);
+      const int4 xj = posfx[js];
+      if (xj.w >= 0) {  // padding slots carry type -1
+        const int64_t jx = (int32_t)((uint32_t)xj.x - (uint32_t)xi[0].x);
+        const int64_t jy = (int32_t)((uint32_t)xj.y - (uint32_t)xi[0].y);
+        const int64_t jz = (int32_t)((uint32_t)xj.z - (uint32_t)xi[0].z);
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const int32_t ix = (int32_t)((uint32_t)xi[q].x - (uint32_t)xj.x);
+          const int32_t iy = (int32_t)((uint32_t)xi[q].y - (uint32_t)xj.y);
+          const int32_t iz = (int32_t)((uint32_t)xi[q].z - (uint32_t)xj.z);
+          const double dx = (double)ix * f.scale[0];
+          const double dy = (double)iy * f.scale[1];
+          const double dz = (double)iz * f.scale[2];
+          const bool nq = q < nri && dx * dx + dy * dy + dz * dz <= il2;
+          near = near || nq;
+          if (nq) {
+            const int64_t qx = (int32_t)((uint32_t)xi[q].x - (uint32_t)xi[0].x);
+            const int64_t qy = (int32_t)((uint32_t)xi[q].y - (uint32_t)xi[0].y);
+            const int64_t qz = (int32_t)((uint32_t)xi[q].z - (uint32_t)xi[0].z);
+            frame_bad = frame_bad || qx - jx != ix || qy - jy != iy || qz - jz != iz || llabs(qx) > flim[0] ||
+                        llabs(jx) > flim[0] || llabs(qy) > flim[1] || llabs(jy) > flim[1] ||
+                        llabs(qz) > flim[2] || llabs(jz) > flim[2];
+          }
+        }
+      }
+    }
+    const unsigned long long ball = __ballot(near);
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
+    if (near && cnt + below < cap) jrow[cnt + below] = js;
+    cnt += __popcll(ball);
   }
-  if (threadIdx.x == 0) {
-    atomicAdd(&stats[0], tot);
-    atomicMax(&stats[1], (unsigned long long)mx);
-  }
+  if (lane == 0) count[ic] = cnt;
+  if (__ballot(frame_bad) != 0ull && lane == 0) stats[2] = 1ull;  // every writer stores the same value
 }
 
 // FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots sit at
@@ -378,15 +477,24 @@ void vcl_build(Ctx &c) {
                      t.orig.p, c.bbox.p, c.c0.p);
   LJG_HIP(hipGetLastError());
   c.list_count.ensure(ncl + 1);
-  c.list_stats.ensure(2);
-  int32_t cap = c.list_cap > 0 ? c.list_cap : 64;
+  c.list_stats.ensure(3);
+  const bool jrows = c.want_jrows;
+  if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
+  // pass 1: cluster-pair lists (j-row build: deduplicated j-cluster indices only)
+  int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : 64) : (c.list_cap > 0 ? c.list_cap : 64);
   for (;;) {
-    c.list.ensure((size_t)ncl * cap + 1);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.list_stats.p);
+    if (jrows) c.jcl.ensure((size_t)ncl * cap + 1);
+    else c.list.ensure((size_t)ncl * cap + 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)3, c.list_stats.p);
     LJG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cluster_lists_tower, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow, g,
-                       c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, c.list_stats.p);
+    if (jrows)
+      hipLaunchKernelGGL(k_cluster_lists_tower<true>, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow, g,
+                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr, c.jcl.p);
+    else
+      hipLaunchKernelGGL(k_cluster_lists_tower<false>, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow,
+                         g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     LJG_HIP(hipGetLastError());
+    count_stats(c, c.list_count.p, ncl, c.list_stats.p);
     unsigned long long st[2] = {0, 0};
     LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
     LJG_HIP(hipStreamSynchronize(c.stream));
@@ -395,14 +503,57 @@ void vcl_build(Ctx &c) {
     if (maxc <= cap) {
       // keep the row capacity for the next build, with headroom for fluctuations
       const int32_t want = maxc + maxc / 8 + 8;
-      c.list_cap = want > cap ? want : (cap > 2 * want ? want : cap);
+      int32_t &keep = jrows ? c.jccap : c.list_cap;
+      keep = want > cap ? want : (cap > 2 * want ? want : cap);
       break;
     }
     cap = maxc + maxc / 8 + 8;
   }
   c.list_row = cap;
+  if (jrows) {
+    // pass 2: expand to j particles on the fixed-point positions of every slot (padding: type -1)
+    verlet_geometry(c);
+    verlet_fixed_slots(c, new_slots);
+    const int32_t ccap = cap;
+    c.jcount.ensure(ncl + 1);
+    int32_t jcap = c.jcap > 0 ? c.jcap : 160;
+    for (;;) {
+      c.jlist.ensure((size_t)ncl * jcap + 1);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)3, c.list_stats.p);
+      LJG_HIP(hipGetLastError());
+      const int nb = blocks_for(ncl, kExpandThreads / kWave);
+      const double ilen2 = c.prm.cutoff + c.prm.skin;
+#define LJG_EXPAND(MV)                                                                                          \
+  hipLaunchKernelGGL(k_jrows_expand<MV>, dim3(nb), dim3(kExpandThreads), 0, c.stream, ncl, c.posfx.p, c.c0.p, ccap, \
+                     c.list_count.p, c.jcl.p, c.fg, ilen2, jcap, c.jcount.p, c.jlist.p, c.list_stats.p)
+      switch (M) {
+        case 1: LJG_EXPAND(1); break;
+        case 2: LJG_EXPAND(2); break;
+        case 4: LJG_EXPAND(4); break;
+        case 8: LJG_EXPAND(8); break;
+        default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", -2};
+      }
+#undef LJG_EXPAND
+      LJG_HIP(hipGetLastError());
+      count_stats(c, c.jcount.p, ncl, c.list_stats.p);
+      unsigned long long st[3] = {0, 0, 0};
+      LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
+      LJG_HIP(hipStreamSynchronize(c.stream));
+      const int32_t maxc = (int32_t)st[1];
+      if (maxc <= jcap) {
+        c.nentries = (int64_t)st[0];
+        c.jrows_frame_ok = st[2] == 0;
+        const int32_t want = maxc + maxc / 8 + 8;
+        c.jcap = want > jcap ? want : (jcap > 2 * want ? want : jcap);
+        break;
+      }
+      jcap = maxc + maxc / 8 + 8;
+    }
+    c.list_row = jcap;
+  }
+  c.jrows_valid = jrows;
   c.layout = Layout::Clusters;
-  vcl_pos32(c);
+  if (!jrows) vcl_pos32(c);
 }
 
 }  // namespace ljg

@@ -25,21 +25,39 @@ __device__ inline uint32_t to_fixed(double x, double mn, double scale_inv) {
   return (uint32_t)(uint64_t)__double2ll_rn((x - mn) * scale_inv);
 }
 
+// Padding slots (orig < 0) get type -1.
 __global__ void k_to_fixed(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-                           const double *__restrict__ z, const int32_t *__restrict__ type, FixGeom f,
-                           int4 *__restrict__ posfx) {
+                           const double *__restrict__ z, const int32_t *__restrict__ type,
+                           const int32_t *__restrict__ orig, FixGeom f, int4 *__restrict__ posfx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   posfx[i] = make_int4((int32_t)to_fixed(x[i], f.min[0], f.inv_scale[0]), (int32_t)to_fixed(y[i], f.min[1], f.inv_scale[1]),
-                       (int32_t)to_fixed(z[i], f.min[2], f.inv_scale[2]), type[i]);
+                       (int32_t)to_fixed(z[i], f.min[2], f.inv_scale[2]), orig[i] < 0 ? -1 : type[i]);
 }
 
-void verlet_fixed(Ctx &c) {
+void verlet_fixed_slots(Ctx &c, int64_t nslots) {
   State &s = c.st[c.cur];
-  c.posfx.ensure(c.n);
-  hipLaunchKernelGGL(k_to_fixed, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
-                     s.type.p, c.fg, c.posfx.p);
+  c.posfx.ensure(nslots);
+  if (nslots <= 0) return;
+  hipLaunchKernelGGL(k_to_fixed, dim3(blocks_for(nslots, 256)), dim3(256), 0, c.stream, nslots, s.x.p, s.y.p, s.z.p,
+                     s.type.p, s.orig.p, c.fg, c.posfx.p);
   LJG_HIP(hipGetLastError());
+}
+
+void verlet_fixed(Ctx &c) { verlet_fixed_slots(c, c.n); }
+
+void verlet_geometry(Ctx &c) {
+  FixGeom &f = c.fg;
+  bool cubic = true;
+  for (int d = 0; d < 3; ++d) {
+    const double L = c.prm.box_max[d] - c.prm.box_min[d];
+    f.min[d] = c.prm.box_min[d];
+    f.scale[d] = L / 4294967296.0;
+    f.inv_scale[d] = 4294967296.0 / L;
+    f.scale32[d] = (float)f.scale[d];
+    cubic = cubic && L == c.prm.box_max[0] - c.prm.box_min[0];
+  }
+  f.cubic = cubic;
 }
 
 // Neighbour lists within ilen = rc + skin from the 27 cells around each particle's cell (cell order
@@ -47,7 +65,7 @@ void verlet_fixed(Ctx &c) {
 __global__ __launch_bounds__(kVThreads) void k_verlet_build(
     int64_t n, const int4 *__restrict__ posfx, const int32_t *__restrict__ cell_of,
     const int64_t *__restrict__ cell_start, CellGeom g, FixGeom f, double ilen2, int32_t cap,
-    int32_t *__restrict__ count, int32_t *__restrict__ list, unsigned long long *__restrict__ stats) {
+    int32_t *__restrict__ count, int32_t *__restrict__ list) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t w = i / kWave;
@@ -95,14 +113,6 @@ __global__ __launch_bounds__(kVThreads) void k_verlet_build(
   const int32_t fill_to = wmax < cap ? wmax : cap;
   const int32_t self = active ? (int32_t)i : 0;
   for (int32_t k = cnt; k < fill_to; ++k) list[(w * cap + k) * kWave + lane] = self;
-  if (lane == 0 && active) {
-    atomicMax(&stats[1], (unsigned long long)wmax);
-  }
-  // total entries: one atomic per wave
-  int32_t tot = cnt;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, kWave);
-  if (lane == 0) atomicAdd(&stats[0], (unsigned long long)tot);
 }
 
 __global__ void k_zero_u64v(int64_t n, unsigned long long *__restrict__ p) {
@@ -111,18 +121,8 @@ __global__ void k_zero_u64v(int64_t n, unsigned long long *__restrict__ p) {
 }
 
 void verlet_build(Ctx &c) {
-  // fixed-point geometry
-  FixGeom &f = c.fg;
-  bool cubic = true;
-  for (int d = 0; d < 3; ++d) {
-    const double L = c.prm.box_max[d] - c.prm.box_min[d];
-    f.min[d] = c.prm.box_min[d];
-    f.scale[d] = L / 4294967296.0;
-    f.inv_scale[d] = 4294967296.0 / L;
-    f.scale32[d] = (float)f.scale[d];
-    cubic = cubic && L == c.prm.box_max[0] - c.prm.box_min[0];
-  }
-  f.cubic = cubic;
+  verlet_geometry(c);
+  const FixGeom &f = c.fg;
   verlet_fixed(c);
   const int64_t n = c.n;
   const int64_t nwaves = (n + kWave - 1) / kWave;
@@ -135,8 +135,9 @@ void verlet_build(Ctx &c) {
     hipLaunchKernelGGL(k_zero_u64v, dim3(1), dim3(64), 0, c.stream, (int64_t)2, c.list_stats.p);
     LJG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_verlet_build, dim3(blocks_for(n, kVThreads)), dim3(kVThreads), 0, c.stream, n, c.posfx.p,
-                       c.cell_of.p, c.cell_start.p, c.cg, f, ilen * ilen, cap, c.vcount.p, c.vlist.p, c.list_stats.p);
+                       c.cell_of.p, c.cell_start.p, c.cg, f, ilen * ilen, cap, c.vcount.p, c.vlist.p);
     LJG_HIP(hipGetLastError());
+    count_stats(c, c.vcount.p, n, c.list_stats.p);
     unsigned long long st[2] = {0, 0};
     LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
     LJG_HIP(hipStreamSynchronize(c.stream));
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
     pe[blockIdx.x] = se[0];
     pv[blockIdx.x] = sv[0];
     ph[blockIdx.x] = sh[0];
-    if (acc) atomicAdd(acc, (unsigned long long)sh[0]);
+    if (acc) acc_add(acc, (unsigned long long)sh[0]);
   }
 }
 

@@ -23,11 +23,13 @@ KERNEL_LC_FP64 = 0
 KERNEL_CLUSTER_FP64 = 1
 KERNEL_CLUSTER_MIXED = 2
 KERNEL_VERLET_MIXED = 3
+KERNEL_CLUSTER_JROWS_MIXED = 4
 KERNEL_NAMES = {
     KERNEL_LC_FP64: "lc_fp64",
     KERNEL_CLUSTER_FP64: "cluster_fp64",
     KERNEL_CLUSTER_MIXED: "cluster_fp32pair_fp64acc",
     KERNEL_VERLET_MIXED: "verlet_fp32pair_fp64acc",
+    KERNEL_CLUSTER_JROWS_MIXED: "cluster_jrows_fp32pair_fp64acc",
 }
 
 _d = ctypes.c_double

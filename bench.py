@@ -79,8 +79,10 @@ def run(sysd, kernel, cluster_size, steps, warmup, timing=True, tile_groups=0):
 # Candidate FP32-pair / FP64-accumulate layouts tried by the tuning pass: (kernel, cluster size,
 # i-clusters per wavefront).
 CANDIDATES = (
-    ("cluster", 4, 8), ("cluster", 4, 4), ("cluster", 8, 2), ("cluster", 2, 8), ("verlet", 1, 0),
+    ("cluster", 4, 8), ("cluster", 8, 2), ("jrows", 4, 8), ("jrows", 4, 16), ("jrows", 8, 4), ("jrows", 8, 8),
+    ("verlet", 1, 0),
 )
+KERNEL_OF = {"cluster": "KERNEL_CLUSTER_MIXED", "jrows": "KERNEL_CLUSTER_JROWS_MIXED", "verlet": "KERNEL_VERLET_MIXED"}
 
 
 def tune(sysd, steps=30):
@@ -88,7 +90,7 @@ def tune(sysd, steps=30):
 
     res = {}
     for kind, m, g in CANDIDATES:
-        kernel = ap.KERNEL_CLUSTER_MIXED if kind == "cluster" else ap.KERNEL_VERLET_MIXED
+        kernel = getattr(ap, KERNEL_OF[kind])
         r = run(sysd, kernel, max(m, 1), steps, 5, timing=False, tile_groups=g)
         res[f"{kind}_M{m}_G{g}"] = (r["pairs_per_s"], kernel, max(m, 1), g)
     best = max(res, key=lambda k: res[k][0])
@@ -127,7 +129,7 @@ def main():
     if args.layout:
         kind, m, g = args.layout.split("_")
         best, table = args.layout, {}
-        kernel = ap.KERNEL_CLUSTER_MIXED if kind == "cluster" else ap.KERNEL_VERLET_MIXED
+        kernel = getattr(ap, KERNEL_OF[kind])
         M, G = int(m[1:]), int(g[1:])
     else:
         best, table = tune(s)

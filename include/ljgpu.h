@@ -39,6 +39,10 @@ typedef int ljg_status;
 #define LJG_KERNEL_CLUSTER_MIXED 2 /* cluster-pair tiles, FP32 pair term, FP64 per-particle sums */
 #define LJG_KERNEL_VERLET_MIXED 3  /* Verlet lists on linked cells (cluster size 1), fixed-point positions,
                                       FP32 pair term, FP64 per-particle sums; boxes >= 3 (rc + skin) */
+#define LJG_KERNEL_CLUSTER_JROWS_MIXED 4 /* Verlet-cluster i-clusters against j-particle rows (the union of
+                                      the members' neighbour lists), fixed-point positions, FP32 pair term,
+                                      FP64 per-particle sums; boxes > 2 (rc + skin) */
+#define LJG_KERNEL_LAST 4
 
 typedef struct {
   double box_min[3];
@@ -125,7 +129,9 @@ ljg_status ljg_cluster_sizes(ljg_ctx *ctx, int64_t *num_slots, int64_t *num_clus
                              int64_t towers_per_dim[2], double side[2], int64_t *towers_per_il);
 /* slot_particle[num_slots] (upload index, -1 for padding), cluster_tower[num_clusters],
  * list_offsets[num_clusters+1], list_j[num_entries], list_shift[num_entries], bbox[6*num_clusters]
- * (xlo,ylo,zlo,xhi,yhi,zhi).  Any pointer may be NULL. */
+ * (xlo,ylo,zlo,xhi,yhi,zhi).  Any pointer may be NULL.  After a build for
+ * LJG_KERNEL_CLUSTER_JROWS_MIXED the entries are j particles: list_j holds their upload index and
+ * list_shift is -1 (minimum image). */
 ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
                             int32_t *list_j, int32_t *list_shift, double *bbox);
 

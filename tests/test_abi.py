@@ -76,3 +76,12 @@ def test_params_struct_matches_header():
     fields = re.findall(r"(double|int32_t)\s+(\w+)(\[\d+\])?;", body)
     assert [f[1] for f in fields] == [f[0] for f in engine.Params._fields_]
     assert ctypes.sizeof(engine.Params) == 8 * 8 + 12 * 4  # 8 doubles, 12 int32
+
+
+def test_kernel_ids_match_header():
+    text = open(os.path.join(ROOT, "include", "ljgpu.h")).read()
+    ids = dict((k, int(v)) for k, v in re.findall(r"#define LJG_KERNEL_(\w+) (\d+)", text))
+    assert ids.pop("LAST") == max(ids.values())
+    for name, val in ids.items():
+        assert getattr(engine, "KERNEL_" + name) == val, name
+    assert set(engine.KERNEL_NAMES) == set(ids.values())

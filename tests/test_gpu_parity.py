@@ -81,8 +81,9 @@ def test_cluster_lists_bit_exact(oracle, M):
         assert np.array_equal(g[k], r[k]), k
 
 
-MIXED = (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED)
-ALL_KERNELS = [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED]
+MIXED = (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED)
+ALL_KERNELS = [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED,
+               ap.KERNEL_CLUSTER_JROWS_MIXED]
 
 
 def test_verlet_lists_exact(oracle):
@@ -98,6 +99,36 @@ def test_verlet_lists_exact(oracle):
     assert np.array_equal(off, roff)
     for p in range(s["n"]):
         assert np.array_equal(np.sort(nb[off[p]:off[p + 1]]), rnb[roff[p]:roff[p + 1]]), p
+
+
+@pytest.mark.parametrize("M,n_side", [(1, 10), (2, 10), (4, 10), (8, 10), (4, 6), (8, 6)])
+def test_cluster_jrows_exact(oracle, M, n_side):
+    # Rows of the j-particle cluster kernel = union of the members' fixed-point Verlet lists (members
+    # included), each j once.  n_side 6 (L = 6.6, just above 2 ilen): the tower window wraps onto itself
+    # and clusters pass the box test under several images, which the build must not expand twice.
+    announce(f"cluster j-rows exact M={M}: {n_side}^3 lattice rho 0.75")
+    s = lattice_box(n_side, 0.75, seed=13)
+    L = s["L"]
+    ids = np.arange(s["n"], dtype=np.int64)
+    with make_engine(L, M=M) as e:
+        e.upload(s["x"], s["y"], s["z"], id=ids)
+        e.force(ap.KERNEL_CLUSTER_JROWS_MIXED, globals=False)
+        g = e.clusters()
+    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), (L, L, L), RC + SKIN, M)
+    for k in ("slot_particle", "cluster_tower", "bbox"):
+        assert np.array_equal(g[k], r[k]), k
+    roff, rnb = oracle.verlet_pairs_fixed(wx, wy, wz, (0, 0, 0), (L, L, L), RC + SKIN)
+    sp, off, lj = g["slot_particle"], g["list_offsets"], g["list_j"]
+    assert np.all(g["list_shift"] == -1)
+    for ci in range(len(off) - 1):
+        members = [int(p) for p in sp[ci * M:(ci + 1) * M] if p >= 0]
+        expect = set(members)
+        for p in members:
+            expect.update(int(q) for q in rnb[roff[p]:roff[p + 1]])
+        row = lj[off[ci]:off[ci + 1]]
+        assert len(row) == len(set(row.tolist())), ci
+        assert set(row.tolist()) == expect, ci
 
 
 @pytest.mark.parametrize("kernel", ALL_KERNELS)
@@ -126,13 +157,18 @@ def test_forces_vs_direct_sum(oracle, kernel, M):
         assert g.pair_count == ref["pairs"]
 
 
-def test_verlet_rejects_small_box():
-    announce("verlet kernel on a box shorter than 3 interaction lengths: rejected")
+@pytest.mark.parametrize("kernel", [ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED])
+def test_fixed_point_kernels_reject_small_box(kernel):
+    # L = 5.29: below 3 ilen (Verlet lists) and below 2 ilen (j-particle rows)
+    announce(f"kernel={ap.KERNEL_NAMES[kernel]} on a box too short for minimum-image lists: rejected")
     s = lattice_box(4, 0.43, seed=5, jitter=0.1)
     with make_engine(s["L"]) as e:
         e.upload(s["x"], s["y"], s["z"])
         with pytest.raises(ap.EngineError) as ei:
-            e.force(ap.KERNEL_VERLET_MIXED)
+            e.force(kernel)
+        assert ei.value.status == 5
+        with pytest.raises(ap.EngineError) as ei:
+            e.step(2, 0.002, kernel)
         assert ei.value.status == 5
 
 
@@ -244,7 +280,32 @@ def test_md_trajectory_lc_fp64(oracle):
     assert np.max(np.abs(g["vx"] - vx)) < 1e-8
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED])
+@pytest.mark.parametrize("n_side", [8, 16])
+@pytest.mark.parametrize("kernel", [ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED])
+def test_md_trajectory_mixed_kernels(kernel, n_side):
+    # 30 velocity-Verlet steps with rebuilds every 10 steps: the FP32-pair kernels (positions refreshed by
+    # the drift in their own formats) follow the FP64 trajectory to within the FP32 force error.  The j-row
+    # kernel takes its exact per-pair variant in the 8^3 box and its first-member-frame variant in 16^3.
+    announce(f"md trajectory kernel={ap.KERNEL_NAMES[kernel]} vs LC FP64: {n_side}^3 lattice, 30 steps")
+    s = lattice_box(n_side, 0.75, seed=4, temperature=0.9)
+    L = s["L"]
+    out = {}
+    for k in (ap.KERNEL_LC_FP64, kernel):
+        with make_engine(L, M=4, rebuild_frequency=10) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            rs = e.step(30, 0.002, k)
+            out[k] = (e.download(), rs)
+    a, b = out[ap.KERNEL_LC_FP64][0], out[kernel][0]
+    d = np.stack([b["x"] - a["x"], b["y"] - a["y"], b["z"] - a["z"]])
+    d -= L * np.round(d / L)
+    # close encounters carry forces ~1e3, whose FP32 error moves a particle by up to a few 1e-6
+    assert np.max(np.abs(d)) < 2e-5
+    assert np.max(np.abs(b["vx"] - a["vx"])) < 1e-3
+    assert out[kernel][1].rebuilds == out[ap.KERNEL_LC_FP64][1].rebuilds
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED,
+                                    ap.KERNEL_CLUSTER_JROWS_MIXED])
 def test_deterministic(kernel):
     announce(f"determinism kernel={ap.KERNEL_NAMES[kernel]}: two identical runs")
     s = lattice_box(8, 0.75, seed=12)
@@ -265,7 +326,8 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     s = lattice_box(32, 0.75, seed=2, temperature=0.9)
     L = s["L"]
     res = {}
-    for kernel in (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64, ap.KERNEL_VERLET_MIXED):
+    for kernel in (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64, ap.KERNEL_VERLET_MIXED,
+                   ap.KERNEL_CLUSTER_JROWS_MIXED):
         with make_engine(L, M=4) as e:
             e.upload(s["x"], s["y"], s["z"])
             g = e.force(kernel, globals=True)
@@ -276,8 +338,11 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     assert rms_rel(res[ap.KERNEL_LC_FP64][0], ref) <= 1e-12
     err = rms_rel(f, ref)
     errv = rms_rel(res[ap.KERNEL_VERLET_MIXED][0], ref)
-    print(f"force RMS relative error vs FP64: cluster FP32-pair {err:.3e}, verlet FP32-pair {errv:.3e}")
+    errj = rms_rel(res[ap.KERNEL_CLUSTER_JROWS_MIXED][0], ref)
+    print(f"force RMS relative error vs FP64: cluster FP32-pair {err:.3e}, verlet FP32-pair {errv:.3e}, "
+          f"cluster j-rows FP32-pair {errj:.3e}")
     assert err <= 1e-5
     assert errv <= 1e-5
+    assert errj <= 1e-5
     assert res[ap.KERNEL_LC_FP64][1].pair_count == ref["pairs"]
     assert res[ap.KERNEL_CLUSTER_FP64][1].pair_count == ref["pairs"]
