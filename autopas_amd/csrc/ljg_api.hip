@@ -156,7 +156,11 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
     LJG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     LJG_HIP(hipEventCreate(&c.ev0));
     LJG_HIP(hipEventCreate(&c.ev1));
-    LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+    LJG_HIP(hipEventCreateWithFlags(&c.ev_drift, hipEventDisableTiming));
+    // coherent, device-mapped: the drift kernel writes the displacement stamp here directly
+    LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c.host_flag, 0, 4 * sizeof(int32_t));
+    LJG_HIP(hipHostGetDevicePointer((void **)&c.host_flag_dev, c.host_flag, 0));
     c.flag.ensure(4);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.flag.p);
     LJG_HIP(hipGetLastError());
@@ -188,6 +192,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.red_out.release(); c.red_hits.release(); c.hits_slots.release(); c.flag.release(); c.scan_tmp.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
+  if (c.ev_drift) (void)hipEventDestroy(c.ev_drift);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
@@ -389,23 +394,44 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     // since the last rebuild stamps flag[0] with this step's number
     const int32_t stamp = ++c.step_stamp;
     md_kick_drift(c, s > 0, true, dt, stamp);
-    bool rebuild = c.steps_since_rebuild >= c.prm.rebuild_frequency;
-    if (!rebuild && c.prm.dynamic_rebuild) {
-      LJG_HIP(hipMemcpyAsync(c.host_flag, c.flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-      LJG_HIP(hipStreamSynchronize(c.stream));
-      rebuild = c.host_flag[0] == stamp;
-    }
-    if (rebuild) {
+    auto rebuild_now = [&]() {
       if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
       c.carry_force = false;  // the force of this step is computed right after the rebuild
       rebuild_for(c, kernel);
       c.carry_force = true;
       if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
       ++rs.rebuilds;
+    };
+    auto force_now = [&]() {
+      if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+      run_force(c, kernel, false);
+      if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+    };
+    if (c.steps_since_rebuild >= c.prm.rebuild_frequency) {
+      rebuild_now();
+      force_now();
+    } else if (c.prm.dynamic_rebuild) {
+      // LogicHandler.h:1000-1015 rebuilds when a particle moved skin/2.  The drift wrote its stamp to
+      // pinned memory; the force is queued before the host reads it, and exits at once on the device if
+      // the stamp is set, so the GPU never waits for the host's decision.
+      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
+      c.skip = SkipIf{c.flag.p, stamp};
+      force_now();
+      c.skip = SkipIf{nullptr, 0};
+      LJG_HIP(hipEventSynchronize(c.ev_drift));
+      if (static_cast<volatile int32_t *>(c.host_flag)[0] == stamp) {
+        if (timing) {  // the skipped launch is not a force evaluation
+          for (int k = 0; k < 2; ++k) {
+            LJG_HIP(hipEventDestroy(fev.back()));
+            fev.pop_back();
+          }
+        }
+        rebuild_now();
+        force_now();
+      }
+    } else {
+      force_now();
     }
-    if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
-    run_force(c, kernel, false);
-    if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
     c.steps_since_rebuild++;
   }
   c.hits_acc = nullptr;

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_lc_fp64(
     const int32_t *__restrict__ type, const int32_t *__restrict__ cell_of, const int64_t *__restrict__ cell_start,
     CellGeom g, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
-    int32_t globals, unsigned long long *__restrict__ acc) {
+    int32_t globals, unsigned long long *__restrict__ acc, SkipIf sk) {
+  if (skip_launch(sk)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ax = 0., ay = 0., az = 0., ae = 0., av = 0.;
   int64_t hits = 0;
@@ -146,8 +147,10 @@ __device__ inline void group_reduce_scatter(double (&v)[NV], int l) {
     const bool upper = (l & step) != 0;
 #pragma unroll
     for (int q = 0; q < half; ++q) {
-      const double send = upper ? v[q] : v[q + half];
-      const double keep = upper ? v[q + half] : v[q];
+      // select values, not addresses: a select of two array slots keeps the array in scratch memory
+      const double lo = v[q], hi = v[q + half];
+      const double send = upper ? lo : hi;
+      const double keep = upper ? hi : lo;
       v[q] = keep + __shfl_xor(send, step, kWave);
     }
   }
@@ -175,7 +178,9 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
-    double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+    double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
+    SkipIf sk) {
+  if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
@@ -324,9 +329,10 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, float rc2, float eps24_1,
     float sig2_1, float shift6_1, int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
-    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk) {
+  if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
-  constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
+  constexpr int NVAL = 3 * M + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
   __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
   if constexpr (MIX) {
@@ -339,128 +345,135 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
   const int grp = lane / LG, l = lane % LG;
   int ic = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
   if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
+  const bool live = ic < nclusters;
+  const int64_t icl = live ? ic : nclusters - 1;  // dead groups (last wave only) read a real cluster, store nothing
   const uint32_t rc2_bits = __float_as_uint(rc2);
-  double e_out = 0., v_out = 0.;
-  int64_t h_out = 0;
-  if (ic < nclusters) {
-    const int nreal = (int)c0[ic].w;
-    const int4 ref = posfx[(int64_t)ic * M];  // the first member is always real
-    float xi[M], yi[M], zi[M];   // FAST: FP32 offsets from the first member (fixed-point units)
-    uint32_t ux[M], uy[M], uz[M];  // exact variant: fixed-point positions
-    int ti[M];
+  const int nreal = (int)c0[icl].w;
+  const int4 ref = posfx[icl * M];  // the first member is always real
+  float xi[M], yi[M], zi[M];     // FAST: FP32 offsets from the first member (fixed-point units)
+  uint32_t ux[M], uy[M], uz[M];  // exact variant: fixed-point positions
+  int ti[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const int4 p = posfx[icl * M + q];
+    const bool pad = q >= nreal;
+    ux[q] = (uint32_t)p.x;
+    uy[q] = (uint32_t)p.y;
+    uz[q] = (uint32_t)p.z;
+    xi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.x - (uint32_t)ref.x);
+    yi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.y - (uint32_t)ref.y);
+    zi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.z - (uint32_t)ref.z);
+    ti[q] = pad ? 0 : p.w;
+  }
+  float ax[M], ay[M], az[M], ae[M], av[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
+  // the pair count is kept per wavefront on the scalar unit: popcount of each ballot of the cutoff mask
+  uint32_t hits = 0;
+  const int32_t *rowp = jlist + icl * row;
+  const int nj = live ? lcount[icl] : 0;  // >= nreal for live groups: every row holds the cluster's own members
+  // the loop runs to the wave's longest row so that it stays uniform (the ballot count needs that)
+  int njmax = nj;
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int off = 32; off >= LG; off >>= 1) {
+      const int o = __shfl_xor(njmax, off, kWave);
+      njmax = o > njmax ? o : njmax;
+    }
+  }
+  const int jlast = nj > 0 ? nj - 1 : 0;
+  int4 p = make_int4(0, 0, 0, 0);
+  if (nj > 0) p = posfx[rowp[l < nj ? l : jlast]];
+  for (int base = 0; base < njmax; base += LG) {
+    const bool valid = base + l < nj;
+    const int4 pj = p;
+    {
+      const int kn = base + LG + l;
+      if (nj > 0) p = posfx[rowp[kn < nj ? kn : jlast]];
+    }
+    const int tj = pj.w < 0 ? 0 : pj.w;
+    float xj = 0.f, yj = 0.f, zj = 0.f;
+    if constexpr (FAST) {
+      xj = valid ? (float)(int32_t)((uint32_t)pj.x - (uint32_t)ref.x) : kPadPos32;
+      yj = (float)(int32_t)((uint32_t)pj.y - (uint32_t)ref.y);
+      zj = (float)(int32_t)((uint32_t)pj.z - (uint32_t)ref.z);
+    }
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int4 p = posfx[(int64_t)ic * M + q];
-      const bool pad = q >= nreal;
-      ux[q] = (uint32_t)p.x;
-      uy[q] = (uint32_t)p.y;
-      uz[q] = (uint32_t)p.z;
-      xi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.x - (uint32_t)ref.x);
-      yi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.y - (uint32_t)ref.y);
-      zi[q] = pad ? -kPadPos32 : (float)(int32_t)((uint32_t)p.z - (uint32_t)ref.z);
-      ti[q] = pad ? 0 : p.w;
-    }
-    float ax[M], ay[M], az[M], ae[M], av[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
-    int hits = 0;
-    const int32_t *rowp = jlist + (int64_t)ic * row;
-    const int nj = lcount[ic];  // >= nreal: every row holds the cluster's own members
-    int4 p = make_int4(0, 0, 0, 0);
-    if (nj > 0) p = posfx[rowp[l < nj ? l : nj - 1]];
-    for (int base = 0; base < nj; base += LG) {
-      const bool valid = base + l < nj;
-      const int4 pj = p;
-      {
-        const int kn = base + LG + l;
-        p = posfx[rowp[kn < nj ? kn : nj - 1]];
-      }
-      const int tj = pj.w;
-      float xj = 0.f, yj = 0.f, zj = 0.f;
+      float dx, dy, dz;
       if constexpr (FAST) {
-        xj = valid ? (float)(int32_t)((uint32_t)pj.x - (uint32_t)ref.x) : kPadPos32;
-        yj = (float)(int32_t)((uint32_t)pj.y - (uint32_t)ref.y);
-        zj = (float)(int32_t)((uint32_t)pj.z - (uint32_t)ref.z);
+        dx = xi[q] - xj; dy = yi[q] - yj; dz = zi[q] - zj;
+      } else {
+        dx = (float)(int32_t)(ux[q] - (uint32_t)pj.x) * f.scale32[0];
+        dy = (float)(int32_t)(uy[q] - (uint32_t)pj.y) * f.scale32[1];
+        dz = (float)(int32_t)(uz[q] - (uint32_t)pj.z) * f.scale32[2];
       }
-#pragma unroll
-      for (int q = 0; q < M; ++q) {
-        float dx, dy, dz;
-        if constexpr (FAST) {
-          dx = xi[q] - xj; dy = yi[q] - yj; dz = zi[q] - zj;
-        } else {
-          dx = (float)(int32_t)(ux[q] - (uint32_t)pj.x) * f.scale32[0];
-          dy = (float)(int32_t)(uy[q] - (uint32_t)pj.y) * f.scale32[1];
-          dz = (float)(int32_t)(uz[q] - (uint32_t)pj.z) * f.scale32[2];
-        }
-        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
-        if constexpr (!FAST) m = m && valid && q < nreal;
-        float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
-        if constexpr (MIX) {
-          const int t = 3 * (ti[q] * ntypes + tj);
-          eps24 = stab[t]; sig2 = stab[t + 1]; shift6 = stab[t + 2];
-        }
-        const float inv = __builtin_amdgcn_rcpf(r2);
-        const float lj2 = sig2 * inv;
-        const float lj6 = lj2 * lj2 * lj2;
-        const float fv = (lj6 * inv) * __builtin_fmaf(eps24 + eps24, lj6, -eps24);
-        const float fac = m ? fv : 0.f;
-        ax[q] = __builtin_fmaf(dx, fac, ax[q]);
-        ay[q] = __builtin_fmaf(dy, fac, ay[q]);
-        az[q] = __builtin_fmaf(dz, fac, az[q]);
-        hits += m ? 1 : 0;
-        if constexpr (GLOB) {
-          ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
-          av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);
-        }
+      const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+      if constexpr (!FAST) m = m && valid && q < nreal;
+      float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
+      if constexpr (MIX) {
+        const int t = 3 * (ti[q] * ntypes + tj);
+        eps24 = stab[t]; sig2 = stab[t + 1]; shift6 = stab[t + 2];
       }
-    }
-    double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = 0.;
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      v[q] = (double)ax[q];
-      v[M + q] = (double)ay[q];
-      v[2 * M + q] = (double)az[q];
-    }
-    v[3 * M] = (double)hits;
-    if constexpr (GLOB) {
-      double se = 0., sv = 0.;
-#pragma unroll
-      for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
-      v[3 * M + 1] = se;
-      v[3 * M + 2] = sv;
-    }
-    group_reduce_scatter<NV, LG>(v, l);
-    constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
-    constexpr int R = NV >> K;
-    const int q0 = group_value_slot<NV, LG>(l) * R;
-    // FAST: the sums are in fixed-point units; F = sum(d_u * fac_u) * (2^32 / L)
-    const double fs = FAST ? f.inv_scale[0] : 1.0;
-    if ((l & ((1 << (KL - K)) - 1)) == 0) {
-      const int64_t s0 = (int64_t)ic * M;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int q = q0 + r;
-        const double val = v[r];
-        if (q < M) {
-          if (q < nreal) fx[s0 + q] = val * fs;
-        } else if (q < 2 * M) {
-          if (q - M < nreal) fy[s0 + q - M] = val * fs;
-        } else if (q < 3 * M) {
-          if (q - 2 * M < nreal) fz[s0 + q - 2 * M] = val * fs;
-        } else if (q == 3 * M) {
-          h_out = (int64_t)val;
-        } else if (GLOB && q == 3 * M + 1) {
-          e_out = val;
-        } else if (GLOB && q == 3 * M + 2) {
-          v_out = val;
-        }
+      const float inv = __builtin_amdgcn_rcpf(r2);
+      const float lj2 = sig2 * inv;
+      const float lj6 = lj2 * lj2 * lj2;
+      const float fv = (lj6 * inv) * __builtin_fmaf(eps24 + eps24, lj6, -eps24);
+      const float fac = m ? fv : 0.f;
+      ax[q] = __builtin_fmaf(dx, fac, ax[q]);
+      ay[q] = __builtin_fmaf(dy, fac, ay[q]);
+      az[q] = __builtin_fmaf(dz, fac, az[q]);
+      hits += (uint32_t)__popcll(__ballot(m));
+      if constexpr (GLOB) {
+        ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
+        av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);
       }
     }
   }
-  block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.;
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    v[q] = (double)ax[q];
+    v[M + q] = (double)ay[q];
+    v[2 * M + q] = (double)az[q];
+  }
+  if constexpr (GLOB) {
+    double se = 0., sv = 0.;
+#pragma unroll
+    for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
+    v[3 * M] = se;
+    v[3 * M + 1] = sv;
+  }
+  group_reduce_scatter<NV, LG>(v, l);
+  constexpr int KL = ilog2(LG), KV = ilog2(NV), K = KL < KV ? KL : KV;
+  constexpr int R = NV >> K;
+  const int q0 = group_value_slot<NV, LG>(l) * R;
+  // FAST: the sums are in fixed-point units; F = sum(d_u * fac_u) * (2^32 / L)
+  const double fs = FAST ? f.inv_scale[0] : 1.0;
+  double e_out = 0., v_out = 0.;
+  if (live && (l & ((1 << (KL - K)) - 1)) == 0) {
+    const int64_t s0 = icl * M;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = q0 + r;
+      const double val = v[r];
+      if (q < M) {
+        if (q < nreal) fx[s0 + q] = val * fs;
+      } else if (q < 2 * M) {
+        if (q - M < nreal) fy[s0 + q - M] = val * fs;
+      } else if (q < 3 * M) {
+        if (q - 2 * M < nreal) fz[s0 + q - 2 * M] = val * fs;
+      } else if (GLOB && q == 3 * M) {
+        e_out = val;
+      } else if (GLOB && q == 3 * M + 1) {
+        v_out = val;
+      }
+    }
+  }
+  block_partials(e_out, v_out, lane == 0 ? (int64_t)hits : 0, pe, pv, ph, GLOB, acc);
 }
 
 // FP64 cluster kernel: same tiles (one i-cluster per wave), absolute FP64 positions with the periodic
@@ -471,7 +484,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
     const int32_t *__restrict__ type, const int32_t *__restrict__ orig, int32_t row,
     const int32_t *__restrict__ lcount, const int4 *__restrict__ list, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
-    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc) {
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk) {
+  if (skip_launch(sk)) return;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
   const int lane = threadIdx.x & (kWave - 1);
@@ -598,11 +612,11 @@ void lc_force(Ctx &c, bool globals) {
   if (mix)
     hipLaunchKernelGGL(k_force_lc_fp64<true>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
                        s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc);
+                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc, c.skip);
   else
     hipLaunchKernelGGL(k_force_lc_fp64<false>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
                        s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc);
+                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc, c.skip);
   LJG_HIP(hipGetLastError());
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }
@@ -620,7 +634,7 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,          \
                      c.nclusters, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,           \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
-                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc)
+                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
   if (mix) {
     if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
   } else {
@@ -640,7 +654,7 @@ static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const d
 #define LJG_LAUNCH(MIXV, GLOBV)                                                                                   \
   hipLaunchKernelGGL((k_force_cluster_fp64<M, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream, c.nclusters, \
                      s.x.p, s.y.p, s.z.p, s.type.p, s.orig.p, c.list_row, c.list_count.p, c.list.p, a, tab, s.fx.p,\
-                     s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc)
+                     s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
   if (mix) {
     if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
   } else {
@@ -667,7 +681,7 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
                      c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, rc2,                 \
                      (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,  \
-                     c.part_v.p, c.part_hits.p, c.hits_acc)
+                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
 #define LJG_LAUNCH2(CUB)                                                              \
   if (mix) {                                                                          \
     if (glob) LJG_LAUNCH(CUB, true, true); else LJG_LAUNCH(CUB, true, false);        \

@@ -108,6 +108,14 @@ struct FixGeom {
   int32_t cubic;
 };
 
+// A force launch that does nothing when *flag == stamp: the step loop launches a step's force before the
+// host has read that step's displacement flag, and a step that needs a rebuild first skips it.
+struct SkipIf {
+  const int32_t *flag;
+  int32_t stamp;
+};
+__device__ inline bool skip_launch(const SkipIf &s) { return s.flag != nullptr && *s.flag == s.stamp; }
+
 // LJ parameters (mixing tables are ntypes x ntypes, row-major).
 struct LJParams {
   int32_t ntypes;
@@ -125,6 +133,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_drift = nullptr;  // recorded after a step's drift (host waits on it, not on the stream)
   bool timing = false;
 
   int64_t n = 0;       // real particles
@@ -189,11 +198,13 @@ struct Ctx {
   DevBuf<int64_t> part_hits;
   int64_t part_items = 0;         // partial-sum slots written by the last force launch
   unsigned long long *hits_acc = nullptr;  // when set, force kernels add their pair counts here (kAccSlots lines)
+  SkipIf skip{nullptr, 0};                 // force launches exit at once when *skip.flag == skip.stamp
   DevBuf<unsigned long long> hits_slots;   // the step loop's pair-count lines
   DevBuf<double> red_out;         // [0]=e [1]=v
   DevBuf<int64_t> red_hits;       // [0]
   DevBuf<int32_t> flag;           // [0] displacement, [1] capacity error
-  int32_t *host_flag = nullptr;   // pinned
+  int32_t *host_flag = nullptr;   // pinned, device-mapped: [0] the drift's displacement stamp
+  int32_t *host_flag_dev = nullptr;  // device address of host_flag
 
   // Scan scratch
   DevBuf<int64_t> scan_tmp;

@@ -133,7 +133,8 @@ struct MdArgs {
 };
 
 __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double4 *__restrict__ c0,
-                             float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ flag) {
+                             float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ flag,
+                             int32_t *__restrict__ hflag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
   if (s.orig[i] < 0) return;
@@ -156,7 +157,10 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
   const double x = s.x[i] + dx, y = s.y[i] + dy, z = s.z[i] + dz;
   s.x[i] = x; s.y[i] = y; s.z[i] = z;
   const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
-  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) flag[0] = a.stamp;
+  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) {
+    flag[0] = a.stamp;
+    hflag[0] = a.stamp;  // pinned host copy, read by the step loop without a device-to-host copy
+  }
   if (a.write_pos32) {
     const double4 r = c0[i / a.M];  // padding slots returned above and keep their sentinel positions
     pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
@@ -186,7 +190,7 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
     a.drift_fac[t] = dt * dt / (2 * m);
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p);
+                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p, c.host_flag_dev);
   LJG_HIP(hipGetLastError());
 }
 

@@ -295,11 +295,11 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
 // length of the exact minimum-image fixed-point separation (the arithmetic of the Verlet-list build; it
 // needs L > 2 ilen).  Survivors are compacted in stream order with a ballot and a prefix popcount, so the
 // row is the union of the members' Verlet lists (members included), each j once.
-// stats[2] is set when, for some member/j pair within ilen, the member's or j's minimum-image offset
-// from the cluster's first member exceeds L/2 - ilen, or the pair separation is not the difference of
-// the two offsets (boxes not much larger than ilen plus the cluster extent).  The margin ilen covers the
-// motion allowed between rebuilds (skin/2 per particle).  The force kernel then uses exact per-pair
-// differences.
+// stats[2] is set unless every member and every row entry lies within L/2 - ilen of the cluster's first
+// member (per dimension, minimum image), the member offsets included: |q| + |j| <= L/2 - ilen makes
+// the separation q - j of the two offsets free of wrap-around, and the margin ilen covers the motion
+// allowed between rebuilds (skin/2 per particle).  The force kernel then works in the first member's
+// frame; otherwise it uses exact per-pair differences.
 constexpr int kExpandThreads = 256;
 
 template <int M>
@@ -311,48 +311,64 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
   const int64_t ic = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kExpandThreads / kWave) + threadIdx.x / kWave));
   if (ic >= nclusters) return;
   const double il2 = ilen * ilen;
-  int64_t flim[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) flim[d] = (int64_t)(2147483648.0 - ilen * f.inv_scale[d]);
   const int nri = (int)c0[ic].w;
   int4 xi[M];
 #pragma unroll
   for (int q = 0; q < M; ++q) xi[q] = posfx[ic * M + q];
+  // frame limits for the row entries: L/2 - ilen minus the largest member offset
+  int64_t jlim[3];
+  bool frame_bad = false;
+  {
+    int64_t qmax[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      if (q < nri) {
+        qmax[0] = max(qmax[0], llabs((int64_t)(int32_t)((uint32_t)xi[q].x - (uint32_t)xi[0].x)));
+        qmax[1] = max(qmax[1], llabs((int64_t)(int32_t)((uint32_t)xi[q].y - (uint32_t)xi[0].y)));
+        qmax[2] = max(qmax[2], llabs((int64_t)(int32_t)((uint32_t)xi[q].z - (uint32_t)xi[0].z)));
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      jlim[d] = (int64_t)(2147483648.0 - ilen * f.inv_scale[d]) - qmax[d];
+      frame_bad = frame_bad || jlim[d] < 0;
+    }
+  }
   const int nflat = ccount[ic] * M;
   const int32_t *crow = jcl + ic * (int64_t)ccap;
   int32_t *jrow = jrows + ic * (int64_t)cap;
   int32_t cnt = 0;
-  bool frame_bad = false;
+  // prefetch the first entry of this lane
+  int32_t jsn = 0;
+  int4 xjn = make_int4(0, 0, 0, -1);
+  if (lane < nflat) {
+    jsn = crow[lane / M] * M + (lane % M);
+    xjn = posfx[jsn];
+  }
   for (int base = 0; base < nflat; base += kWave) {
-    const int k = base + lane;
+    const int32_t js = jsn;
+    const int4 xj = xjn;
+    const bool inrow = base + lane < nflat;
+    {
+      const int kn = base + kWave + lane;
+      if (kn < nflat) {
+        jsn = crow[kn / M] * M + (kn % M);
+        xjn = posfx[jsn];
+      }
+    }
     bool near = false;
-    int32_t js = 0;
-    if (k < nflat) {
-      js = crow[k / M] * M + (k % M);
-      const int4 xj = posfx[js];
-      if (xj.w >= 0) {  // padding slots carry type -1
-        const int64_t jx = (int32_t)((uint32_t)xj.x - (uint32_t)xi[0].x);
-        const int64_t jy = (int32_t)((uint32_t)xj.y - (uint32_t)xi[0].y);
-        const int64_t jz = (int32_t)((uint32_t)xj.z - (uint32_t)xi[0].z);
+    if (inrow && xj.w >= 0) {  // padding slots carry type -1
 #pragma unroll
-        for (int q = 0; q < M; ++q) {
-          const int32_t ix = (int32_t)((uint32_t)xi[q].x - (uint32_t)xj.x);
-          const int32_t iy = (int32_t)((uint32_t)xi[q].y - (uint32_t)xj.y);
-          const int32_t iz = (int32_t)((uint32_t)xi[q].z - (uint32_t)xj.z);
-          const double dx = (double)ix * f.scale[0];
-          const double dy = (double)iy * f.scale[1];
-          const double dz = (double)iz * f.scale[2];
-          const bool nq = q < nri && dx * dx + dy * dy + dz * dz <= il2;
-          near = near || nq;
-          if (nq) {
-            const int64_t qx = (int32_t)((uint32_t)xi[q].x - (uint32_t)xi[0].x);
-            const int64_t qy = (int32_t)((uint32_t)xi[q].y - (uint32_t)xi[0].y);
-            const int64_t qz = (int32_t)((uint32_t)xi[q].z - (uint32_t)xi[0].z);
-            frame_bad = frame_bad || qx - jx != ix || qy - jy != iy || qz - jz != iz || llabs(qx) > flim[0] ||
-                        llabs(jx) > flim[0] || llabs(qy) > flim[1] || llabs(jy) > flim[1] ||
-                        llabs(qz) > flim[2] || llabs(jz) > flim[2];
-          }
-        }
+      for (int q = 0; q < M; ++q) {
+        const double dx = (double)(int32_t)((uint32_t)xi[q].x - (uint32_t)xj.x) * f.scale[0];
+        const double dy = (double)(int32_t)((uint32_t)xi[q].y - (uint32_t)xj.y) * f.scale[1];
+        const double dz = (double)(int32_t)((uint32_t)xi[q].z - (uint32_t)xj.z) * f.scale[2];
+        near = near || (q < nri && dx * dx + dy * dy + dz * dz <= il2);
+      }
+      if (near) {
+        frame_bad = frame_bad || llabs((int64_t)(int32_t)((uint32_t)xj.x - (uint32_t)xi[0].x)) > jlim[0] ||
+                    llabs((int64_t)(int32_t)((uint32_t)xj.y - (uint32_t)xi[0].y)) > jlim[1] ||
+                    llabs((int64_t)(int32_t)((uint32_t)xj.z - (uint32_t)xi[0].z)) > jlim[2];
       }
     }
     const unsigned long long ball = __ballot(near);

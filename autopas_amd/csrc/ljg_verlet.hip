@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
     const int32_t *__restrict__ list, FixGeom f, float rc2, float eps24_1, float sig2_1, float shift6_1,
     int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
-    unsigned long long *__restrict__ acc) {
+    unsigned long long *__restrict__ acc, SkipIf sk) {
+  if (skip_launch(sk)) return;
   __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
   if constexpr (MIX) {
     // sigma^2 in the units the distances are computed in
@@ -293,7 +294,7 @@ void verlet_force(Ctx &c, bool globals) {
 #define LJG_LAUNCH(CUB, MIXV, GLOBV)                                                                             \
   hipLaunchKernelGGL((k_force_verlet<CUB, MIXV, GLOBV>), dim3(nb), dim3(kVThreads), 0, c.stream, c.n, c.posfx.p, \
                      c.vrow, c.vcount.p, c.vlist.p, f, rc2, (float)c.lj.eps24[0], sig2, (float)c.lj.shift6[0],   \
-                     c.lj.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc)
+                     c.lj.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
 #define LJG_LAUNCH2(CUB)                                                 \
   if (mix) {                                                             \
     if (globals) LJG_LAUNCH(CUB, true, true); else LJG_LAUNCH(CUB, true, false); \

@@ -280,6 +280,39 @@ def test_md_trajectory_lc_fp64(oracle):
     assert np.max(np.abs(g["vx"] - vx)) < 1e-8
 
 
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64])
+def test_dynamic_rebuild_trajectory(oracle, kernel):
+    # Hot system, no periodic rebuilds: every rebuild comes from the skin/2 displacement trigger
+    # (LogicHandler.h:1000-1015), whose step launches its force before the host has seen the trigger and
+    # must skip that launch.  Positions follow the oracle's direct-sum trajectory and the pair count
+    # summed over the steps is the oracle's exactly (skipped launches count nothing).
+    announce(f"dynamic rebuild trajectory kernel={ap.KERNEL_NAMES[kernel]}: 6^3 lattice T 4, 60 steps dt 0.004")
+    s = lattice_box(6, 0.75, seed=21, temperature=4.0)
+    L = s["L"]
+    dt = 0.004
+    with make_engine(L, M=4, rebuild_frequency=100000, dynamic_rebuild=True) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        rs = e.step(60, dt, kernel)
+        g = e.download()
+    x, y, z = s["x"].copy(), s["y"].copy(), s["z"].copy()
+    vx, vy, vz = s["vx"].copy(), s["vy"].copy(), s["vz"].copy()
+    fx, fy, fz = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    m = np.array([1.0])
+    pairs = 0
+    for _ in range(60):
+        ofx, ofy, ofz = oracle.verlet_positions(x, y, z, vx, vy, vz, fx, fy, fz, m, None, dt)
+        x, y, z = oracle.wrap(x, y, z, (0, 0, 0), (L, L, L))
+        r = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L))
+        fx, fy, fz = r["fx"], r["fy"], r["fz"]
+        pairs += r["pairs"]
+        oracle.verlet_velocities(vx, vy, vz, fx, fy, fz, ofx, ofy, ofz, m, None, dt)
+    assert rs.rebuilds >= 3  # the initial build plus displacement-triggered ones
+    assert rs.pair_interactions == pairs
+    d = np.stack([g["x"] - x, g["y"] - y, g["z"] - z])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-8
+
+
 @pytest.mark.parametrize("n_side", [8, 16])
 @pytest.mark.parametrize("kernel", [ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED])
 def test_md_trajectory_mixed_kernels(kernel, n_side):
