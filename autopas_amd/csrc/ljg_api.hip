@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <new>
+#include <utility>
 
 #include "ljg_internal.h"
 
@@ -389,22 +390,41 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     ++rs.rebuilds;
   }
   c.hits_acc = c.hits_slots.p;
+  auto rebuild_now = [&]() {
+    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    c.carry_force = false;  // the force of this step is computed right after the rebuild
+    rebuild_for(c, kernel);
+    c.carry_force = true;
+    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    ++rs.rebuilds;
+  };
+  // The j-row kernel advances the particles in its epilogue (kick of step s, drift of step s + 1), so
+  // only the first drift runs as a kernel of its own.
+  const bool fused = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
+  int32_t stamp = 0;
+  if (fused && nsteps > 0) {
+    stamp = ++c.step_stamp;
+    md_kick_drift(c, false, true, dt, stamp);
+    LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
+  }
   for (int32_t s = 0; s < nsteps; ++s) {
-    // kick of the previous step (if any) fused with this step's drift; a particle that moved skin/2
-    // since the last rebuild stamps flag[0] with this step's number
-    const int32_t stamp = ++c.step_stamp;
-    md_kick_drift(c, s > 0, true, dt, stamp);
-    auto rebuild_now = [&]() {
-      if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
-      c.carry_force = false;  // the force of this step is computed right after the rebuild
-      rebuild_for(c, kernel);
-      c.carry_force = true;
-      if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
-      ++rs.rebuilds;
-    };
+    const bool last = s + 1 == nsteps;
+    if (!fused) {
+      // kick of the previous step (if any) fused with this step's drift; a particle that moved skin/2
+      // since the last rebuild stamps flag[0] with this step's number
+      stamp = ++c.step_stamp;
+      md_kick_drift(c, s > 0, true, dt, stamp);
+      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
+    }
+    const int32_t next_stamp = fused && !last ? ++c.step_stamp : 0;
     auto force_now = [&]() {
       if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
-      run_force(c, kernel, false);
+      if (fused) {
+        const IntegArgs ia = integ_args(c, true, !last, dt, next_stamp);
+        cluster_jrows_force(c, false, &ia);
+      } else {
+        run_force(c, kernel, false);
+      }
       if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
     };
     if (c.steps_since_rebuild >= c.prm.rebuild_frequency) {
@@ -414,7 +434,6 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
       // LogicHandler.h:1000-1015 rebuilds when a particle moved skin/2.  The drift wrote its stamp to
       // pinned memory; the force is queued before the host reads it, and exits at once on the device if
       // the stamp is set, so the GPU never waits for the host's decision.
-      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
       c.skip = SkipIf{c.flag.p, stamp};
       force_now();
       c.skip = SkipIf{nullptr, 0};
@@ -432,10 +451,16 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     } else {
       force_now();
     }
+    if (fused && !last) {
+      // the launch wrote the next step's positions into the second buffer
+      std::swap(c.posfx, c.posfx_next);
+      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
+      stamp = next_stamp;
+    }
     c.steps_since_rebuild++;
   }
   c.hits_acc = nullptr;
-  if (nsteps > 0) md_kick_drift(c, true, false, dt, 0);  // final velocity update
+  if (nsteps > 0 && !fused) md_kick_drift(c, true, false, dt, 0);  // final velocity update
   c.timing = saved_timing;
   unsigned long long slots[kAccSlots * kAccStride];
   LJG_HIP(hipMemcpyAsync(slots, c.hits_slots.p, sizeof slots, hipMemcpyDeviceToHost, c.stream));

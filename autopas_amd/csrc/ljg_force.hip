@@ -323,13 +323,17 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
 //    and fails the bit-pattern cutoff test.
 //  * otherwise: every pair separation is the exact minimum-image int32 difference x_q - x_j, scaled per
 //    dimension; padding members and lanes past the end of the row are masked.
-template <int M, int G, bool FAST, bool MIX, bool GLOB>
+//  * INTEG: the epilogue also advances the i-members by velocity Verlet exactly as k_kick_drift does
+//    (kick of this step, drift of the next, same FP64 operations in the same order), writes their
+//    fixed-point positions for the next step to the second buffer (other workgroups still read this
+//    step's), and raises the displacement flag.
+template <int M, int G, bool FAST, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t row,
     const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, float rc2, float eps24_1,
     float sig2_1, float shift6_1, int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
-    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk) {
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, IntegArgs ia) {
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + (GLOB ? 2 : 0);
@@ -461,15 +465,68 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
       const int q = q0 + r;
       const double val = v[r];
       if (q < M) {
-        if (q < nreal) fx[s0 + q] = val * fs;
+        if (!INTEG && q < nreal) fx[s0 + q] = val * fs;
       } else if (q < 2 * M) {
-        if (q - M < nreal) fy[s0 + q - M] = val * fs;
+        if (!INTEG && q - M < nreal) fy[s0 + q - M] = val * fs;
       } else if (q < 3 * M) {
-        if (q - 2 * M < nreal) fz[s0 + q - 2 * M] = val * fs;
+        if (!INTEG && q - 2 * M < nreal) fz[s0 + q - 2 * M] = val * fs;
       } else if (GLOB && q == 3 * M) {
         e_out = val;
       } else if (GLOB && q == 3 * M + 1) {
         v_out = val;
+      }
+    }
+  }
+  if constexpr (INTEG) {
+    // the group's forces go through LDS so that each lane then advances one whole member: the state
+    // loads and stores of a wave cover consecutive slots, as in k_kick_drift
+    __shared__ double sF[kWavesPerBlock * G * 3 * M];
+    double *fsh = sF + ((threadIdx.x / kWave) * G + grp) * (3 * M);
+    if ((l & ((1 << (KL - K)) - 1)) == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = q0 + r;
+        if (q < 3 * M) fsh[q] = v[r] * fs;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (live) {
+      for (int m = l; m < nreal; m += LG) {
+#pragma clang fp contract(off)
+        const int64_t i = icl * M + m;
+        const double Fx = fsh[m], Fy = fsh[M + m], Fz = fsh[2 * M + m];
+        const int4 pf = posfx[i];
+        const int t = pf.w;
+        fx[i] = Fx;
+        fy[i] = Fy;
+        fz[i] = Fz;
+        double vx = ia.vx[i], vy = ia.vy[i], vz = ia.vz[i];
+        if (ia.kick) {
+          const double kf = ia.kick_fac[t];
+          vx += (Fx + ia.ofx[i]) * kf;
+          vy += (Fy + ia.ofy[i]) * kf;
+          vz += (Fz + ia.ofz[i]) * kf;
+          ia.vx[i] = vx; ia.vy[i] = vy; ia.vz[i] = vz;
+        }
+        if (ia.drift) {
+          const double ff = ia.drift_fac[t];
+          const double dx = vx * ia.dt + Fx * ff;
+          const double dy = vy * ia.dt + Fy * ff;
+          const double dz = vz * ia.dt + Fz * ff;
+          ia.ofx[i] = Fx; ia.ofy[i] = Fy; ia.ofz[i] = Fz;
+          const double x = ia.x[i] + dx, y = ia.y[i] + dy, z = ia.z[i] + dz;
+          ia.x[i] = x; ia.y[i] = y; ia.z[i] = z;
+          const double ex = x - ia.rx[i], ey = y - ia.ry[i], ez = z - ia.rz[i];
+          if (ex * ex + ey * ey + ez * ez >= ia.half_skin2) {
+            ia.flag[0] = ia.stamp;
+            ia.hflag[0] = ia.stamp;
+          }
+          ia.posfx_next[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - f.min[0]) * f.inv_scale[0]),
+                                       (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - f.min[1]) * f.inv_scale[1]),
+                                       (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - f.min[2]) * f.inv_scale[2]), t);
+        }
       }
     }
   }
@@ -664,7 +721,7 @@ static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const d
 }
 
 template <int M, int G>
-static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
+static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, const IntegArgs *integ) {
   constexpr int per_block = kWavesPerBlock * G;
   const int nb = blocks_for(c.nclusters, per_block);
   c.part_e.ensure(nb);
@@ -677,16 +734,19 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   const float rc2 = (float)(a.rc2 * u2);
   const float sig2 = (float)(a.sig2 * u2);
   State &s = c.st[c.cur];
-#define LJG_LAUNCH(CUB, MIXV, GLOBV)                                                                              \
-  hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
-                     c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, rc2,                 \
-                     (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,  \
-                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
-#define LJG_LAUNCH2(CUB)                                                              \
-  if (mix) {                                                                          \
-    if (glob) LJG_LAUNCH(CUB, true, true); else LJG_LAUNCH(CUB, true, false);        \
-  } else {                                                                            \
-    if (glob) LJG_LAUNCH(CUB, false, true); else LJG_LAUNCH(CUB, false, false);      \
+  const IntegArgs ia = integ ? *integ : IntegArgs{};
+#define LJG_LAUNCH(CUB, MIXV, GLOBV, INTV)                                                                          \
+  hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream, \
+                     c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, rc2,                   \
+                     (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,    \
+                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia)
+#define LJG_LAUNCH2(CUB)                                                                                 \
+  if (integ) {                                                                                           \
+    if (mix) LJG_LAUNCH(CUB, true, false, true); else LJG_LAUNCH(CUB, false, false, true);              \
+  } else if (mix) {                                                                                      \
+    if (glob) LJG_LAUNCH(CUB, true, true, false); else LJG_LAUNCH(CUB, true, false, false);             \
+  } else {                                                                                               \
+    if (glob) LJG_LAUNCH(CUB, false, true, false); else LJG_LAUNCH(CUB, false, false, false);           \
   }
   if (fast) {
     LJG_LAUNCH2(true)
@@ -732,14 +792,14 @@ void cluster_force(Ctx &c, bool mixed, bool globals) {
 // i-clusters per wave for the j-row kernel (rows are ~1.5x longer than a member's own list).
 static int default_jrow_groups(int M) { return M >= 8 ? 4 : M == 4 ? 8 : M == 2 ? 16 : 32; }
 
-void cluster_jrows_force(Ctx &c, bool globals) {
+void cluster_jrows_force(Ctx &c, bool globals, const IntegArgs *integ) {
   const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
   const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_jrow_groups(c.M);
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
 #define LJG_CASE(MV, GV) \
-  if (c.M == MV && G == GV) { launch_jrows<MV, GV>(c, mix, globals, a, tab); goto launched; }
+  if (c.M == MV && G == GV) { launch_jrows<MV, GV>(c, mix, globals && !integ, a, tab, integ); goto launched; }
   LJG_CASE(8, 2) LJG_CASE(8, 4) LJG_CASE(8, 8)
   LJG_CASE(4, 4) LJG_CASE(4, 8) LJG_CASE(4, 16)
   LJG_CASE(2, 8) LJG_CASE(2, 16)

@@ -116,6 +116,20 @@ struct SkipIf {
 };
 __device__ inline bool skip_launch(const SkipIf &s) { return s.flag != nullptr && *s.flag == s.stamp; }
 
+// Velocity-Verlet update fused into a force kernel's epilogue (kick of this step, drift of the next):
+// the FP64 state of the current layout, the fixed-point positions to write for the next step, the
+// displacement flag (device and pinned host copy) and the per-type factors.
+struct IntegArgs {
+  double *vx, *vy, *vz, *ofx, *ofy, *ofz, *x, *y, *z;
+  const double *rx, *ry, *rz;
+  int4 *posfx_next;
+  int32_t *flag, *hflag;
+  double dt, half_skin2;
+  int32_t kick, drift, stamp;
+  double kick_fac[kMaxTypes];   // dt / (2m)
+  double drift_fac[kMaxTypes];  // dt * dt / (2m)
+};
+
 // LJ parameters (mixing tables are ntypes x ntypes, row-major).
 struct LJParams {
   int32_t ntypes;
@@ -179,6 +193,7 @@ struct Ctx {
   // Verlet lists on the linked cells (fixed-point positions, transposed rows per wavefront)
   FixGeom fg{};
   DevBuf<int4> posfx;            // per slot: fixed-point x, y, z and the type
+  DevBuf<int4> posfx_next;       // the fused integrator writes the next step's fixed-point positions here
   DevBuf<int32_t> jcl;           // j-row build, first pass: per i-cluster row of j-cluster indices
   int32_t jccap = 0;
   DevBuf<int32_t> jlist;         // cluster layout: per i-cluster row of j slots (union of members' lists)
@@ -230,7 +245,8 @@ void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
 void verlet_force(Ctx &c, bool globals);   // ljg_verlet.hip
 void verlet_geometry(Ctx &c);              // ljg_verlet.hip: fixed-point scales of the box
 void verlet_fixed_slots(Ctx &c, int64_t nslots);  // ljg_verlet.hip: posfx for slots [0, nslots)
-void cluster_jrows_force(Ctx &c, bool globals);   // ljg_force.hip
+void cluster_jrows_force(Ctx &c, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
+IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);    // ljg_md.hip
 void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
 void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip

@@ -194,6 +194,31 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   LJG_HIP(hipGetLastError());
 }
 
+IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
+  State &s = c.st[c.cur];
+  IntegArgs a{};
+  a.vx = s.vx.p; a.vy = s.vy.p; a.vz = s.vz.p;
+  a.ofx = s.ofx.p; a.ofy = s.ofy.p; a.ofz = s.ofz.p;
+  a.x = s.x.p; a.y = s.y.p; a.z = s.z.p;
+  a.rx = s.rx.p; a.ry = s.ry.p; a.rz = s.rz.p;
+  c.posfx_next.ensure(c.nslots);
+  a.posfx_next = c.posfx_next.p;
+  a.flag = c.flag.p;
+  a.hflag = c.host_flag_dev;
+  a.dt = dt;
+  const double hs = c.prm.skin * 0.5;
+  a.half_skin2 = hs * hs;
+  a.kick = kick;
+  a.drift = drift;
+  a.stamp = stamp;
+  for (int t = 0; t < kMaxTypes; ++t) {
+    const double m = c.lj.mass[t] > 0. ? c.lj.mass[t] : 1.;
+    a.kick_fac[t] = dt / (2 * m);
+    a.drift_fac[t] = dt * dt / (2 * m);
+  }
+  return a;
+}
+
 __global__ void k_set_rebuild_ref(int64_t nslots, StateView s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
