@@ -13,5 +13,6 @@ from .engine import (  # noqa: F401
     Engine,
     EngineError,
     generate_lattice,
+    generate_slab,
     load_library,
 )

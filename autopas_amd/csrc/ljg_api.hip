@@ -146,6 +146,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
     return LJG_ERR_UNSUPPORTED;
   if (p->rebuild_frequency < 1) return LJG_ERR_ARG;
   if (p->tile_groups < 0 || p->tile_groups > 64) return LJG_ERR_ARG;
+  if (p->build_groups < 0 || p->build_groups > 64) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;
@@ -189,8 +190,8 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
-  c.red_out.release(); c.red_hits.release(); c.hits_slots.release(); c.flag.release(); c.scan_tmp.release();
+  c.pos32.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev_drift) (void)hipEventDestroy(c.ev_drift);
@@ -249,6 +250,26 @@ ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const do
   LJG_HIP(hipMemcpyAsync(c.tab.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
   c.types_set = true;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max[3]) {
+  if (!h || !box_min || !box_max) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  for (int d = 0; d < 3; ++d)
+    if (!(box_max[d] - box_min[d] >= ilen)) return fail(c, LJG_ERR_ARG, "box shorter than rc + skin");
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  for (int d = 0; d < 3; ++d) {
+    c.prm.box_min[d] = box_min[d];
+    c.prm.box_max[d] = box_max[d];
+  }
+  c.n = 0;
+  c.nslots = 0;
+  c.layout = Layout::None;
+  c.have_forces = false;
+  c.steps_since_rebuild = 0;
   return LJG_OK;
   LJG_API_END
 }

@@ -182,7 +182,8 @@ struct Ctx {
   DevBuf<double> bbox;          // 6 * nclusters
   DevBuf<double4> c0;           // nclusters: cluster reference point (bbox centre at build)
   DevBuf<int32_t> list_count;   // entries per i-cluster
-  DevBuf<unsigned long long> list_stats;  // [0] total entries, [1] longest list
+  DevBuf<unsigned long long> list_stats;  // list build statistics (totals, longest rows, flags)
+  DevBuf<unsigned long long> tstats;      // tower statistics: [0] particles, [1] longest tower, [2] clusters
   int32_t list_row = 0;         // row capacity of the current lists: cluster ci owns [ci*row, ci*row + count)
   int32_t list_cap = 0;         // row capacity to start the next build with
   // List entry: x = j cluster | shift id << 27; y,z,w = FP32 bits of c0_j + shift - c0_i (the offset

@@ -53,7 +53,10 @@ __global__ void k_tower_nclusters(int64_t ntowers, const int32_t *__restrict__ c
 
 // One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
 // are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.
+// Two instances: CAP = kSortSmall takes the towers up to that size (12 KB of LDS, so many workgroups
+// share a CU), CAP = kTowerSortCap the rare larger ones; each skips the towers of the other.
 constexpr int kSortThreads = 256;
+constexpr int kSortSmall = 1024;
 
 __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, const int64_t *__restrict__ id) {
   if (sb < 0) return sa >= 0;  // padding sorts last
@@ -63,17 +66,20 @@ __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, co
   return id[sa] < id[sb];
 }
 
+template <int CAP>
 __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
     int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
     const double *__restrict__ z, const int64_t *__restrict__ id, const int64_t *__restrict__ cstart, int32_t M,
     int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, int32_t *__restrict__ err) {
-  __shared__ double sz[kTowerSortCap];
-  __shared__ int32_t ss[kTowerSortCap];
+  __shared__ double sz[CAP];
+  __shared__ int32_t ss[CAP];
   const int64_t t = blockIdx.x;
   if (t >= ntowers) return;
   const int64_t b = start[t];
   const int n = (int)(start[t + 1] - b);
-  if (n > kTowerSortCap) {
+  if (CAP > kSortSmall && n <= kSortSmall) return;  // sorted by the small instance
+  if (CAP == kSortSmall && n > kSortSmall) return;  // left to the large instance
+  if (n > CAP) {
     if (threadIdx.x == 0) err[1] = 1;
     return;
   }
@@ -162,6 +168,7 @@ __device__ inline double box_dist2(const double *a, const double *b, double shx,
 // i-cluster is a binary search on the monotone upper z-extents held in LDS.  count[ci] is the row's full
 // length even when it exceeds cap, so the host can grow cap and rebuild if a row overflowed.
 constexpr int kListThreads = 64;
+constexpr int kJclSplit = 4;   // wavefronts per tower in the first pass of the j-row build
 constexpr int kListLds = 128;  // j-clusters staged per chunk
 
 // JCL = true (the j-particle row build, first pass): entries are the j-cluster index only (int32), and a
@@ -197,13 +204,27 @@ __device__ inline bool passed_earlier(const double *bi, const double *bj, const 
   return false;
 }
 
-template <bool JCL>
-__global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
+// LDS exchange among the lanes of one wavefront (the waves of a workgroup stage different towers).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// NS wavefronts per tower split the neighbour-tower visits (oy, ox) of the traversal into NS consecutive
+// ranges; wave w writes its entries for i-cluster ci into sub-row w, [ci*cap + w*cap/NS, ...), with its
+// length in count[ci*NS + w], so the sub-rows read in order give the single-wave traversal order.
+// The cluster-entry lists (JCL = false, compared entry by entry with the oracle) use NS = 1.
+template <bool JCL, int NS>
+__global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
     int64_t ntowers, TowerGeom g, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
     const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
     int32_t *__restrict__ jcl) {
-  __shared__ double sb[kListLds * 6];
-  __shared__ double4 sc[kListLds];
+  __shared__ double sbw[NS][kListLds * 6];
+  __shared__ double4 scw[NS][kListLds];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  double *sb = sbw[wv];
+  double4 *sc = scw[wv];
   const int64_t t = blockIdx.x;
   if (t >= ntowers) return;
   const int tx = (int)(t % g.nt[0]), ty = (int)(t / g.nt[0]);
@@ -211,8 +232,11 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
   const double il2 = g.ilen * g.ilen;
   const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
   const bool xy_repeats = g.nt[0] < 2 * g.R + 1 || g.nt[1] < 2 * g.R + 1;
+  const int side = 2 * g.R + 1, nvis = side * side;
+  const int vbeg = (int)((int64_t)nvis * wv / NS), vend = (int)((int64_t)nvis * (wv + 1) / NS);
+  const int32_t subcap = cap / NS;
   for (int64_t base = ib; base < ie; base += kListThreads) {
-    const int64_t ci = base + threadIdx.x;
+    const int64_t ci = base + lane;
     const bool act = ci < ie;
     double bi[6] = {0., 0., 0., 0., 0., 0.};
     double4 ai = make_double4(0., 0., 0., 0.);
@@ -221,72 +245,71 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
       for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
       ai = c0[ci];
     }
-    const int64_t w0 = ci * (int64_t)cap;
+    const int64_t w0 = ci * (int64_t)cap + (int64_t)wv * subcap;
     int32_t cnt = 0;
-    for (int oy = -g.R; oy <= g.R; ++oy) {
+    for (int v = vbeg; v < vend; ++v) {
+      const int oy = v / side - g.R, ox = v % side - g.R;
       const int aoy = oy < 0 ? -oy : oy;
       const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
-      for (int ox = -g.R; ox <= g.R; ++ox) {
-        const int aox = ox < 0 ? -ox : ox;
-        const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
-        if (dtx * dtx + dty * dty > il2) continue;
-        int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
-        while (jx < 0) { jx += g.nt[0]; --sx; }
-        while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
-        while (jy < 0) { jy += g.nt[1]; --sy; }
-        while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
-        const int64_t tj = jx + (int64_t)jy * g.nt[0];
-        const int64_t cb = cstart[tj], ce = cstart[tj + 1];
-        if (cb == ce) continue;
-        const bool single = ce - cb <= kListLds;
-        for (int sz = -1; sz <= 1; ++sz) {
-          const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
-          const double zmin = bi[2] - zwin - shz;  // window in the j tower's own frame
-          const double zmax = bi[5] + zwin - shz;
-          for (int64_t jb = cb; jb < ce; jb += kListLds) {
-            const int nj = (int)(ce - jb < kListLds ? ce - jb : kListLds);
-            if (!single || (sz == -1 && jb == cb)) {
-              __syncthreads();
-              for (int k = threadIdx.x; k < 6 * nj; k += kListThreads) sb[k] = bbox[6 * jb + k];
-              for (int k = threadIdx.x; k < nj; k += kListThreads) sc[k] = c0[jb + k];
-              __syncthreads();
-            }
-            if (!act) continue;
-            // lower bound of zhi >= zmin in the chunk
-            int lo = 0, hi = nj;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (sb[6 * mid + 5] < zmin) lo = mid + 1;
-              else hi = mid;
-            }
-            for (int k = lo; k < nj; ++k) {
-              const double *bj = sb + 6 * k;
-              if (bj[2] > zmax) break;
-              if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
-                if constexpr (JCL) {
-                  if (!passed_earlier(bi, bj, g, tx, ty, oy, ox, sz, xy_repeats, il2)) {
-                    if (cnt < cap) jcl[w0 + cnt] = (int32_t)(jb + k);
-                    ++cnt;
-                  }
-                } else {
-                  if (cnt < cap) {
-                    const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-                    const double4 b = sc[k];
-                    const float dx = (float)((b.x + shx) - ai.x);
-                    const float dy = (float)((b.y + shy) - ai.y);
-                    const float dz = (float)((b.z + shz) - ai.z);
-                    list[w0 + cnt] = make_int4((int32_t)(jb + k) | (sid << kShiftBits), __float_as_int(dx),
-                                               __float_as_int(dy), __float_as_int(dz));
-                  }
+      const int aox = ox < 0 ? -ox : ox;
+      const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+      if (dtx * dtx + dty * dty > il2) continue;
+      int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+      while (jx < 0) { jx += g.nt[0]; --sx; }
+      while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+      while (jy < 0) { jy += g.nt[1]; --sy; }
+      while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+      const int64_t tj = jx + (int64_t)jy * g.nt[0];
+      const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+      if (cb == ce) continue;
+      const bool single = ce - cb <= kListLds;
+      for (int sz = -1; sz <= 1; ++sz) {
+        const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
+        const double zmin = bi[2] - zwin - shz;  // window in the j tower's own frame
+        const double zmax = bi[5] + zwin - shz;
+        for (int64_t jb = cb; jb < ce; jb += kListLds) {
+          const int nj = (int)(ce - jb < kListLds ? ce - jb : kListLds);
+          if (!single || (sz == -1 && jb == cb)) {
+            wave_sync();
+            for (int k = lane; k < 6 * nj; k += kWave) sb[k] = bbox[6 * jb + k];
+            for (int k = lane; k < nj; k += kWave) sc[k] = c0[jb + k];
+            wave_sync();
+          }
+          if (!act) continue;
+          // lower bound of zhi >= zmin in the chunk
+          int lo = 0, hi = nj;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sb[6 * mid + 5] < zmin) lo = mid + 1;
+            else hi = mid;
+          }
+          for (int k = lo; k < nj; ++k) {
+            const double *bj = sb + 6 * k;
+            if (bj[2] > zmax) break;
+            if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+              if constexpr (JCL) {
+                if (!passed_earlier(bi, bj, g, tx, ty, oy, ox, sz, xy_repeats, il2)) {
+                  if (cnt < subcap) jcl[w0 + cnt] = (int32_t)(jb + k);
                   ++cnt;
                 }
+              } else {
+                if (cnt < subcap) {
+                  const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                  const double4 b = sc[k];
+                  const float dx = (float)((b.x + shx) - ai.x);
+                  const float dy = (float)((b.y + shy) - ai.y);
+                  const float dz = (float)((b.z + shz) - ai.z);
+                  list[w0 + cnt] = make_int4((int32_t)(jb + k) | (sid << kShiftBits), __float_as_int(dx),
+                                             __float_as_int(dy), __float_as_int(dz));
+                }
+                ++cnt;
               }
             }
           }
         }
       }
     }
-    if (act) count[ci] = cnt;
+    if (act) count[ci * NS + wv] = cnt;
   }
 }
 
@@ -302,19 +325,23 @@ __global__ __launch_bounds__(kListThreads) void k_cluster_lists_tower(
 // frame; otherwise it uses exact per-pair differences.
 constexpr int kExpandThreads = 256;
 
-template <int M>
+template <int M, int NS, int G>
 __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t ccap,
     const int32_t *__restrict__ ccount, const int32_t *__restrict__ jcl, FixGeom f, double ilen, int32_t cap,
     int32_t *__restrict__ count, int32_t *__restrict__ jrows, unsigned long long *__restrict__ stats) {
+  constexpr int LG = kWave / G;  // lanes per i-cluster
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t ic = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kExpandThreads / kWave) + threadIdx.x / kWave));
-  if (ic >= nclusters) return;
+  const int grp = lane / LG, l = lane % LG;
+  int64_t ic = (int64_t)(blockIdx.x * (kExpandThreads / kWave) + threadIdx.x / kWave) * G + grp;
+  if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane((int)ic);
+  const bool live = ic < nclusters;
+  const int64_t icl = live ? ic : nclusters - 1;
   const double il2 = ilen * ilen;
-  const int nri = (int)c0[ic].w;
+  const int nri = (int)c0[icl].w;
   int4 xi[M];
 #pragma unroll
-  for (int q = 0; q < M; ++q) xi[q] = posfx[ic * M + q];
+  for (int q = 0; q < M; ++q) xi[q] = posfx[icl * M + q];
   // frame limits for the row entries: L/2 - ilen minus the largest member offset
   int64_t jlim[3];
   bool frame_bad = false;
@@ -331,28 +358,57 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       jlim[d] = (int64_t)(2147483648.0 - ilen * f.inv_scale[d]) - qmax[d];
-      frame_bad = frame_bad || jlim[d] < 0;
+      frame_bad = frame_bad || (live && jlim[d] < 0);
     }
   }
-  const int nflat = ccount[ic] * M;
-  const int32_t *crow = jcl + ic * (int64_t)ccap;
-  int32_t *jrow = jrows + ic * (int64_t)cap;
+  // the NS sub-rows of the first pass, in order (an overflowed sub-row is rebuilt anyway)
+  const int32_t subcap = ccap / NS;
+  int pre[NS + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int w = 0; w < NS; ++w) {
+    const int32_t cw = ccount[icl * NS + w];
+    pre[w + 1] = pre[w] + (cw < subcap ? cw : subcap);
+  }
+  const int nflat = live ? pre[NS] * M : 0;
+  // the loop runs to the wave's longest stream so that the ballots stay uniform
+  int nmax = nflat;
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int off = 32; off >= LG; off >>= 1) {
+      const int o = __shfl_xor(nmax, off, kWave);
+      nmax = o > nmax ? o : nmax;
+    }
+  }
+  const int32_t *crow = jcl + icl * (int64_t)ccap;
+  // entry e of the concatenated sub-rows
+  auto centry = [&](int e) {
+    int w = 0;
+#pragma unroll
+    for (int k = 1; k < NS; ++k) w += e >= pre[k] ? 1 : 0;
+    int off = e;
+#pragma unroll
+    for (int k = 1; k < NS; ++k) off -= (w == k) ? pre[k] : 0;
+    return crow[w * subcap + off];
+  };
+  int32_t *jrow = jrows + icl * (int64_t)cap;
+  const unsigned long long gmask = (LG == kWave ? ~0ull : ((1ull << LG) - 1ull)) << (grp * LG);
   int32_t cnt = 0;
   // prefetch the first entry of this lane
   int32_t jsn = 0;
   int4 xjn = make_int4(0, 0, 0, -1);
-  if (lane < nflat) {
-    jsn = crow[lane / M] * M + (lane % M);
+  if (l < nflat) {
+    jsn = centry(l / M) * M + (l % M);
     xjn = posfx[jsn];
   }
-  for (int base = 0; base < nflat; base += kWave) {
+  for (int base = 0; base < nmax; base += LG) {
     const int32_t js = jsn;
     const int4 xj = xjn;
-    const bool inrow = base + lane < nflat;
+    const bool inrow = base + l < nflat;
     {
-      const int kn = base + kWave + lane;
+      const int kn = base + LG + l;
       if (kn < nflat) {
-        jsn = crow[kn / M] * M + (kn % M);
+        jsn = centry(kn / M) * M + (kn % M);
         xjn = posfx[jsn];
       }
     }
@@ -371,12 +427,12 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
                     llabs((int64_t)(int32_t)((uint32_t)xj.z - (uint32_t)xi[0].z)) > jlim[2];
       }
     }
-    const unsigned long long ball = __ballot(near);
+    const unsigned long long ball = __ballot(near) & gmask;
     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
     if (near && cnt + below < cap) jrow[cnt + below] = js;
     cnt += __popcll(ball);
   }
-  if (lane == 0) count[ic] = cnt;
+  if (live && l == 0) count[ic] = cnt;
   if (__ballot(frame_bad) != 0ull && lane == 0) stats[2] = 1ull;  // every writer stores the same value
 }
 
@@ -402,6 +458,10 @@ void vcl_pos32(Ctx &c) {
   hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
                      s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p);
   LJG_HIP(hipGetLastError());
+}
+
+__global__ void k_put_i64(const int64_t *__restrict__ from, unsigned long long *__restrict__ to) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) to[0] = (unsigned long long)from[0];
 }
 
 __global__ void k_zero_u64(int64_t n, unsigned long long *__restrict__ p) {
@@ -470,20 +530,33 @@ void vcl_build(Ctx &c) {
                      c.ctow_count.p);
   LJG_HIP(hipGetLastError());
   exclusive_scan_i32_to_i64(c, c.ctow_count.p, c.ctow_start.p, ntow);
-  int64_t ncl = 0;
-  LJG_HIP(hipMemcpyAsync(&ncl, c.ctow_start.p + ntow, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  // one synchronisation for the sizes: the cluster count and the longest tower
+  c.tstats.ensure(4);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.tstats.p);
+  LJG_HIP(hipGetLastError());
+  count_stats(c, c.tower_count.p, ntow, c.tstats.p);
+  hipLaunchKernelGGL(k_put_i64, dim3(1), dim3(1), 0, c.stream, c.ctow_start.p + ntow, c.tstats.p + 2);
+  LJG_HIP(hipGetLastError());
+  unsigned long long ts[3] = {0, 0, 0};
+  LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
+  const int64_t ncl = (int64_t)ts[2];
+  if (ts[1] > (unsigned long long)kTowerSortCap)
+    throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
   c.nclusters = ncl;
   const int64_t new_slots = ncl * M;
   c.perm.ensure(new_slots);
   c.cluster_tower.ensure(ncl);
-  hipLaunchKernelGGL(k_tower_sort, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow, c.tower_start.p,
-                     c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p, c.flag.p);
+  hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
+                     c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p,
+                     c.flag.p);
   LJG_HIP(hipGetLastError());
-  int32_t hflag[2] = {0, 0};
-  LJG_HIP(hipMemcpyAsync(hflag, c.flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
-  if (hflag[1]) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
+  if (ts[1] > (unsigned long long)kSortSmall) {
+    hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
+                       c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p,
+                       c.cluster_tower.p, c.flag.p);
+    LJG_HIP(hipGetLastError());
+  }
   permute_state(c, c.perm.p, new_slots);
   c.M = M;
   State &t = c.st[c.cur];
@@ -492,80 +565,89 @@ void vcl_build(Ctx &c) {
   hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(ncl, 256)), dim3(256), 0, c.stream, ncl, M, t.x.p, t.y.p, t.z.p,
                      t.orig.p, c.bbox.p, c.c0.p);
   LJG_HIP(hipGetLastError());
-  c.list_count.ensure(ncl + 1);
-  c.list_stats.ensure(3);
+  c.list_count.ensure(ncl * kJclSplit + 1);
+  // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
+  // pass, [3], [4] total and longest j-particle row
+  c.list_stats.ensure(5);
   const bool jrows = c.want_jrows;
   if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
-  // pass 1: cluster-pair lists (j-row build: deduplicated j-cluster indices only)
-  int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : 64) : (c.list_cap > 0 ? c.list_cap : 64);
-  for (;;) {
-    if (jrows) c.jcl.ensure((size_t)ncl * cap + 1);
-    else c.list.ensure((size_t)ncl * cap + 1);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)3, c.list_stats.p);
-    LJG_HIP(hipGetLastError());
-    if (jrows)
-      hipLaunchKernelGGL(k_cluster_lists_tower<true>, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow, g,
-                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr, c.jcl.p);
-    else
-      hipLaunchKernelGGL(k_cluster_lists_tower<false>, dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream, ntow,
-                         g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
-    LJG_HIP(hipGetLastError());
-    count_stats(c, c.list_count.p, ncl, c.list_stats.p);
-    unsigned long long st[2] = {0, 0};
-    LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
-    LJG_HIP(hipStreamSynchronize(c.stream));
-    c.nentries = (int64_t)st[0];
-    const int32_t maxc = (int32_t)st[1];
-    if (maxc <= cap) {
-      // keep the row capacity for the next build, with headroom for fluctuations
-      const int32_t want = maxc + maxc / 8 + 8;
-      int32_t &keep = jrows ? c.jccap : c.list_cap;
-      keep = want > cap ? want : (cap > 2 * want ? want : cap);
-      break;
-    }
-    cap = maxc + maxc / 8 + 8;
-  }
-  c.list_row = cap;
   if (jrows) {
-    // pass 2: expand to j particles on the fixed-point positions of every slot (padding: type -1)
+    // fixed-point positions of every slot (padding: type -1) for the expansion pass
     verlet_geometry(c);
     verlet_fixed_slots(c, new_slots);
-    const int32_t ccap = cap;
     c.jcount.ensure(ncl + 1);
-    int32_t jcap = c.jcap > 0 ? c.jcap : 160;
-    for (;;) {
+  }
+  // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
+  // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
+  int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24) : (c.list_cap > 0 ? c.list_cap : 64);
+  int32_t jcap = c.jcap > 0 ? c.jcap : 160;
+  for (;;) {
+    if (jrows) {
+      c.jcl.ensure((size_t)ncl * cap + 1);
       c.jlist.ensure((size_t)ncl * jcap + 1);
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)3, c.list_stats.p);
-      LJG_HIP(hipGetLastError());
-      const int nb = blocks_for(ncl, kExpandThreads / kWave);
-      const double ilen2 = c.prm.cutoff + c.prm.skin;
-#define LJG_EXPAND(MV)                                                                                          \
-  hipLaunchKernelGGL(k_jrows_expand<MV>, dim3(nb), dim3(kExpandThreads), 0, c.stream, ncl, c.posfx.p, c.c0.p, ccap, \
-                     c.list_count.p, c.jcl.p, c.fg, ilen2, jcap, c.jcount.p, c.jlist.p, c.list_stats.p)
-      switch (M) {
-        case 1: LJG_EXPAND(1); break;
-        case 2: LJG_EXPAND(2); break;
-        case 4: LJG_EXPAND(4); break;
-        case 8: LJG_EXPAND(8); break;
-        default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", -2};
-      }
-#undef LJG_EXPAND
-      LJG_HIP(hipGetLastError());
-      count_stats(c, c.jcount.p, ncl, c.list_stats.p);
-      unsigned long long st[3] = {0, 0, 0};
-      LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
-      LJG_HIP(hipStreamSynchronize(c.stream));
-      const int32_t maxc = (int32_t)st[1];
-      if (maxc <= jcap) {
-        c.nentries = (int64_t)st[0];
-        c.jrows_frame_ok = st[2] == 0;
-        const int32_t want = maxc + maxc / 8 + 8;
-        c.jcap = want > jcap ? want : (jcap > 2 * want ? want : jcap);
-        break;
-      }
-      jcap = maxc + maxc / 8 + 8;
+    } else {
+      c.list.ensure((size_t)ncl * cap + 1);
     }
-    c.list_row = jcap;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)5, c.list_stats.p);
+    LJG_HIP(hipGetLastError());
+    if (jrows)
+      hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit>), dim3((unsigned)ntow), dim3(kListThreads * kJclSplit),
+                         0, c.stream, ntow, g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
+                         c.jcl.p);
+    else
+      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
+                         ntow, g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
+    LJG_HIP(hipGetLastError());
+    count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
+    if (jrows) {
+      // i-clusters per wavefront of the expansion pass: two for M >= 4 (rows of ~100-250 candidates),
+      // more for the short streams of small clusters
+      const int eg = c.prm.build_groups > 0 ? c.prm.build_groups : (M >= 4 ? 2 : M == 2 ? 4 : 8);
+#define LJG_EXPAND(MV, GV)                                                                                        \
+  if (M == MV && eg == GV) {                                                                                      \
+    const int nb = blocks_for(ncl, (kExpandThreads / kWave) * GV);                                               \
+    hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV>), dim3(nb), dim3(kExpandThreads), 0, c.stream, ncl,    \
+                       c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, jcap, c.jcount.p, c.jlist.p, \
+                       c.list_stats.p);                                                                          \
+    goto expanded;                                                                                                \
+  }
+      LJG_EXPAND(1, 4) LJG_EXPAND(1, 8) LJG_EXPAND(2, 2) LJG_EXPAND(2, 4) LJG_EXPAND(4, 1) LJG_EXPAND(4, 2)
+      LJG_EXPAND(4, 4) LJG_EXPAND(8, 1) LJG_EXPAND(8, 2) LJG_EXPAND(8, 4)
+#undef LJG_EXPAND
+      throw HipError{hipErrorInvalidValue, "unsupported (cluster size, expansion groups) combination", -2};
+    expanded:
+      LJG_HIP(hipGetLastError());
+      count_stats(c, c.jcount.p, ncl, c.list_stats.p + 3);
+    }
+    unsigned long long st[5] = {0, 0, 0, 0, 0};
+    int32_t sort_err = 0;
+    LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
+    LJG_HIP(hipMemcpyAsync(&sort_err, c.flag.p + 1, sizeof sort_err, hipMemcpyDeviceToHost, c.stream));
+    LJG_HIP(hipStreamSynchronize(c.stream));
+    if (sort_err) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
+    const int32_t maxc = (int32_t)st[1], maxj = (int32_t)st[4];
+    const int32_t split = jrows ? kJclSplit : 1;  // first-pass rows hold `split` sub-rows of cap/split
+    const bool ok1 = maxc <= cap / split, ok2 = !jrows || maxj <= jcap;
+    if (!ok1) cap = split * (maxc + maxc / 8 + 8);
+    if (jrows && ok1 && !ok2) jcap = maxj + maxj / 8 + 8;
+    if (!(ok1 && ok2)) continue;
+    // keep the row capacities for the next build, with headroom for fluctuations
+    auto keep = [](int32_t used, int32_t had) {
+      const int32_t want = used + used / 8 + 8;
+      return want > had ? want : (had > 2 * want ? want : had);
+    };
+    if (jrows) {
+      c.jccap = kJclSplit * keep(maxc, cap / kJclSplit);
+      c.jcap = keep(maxj, jcap);
+      c.nentries = (int64_t)st[3];
+      c.jrows_frame_ok = st[2] == 0;
+      c.list_row = jcap;
+    } else {
+      c.list_cap = keep(maxc, cap);
+      c.nentries = (int64_t)st[0];
+      c.list_row = cap;
+    }
+    break;
   }
   c.jrows_valid = jrows;
   c.layout = Layout::Clusters;

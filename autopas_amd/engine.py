@@ -50,7 +50,8 @@ class Params(ctypes.Structure):
         ("apply_shift", _i32),
         ("device", _i32),
         ("tile_groups", _i32),
-        ("reserved", _i32 * 6),
+        ("build_groups", _i32),
+        ("reserved", _i32 * 5),
     ]
 
 
@@ -81,6 +82,7 @@ SIGNATURES = {
     "ljg_destroy": (_i32, [_p]),
     "ljg_last_error": (ctypes.c_char_p, [_p]),
     "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
+    "ljg_set_box": (_i32, [_p, _p, _p]),
     "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "ljg_set_forces": (_i32, [_p, _p, _p, _p]),
@@ -95,6 +97,7 @@ SIGNATURES = {
     "ljg_get_clusters": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
     "ljg_get_verlet_lists": (_i32, [_p, _p, _p]),
     "ljg_generate_lattice": (_i32, [ctypes.c_uint64, _i64, _i64, _i64, _i64, _d, _p, _d, _d, _i32] + [_p] * 7),
+    "ljg_generate_slab": (_i32, [ctypes.c_uint64, _p, _d, _d, _i64, _i64, _d, _i32, _d, _i64] + [_p] * 7),
 }
 
 _lib = None
@@ -143,11 +146,27 @@ def generate_lattice(seed, n, dims, spacing, box_min=(0.0, 0.0, 0.0), jitter=0.0
     return x, y, z, vx, vy, vz, t
 
 
+def generate_slab(seed, box, slab, n_liquid, n_gas, dmin=0.85, ntypes=1, temperature=0.0, max_attempts=None):
+    """Seeded two-phase system: a liquid slab (z in [slab[0], slab[1])) in a gas, by random sequential
+    placement with minimum distance dmin (host-side generator in the engine library)."""
+    lib = load_library()
+    n = n_liquid + n_gas
+    x, y, z, vx, vy, vz = (np.empty(n) for _ in range(6))
+    t = np.empty(n, dtype=np.int32)
+    b = np.asarray(box, dtype=np.float64)
+    attempts = max_attempts if max_attempts is not None else 200 * n
+    st = lib.ljg_generate_slab(seed, _ptr(b), slab[0], slab[1], n_liquid, n_gas, dmin, ntypes, temperature,
+                               attempts, _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(t))
+    if st != OK:
+        raise EngineError(st, "ljg_generate_slab")
+    return x, y, z, vx, vy, vz, t
+
+
 class Engine:
     """One periodic box on one GPU."""
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
-                 dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0):
+                 dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -160,6 +179,7 @@ class Engine:
         p.apply_shift = int(bool(apply_shift))
         p.device = device
         p.tile_groups = tile_groups
+        p.build_groups = build_groups
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))
@@ -194,6 +214,13 @@ class Engine:
         buf = ctypes.create_string_buffer(256)
         self._check(self.lib.ljg_device_name(self.h, buf, 256), "ljg_device_name")
         return buf.value.decode()
+
+    def set_box(self, box_min, box_max):
+        """Move the context to a new periodic box; the particles are dropped (upload the next system)."""
+        lo = np.ascontiguousarray(box_min, dtype=np.float64)
+        hi = np.ascontiguousarray(box_max, dtype=np.float64)
+        self._check(self.lib.ljg_set_box(self.h, _ptr(lo), _ptr(hi)), "ljg_set_box")
+        self.n = 0
 
     def set_types(self, epsilon, sigma, mass=None):
         e = np.ascontiguousarray(epsilon, dtype=np.float64)

@@ -1,0 +1,192 @@
+"""Job serving on one GPU: independent MD jobs (small periodic LJ boxes) arrive over time and are run by
+a pool of workers, each owning one engine context (its own HIP stream) that is reused from job to job.
+
+The unit of work is md-flexible's whole simulation loop for one box (``Simulation.cpp:230-350``): the
+particles are uploaded, ``steps`` velocity-Verlet steps run on the device with lists rebuilt every
+``rebuild_frequency`` steps or on a skin/2 displacement (``LogicHandler.h:987-1015``), and the final
+state is downloaded.  Latency is measured from a job's arrival to the end of its download.
+
+Job parameters follow the serving workload: N log-uniform in [256, 65536], density uniform in
+[0.4, 0.85], steps uniform in {10, ..., 1000}, temperature uniform in [0.7, 1.5], a jittered simple-cubic
+lattice (jitter +-0.05 sigma) with ceil(N^(1/3)) sites per side (the first N sites) filling the cube
+L = (N / rho)^(1/3), and a per-job seed 5000 + j.  Arrivals are a Poisson process.  All draws come from
+numpy's Philox4x32 generator.
+
+Worker threads spend their time inside the C ABI, which releases the GIL, so the contexts' streams run
+concurrently on the device.
+"""
+import math
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine
+
+RC = 2.5
+SKIN = 0.3
+DT = 0.002
+
+
+@dataclass
+class Job:
+    index: int
+    n: int
+    density: float
+    steps: int
+    temperature: float
+    seed: int
+    arrival: float = 0.0  # seconds after the start of the run
+    # filled in by the server
+    start: float = math.nan
+    finish: float = math.nan
+    pairs: int = 0
+    rebuilds: int = 0
+    worker: int = -1
+    result: dict = field(default_factory=dict)
+
+    @property
+    def box_length(self):
+        return (self.n / self.density) ** (1.0 / 3.0)
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def make_jobs(count, rate=None, seed=5, n_range=(256, 65536), density_range=(0.4, 0.85), steps_range=(10, 1000),
+              temperature_range=(0.7, 1.5), first_seed=5000):
+    """Draw ``count`` jobs; with ``rate`` (jobs/s) their arrival times form a Poisson process, else all
+    arrive at t = 0 (closed loop)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    jobs = []
+    t = 0.0
+    for j in range(count):
+        n = int(round(math.exp(rng.uniform(lo, hi))))
+        n = min(max(n, n_range[0]), n_range[1])
+        job = Job(index=j, n=n, density=float(rng.uniform(*density_range)),
+                  steps=int(rng.integers(steps_range[0], steps_range[1] + 1)),
+                  temperature=float(rng.uniform(*temperature_range)), seed=first_seed + j)
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+            job.arrival = t
+        jobs.append(job)
+    return jobs
+
+
+def job_system(job):
+    """Jittered simple-cubic lattice of the job (positions, velocities) in the cube [0, L)^3."""
+    k = int(math.ceil(job.n ** (1.0 / 3.0) - 1e-9))
+    L = job.box_length
+    a = L / k
+    x, y, z, vx, vy, vz, _ = engine.generate_lattice(job.seed, job.n, (k, k, k), a, (0.0, 0.0, 0.0), 0.05,
+                                                     job.temperature)
+    return L, (x, y, z, vx, vy, vz)
+
+
+def run_job(eng, job, kernel):
+    """Run one job on an engine context (box, upload, steps, download); returns (pairs, rebuilds, state)."""
+    L, (x, y, z, vx, vy, vz) = job_system(job)
+    eng.set_box((0.0, 0.0, 0.0), (L, L, L))
+    eng.upload(x, y, z, vx, vy, vz)
+    rs = eng.step(job.steps, DT, kernel)
+    return rs.pair_interactions, rs.rebuilds, eng.download()
+
+
+class Server:
+    """A pool of ``workers`` threads, each with one engine context, serving jobs first come first served."""
+
+    def __init__(self, workers=4, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=4, tile_groups=0,
+                 keep_results=False, make_engine=None):
+        self.workers = workers
+        self.kernel = kernel
+        self.keep_results = keep_results
+        if make_engine is None:
+            def make_engine():
+                return engine.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), RC, SKIN, cluster_size=cluster_size,
+                                     rebuild_frequency=10, dynamic_rebuild=True, tile_groups=tile_groups)
+        self._make_engine = make_engine
+        self._engines = [make_engine() for _ in range(workers)]
+        self.errors = []
+
+    def close(self):
+        for e in self._engines:
+            e.close()
+        self._engines = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, jobs, run=run_job):
+        """Serve ``jobs`` (arrival times relative to now); returns the wall time from start to the last
+        completion.  Each job gets start/finish/latency, pairs and rebuilds filled in."""
+        q = queue.Queue()
+        self.errors = []
+        t0 = time.perf_counter()
+
+        def feeder():
+            for job in sorted(jobs, key=lambda j: j.arrival):
+                delay = job.arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                q.put(job)
+            for _ in range(self.workers):
+                q.put(None)
+
+        def worker(w):
+            eng = self._engines[w]
+            while True:
+                job = q.get()
+                if job is None:
+                    return
+                job.worker = w
+                job.start = time.perf_counter() - t0
+                try:
+                    pairs, rebuilds, state = run(eng, job, self.kernel)
+                except Exception as exc:  # recorded and re-raised by run()
+                    self.errors.append((job.index, exc))
+                    pairs, rebuilds, state = 0, 0, {}
+                job.finish = time.perf_counter() - t0
+                job.pairs, job.rebuilds = pairs, rebuilds
+                if self.keep_results:
+                    job.result = state
+
+        threads = [threading.Thread(target=worker, args=(w,), daemon=True) for w in range(self.workers)]
+        for th in threads:
+            th.start()
+        feeder()
+        for th in threads:
+            th.join()
+        if self.errors:
+            idx, exc = self.errors[0]
+            raise RuntimeError(f"job {idx} failed: {exc}") from exc
+        return max(j.finish for j in jobs) if jobs else 0.0
+
+
+def latency_summary(jobs, wall):
+    lat = np.array([j.latency for j in jobs])
+    pairs = sum(j.pairs for j in jobs)
+    return dict(jobs=len(jobs), p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_ms=float(lat.mean() * 1e3), wall_s=wall, pairs=int(pairs), pairs_per_s=pairs / wall if wall else 0.0,
+                jobs_per_s=len(jobs) / wall if wall else 0.0)
+
+
+def serve_open_loop(count, workers=4, kernel=engine.KERNEL_CLUSTER_MIXED, load=0.8, seed=5, cluster_size=4,
+                    calibration_jobs=None, **job_kw):
+    """The serving measurement: calibrate the pool's throughput on a closed-loop batch, then serve
+    ``count`` jobs arriving as a Poisson process at ``load`` times that throughput."""
+    with Server(workers, kernel, cluster_size) as srv:
+        calib = make_jobs(calibration_jobs or max(8, count // 4), None, seed=seed + 1, **job_kw)
+        wall = srv.run(calib)
+        capacity = len(calib) / wall
+        jobs = make_jobs(count, load * capacity, seed=seed, **job_kw)
+        wall = srv.run(jobs)
+    out = latency_summary(jobs, wall)
+    out.update(workers=workers, load=load, capacity_jobs_per_s=capacity, kernel=engine.KERNEL_NAMES[kernel])
+    return out, jobs

@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--plain-steps", type=int, default=200)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--serve-jobs", type=int, default=200, help="jobs in the serving measurement (0: skip)")
+    p.add_argument("--serve-workers", type=int, default=4)
     p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
     return p.parse_args()
 
@@ -115,6 +117,25 @@ def cpu_baseline(seconds):
                 kind="port",
                 sample=f"32768-particle box (32^3 lattice, rho 0.75, seed 2), {reps} C08 traversals, "
                        f"{r['pairs']} pairs each, {r['traverse_seconds'] * 1e3:.2f} ms per traversal")
+
+
+def serving(jobs, workers):
+    """p50/p99 job latency: independent small boxes (N log-uniform in [256, 65536], rho in [0.4, 0.85],
+    10-1000 steps) arriving as a Poisson process at 80% of the measured pool throughput, served by
+    `workers` reused contexts with the j-row kernel; beside it the same jobs run one at a time with the
+    plain FP64 kernel (closed loop, so its latency is the bare service time)."""
+    import autopas_amd as ap
+    from autopas_amd import serve
+
+    out, _ = serve.serve_open_loop(jobs, workers=workers, kernel=ap.KERNEL_CLUSTER_JROWS_MIXED, load=0.8, seed=5,
+                                   cluster_size=8)
+    base_jobs = serve.make_jobs(max(jobs // 4, 8), None, seed=5)
+    with serve.Server(1, ap.KERNEL_LC_FP64, 4) as srv:
+        wall = srv.run(base_jobs)
+    base = serve.latency_summary(base_jobs, wall)
+    out["fp64_one_at_a_time"] = dict(jobs=base["jobs"], p50_ms=base["p50_ms"], p99_ms=base["p99_ms"],
+                                     pairs_per_s=base["pairs_per_s"], jobs_per_s=base["jobs_per_s"])
+    return out
 
 
 def main():
@@ -198,6 +219,8 @@ def main():
                     "the non-packed FP32 issue rate these kernels use is 78.6 TFLOP/s",
         },
     }
+    if args.serve_jobs > 0:
+        out["serving"] = serving(args.serve_jobs, args.serve_workers)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

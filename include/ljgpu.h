@@ -54,8 +54,9 @@ typedef struct {
   int32_t dynamic_rebuild;   /* also rebuild once any particle moved >= skin/2 (LogicHandler.h:1000-1015) */
   int32_t apply_shift;    /* truncated-shifted potential energy (LJFunctor applyShift) */
   int32_t device;         /* HIP device ordinal */
-  int32_t tile_groups;    /* i-clusters per wavefront in the FP32 cluster kernel; 0 = default per cluster size */
-  int32_t reserved[6];
+  int32_t tile_groups;    /* i-clusters per wavefront in the FP32 cluster kernels; 0 = default per cluster size */
+  int32_t build_groups;   /* i-clusters per wavefront in the j-row expansion pass; 0 = default per cluster size */
+  int32_t reserved[5];
 } ljg_params;
 
 typedef struct {
@@ -82,6 +83,11 @@ ljg_status ljg_destroy(ljg_ctx *ctx);
 const char *ljg_last_error(const ljg_ctx *ctx);
 /* Human-readable name of the device the context runs on (for logs). */
 ljg_status ljg_device_name(const ljg_ctx *ctx, char *buf, int32_t buflen);
+
+/* Change the periodic box of a context (AutoPas::setBoxMin/setBoxMax, AutoPasDecl.h:589-595, or
+ * resizeBox, :116).  The particles are dropped: upload the next system afterwards.  Device buffers are
+ * kept, so one context can serve many jobs of different sizes without reallocating. */
+ljg_status ljg_set_box(ljg_ctx *ctx, const double box_min[3], const double box_max[3]);
 
 /* Particle types: ParticlePropertiesLibrary::addSiteType + calculateMixingCoefficients
  * (ParticlePropertiesLibrary.h:443-489): Lorentz-Berthelot mixing is done here, in FP64. */
@@ -148,6 +154,14 @@ ljg_status ljg_get_verlet_lists(ljg_ctx *ctx, int64_t *offsets, int32_t *neighbo
 ljg_status ljg_generate_lattice(uint64_t seed, int64_t n, int64_t nx, int64_t ny, int64_t nz, double a,
                                 const double box_min[3], double jitter, double temperature, int32_t ntypes,
                                 double *x, double *y, double *z, double *vx, double *vy, double *vz, int32_t *type);
+
+/* Two-phase system: n_liquid particles uniform in the slab slab_lo <= z < slab_hi, then n_gas uniform
+ * in the rest of the periodic box [0, box), placed one by one with no pair closer than dmin (random
+ * sequential placement, periodic minimum image).  Types uniform in [0, ntypes); velocities as for the
+ * lattice.  Returns LJG_ERR_CAPACITY if max_attempts candidates were not enough. */
+ljg_status ljg_generate_slab(uint64_t seed, const double box[3], double slab_lo, double slab_hi, int64_t n_liquid,
+                             int64_t n_gas, double dmin, int32_t ntypes, double temperature, int64_t max_attempts,
+                             double *x, double *y, double *z, double *vx, double *vy, double *vz, int32_t *type);
 
 #ifdef __cplusplus
 }

@@ -85,3 +85,43 @@ def test_kernel_ids_match_header():
     for name, val in ids.items():
         assert getattr(engine, "KERNEL_" + name) == val, name
     assert set(engine.KERNEL_NAMES) == set(ids.values())
+
+
+def slab_box(scale):
+    """The two-phase slab system scaled down by `scale` per dimension: box 200 x 200 x 400 / scale, a liquid
+    slab at density 0.8 (thickness 53.85 / scale) centred in z, gas at 0.02 elsewhere."""
+    box = np.array([200.0, 200.0, 400.0]) / scale
+    thick = 53.85 / scale
+    lo, hi = box[2] / 2 - thick / 2, box[2] / 2 + thick / 2
+    n_l = int(round(0.8 * box[0] * box[1] * thick))
+    n_g = int(round(0.02 * (box.prod() - box[0] * box[1] * thick)))
+    return box, (lo, hi), n_l, n_g
+
+
+def test_slab_generator_min_distance_and_phases():
+    from scipy.spatial import cKDTree
+
+    box, slab, n_l, n_g = slab_box(8.0)
+    x, y, z, vx, vy, vz, t = ap.generate_slab(3, box, slab, n_l, n_g, dmin=0.85, ntypes=4, temperature=0.9)
+    pos = np.stack([x, y, z], axis=1)
+    assert np.all(pos >= 0) and np.all(pos < box)
+    # no pair closer than dmin, periodic (brute-force tree check)
+    tree = cKDTree(pos, boxsize=box)
+    assert len(tree.query_pairs(0.85 - 1e-12)) == 0
+    # the first n_l particles are in the slab, the rest outside
+    assert np.all((z[:n_l] >= slab[0]) & (z[:n_l] < slab[1]))
+    assert np.all((z[n_l:] < slab[0]) | (z[n_l:] >= slab[1]))
+    # types uniform over 4, exact temperature and zero momentum
+    counts = np.bincount(t, minlength=4)
+    assert counts.min() > 0.2 * len(t)
+    assert abs(vx.sum()) < 1e-9 and abs(vz.sum()) < 1e-9
+    assert (vx @ vx + vy @ vy + vz @ vz) / (3 * len(x) - 3) == pytest.approx(0.9, rel=1e-12)
+    again = ap.generate_slab(3, box, slab, n_l, n_g, dmin=0.85, ntypes=4, temperature=0.9)
+    assert np.array_equal(again[0], x) and np.array_equal(again[6], t)
+
+
+def test_slab_generator_reports_exhaustion():
+    box, slab, n_l, n_g = slab_box(8.0)
+    with pytest.raises(ap.EngineError) as ei:
+        ap.generate_slab(3, box, slab, n_l, n_g, dmin=0.85, max_attempts=n_l // 2)
+    assert ei.value.status == engine.ERR_CAPACITY
