@@ -600,9 +600,8 @@ void vcl_build(Ctx &c) {
     LJG_HIP(hipGetLastError());
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
     if (jrows) {
-      // i-clusters per wavefront of the expansion pass: two for M >= 4 (rows of ~100-250 candidates),
-      // more for the short streams of small clusters
-      const int eg = c.prm.build_groups > 0 ? c.prm.build_groups : (M >= 4 ? 2 : M == 2 ? 4 : 8);
+      // i-clusters per wavefront of the expansion pass (tuned on the 10^6-particle liquid: 4 for M >= 4)
+      const int eg = c.prm.build_groups > 0 ? c.prm.build_groups : (M >= 4 ? 4 : M == 2 ? 4 : 8);
 #define LJG_EXPAND(MV, GV)                                                                                        \
   if (M == MV && eg == GV) {                                                                                      \
     const int nb = blocks_for(ncl, (kExpandThreads / kWave) * GV);                                               \
@@ -632,8 +631,10 @@ void vcl_build(Ctx &c) {
     if (jrows && ok1 && !ok2) jcap = maxj + maxj / 8 + 8;
     if (!(ok1 && ok2)) continue;
     // keep the row capacities for the next build, with headroom for fluctuations
+    // (a melting lattice lengthens its rows by ~10% between rebuilds early on: a quarter of headroom keeps
+    // the two passes from running twice)
     auto keep = [](int32_t used, int32_t had) {
-      const int32_t want = used + used / 8 + 8;
+      const int32_t want = used + used / 4 + 8;
       return want > had ? want : (had > 2 * want ? want : had);
     };
     if (jrows) {
