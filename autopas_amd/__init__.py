@@ -15,4 +15,6 @@ from .engine import (  # noqa: F401
     generate_lattice,
     generate_slab,
     load_library,
+    read_checkpoint,
+    write_checkpoint,
 )

@@ -35,6 +35,7 @@ SOURCES = {
     "ljg_force.hip": [],
     "ljg_verlet.hip": ["-ffp-contract=off"],
     "ljg_generate.cpp": ["-ffp-contract=off"],
+    "ljg_checkpoint.cpp": [],
 }
 
 

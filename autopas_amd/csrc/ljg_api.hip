@@ -86,8 +86,7 @@ static void rebuild_for(Ctx &c, int kernel) {
     lc_bin(c);
     if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_build(c);
   }
-  md_set_rebuild_reference(c);
-  c.steps_since_rebuild = 0;
+  c.steps_since_rebuild = 0;  // the rebuild's permutation set the reference positions
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {

@@ -249,9 +249,9 @@ void verlet_fixed_slots(Ctx &c, int64_t nslots);  // ljg_verlet.hip: posfx for s
 void cluster_jrows_force(Ctx &c, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
 IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);    // ljg_md.hip
 void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
-void md_set_rebuild_reference(Ctx &c);     // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip
-void permute_state(Ctx &c, const int32_t *src, int64_t new_slots);  // ljg_md.hip
+// Rebuild permutation (also sets the rebuild reference positions; writes posfx when given).
+void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx = nullptr);  // ljg_md.hip
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip

@@ -84,19 +84,27 @@ void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t 
 
 // ---------------------------------------------------------------------------------------------------
 // State permutation: new slot s takes the particle of old slot src[s]; src[s] < 0 makes a padding slot.
-// The rebuild reference positions are not carried: every rebuild resets them from the new positions.
-// Inside the step loop the force is recomputed right after a rebuild, so it is not carried either.
+// Every permutation is a rebuild, so the rebuild reference positions are set to the positions here, and
+// when posfx is given the slots' 32-bit fixed-point positions are written too (type -1 for padding).
+// Inside the step loop the force is recomputed right after a rebuild, so it is not carried.
 __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateView o, StateView n,
-                          int32_t carry_force) {
+                          int32_t carry_force, int4 *__restrict__ posfx, FixGeom f) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnew) return;
   const int32_t k = src[s];
   if (k >= 0) {
-    n.x[s] = o.x[k];   n.y[s] = o.y[k];   n.z[s] = o.z[k];
+    const double x = o.x[k], y = o.y[k], z = o.z[k];
+    const int32_t t = o.type[k];
+    n.x[s] = x;   n.y[s] = y;   n.z[s] = z;
+    n.rx[s] = x;  n.ry[s] = y;  n.rz[s] = z;
     n.vx[s] = o.vx[k]; n.vy[s] = o.vy[k]; n.vz[s] = o.vz[k];
     if (carry_force) { n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k]; }
     n.ofx[s] = o.ofx[k]; n.ofy[s] = o.ofy[k]; n.ofz[s] = o.ofz[k];
-    n.type[s] = o.type[k]; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
+    n.type[s] = t; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
+    if (posfx)
+      posfx[s] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - f.min[0]) * f.inv_scale[0]),
+                           (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - f.min[1]) * f.inv_scale[1]),
+                           (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - f.min[2]) * f.inv_scale[2]), t);
   } else {
     n.x[s] = kDummyPos; n.y[s] = kDummyPos; n.z[s] = kDummyPos;
     n.vx[s] = 0.; n.vy[s] = 0.; n.vz[s] = 0.;
@@ -104,14 +112,15 @@ __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateVi
     n.ofx[s] = 0.; n.ofy[s] = 0.; n.ofz[s] = 0.;
     n.rx[s] = kDummyPos; n.ry[s] = kDummyPos; n.rz[s] = kDummyPos;
     n.type[s] = 0; n.orig[s] = -1; n.id[s] = -1;
+    if (posfx) posfx[s] = make_int4(0, 0, 0, -1);
   }
 }
 
-void permute_state(Ctx &c, const int32_t *src, int64_t new_slots) {
+void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx) {
   const int other = 1 - c.cur;
   c.st[other].ensure((size_t)new_slots);
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, src,
-                     view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force);
+                     view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force, posfx, c.fg);
   LJG_HIP(hipGetLastError());
   c.cur = other;
   c.nslots = new_slots;
@@ -219,17 +228,6 @@ IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   return a;
 }
 
-__global__ void k_set_rebuild_ref(int64_t nslots, StateView s) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  s.rx[i] = s.x[i]; s.ry[i] = s.y[i]; s.rz[i] = s.z[i];
-}
-
-void md_set_rebuild_reference(Ctx &c) {
-  hipLaunchKernelGGL(k_set_rebuild_ref, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]));
-  LJG_HIP(hipGetLastError());
-}
 
 // Periodic wrap of particles that left the box, RegularGridDecomposition.cpp:563-587.
 struct WrapArgs {

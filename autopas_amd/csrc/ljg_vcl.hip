@@ -557,7 +557,13 @@ void vcl_build(Ctx &c) {
                        c.cluster_tower.p, c.flag.p);
     LJG_HIP(hipGetLastError());
   }
-  permute_state(c, c.perm.p, new_slots);
+  const bool jrows = c.want_jrows;
+  if (jrows) {
+    // fixed-point positions of every slot (padding: type -1), written by the permutation
+    verlet_geometry(c);
+    c.posfx.ensure(new_slots);
+  }
+  permute_state(c, c.perm.p, new_slots, jrows ? c.posfx.p : nullptr);
   c.M = M;
   State &t = c.st[c.cur];
   c.bbox.ensure(6 * ncl);
@@ -569,14 +575,8 @@ void vcl_build(Ctx &c) {
   // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
   // pass, [3], [4] total and longest j-particle row
   c.list_stats.ensure(5);
-  const bool jrows = c.want_jrows;
   if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
-  if (jrows) {
-    // fixed-point positions of every slot (padding: type -1) for the expansion pass
-    verlet_geometry(c);
-    verlet_fixed_slots(c, new_slots);
-    c.jcount.ensure(ncl + 1);
-  }
+  if (jrows) c.jcount.ensure(ncl + 1);
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
   // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
   int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24) : (c.list_cap > 0 ? c.list_cap : 64);

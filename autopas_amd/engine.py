@@ -98,6 +98,9 @@ SIGNATURES = {
     "ljg_get_verlet_lists": (_i32, [_p, _p, _p]),
     "ljg_generate_lattice": (_i32, [ctypes.c_uint64, _i64, _i64, _i64, _i64, _d, _p, _d, _d, _i32] + [_p] * 7),
     "ljg_generate_slab": (_i32, [ctypes.c_uint64, _p, _d, _d, _i64, _i64, _d, _i32, _d, _i64] + [_p] * 7),
+    "ljg_write_checkpoint": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i64, _i32, _i64] + [_p] * 11
+                             + [ctypes.c_char_p, _i32]),
+    "ljg_read_checkpoint": (_i32, [ctypes.c_char_p, _p] + [_p] * 11),
 }
 
 _lib = None
@@ -160,6 +163,42 @@ def generate_slab(seed, box, slab, n_liquid, n_gas, dmin=0.85, ntypes=1, tempera
     if st != OK:
         raise EngineError(st, "ljg_generate_slab")
     return x, y, z, vx, vy, vz, t
+
+
+STATE_KEYS = ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz")
+
+
+def write_checkpoint(directory, name, iteration, state, type=None, id=None, digits=None):
+    """Write state (dict of the nine FP64 arrays) as an md-flexible checkpoint; returns the .pvtu path."""
+    lib = load_library()
+    arrs = [np.ascontiguousarray(state[k], dtype=np.float64) for k in STATE_KEYS]
+    n = len(arrs[0])
+    t = np.zeros(n, dtype=np.int32) if type is None else np.ascontiguousarray(type, dtype=np.int32)
+    i = np.arange(n, dtype=np.int64) if id is None else np.ascontiguousarray(id, dtype=np.int64)
+    digits = digits if digits is not None else max(len(str(iteration)), 1)
+    buf = ctypes.create_string_buffer(4096)
+    st = lib.ljg_write_checkpoint(str(directory).encode(), str(name).encode(), iteration, digits, n,
+                                  *[_ptr(a) for a in arrs], _ptr(t), _ptr(i), buf, len(buf))
+    if st != OK:
+        raise EngineError(st, "ljg_write_checkpoint")
+    return buf.value.decode()
+
+
+def read_checkpoint(path):
+    """Read all pieces of an md-flexible checkpoint: (state dict, type, id)."""
+    lib = load_library()
+    n = np.zeros(1, dtype=np.int64)
+    st = lib.ljg_read_checkpoint(str(path).encode(), _ptr(n), *([None] * 11))
+    if st != OK:
+        raise EngineError(st, "ljg_read_checkpoint")
+    cnt = int(n[0])
+    arrs = [np.empty(cnt) for _ in STATE_KEYS]
+    t = np.empty(cnt, dtype=np.int32)
+    i = np.empty(cnt, dtype=np.int64)
+    st = lib.ljg_read_checkpoint(str(path).encode(), _ptr(n), *[_ptr(a) for a in arrs], _ptr(t), _ptr(i))
+    if st != OK:
+        raise EngineError(st, "ljg_read_checkpoint")
+    return dict(zip(STATE_KEYS, arrs)), t, i
 
 
 class Engine:
@@ -261,6 +300,20 @@ class Engine:
         rs = RunStats()
         self._check(self.lib.ljg_step(self.h, nsteps, dt, kernel, ctypes.byref(rs)), "ljg_step")
         return rs
+
+    def write_checkpoint(self, directory, name, iteration, digits=None):
+        """Write the current state (upload order, uploaded types and ids) as an md-flexible checkpoint."""
+        st = self.download()
+        t, i = self._keep[6], self._keep[7]
+        return write_checkpoint(directory, name, iteration, st, type=t, id=i, digits=digits)
+
+    def load_checkpoint(self, path):
+        """Upload positions, velocities, types and ids from an md-flexible checkpoint and restore its
+        forces as the current force (MDFlexConfig.cpp:173), so the next step's drift uses them."""
+        st, t, i = read_checkpoint(path)
+        self.upload(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], type=t, id=i)
+        self.set_forces(st["fx"], st["fy"], st["fz"])
+        return len(t)
 
     def download(self):
         out = [np.empty(self.n) for _ in range(9)]

@@ -146,6 +146,22 @@ ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *clust
  * (offsets[n] is the total). */
 ljg_status ljg_get_verlet_lists(ljg_ctx *ctx, int64_t *offsets, int32_t *neighbors);
 
+/* ---- checkpoints (host side; md-flexible's .pvtu/.vtu format) --------------------------------- */
+/* Write dir/<name>_Particles_<iteration>.pvtu and dir/data/<name>_Particles_0_<iteration>.vtu, the
+ * iteration zero-padded to `digits` (ParallelVtkWriter.cpp:60-200,309-358,438-442), with 17 significant
+ * digits so FP64 state round-trips.  Arrays in caller order.  The index path is copied to pvtu_path if
+ * given. */
+ljg_status ljg_write_checkpoint(const char *dir, const char *name, int64_t iteration, int32_t digits, int64_t n,
+                                const double *x, const double *y, const double *z, const double *vx,
+                                const double *vy, const double *vz, const double *fx, const double *fy,
+                                const double *fz, const int32_t *type, const int64_t *id, char *pvtu_path,
+                                int32_t pvtu_path_len);
+/* Read every piece of a checkpoint (MDFlexConfig::loadParticlesFromCheckpoint, MDFlexConfig.cpp:33-183,
+ * 648-670).  With x == NULL only *n is set (the total); otherwise *n is the capacity of the arrays on
+ * input and the count on output.  Pieces are concatenated in rank order. */
+ljg_status ljg_read_checkpoint(const char *pvtu, int64_t *n, double *x, double *y, double *z, double *vx, double *vy,
+                               double *vz, double *fx, double *fy, double *fz, int32_t *type, int64_t *id);
+
 /* ---- synthetic systems (host-side, seeded Philox4x32-10) ------------------------------------- */
 /* Simple-cubic lattice of nx*ny*nz sites, spacing a, positions box_min + (i+1/2)a plus uniform jitter in
  * [-jitter, jitter]; the first n sites are used (n <= nx*ny*nz).  Velocities Maxwell-Boltzmann at

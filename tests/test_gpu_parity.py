@@ -386,3 +386,31 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     assert errj <= 1e-5
     assert res[ap.KERNEL_LC_FP64][1].pair_count == ref["pairs"]
     assert res[ap.KERNEL_CLUSTER_FP64][1].pair_count == ref["pairs"]
+
+
+def test_checkpoint_restart_continues_the_trajectory(tmp_path):
+    # 30 steps in one go vs 15 steps, checkpoint, a fresh context restored from it, 15 more steps (FP64)
+    announce("checkpoint restart: 6^3 lattice, 15 + 15 steps vs 30 steps, LC FP64")
+    s = lattice_box(6, 0.75, seed=17, temperature=0.9)
+    L = s["L"]
+    with make_engine(L, M=4, rebuild_frequency=5) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        e.step(30, 0.002, ap.KERNEL_LC_FP64)
+        ref = e.download()
+    with make_engine(L, M=4, rebuild_frequency=5) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        e.step(15, 0.002, ap.KERNEL_LC_FP64)
+        mid = e.download()
+        path = e.write_checkpoint(tmp_path, "lj", 15, digits=4)
+    back, _, ids = ap.read_checkpoint(path)
+    for k in mid:
+        assert np.array_equal(back[k], mid[k]), k
+    assert np.array_equal(ids, np.arange(len(ids)))
+    with make_engine(L, M=4, rebuild_frequency=5) as e:
+        assert e.load_checkpoint(path) == s["n"]
+        e.step(15, 0.002, ap.KERNEL_LC_FP64)
+        out = e.download()
+    d = np.stack([out["x"] - ref["x"], out["y"] - ref["y"], out["z"] - ref["z"]])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-10
+    assert np.max(np.abs(out["vx"] - ref["vx"])) < 1e-9
