@@ -4,6 +4,7 @@ The compute lives in ``libljgpu.so`` (HIP kernels for gfx950 behind the C ABI in
 this package only binds it.
 """
 from .engine import (  # noqa: F401
+    KERNEL_AUTO,
     KERNEL_CLUSTER_FP64,
     KERNEL_CLUSTER_JROWS_MIXED,
     KERNEL_CLUSTER_MIXED,

@@ -91,7 +91,8 @@ static void rebuild_for(Ctx &c, int kernel) {
 
 static bool layout_matches(const Ctx &c, int kernel) {
   if (is_cluster_kernel(kernel))
-    return c.layout == Layout::Clusters && c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED);
+    return c.layout == Layout::Clusters && c.built_M == c.M &&
+           c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED);
   if (kernel == LJG_KERNEL_VERLET_MIXED) return c.layout == Layout::Verlet;
   return c.layout == Layout::Cells;
 }
@@ -382,14 +383,9 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
   return LJG_OK;
 }
 
-ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out) {
-  if (!h || nsteps < 0 || kernel < 0 || kernel > LJG_KERNEL_LAST || !(dt > 0.)) return LJG_ERR_ARG;
-  LJG_API_BEGIN(h)
-  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
-  if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
-  const auto t0 = std::chrono::steady_clock::now();
-  ljg_run_stats rs{};
-  const bool timing = c.timing;
+// nsteps velocity-Verlet steps with one kernel (LogicHandler.h:987-1015 rebuild policy); the statistics
+// are added to rs.  HIP-event timing of force launches and rebuilds when `timing`.
+static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool timing, ljg_run_stats &rs) {
   std::vector<hipEvent_t> fev, rev;  // pairs (start, stop) per force launch / rebuild
   auto new_event = [](std::vector<hipEvent_t> &v) {
     hipEvent_t e;
@@ -487,8 +483,8 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
   LJG_HIP(hipStreamSynchronize(c.stream));
   int64_t hits2 = 0;
   for (int k = 0; k < kAccSlots; ++k) hits2 += (int64_t)slots[k * kAccStride];
-  rs.steps = nsteps;
-  rs.pair_interactions = hits2 / 2;
+  rs.steps += nsteps;
+  rs.pair_interactions += hits2 / 2;
   for (size_t k = 0; k + 1 < fev.size(); k += 2) {
     float ms = 0.f;
     LJG_HIP(hipEventElapsedTime(&ms, fev[k], fev[k + 1]));
@@ -503,10 +499,143 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
   for (auto e : fev) (void)hipEventDestroy(e);
   for (auto e : rev) (void)hipEventDestroy(e);
   c.have_forces = true;
+}
+
+
+// ---- layout tuning (AutoTuner.cpp:85-175,250-369 in spirit) ----------------------------------------
+// Candidates: FP32-pair/FP64-accumulate layouts (kernel, cluster size, i-clusters per wavefront of the
+// force kernel, of the j-row expansion pass).  A tuning phase runs each candidate the box supports for
+// tuning_samples steps of the real simulation and scores it as AutoTuner's estimateRuntimeFromSamples
+// does with the mean selector: rebuild time / rebuild_frequency + the rest of the step.  The lowest
+// score is used until tuning_interval steps later (0: never again).
+struct TuneCandidate {
+  int32_t kernel, M, G, B;
+};
+static const TuneCandidate kTuneCandidates[] = {
+    {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 8, 4}, {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 4, 4},
+    {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 8, 4}, {LJG_KERNEL_CLUSTER_MIXED, 4, 8, 0},
+    {LJG_KERNEL_CLUSTER_MIXED, 8, 2, 0},       {LJG_KERNEL_VERLET_MIXED, 4, 0, 0},
+};
+constexpr int kNumTuneCandidates = (int)(sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]));
+static_assert(kNumTuneCandidates <= LJG_MAX_TUNE_CANDIDATES, "ljg_tuning holds every candidate");
+
+static void apply_candidate(Ctx &c, const TuneCandidate &k) {
+  c.M = k.M;
+  c.prm.tile_groups = k.G;
+  c.prm.build_groups = k.B;
+}
+
+static int next_candidate(const Ctx &c, int from) {
+  for (int i = from; i < kNumTuneCandidates; ++i)
+    if (!kernel_unsupported(c, kTuneCandidates[i].kernel)) return i;
+  return -1;
+}
+
+static void run_auto(Ctx &c, int32_t nsteps, double dt, ljg_run_stats &rs) {
+  TuneState &t = c.tune;
+  const int32_t samples = c.prm.tuning_samples > 0 ? c.prm.tuning_samples
+                                                    : (c.prm.rebuild_frequency > 3 ? c.prm.rebuild_frequency : 3);
+  int32_t left = nsteps;
+  while (left > 0) {
+    if (!t.active && (t.best < 0 || (c.prm.tuning_interval > 0 && t.steps_since >= c.prm.tuning_interval))) {
+      t.active = true;
+      t.ci = next_candidate(c, 0);
+      t.done = 0;
+      t.wall_ms = t.rebuild_ms = 0.;
+      t.rebuilds = 0;
+      for (double &e : t.effective_ms) e = -1.;
+      ++t.phases;
+    }
+    if (t.active) {
+      const TuneCandidate &k = kTuneCandidates[t.ci];
+      apply_candidate(c, k);
+      // every candidate's samples start with a rebuild, so each score carries its rebuild cost
+      if (t.done == 0) c.steps_since_rebuild = c.prm.rebuild_frequency;
+      const int32_t run = left < samples - t.done ? left : samples - t.done;
+      ljg_run_stats sub{};
+      const auto t0 = std::chrono::steady_clock::now();
+      run_steps(c, k.kernel, run, dt, true, sub);  // ends with a synchronisation
+      t.wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      t.rebuild_ms += sub.rebuild_ms;
+      t.rebuilds += (int32_t)sub.rebuilds;
+      t.done += run;
+      left -= run;
+      rs.steps += sub.steps;
+      rs.rebuilds += sub.rebuilds;
+      rs.pair_interactions += sub.pair_interactions;
+      if (c.timing) {
+        rs.force_ms += sub.force_ms;
+        rs.force_launches += sub.force_launches;
+        rs.rebuild_ms += sub.rebuild_ms;
+      }
+      if (t.done == samples) {
+        const double per_rebuild = t.rebuilds > 0 ? t.rebuild_ms / t.rebuilds : 0.;
+        t.effective_ms[t.ci] = (t.wall_ms - t.rebuild_ms) / samples + per_rebuild / c.prm.rebuild_frequency;
+        const int nxt = next_candidate(c, t.ci + 1);
+        if (nxt >= 0) {
+          t.ci = nxt;
+          t.done = 0;
+          t.wall_ms = t.rebuild_ms = 0.;
+          t.rebuilds = 0;
+        } else {
+          int best = -1;
+          for (int i = 0; i < kNumTuneCandidates; ++i)
+            if (t.effective_ms[i] >= 0. && (best < 0 || t.effective_ms[i] < t.effective_ms[best])) best = i;
+          t.best = best;
+          t.active = false;
+          t.steps_since = 0;
+        }
+      }
+    } else {
+      const TuneCandidate &k = kTuneCandidates[t.best];
+      apply_candidate(c, k);
+      int32_t run = left;
+      if (c.prm.tuning_interval > 0 && c.prm.tuning_interval - t.steps_since < run)
+        run = (int32_t)(c.prm.tuning_interval - t.steps_since);
+      run_steps(c, k.kernel, run, dt, c.timing, rs);
+      t.steps_since += run;
+      left -= run;
+    }
+  }
+}
+
+ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out) {
+  if (!h || nsteps < 0 || kernel < 0 || (kernel > LJG_KERNEL_LAST && kernel != LJG_KERNEL_AUTO) || !(dt > 0.))
+    return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (kernel != LJG_KERNEL_AUTO)
+    if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
+  const auto t0 = std::chrono::steady_clock::now();
+  ljg_run_stats rs{};
+  if (kernel == LJG_KERNEL_AUTO) {
+    if (next_candidate(c, 0) < 0) return fail(c, LJG_ERR_UNSUPPORTED, "no tuning candidate fits this box");
+    run_auto(c, nsteps, dt, rs);
+  } else {
+    run_steps(c, kernel, nsteps, dt, c.timing, rs);
+  }
   rs.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (out) *out = rs;
   return LJG_OK;
   LJG_API_END
+}
+
+ljg_status ljg_get_tuning(ljg_ctx *h, ljg_tuning *out) {
+  if (!h || !out) return LJG_ERR_ARG;
+  Ctx &c = h->c;
+  std::memset(out, 0, sizeof *out);
+  out->candidates = kNumTuneCandidates;
+  out->best = c.tune.best;
+  out->phases = c.tune.phases;
+  out->tuning = c.tune.active ? 1 : 0;
+  for (int i = 0; i < kNumTuneCandidates; ++i) {
+    out->kernel[i] = kTuneCandidates[i].kernel;
+    out->cluster_size[i] = kTuneCandidates[i].M;
+    out->tile_groups[i] = kTuneCandidates[i].G;
+    out->build_groups[i] = kTuneCandidates[i].B;
+    out->effective_ms[i] = c.tune.effective_ms[i];
+  }
+  return LJG_OK;
 }
 
 ljg_status ljg_download(ljg_ctx *h, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *fx,

@@ -141,6 +141,19 @@ struct LJParams {
 
 enum class Layout { None, Upload, Cells, Clusters, Verlet };
 
+// Layout tuning (LJG_KERNEL_AUTO): progress of the current phase and the scores of the last one.
+struct TuneState {
+  bool active = false;
+  int ci = 0;                  // candidate being sampled
+  int32_t done = 0;            // its steps so far
+  double wall_ms = 0., rebuild_ms = 0.;
+  int32_t rebuilds = 0;
+  int best = -1;               // chosen candidate
+  int32_t phases = 0;
+  int64_t steps_since = 0;     // steps since the last phase ended
+  double effective_ms[LJG_MAX_TUNE_CANDIDATES] = {};
+};
+
 struct Ctx {
   ljg_params prm{};
   std::string err;
@@ -226,6 +239,8 @@ struct Ctx {
   DevBuf<int64_t> scan_tmp;
 
   bool have_forces = false;  // st[cur].fx holds the force of the current positions
+  int32_t built_M = 0;       // cluster size of the current cluster layout
+  TuneState tune;
   int64_t steps_since_rebuild = 0;
   int32_t step_stamp = 0;    // running step number written into flag[0] by the displacement check
   bool carry_force = true;   // permutes keep the current force (false inside the step loop)

@@ -651,6 +651,7 @@ void vcl_build(Ctx &c) {
     break;
   }
   c.jrows_valid = jrows;
+  c.built_M = M;
   c.layout = Layout::Clusters;
   if (!jrows) vcl_pos32(c);
 }

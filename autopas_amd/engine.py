@@ -24,6 +24,8 @@ KERNEL_CLUSTER_FP64 = 1
 KERNEL_CLUSTER_MIXED = 2
 KERNEL_VERLET_MIXED = 3
 KERNEL_CLUSTER_JROWS_MIXED = 4
+KERNEL_AUTO = 100
+MAX_TUNE_CANDIDATES = 16
 KERNEL_NAMES = {
     KERNEL_LC_FP64: "lc_fp64",
     KERNEL_CLUSTER_FP64: "cluster_fp64",
@@ -51,7 +53,9 @@ class Params(ctypes.Structure):
         ("device", _i32),
         ("tile_groups", _i32),
         ("build_groups", _i32),
-        ("reserved", _i32 * 5),
+        ("tuning_samples", _i32),
+        ("tuning_interval", _i32),
+        ("reserved", _i32 * 3),
     ]
 
 
@@ -83,6 +87,7 @@ SIGNATURES = {
     "ljg_last_error": (ctypes.c_char_p, [_p]),
     "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
     "ljg_set_box": (_i32, [_p, _p, _p]),
+    "ljg_get_tuning": (_i32, [_p, _p]),
     "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "ljg_set_forces": (_i32, [_p, _p, _p, _p]),
@@ -201,11 +206,21 @@ def read_checkpoint(path):
     return dict(zip(STATE_KEYS, arrs)), t, i
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("candidates", _i32), ("best", _i32), ("phases", _i32), ("tuning", _i32),
+        ("kernel", _i32 * MAX_TUNE_CANDIDATES), ("cluster_size", _i32 * MAX_TUNE_CANDIDATES),
+        ("tile_groups", _i32 * MAX_TUNE_CANDIDATES), ("build_groups", _i32 * MAX_TUNE_CANDIDATES),
+        ("effective_ms", _d * MAX_TUNE_CANDIDATES),
+    ]
+
+
 class Engine:
     """One periodic box on one GPU."""
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
-                 dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0):
+                 dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
+                 tuning_interval=0):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -219,6 +234,8 @@ class Engine:
         p.device = device
         p.tile_groups = tile_groups
         p.build_groups = build_groups
+        p.tuning_samples = tuning_samples
+        p.tuning_interval = tuning_interval
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))
@@ -253,6 +270,15 @@ class Engine:
         buf = ctypes.create_string_buffer(256)
         self._check(self.lib.ljg_device_name(self.h, buf, 256), "ljg_device_name")
         return buf.value.decode()
+
+    def tuning(self):
+        """State of the layout tuning (kernel=KERNEL_AUTO): candidates with their scores and the choice."""
+        t = Tuning()
+        self._check(self.lib.ljg_get_tuning(self.h, ctypes.byref(t)), "ljg_get_tuning")
+        n = t.candidates
+        cands = [dict(kernel=t.kernel[i], cluster_size=t.cluster_size[i], tile_groups=t.tile_groups[i],
+                      build_groups=t.build_groups[i], effective_ms=t.effective_ms[i]) for i in range(n)]
+        return dict(best=t.best, phases=t.phases, tuning=bool(t.tuning), candidates=cands)
 
     def set_box(self, box_min, box_max):
         """Move the context to a new periodic box; the particles are dropped (upload the next system)."""

@@ -43,6 +43,10 @@ typedef int ljg_status;
                                       the members' neighbour lists), fixed-point positions, FP32 pair term,
                                       FP64 per-particle sums; boxes > 2 (rc + skin) */
 #define LJG_KERNEL_LAST 4
+/* ljg_step only: tune the layout while stepping (AutoPas's tuning phases, AutoTuner.cpp:85-175) and use
+ * the fastest; see ljg_params.tuning_samples / tuning_interval and ljg_get_tuning. */
+#define LJG_KERNEL_AUTO 100
+#define LJG_MAX_TUNE_CANDIDATES 16
 
 typedef struct {
   double box_min[3];
@@ -56,7 +60,11 @@ typedef struct {
   int32_t device;         /* HIP device ordinal */
   int32_t tile_groups;    /* i-clusters per wavefront in the FP32 cluster kernels; 0 = default per cluster size */
   int32_t build_groups;   /* i-clusters per wavefront in the j-row expansion pass; 0 = default per cluster size */
-  int32_t reserved[5];
+  int32_t tuning_samples; /* LJG_KERNEL_AUTO: steps per candidate in a tuning phase (AutoPas maxSamples);
+                             0 = max(3, rebuild_frequency) */
+  int32_t tuning_interval; /* LJG_KERNEL_AUTO: steps between tuning phases (AutoPas tuningInterval); 0 = tune
+                              once */
+  int32_t reserved[3];
 } ljg_params;
 
 typedef struct {
@@ -75,6 +83,21 @@ typedef struct {
   double rebuild_ms;         /* summed HIP-event time of rebuilds (bin/sort/list) (when timing is on) */
   double wall_ms;            /* host wall time of the call */
 } ljg_run_stats;
+
+/* Result of the layout tuning (LJG_KERNEL_AUTO): the candidates, their scores (ms per step: rebuild time
+ * / rebuild_frequency + the rest of the step, as AutoTuner::estimateRuntimeFromSamples with the mean
+ * selector; -1 if not measured, e.g. the box is too small for that kernel) and the chosen one. */
+typedef struct {
+  int32_t candidates;
+  int32_t best;    /* index of the chosen candidate, -1 before the first tuning phase ends */
+  int32_t phases;  /* tuning phases started */
+  int32_t tuning;  /* 1 while a tuning phase is in progress */
+  int32_t kernel[LJG_MAX_TUNE_CANDIDATES];
+  int32_t cluster_size[LJG_MAX_TUNE_CANDIDATES];
+  int32_t tile_groups[LJG_MAX_TUNE_CANDIDATES];
+  int32_t build_groups[LJG_MAX_TUNE_CANDIDATES];
+  double effective_ms[LJG_MAX_TUNE_CANDIDATES];
+} ljg_tuning;
 
 /* Create/destroy a context.  Replaces AutoPas::init() for the container this engine provides
  * (AutoPasImpl.h:55-113; LogicHandler.h:58-100, checkMinimalSize :967-983). */
@@ -118,6 +141,9 @@ ljg_status ljg_force(ljg_ctx *ctx, int32_t kernel, int32_t calc_globals, ljg_glo
 /* nsteps velocity-Verlet steps (TimeDiscretization.cpp:17-69, :152-165) with the rebuild policy of
  * LogicHandler.h:987-1015, all on the device.  out may be NULL. */
 ljg_status ljg_step(ljg_ctx *ctx, int32_t nsteps, double dt, int32_t kernel, ljg_run_stats *out);
+
+/* Layout tuning state and scores (AutoTuner's evidence, AutoTuner.cpp:250-369). */
+ljg_status ljg_get_tuning(ljg_ctx *ctx, ljg_tuning *out);
 
 /* Turn HIP-event timing of force kernels and rebuilds on/off (events on the context's stream). */
 ljg_status ljg_set_timing(ljg_ctx *ctx, int32_t enabled);

@@ -81,6 +81,7 @@ def test_params_struct_matches_header():
 def test_kernel_ids_match_header():
     text = open(os.path.join(ROOT, "include", "ljgpu.h")).read()
     ids = dict((k, int(v)) for k, v in re.findall(r"#define LJG_KERNEL_(\w+) (\d+)", text))
+    assert ids.pop("AUTO") == engine.KERNEL_AUTO
     assert ids.pop("LAST") == max(ids.values())
     for name, val in ids.items():
         assert getattr(engine, "KERNEL_" + name) == val, name

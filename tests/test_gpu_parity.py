@@ -414,3 +414,35 @@ def test_checkpoint_restart_continues_the_trajectory(tmp_path):
     d -= L * np.round(d / L)
     assert np.max(np.abs(d)) < 1e-10
     assert np.max(np.abs(out["vx"] - ref["vx"])) < 1e-9
+
+
+def test_auto_layout_tuning():
+    # KERNEL_AUTO: a tuning phase samples every candidate that fits the box for tuning_samples steps of
+    # the real run, picks the lowest score and keeps it; a second phase starts after tuning_interval.
+    announce("layout tuning: 20^3 lattice, 6 candidates x 4 steps, interval 10")
+    s = lattice_box(20, 0.75, seed=6, temperature=0.9)
+    L = s["L"]
+    with make_engine(L, M=4, rebuild_frequency=4, tuning_samples=4, tuning_interval=10) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        t0 = e.tuning()
+        assert t0["best"] == -1 and t0["phases"] == 0
+        rs = e.step(24, 0.002, ap.KERNEL_AUTO)  # exactly one phase: 6 candidates x 4 steps
+        assert rs.steps == 24
+        t1 = e.tuning()
+        n = len(t1["candidates"])
+        assert not t1["tuning"] and t1["phases"] == 1 and 0 <= t1["best"] < n
+        scores = [c["effective_ms"] for c in t1["candidates"]]
+        assert all(x > 0 for x in scores)  # the 22-sigma box fits every candidate
+        assert scores[t1["best"]] == min(scores)
+        rs = e.step(16, 0.002, ap.KERNEL_AUTO)  # 10 steps with the choice reach the interval, then a new phase
+        assert rs.steps == 16
+        assert e.tuning()["phases"] == 2 and e.tuning()["tuning"]
+        st = e.download()
+    # the run stays on the physical trajectory: compare with the FP64 linked-cells kernel over 40 steps
+    with make_engine(L, M=4, rebuild_frequency=4) as e:
+        e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+        e.step(40, 0.002, ap.KERNEL_LC_FP64)
+        ref = e.download()
+    d = np.stack([st["x"] - ref["x"], st["y"] - ref["y"], st["z"] - ref["z"]])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 2e-5
