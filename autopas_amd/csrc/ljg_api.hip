@@ -78,11 +78,11 @@ static bool is_cluster_kernel(int k) {
 }
 
 static void rebuild_for(Ctx &c, int kernel) {
-  wrap_positions(c);
-  if (is_cluster_kernel(kernel)) {
+  if (is_cluster_kernel(kernel)) {  // the tower histogram wraps the positions
     c.want_jrows = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
     vcl_build(c);
   } else {
+    wrap_positions(c);
     lc_bin(c);
     if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_build(c);
   }
@@ -385,6 +385,8 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 
 // nsteps velocity-Verlet steps with one kernel (LogicHandler.h:987-1015 rebuild policy); the statistics
 // are added to rs.  HIP-event timing of force launches and rebuilds when `timing`.
+constexpr int32_t kTimingStride = 8;
+
 static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool timing, ljg_run_stats &rs) {
   std::vector<hipEvent_t> fev, rev;  // pairs (start, stop) per force launch / rebuild
   auto new_event = [](std::vector<hipEvent_t> &v) {
@@ -433,15 +435,18 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
       LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
     }
     const int32_t next_stamp = fused && !last ? ++c.step_stamp : 0;
+    // Each event is a marker the GPU processes between kernels (~5 us), so force launches are timed on
+    // every kTimingStride-th step only; the statistics report the mean over the timed launches.
+    const bool time_force = timing && s % kTimingStride == 0;
     auto force_now = [&]() {
-      if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+      if (time_force) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
       if (fused) {
         const IntegArgs ia = integ_args(c, true, !last, dt, next_stamp);
         cluster_jrows_force(c, false, &ia);
       } else {
         run_force(c, kernel, false);
       }
-      if (timing) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
+      if (time_force) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
     };
     if (c.steps_since_rebuild >= c.prm.rebuild_frequency) {
       rebuild_now();
@@ -455,7 +460,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
       c.skip = SkipIf{nullptr, 0};
       LJG_HIP(hipEventSynchronize(c.ev_drift));
       if (static_cast<volatile int32_t *>(c.host_flag)[0] == stamp) {
-        if (timing) {  // the skipped launch is not a force evaluation
+        if (time_force) {  // the skipped launch is not a force evaluation
           for (int k = 0; k < 2; ++k) {
             LJG_HIP(hipEventDestroy(fev.back()));
             fev.pop_back();

@@ -34,7 +34,8 @@ struct HipError {
     if (e_ != hipSuccess) throw ::ljg::HipError{e_, #call, __LINE__}; \
   } while (0)
 
-// Grow-only device buffer.
+// Grow-only device buffer.  Growth leaves half again as much room: freeing device memory waits for the
+// device, so a rebuild that reallocates stalls the stream (lists lengthen while a lattice melts).
 template <class T>
 struct DevBuf {
   T *p = nullptr;
@@ -43,7 +44,7 @@ struct DevBuf {
     if (n <= cap) return;
     if (p) LJG_HIP(hipFree(p));
     p = nullptr;
-    size_t c = n + n / 4 + 64;
+    size_t c = n + n / 2 + 64;
     LJG_HIP(hipMalloc(&p, c * sizeof(T)));
     cap = c;
   }
@@ -129,6 +130,25 @@ struct IntegArgs {
   double kick_fac[kMaxTypes];   // dt / (2m)
   double drift_fac[kMaxTypes];  // dt * dt / (2m)
 };
+
+// Periodic wrap of positions, as md-flexible's RegularGridDecomposition.cpp:563-587: a coordinate below
+// the box moves up by L (at most to the largest value below the upper bound), one at or above the upper
+// bound moves down by L (at least to the lower bound).
+struct WrapArgs {
+  double mn[3], mx[3], L[3], just_inside[3];
+};
+WrapArgs wrap_args(const ljg_params &p);
+__device__ inline double wrap_coord(double p, int d, const WrapArgs &w) {
+  if (p < w.mn[d]) {
+    const double q = p + w.L[d];
+    return q < w.just_inside[d] ? q : w.just_inside[d];
+  }
+  if (p >= w.mx[d]) {
+    const double q = p - w.L[d];
+    return q > w.mn[d] ? q : w.mn[d];
+  }
+  return p;
+}
 
 // LJ parameters (mixing tables are ntypes x ntypes, row-major).
 struct LJParams {

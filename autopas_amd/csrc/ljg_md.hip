@@ -230,39 +230,31 @@ IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
 
 
 // Periodic wrap of particles that left the box, RegularGridDecomposition.cpp:563-587.
-struct WrapArgs {
-  double mn[3], mx[3], L[3], just_inside[3];
-};
-
 __global__ void k_wrap(int64_t nslots, StateView s, WrapArgs w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots || s.orig[i] < 0) return;
   double *c[3] = {s.x, s.y, s.z};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double p = c[d][i];
-    if (p < w.mn[d]) {
-      const double q = p + w.L[d];
-      p = q < w.just_inside[d] ? q : w.just_inside[d];
-      c[d][i] = p;
-    } else if (p >= w.mx[d]) {
-      const double q = p - w.L[d];
-      p = q > w.mn[d] ? q : w.mn[d];
-      c[d][i] = p;
-    }
+    const double p = c[d][i], q = wrap_coord(p, d, w);
+    if (q != p) c[d][i] = q;
   }
 }
 
-void wrap_positions(Ctx &c) {
+WrapArgs wrap_args(const ljg_params &p) {
   WrapArgs w{};
   for (int d = 0; d < 3; ++d) {
-    w.mn[d] = c.prm.box_min[d];
-    w.mx[d] = c.prm.box_max[d];
-    w.L[d] = c.prm.box_max[d] - c.prm.box_min[d];
-    w.just_inside[d] = std::nextafter(c.prm.box_max[d], c.prm.box_min[d]);
+    w.mn[d] = p.box_min[d];
+    w.mx[d] = p.box_max[d];
+    w.L[d] = p.box_max[d] - p.box_min[d];
+    w.just_inside[d] = std::nextafter(p.box_max[d], p.box_min[d]);
   }
+  return w;
+}
+
+void wrap_positions(Ctx &c) {
   hipLaunchKernelGGL(k_wrap, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), w);
+                     view(c.st[c.cur]), wrap_args(c.prm));
   LJG_HIP(hipGetLastError());
 }
 

@@ -23,12 +23,23 @@ __device__ inline int32_t tower_index(const TowerGeom &g, double px, double py) 
   return t2[0] + t2[1] * g.nt[0];
 }
 
-__global__ void k_tower_hist(int64_t nslots, const double *__restrict__ x, const double *__restrict__ y,
-                             const int32_t *__restrict__ orig, TowerGeom g, int32_t *__restrict__ tower_of,
-                             int32_t *__restrict__ rank, int32_t *__restrict__ count) {
+// Also wraps the positions into the box first (the rebuild's wrap, RegularGridDecomposition.cpp:563-587).
+__global__ void k_tower_hist(int64_t nslots, double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                             const int32_t *__restrict__ orig, TowerGeom g, WrapArgs w,
+                             int32_t *__restrict__ tower_of, int32_t *__restrict__ rank, int32_t *__restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < nslots && orig[i] >= 0;
-  const int32_t t = active ? tower_index(g, x[i], y[i]) : -1;
+  double px = 0., py = 0.;
+  if (active) {
+    const double ox = x[i], oy = y[i], oz = z[i];
+    px = wrap_coord(ox, 0, w);
+    py = wrap_coord(oy, 1, w);
+    const double pz = wrap_coord(oz, 2, w);
+    if (px != ox) x[i] = px;
+    if (py != oy) y[i] = py;
+    if (pz != oz) z[i] = pz;
+  }
+  const int32_t t = active ? tower_index(g, px, py) : -1;
   const int32_t r = wave_aggregated_rank(t, active, count);
   if (i < nslots) {
     tower_of[i] = t;
@@ -45,10 +56,62 @@ __global__ void k_tower_scatter(int64_t nslots, const int32_t *__restrict__ towe
   items[start[t] + rank[i]] = (int32_t)i;
 }
 
-__global__ void k_tower_nclusters(int64_t ntowers, const int32_t *__restrict__ count, int32_t M,
-                                  int32_t *__restrict__ ccount) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ntowers) ccount[t] = (count[t] + M - 1) / M;
+// One workgroup: exclusive scans of the tower sizes (particle offsets) and of the clusters per tower,
+// ceil(count / M), with the totals at index ntowers; stats = {particles, longest tower, clusters}.  Each thread scans a contiguous run of towers, then the run totals are
+// scanned in LDS.
+constexpr int kLayoutThreads = 1024;
+
+__global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers, const int32_t *__restrict__ count,
+                                                                 int32_t M, int64_t *__restrict__ start,
+                                                                 int32_t *__restrict__ ccount,
+                                                                 int64_t *__restrict__ cstart,
+                                                                 unsigned long long *__restrict__ stats) {
+  __shared__ int64_t sp[kLayoutThreads], sc[kLayoutThreads];
+  __shared__ int32_t sm[kLayoutThreads];
+  const int tid = threadIdx.x;
+  const int64_t per = (ntowers + kLayoutThreads - 1) / kLayoutThreads;
+  const int64_t b = tid * per, e = b + per < ntowers ? b + per : ntowers;
+  int64_t np = 0, nc = 0;
+  int32_t mx = 0;
+  for (int64_t t = b; t < e; ++t) {
+    const int32_t k = count[t];
+    np += k;
+    nc += (k + M - 1) / M;
+    mx = k > mx ? k : mx;
+  }
+  sp[tid] = np;
+  sc[tid] = nc;
+  sm[tid] = mx;
+  __syncthreads();
+  // inclusive scan of the run totals (Hillis-Steele), maximum by tree
+  for (int off = 1; off < kLayoutThreads; off <<= 1) {
+    const int64_t ap = tid >= off ? sp[tid - off] : 0, ac = tid >= off ? sc[tid - off] : 0;
+    __syncthreads();
+    sp[tid] += ap;
+    sc[tid] += ac;
+    __syncthreads();
+  }
+  for (int s = kLayoutThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) sm[tid] = sm[tid + s] > sm[tid] ? sm[tid + s] : sm[tid];
+    __syncthreads();
+  }
+  int64_t op = sp[tid] - np, oc = sc[tid] - nc;
+  for (int64_t t = b; t < e; ++t) {
+    const int32_t k = count[t];
+    const int32_t kc = (k + M - 1) / M;
+    start[t] = op;
+    cstart[t] = oc;
+    ccount[t] = kc;
+    op += k;
+    oc += kc;
+  }
+  if (tid == kLayoutThreads - 1) {
+    start[ntowers] = sp[tid];
+    cstart[ntowers] = sc[tid];
+    stats[0] = (unsigned long long)sp[tid];
+    stats[2] = (unsigned long long)sc[tid];
+  }
+  if (tid == 0) stats[1] = (unsigned long long)sm[0];
 }
 
 // One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
@@ -70,7 +133,7 @@ template <int CAP>
 __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
     int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
     const double *__restrict__ z, const int64_t *__restrict__ id, const int64_t *__restrict__ cstart, int32_t M,
-    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, int32_t *__restrict__ err) {
+    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err) {
   __shared__ double sz[CAP];
   __shared__ int32_t ss[CAP];
   const int64_t t = blockIdx.x;
@@ -80,7 +143,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
   if (CAP > kSortSmall && n <= kSortSmall) return;  // sorted by the small instance
   if (CAP == kSortSmall && n > kSortSmall) return;  // left to the large instance
   if (n > CAP) {
-    if (threadIdx.x == 0) err[1] = 1;
+    if (threadIdx.x == 0) err[0] = 1ull;
     return;
   }
   int P = 1;
@@ -460,10 +523,6 @@ void vcl_pos32(Ctx &c) {
   LJG_HIP(hipGetLastError());
 }
 
-__global__ void k_put_i64(const int64_t *__restrict__ from, unsigned long long *__restrict__ to) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) to[0] = (unsigned long long)from[0];
-}
-
 __global__ void k_zero_u64(int64_t n, unsigned long long *__restrict__ p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -518,24 +577,16 @@ void vcl_build(Ctx &c) {
   c.ctow_start.ensure(ntow + 1);
   c.flag.ensure(4);
   fill_i32(c, c.tower_count.p, ntow + 1, 0);
-  fill_i32(c, c.flag.p + 1, 1, 0);
-  hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.orig.p, g,
-                     c.tower_of.p, c.rank.p, c.tower_count.p);
+  hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
+                     s.orig.p, g, wrap_args(c.prm), c.tower_of.p, c.rank.p, c.tower_count.p);
   LJG_HIP(hipGetLastError());
-  exclusive_scan_i32_to_i64(c, c.tower_count.p, c.tower_start.p, ntow);
+  // tower offsets, clusters per tower and their offsets, and the sizes for the one synchronisation
+  c.tstats.ensure(4);
+  hipLaunchKernelGGL(k_tower_layout, dim3(1), dim3(kLayoutThreads), 0, c.stream, ntow, c.tower_count.p, M,
+                     c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
+  LJG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.tower_of.p, c.rank.p,
                      c.tower_start.p, c.tower_items.p);
-  LJG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_tower_nclusters, dim3(blocks_for(ntow, 256)), dim3(256), 0, c.stream, ntow, c.tower_count.p, M,
-                     c.ctow_count.p);
-  LJG_HIP(hipGetLastError());
-  exclusive_scan_i32_to_i64(c, c.ctow_count.p, c.ctow_start.p, ntow);
-  // one synchronisation for the sizes: the cluster count and the longest tower
-  c.tstats.ensure(4);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.tstats.p);
-  LJG_HIP(hipGetLastError());
-  count_stats(c, c.tower_count.p, ntow, c.tstats.p);
-  hipLaunchKernelGGL(k_put_i64, dim3(1), dim3(1), 0, c.stream, c.ctow_start.p + ntow, c.tstats.p + 2);
   LJG_HIP(hipGetLastError());
   unsigned long long ts[3] = {0, 0, 0};
   LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
@@ -547,14 +598,19 @@ void vcl_build(Ctx &c) {
   const int64_t new_slots = ncl * M;
   c.perm.ensure(new_slots);
   c.cluster_tower.ensure(ncl);
+  // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
+  // pass, [3], [4] total and longest j-particle row, [5] sort tile exceeded; read back once at the end
+  c.list_stats.ensure(6);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)6, c.list_stats.p);
+  LJG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
                      c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p,
-                     c.flag.p);
+                     c.list_stats.p + 5);
   LJG_HIP(hipGetLastError());
   if (ts[1] > (unsigned long long)kSortSmall) {
     hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
                        c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p,
-                       c.cluster_tower.p, c.flag.p);
+                       c.cluster_tower.p, c.list_stats.p + 5);
     LJG_HIP(hipGetLastError());
   }
   const bool jrows = c.want_jrows;
@@ -572,9 +628,6 @@ void vcl_build(Ctx &c) {
                      t.orig.p, c.bbox.p, c.c0.p);
   LJG_HIP(hipGetLastError());
   c.list_count.ensure(ncl * kJclSplit + 1);
-  // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
-  // pass, [3], [4] total and longest j-particle row
-  c.list_stats.ensure(5);
   if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
   if (jrows) c.jcount.ensure(ncl + 1);
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
@@ -618,12 +671,10 @@ void vcl_build(Ctx &c) {
       LJG_HIP(hipGetLastError());
       count_stats(c, c.jcount.p, ncl, c.list_stats.p + 3);
     }
-    unsigned long long st[5] = {0, 0, 0, 0, 0};
-    int32_t sort_err = 0;
+    unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
     LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
-    LJG_HIP(hipMemcpyAsync(&sort_err, c.flag.p + 1, sizeof sort_err, hipMemcpyDeviceToHost, c.stream));
     LJG_HIP(hipStreamSynchronize(c.stream));
-    if (sort_err) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
+    if (st[5]) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
     const int32_t maxc = (int32_t)st[1], maxj = (int32_t)st[4];
     const int32_t split = jrows ? kJclSplit : 1;  // first-pass rows hold `split` sub-rows of cap/split
     const bool ok1 = maxc <= cap / split, ok2 = !jrows || maxj <= jcap;

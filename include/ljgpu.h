@@ -78,7 +78,8 @@ typedef struct {
   int64_t steps;
   int64_t rebuilds;
   int64_t pair_interactions; /* sum over steps of the unique pairs within rc */
-  double force_ms;           /* summed HIP-event time of the force kernel launches (when timing is on) */
+  double force_ms;           /* summed HIP-event time of the timed force launches (when timing is on: every
+                                8th step's launch, so that the event markers stay off most steps) */
   int64_t force_launches;
   double rebuild_ms;         /* summed HIP-event time of rebuilds (bin/sort/list) (when timing is on) */
   double wall_ms;            /* host wall time of the call */
