@@ -34,6 +34,7 @@ SOURCES = {
     "ljg_md.hip": ["-ffp-contract=off"],
     "ljg_force.hip": [],
     "ljg_verlet.hip": ["-ffp-contract=off"],
+    "ljg_ensemble.hip": ["-ffp-contract=off"],
     "ljg_generate.cpp": ["-ffp-contract=off"],
     "ljg_checkpoint.cpp": [],
 }

@@ -100,6 +100,8 @@ static bool layout_matches(const Ctx &c, int kernel) {
 // The fixed-point kernels rely on minimum-image differences: the Verlet-list kernel also on 27 distinct
 // neighbour cells (box >= 3 ilen), the j-row cluster kernel on a single image within ilen (box > 2 ilen).
 static const char *kernel_unsupported(const Ctx &c, int kernel) {
+  if (c.nboxes > 0 && kernel != LJG_KERNEL_CLUSTER_MIXED)
+    return "an ensemble of boxes runs the FP32 cluster kernel only";
   const double ilen = c.prm.cutoff + c.prm.skin;
   for (int d = 0; d < 3; ++d) {
     const double L = c.prm.box_max[d] - c.prm.box_min[d];
@@ -265,11 +267,78 @@ ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max
     c.prm.box_min[d] = box_min[d];
     c.prm.box_max[d] = box_max[d];
   }
+  c.nboxes = 0;
+  c.box_glob_valid = false;
   c.n = 0;
   c.nslots = 0;
   c.layout = Layout::None;
   c.have_forces = false;
   c.steps_since_rebuild = 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, const double *box_max,
+                         const int64_t *count) {
+  if (!h || nboxes < 1 || !box_min || !box_max || !count) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  int64_t total = 0;
+  for (int32_t b = 0; b < nboxes; ++b) {
+    if (count[b] < 1) return fail(c, LJG_ERR_ARG, "a box without particles");
+    for (int d = 0; d < 3; ++d)
+      if (!(box_max[3 * b + d] - box_min[3 * b + d] >= ilen)) return fail(c, LJG_ERR_ARG, "box shorter than rc + skin");
+    total += count[b];
+  }
+  if (total > (int64_t)INT32_MAX / 2) return fail(c, LJG_ERR_ARG, "too many particles");
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.box_lo.assign(box_min, box_min + 3 * (size_t)nboxes);
+  c.box_hi.assign(box_max, box_max + 3 * (size_t)nboxes);
+  c.box_count.assign(count, count + nboxes);
+  std::vector<int64_t> pstart((size_t)nboxes + 1);
+  std::vector<WrapArgs> wr((size_t)nboxes);
+  pstart[0] = 0;
+  for (int32_t b = 0; b < nboxes; ++b) {
+    pstart[b + 1] = pstart[b] + count[b];
+    wr[b] = wrap_args(&box_min[3 * b], &box_max[3 * b]);
+  }
+  c.box_pstart.ensure(pstart.size());
+  c.box_wrap.ensure(wr.size());
+  LJG_HIP(hipMemcpyAsync(c.box_pstart.p, pstart.data(), pstart.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                         c.stream));
+  LJG_HIP(hipMemcpyAsync(c.box_wrap.p, wr.data(), wr.size() * sizeof(WrapArgs), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  // the single-box parameters keep the first box (the fixed-point and cell geometry never run here)
+  for (int d = 0; d < 3; ++d) {
+    c.prm.box_min[d] = box_min[d];
+    c.prm.box_max[d] = box_max[d];
+  }
+  c.nboxes = nboxes;
+  c.box_geom_M = 0;
+  c.box_glob_valid = false;
+  c.n = 0;
+  c.nslots = 0;
+  c.layout = Layout::None;
+  c.have_forces = false;
+  c.steps_since_rebuild = 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_get_box_globals(ljg_ctx *h, int32_t nboxes, double *potential_energy, double *virial,
+                               int64_t *pair_count) {
+  if (!h) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.nboxes == 0 || nboxes != c.nboxes) return fail(c, LJG_ERR_ARG, "not an ensemble of that many boxes");
+  if (!c.box_glob_valid) return fail(c, LJG_ERR_STATE, "no force evaluation with globals since the last change");
+  std::vector<double> g(3 * (size_t)nboxes);
+  LJG_HIP(hipMemcpyAsync(g.data(), c.box_glob.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  for (int32_t b = 0; b < nboxes; ++b) {
+    if (potential_energy) potential_energy[b] = g[3 * b];
+    if (virial) virial[b] = g[3 * b + 1];
+    if (pair_count) pair_count[b] = (int64_t)std::llround(g[3 * b + 2]);
+  }
   return LJG_OK;
   LJG_API_END
 }
@@ -282,6 +351,12 @@ ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, c
       if (type[i] < 0 || type[i] >= h->c.lj.ntypes) return LJG_ERR_ARG;
   }
   LJG_API_BEGIN(h)
+  if (c.nboxes > 0) {
+    int64_t total = 0;
+    for (int64_t k : c.box_count) total += k;
+    if (n != total) return fail(c, LJG_ERR_ARG, "particle count differs from the ensemble's boxes");
+  }
+  c.box_glob_valid = false;
   c.cur = 0;
   c.st[0].ensure((size_t)n);
   c.n = n;
@@ -311,7 +386,8 @@ ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, c
   hipLaunchKernelGGL(k_init_upload, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, view(s), did,
                      (int32_t)(id != nullptr));
   LJG_HIP(hipGetLastError());
-  wrap_positions(c);
+  if (c.nboxes > 0) ensemble_wrap(c);
+  else wrap_positions(c);
   LJG_HIP(hipStreamSynchronize(c.stream));
   c.layout = Layout::Upload;
   c.have_forces = false;
@@ -342,6 +418,7 @@ ljg_status ljg_set_forces(ljg_ctx *h, const double *fx, const double *fy, const 
 ljg_status ljg_bin(ljg_ctx *h) {
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (c.nboxes > 0) return fail(c, LJG_ERR_UNSUPPORTED, "linked cells on an ensemble of boxes");
   rebuild_for(c, LJG_KERNEL_LC_FP64);
   LJG_HIP(hipStreamSynchronize(c.stream));
   return LJG_OK;
@@ -364,6 +441,8 @@ ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globa
   if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
   if (!layout_matches(c, kernel)) rebuild_for(c, kernel);
   run_force(c, kernel, calc_globals != 0);
+  if (c.nboxes > 0 && calc_globals) ensemble_box_globals(c);
+  c.box_glob_valid = c.nboxes > 0 && calc_globals;
   ljg_globals g{};
   reduce_partials(c, partial_items(c, kernel), &g, calc_globals != 0);
   g.candidate_evals = candidate_evals(c, kernel);
@@ -613,6 +692,7 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
   const auto t0 = std::chrono::steady_clock::now();
   ljg_run_stats rs{};
+  c.box_glob_valid = false;
   if (kernel == LJG_KERNEL_AUTO) {
     if (next_candidate(c, 0) < 0) return fail(c, LJG_ERR_UNSUPPORTED, "no tuning candidate fits this box");
     run_auto(c, nsteps, dt, rs);

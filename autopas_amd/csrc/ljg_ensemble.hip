@@ -1,0 +1,157 @@
+// ljg_ensemble.hip -- many independent periodic boxes in one context.
+//
+// md-flexible runs one AutoPas instance per simulation (AutoPasImpl.h:55-113), each with its own tower
+// grid (ClusterTowerBlock2D.h:140-168), cluster lists and rebuilds.  Here the boxes' particles are
+// concatenated box by box and every launch covers all boxes:
+//   * each box gets the tower grid it would have alone (make_tower_geom on its own box and count), and
+//     the boxes' towers are numbered one box after the other, so clusters and slots stay box-major;
+//   * the tower histogram wraps each particle into its own box (RegularGridDecomposition.cpp:563-587);
+//   * the cluster-pair list pass reads the grid of the tower's box (k_cluster_lists_tower<.., MB>), and
+//     its list entries carry the image offset c0_j + shift - c0_i formed with that box's lengths, so the
+//     FP32 cluster force kernel runs on the ensemble unchanged;
+//   * per-box potential energy, virial and pair count are reduced from per-cluster sums in a fixed order.
+// Compiled with -ffp-contract=off like the single-box build: a box's grid, wrap and tower indices round
+// exactly as they do when the box is alone.
+#include <cmath>
+
+#include "ljg_internal.h"
+
+namespace ljg {
+
+// Box of upload index o: the last b with pstart[b] <= o.
+__device__ inline int32_t box_of(int64_t o, int32_t nboxes, const int64_t *__restrict__ pstart) {
+  int32_t lo = 0, hi = nboxes - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (pstart[mid] <= o) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Tower histogram over the ensemble: wrap into the particle's own box, then that box's tower index
+// offset by the box's first tower.
+__global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, double *__restrict__ y,
+                                   double *__restrict__ z, const int32_t *__restrict__ orig, int32_t nboxes,
+                                   const int64_t *__restrict__ pstart, const TowerGeom *__restrict__ tg,
+                                   const WrapArgs *__restrict__ wr, const int64_t *__restrict__ tbase,
+                                   int32_t *__restrict__ tower_of, int32_t *__restrict__ rank,
+                                   int32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < nslots && orig[i] >= 0;
+  int32_t t = -1;
+  if (active) {
+    const int32_t b = box_of(orig[i], nboxes, pstart);
+    const WrapArgs w = wr[b];
+    const double ox = x[i], oy = y[i], oz = z[i];
+    const double px = wrap_coord(ox, 0, w), py = wrap_coord(oy, 1, w), pz = wrap_coord(oz, 2, w);
+    if (px != ox) x[i] = px;
+    if (py != oy) y[i] = py;
+    if (pz != oz) z[i] = pz;
+    t = (int32_t)(tbase[b] + tower_index(tg[b], px, py));
+  }
+  const int32_t r = wave_aggregated_rank(t, active, count);
+  if (i < nslots) {
+    tower_of[i] = t;
+    rank[i] = r;
+  }
+}
+
+__global__ void k_wrap_boxes(int64_t nslots, StateView s, int32_t nboxes, const int64_t *__restrict__ pstart,
+                             const WrapArgs *__restrict__ wr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots || s.orig[i] < 0) return;
+  const WrapArgs w = wr[box_of(s.orig[i], nboxes, pstart)];
+  double *c[3] = {s.x, s.y, s.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double p = c[d][i], q = wrap_coord(p, d, w);
+    if (q != p) c[d][i] = q;
+  }
+}
+
+// One wavefront per box: the sums of the box's i-clusters [cstart[tbase[b]], cstart[tbase[b+1]]) in a
+// fixed lane assignment and butterfly, then LJFunctor::endTraversal's scaling for full lists (halve
+// everything, energy / 6; LJFunctor.h:661-685).
+constexpr int kBoxReduceThreads = 256;
+
+__global__ __launch_bounds__(kBoxReduceThreads) void k_box_reduce(int32_t nboxes, const int64_t *__restrict__ tbase,
+                                                                  const int64_t *__restrict__ cstart,
+                                                                  const double *__restrict__ cl,
+                                                                  double *__restrict__ out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int32_t b = (int32_t)(blockIdx.x * (kBoxReduceThreads / kWave) + threadIdx.x / kWave);
+  if (b >= nboxes) return;
+  const int64_t cb = cstart[tbase[b]], ce = cstart[tbase[b + 1]];
+  double e = 0., v = 0., h = 0.;
+  for (int64_t ci = cb + lane; ci < ce; ci += kWave) {
+    e += cl[3 * ci];
+    v += cl[3 * ci + 1];
+    h += cl[3 * ci + 2];
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, kWave);
+    v += __shfl_xor(v, off, kWave);
+    h += __shfl_xor(h, off, kWave);
+  }
+  if (lane == 0) {
+    out[3 * b] = e * 0.5 / 6.;
+    out[3 * b + 1] = v * 0.5;
+    out[3 * b + 2] = h * 0.5;
+  }
+}
+
+void ensemble_geometry(Ctx &c) {
+  if (c.box_geom_M == c.M) return;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  const int32_t nb = c.nboxes;
+  std::vector<TowerGeom> tg((size_t)nb);
+  std::vector<int64_t> tbase((size_t)nb + 1);
+  tbase[0] = 0;
+  for (int32_t b = 0; b < nb; ++b) {
+    tg[b] = make_tower_geom(&c.box_lo[3 * b], &c.box_hi[3 * b], c.box_count[b], c.M, ilen);
+    tbase[b + 1] = tbase[b] + (int64_t)tg[b].nt[0] * tg[b].nt[1];
+  }
+  const int64_t ntow = tbase[nb];
+  if (ntow > INT32_MAX / 2) throw HipError{hipErrorInvalidValue, "too many towers in the ensemble", -1};
+  std::vector<int32_t> tbox((size_t)ntow);
+  for (int32_t b = 0; b < nb; ++b)
+    for (int64_t t = tbase[b]; t < tbase[b + 1]; ++t) tbox[t] = b;
+  c.box_tg.ensure((size_t)nb);
+  c.box_tbase.ensure((size_t)nb + 1);
+  c.tower_box.ensure((size_t)ntow);
+  LJG_HIP(hipMemcpyAsync(c.box_tg.p, tg.data(), tg.size() * sizeof(TowerGeom), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipMemcpyAsync(c.box_tbase.p, tbase.data(), tbase.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                         c.stream));
+  LJG_HIP(hipMemcpyAsync(c.tower_box.p, tbox.data(), tbox.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
+  c.box_ntowers = ntow;
+  c.tg = tg[0];
+  c.box_geom_M = c.M;
+}
+
+void ensemble_tower_hist(Ctx &c) {
+  State &s = c.st[c.cur];
+  const int64_t ns = c.nslots;
+  hipLaunchKernelGGL(k_tower_hist_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
+                     s.orig.p, c.nboxes, c.box_pstart.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, c.tower_of.p,
+                     c.rank.p, c.tower_count.p);
+  LJG_HIP(hipGetLastError());
+}
+
+void ensemble_wrap(Ctx &c) {
+  hipLaunchKernelGGL(k_wrap_boxes, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                     view(c.st[c.cur]), c.nboxes, c.box_pstart.p, c.box_wrap.p);
+  LJG_HIP(hipGetLastError());
+}
+
+void ensemble_box_globals(Ctx &c) {
+  c.box_glob.ensure(3 * (size_t)c.nboxes);
+  constexpr int per_block = kBoxReduceThreads / kWave;
+  hipLaunchKernelGGL(k_box_reduce, dim3(blocks_for(c.nboxes, per_block)), dim3(kBoxReduceThreads), 0, c.stream,
+                     c.nboxes, c.box_tbase.p, c.ctow_start.p, c.cl_glob.p, c.box_glob.p);
+  LJG_HIP(hipGetLastError());
+}
+
+}  // namespace ljg

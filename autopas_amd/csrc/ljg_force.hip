@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
-    SkipIf sk) {
+    SkipIf sk, double *__restrict__ clg) {
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
@@ -300,10 +300,13 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
           if ((real >> (q - 2 * M)) & 1u) fz[s0 + q - 2 * M] = val;
         } else if (q == 3 * M) {
           h_out = (int64_t)val;
+          if (GLOB && clg) clg[3 * (int64_t)ic + 2] = val;  // per-cluster sums for per-box globals
         } else if (GLOB && q == 3 * M + 1) {
           e_out = val;
+          if (clg) clg[3 * (int64_t)ic] = val;
         } else if (GLOB && q == 3 * M + 2) {
           v_out = val;
+          if (clg) clg[3 * (int64_t)ic + 1] = val;
         }
       }
     }
@@ -687,11 +690,16 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   c.part_hits.ensure(nb);
   c.part_items = nb;
   const float rc2 = (float)a.rc2;
+  double *clg = nullptr;
+  if (glob && c.nboxes > 0) {
+    c.cl_glob.ensure(3 * (size_t)c.nclusters);
+    clg = c.cl_glob.p;
+  }
 #define LJG_LAUNCH(MIXV, GLOBV)                                                                                    \
   hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,          \
                      c.nclusters, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,           \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
-                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
+                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, clg)
   if (mix) {
     if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
   } else {

@@ -102,6 +102,24 @@ struct TowerGeom {
   double ilen;
 };
 
+// Tower grid of a box with n particles for cluster size M (ClusterTowerBlock2D::estimateOptimalGridSideLength
+// and resize, ClusterTowerBlock2D.h:89-114,140-168, owned towers only).
+TowerGeom make_tower_geom(const double lo[3], const double hi[3], int64_t n, int32_t M, double ilen);
+
+// Tower of a position (ClusterTowerBlock2D::getTowerIndex2DAtPosition, ClusterTowerBlock2D.h:220-245).
+__device__ inline int32_t tower_index(const TowerGeom &g, double px, double py) {
+  const double p[2] = {px, py};
+  int32_t t2[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    long t = (long)floor((p[d] - g.box_min[d]) * g.recip[d]);
+    if (t < 0) t = 0;
+    if (t > g.nt[d] - 1) t = g.nt[d] - 1;
+    t2[d] = (int32_t)t;
+  }
+  return t2[0] + t2[1] * g.nt[0];
+}
+
 // 32-bit fixed-point coordinates for the Verlet-list kernels: X = round((x - min) * inv_scale) mod 2^32.
 struct FixGeom {
   double min[3], scale[3], inv_scale[3];  // scale = L / 2^32
@@ -138,6 +156,7 @@ struct WrapArgs {
   double mn[3], mx[3], L[3], just_inside[3];
 };
 WrapArgs wrap_args(const ljg_params &p);
+WrapArgs wrap_args(const double lo[3], const double hi[3]);
 __device__ inline double wrap_coord(double p, int d, const WrapArgs &w) {
   if (p < w.mn[d]) {
     const double q = p + w.L[d];
@@ -266,6 +285,22 @@ struct Ctx {
   bool carry_force = true;   // permutes keep the current force (false inside the step loop)
   int64_t last_pairs = 0;
   int64_t last_evals = 0;
+
+  // Ensemble of independent boxes (nboxes == 0: the single box of prm).  Upload order is box-major;
+  // towers, clusters and slots are too, each box's tower grid being the one it would have alone.
+  int32_t nboxes = 0;
+  std::vector<double> box_lo, box_hi;  // 3 per box
+  std::vector<int64_t> box_count;      // particles per box
+  int32_t box_geom_M = 0;              // cluster size the per-box tower grids were made for (0: none yet)
+  int64_t box_ntowers = 0;
+  DevBuf<TowerGeom> box_tg;            // per box
+  DevBuf<WrapArgs> box_wrap;           // per box
+  DevBuf<int64_t> box_pstart;          // nboxes + 1: upload-index range of each box
+  DevBuf<int64_t> box_tbase;           // nboxes + 1: first tower of each box
+  DevBuf<int32_t> tower_box;           // box of each tower
+  DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch
+  DevBuf<double> box_glob;             // per box (energy, virial, pairs) after endTraversal scaling
+  bool box_glob_valid = false;
 };
 
 // ---- launchers (defined in the .hip translation units) -------------------------------------------
@@ -290,6 +325,12 @@ void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx = 
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
+// Ensemble (ljg_ensemble.hip): per-box tower grids for cluster size c.M, the tower histogram that wraps
+// each position into its own box, the wrap alone, and the per-box reduction of cl_glob into box_glob.
+void ensemble_geometry(Ctx &c);
+void ensemble_tower_hist(Ctx &c);
+void ensemble_wrap(Ctx &c);
+void ensemble_box_globals(Ctx &c);
 
 constexpr int kShiftBits = 27;
 constexpr int32_t kClusterMask = (1 << kShiftBits) - 1;

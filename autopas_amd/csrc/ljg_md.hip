@@ -241,16 +241,18 @@ __global__ void k_wrap(int64_t nslots, StateView s, WrapArgs w) {
   }
 }
 
-WrapArgs wrap_args(const ljg_params &p) {
+WrapArgs wrap_args(const double lo[3], const double hi[3]) {
   WrapArgs w{};
   for (int d = 0; d < 3; ++d) {
-    w.mn[d] = p.box_min[d];
-    w.mx[d] = p.box_max[d];
-    w.L[d] = p.box_max[d] - p.box_min[d];
-    w.just_inside[d] = std::nextafter(p.box_max[d], p.box_min[d]);
+    w.mn[d] = lo[d];
+    w.mx[d] = hi[d];
+    w.L[d] = hi[d] - lo[d];
+    w.just_inside[d] = std::nextafter(hi[d], lo[d]);
   }
   return w;
 }
+
+WrapArgs wrap_args(const ljg_params &p) { return wrap_args(p.box_min, p.box_max); }
 
 void wrap_positions(Ctx &c) {
   hipLaunchKernelGGL(k_wrap, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,

@@ -10,17 +10,27 @@
 
 namespace ljg {
 
-__device__ inline int32_t tower_index(const TowerGeom &g, double px, double py) {
-  const double p[2] = {px, py};
-  int32_t t2[2];
-#pragma unroll
-  for (int d = 0; d < 2; ++d) {
-    long t = (long)floor((p[d] - g.box_min[d]) * g.recip[d]);
-    if (t < 0) t = 0;
-    if (t > g.nt[d] - 1) t = g.nt[d] - 1;
-    t2[d] = (int32_t)t;
+TowerGeom make_tower_geom(const double lo[3], const double hi[3], int64_t n, int32_t M, double ilen) {
+  TowerGeom g{};
+  double L[3];
+  for (int d = 0; d < 3; ++d) {
+    L[d] = hi[d] - lo[d];
+    g.box_min[d] = lo[d];
+    g.L[d] = L[d];
   }
-  return t2[0] + t2[1] * g.nt[0];
+  const double density = (double)n / (L[0] * L[1] * L[2]);
+  const double side0 = std::cbrt((double)M / density);
+  for (int d = 0; d < 2; ++d) {
+    long t = (long)std::ceil(L[d] / side0);
+    if (t < 1) t = 1;
+    g.nt[d] = (int32_t)t;
+    g.side[d] = L[d] / (double)t;
+    g.recip[d] = 1. / g.side[d];
+  }
+  const long r0 = (long)std::ceil(ilen / g.side[0]), r1 = (long)std::ceil(ilen / g.side[1]);
+  g.R = (int32_t)(r0 > r1 ? r0 : r1);
+  g.ilen = ilen;
+  return g;
 }
 
 // Also wraps the positions into the box first (the rebuild's wrap, RegularGridDecomposition.cpp:563-587).
@@ -278,9 +288,12 @@ __device__ inline void wave_sync() {
 // ranges; wave w writes its entries for i-cluster ci into sub-row w, [ci*cap + w*cap/NS, ...), with its
 // length in count[ci*NS + w], so the sub-rows read in order give the single-wave traversal order.
 // The cluster-entry lists (JCL = false, compared entry by entry with the oracle) use NS = 1.
-template <bool JCL, int NS>
+// MB (an ensemble of boxes): tower t belongs to box tbox[t], whose grid is tgs[box] and whose towers
+// start at tbase[box]; the traversal inside the box is the single-box one.
+template <bool JCL, int NS, bool MB>
 __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
-    int64_t ntowers, TowerGeom g, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
+    int64_t ntowers, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
+    const int32_t *__restrict__ tbox, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
     const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
     int32_t *__restrict__ jcl) {
   __shared__ double sbw[NS][kListLds * 6];
@@ -290,7 +303,15 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
   double4 *sc = scw[wv];
   const int64_t t = blockIdx.x;
   if (t >= ntowers) return;
-  const int tx = (int)(t % g.nt[0]), ty = (int)(t / g.nt[0]);
+  TowerGeom g = g1;
+  int64_t tb = 0;
+  if constexpr (MB) {
+    const int32_t b = tbox[t];
+    g = tgs[b];
+    tb = tbase[b];
+  }
+  const int64_t tl = t - tb;
+  const int tx = (int)(tl % g.nt[0]), ty = (int)(tl / g.nt[0]);
   const int64_t ib = cstart[t], ie = cstart[t + 1];
   const double il2 = g.ilen * g.ilen;
   const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
@@ -322,7 +343,7 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
       while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
       while (jy < 0) { jy += g.nt[1]; --sy; }
       while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
-      const int64_t tj = jx + (int64_t)jy * g.nt[0];
+      const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
       const int64_t cb = cstart[tj], ce = cstart[tj + 1];
       if (cb == ce) continue;
       const bool single = ce - cb <= kListLds;
@@ -543,27 +564,16 @@ void vcl_build(Ctx &c) {
   const int M = c.M;
   const double ilen = c.prm.cutoff + c.prm.skin;
   // Tower grid: ClusterTowerBlock2D::estimateOptimalGridSideLength (ClusterTowerBlock2D.h:140-168) with
-  // the owned towers only (periodic images come from shift vectors) and resize (:89-114).
-  TowerGeom &g = c.tg;
-  double L[3];
-  for (int d = 0; d < 3; ++d) {
-    L[d] = c.prm.box_max[d] - c.prm.box_min[d];
-    g.box_min[d] = c.prm.box_min[d];
-    g.L[d] = L[d];
+  // the owned towers only (periodic images come from shift vectors) and resize (:89-114); one per box of
+  // an ensemble, numbered box by box.
+  const bool ens = c.nboxes > 0;
+  if (ens) {
+    ensemble_geometry(c);
+  } else {
+    c.tg = make_tower_geom(c.prm.box_min, c.prm.box_max, c.n, M, ilen);
   }
-  const double density = (double)c.n / (L[0] * L[1] * L[2]);
-  const double side0 = std::cbrt((double)M / density);
-  for (int d = 0; d < 2; ++d) {
-    long t = (long)std::ceil(L[d] / side0);
-    if (t < 1) t = 1;
-    g.nt[d] = (int32_t)t;
-    g.side[d] = L[d] / (double)t;
-    g.recip[d] = 1. / g.side[d];
-  }
-  const long r0 = (long)std::ceil(ilen / g.side[0]), r1 = (long)std::ceil(ilen / g.side[1]);
-  g.R = (int32_t)(r0 > r1 ? r0 : r1);
-  g.ilen = ilen;
-  const int64_t ntow = (int64_t)g.nt[0] * g.nt[1];
+  const TowerGeom &g = c.tg;
+  const int64_t ntow = ens ? c.box_ntowers : (int64_t)g.nt[0] * g.nt[1];
   c.ntowers = ntow;
 
   const int64_t ns = c.nslots;
@@ -577,9 +587,13 @@ void vcl_build(Ctx &c) {
   c.ctow_start.ensure(ntow + 1);
   c.flag.ensure(4);
   fill_i32(c, c.tower_count.p, ntow + 1, 0);
-  hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
-                     s.orig.p, g, wrap_args(c.prm), c.tower_of.p, c.rank.p, c.tower_count.p);
-  LJG_HIP(hipGetLastError());
+  if (ens) {
+    ensemble_tower_hist(c);
+  } else {
+    hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
+                       s.orig.p, g, wrap_args(c.prm), c.tower_of.p, c.rank.p, c.tower_count.p);
+    LJG_HIP(hipGetLastError());
+  }
   // tower offsets, clusters per tower and their offsets, and the sizes for the one synchronisation
   c.tstats.ensure(4);
   hipLaunchKernelGGL(k_tower_layout, dim3(1), dim3(kLayoutThreads), 0, c.stream, ntow, c.tower_count.p, M,
@@ -614,6 +628,7 @@ void vcl_build(Ctx &c) {
     LJG_HIP(hipGetLastError());
   }
   const bool jrows = c.want_jrows;
+  if (jrows && ens) throw HipError{hipErrorInvalidValue, "j-particle rows on an ensemble of boxes", -2};
   if (jrows) {
     // fixed-point positions of every slot (padding: type -1), written by the permutation
     verlet_geometry(c);
@@ -644,12 +659,18 @@ void vcl_build(Ctx &c) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)5, c.list_stats.p);
     LJG_HIP(hipGetLastError());
     if (jrows)
-      hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit>), dim3((unsigned)ntow), dim3(kListThreads * kJclSplit),
-                         0, c.stream, ntow, g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
-                         c.jcl.p);
+      hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit, false>), dim3((unsigned)ntow),
+                         dim3(kListThreads * kJclSplit), 0, c.stream, ntow, g, (const TowerGeom *)nullptr,
+                         (const int64_t *)nullptr, (const int32_t *)nullptr, c.ctow_start.p, c.bbox.p, c.c0.p, cap,
+                         c.list_count.p, (int4 *)nullptr, c.jcl.p);
+    else if (ens)
+      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
+                         ntow, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.ctow_start.p, c.bbox.p, c.c0.p, cap,
+                         c.list_count.p, c.list.p, (int32_t *)nullptr);
     else
-      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
-                         ntow, g, c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
+      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, false>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
+                         ntow, g, (const TowerGeom *)nullptr, (const int64_t *)nullptr, (const int32_t *)nullptr,
+                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     LJG_HIP(hipGetLastError());
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
     if (jrows) {

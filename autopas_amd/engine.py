@@ -87,6 +87,8 @@ SIGNATURES = {
     "ljg_last_error": (ctypes.c_char_p, [_p]),
     "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
     "ljg_set_box": (_i32, [_p, _p, _p]),
+    "ljg_set_boxes": (_i32, [_p, _i32, _p, _p, _p]),
+    "ljg_get_box_globals": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_get_tuning": (_i32, [_p, _p]),
     "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -216,7 +218,7 @@ class Tuning(ctypes.Structure):
 
 
 class Engine:
-    """One periodic box on one GPU."""
+    """One periodic box on one GPU, or an ensemble of independent periodic boxes (set_boxes)."""
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
@@ -285,7 +287,29 @@ class Engine:
         lo = np.ascontiguousarray(box_min, dtype=np.float64)
         hi = np.ascontiguousarray(box_max, dtype=np.float64)
         self._check(self.lib.ljg_set_box(self.h, _ptr(lo), _ptr(hi)), "ljg_set_box")
+        self.nboxes = 0
         self.n = 0
+
+    def set_boxes(self, box_min, box_max, counts):
+        """Turn the context into an ensemble of independent periodic boxes: box b spans
+        [box_min[b], box_max[b]) (arrays of shape (nboxes, 3)) and holds counts[b] particles; the next
+        upload takes the boxes' particles concatenated in box order.  The particles are dropped."""
+        lo = np.ascontiguousarray(box_min, dtype=np.float64).reshape(-1, 3)
+        hi = np.ascontiguousarray(box_max, dtype=np.float64).reshape(-1, 3)
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        if not (len(lo) == len(hi) == len(cnt)):
+            raise ValueError("box_min, box_max and counts describe different numbers of boxes")
+        self._check(self.lib.ljg_set_boxes(self.h, len(cnt), _ptr(lo), _ptr(hi), _ptr(cnt)), "ljg_set_boxes")
+        self.nboxes = len(cnt)
+        self.n = 0
+
+    def box_globals(self):
+        """Per-box potential energy, virial and pair count of the last force(globals=True) on an ensemble."""
+        nb = getattr(self, "nboxes", 0)
+        e, v = np.empty(nb), np.empty(nb)
+        pc = np.empty(nb, dtype=np.int64)
+        self._check(self.lib.ljg_get_box_globals(self.h, nb, _ptr(e), _ptr(v), _ptr(pc)), "ljg_get_box_globals")
+        return dict(potential_energy=e, virial=v, pair_count=pc)
 
     def set_types(self, epsilon, sigma, mass=None):
         e = np.ascontiguousarray(epsilon, dtype=np.float64)

@@ -113,6 +113,22 @@ ljg_status ljg_device_name(const ljg_ctx *ctx, char *buf, int32_t buflen);
  * kept, so one context can serve many jobs of different sizes without reallocating. */
 ljg_status ljg_set_box(ljg_ctx *ctx, const double box_min[3], const double box_max[3]);
 
+/* Ensemble of independent periodic boxes in one context.  md-flexible runs one AutoPas instance per
+ * simulation (AutoPasImpl.h:55-113); here the boxes share every launch.  Box b spans
+ * [box_min[3b..3b+2], box_max[3b..3b+2]) and holds count[b] particles.  The next ljg_upload takes the
+ * boxes' particles concatenated in box order (n = sum of count).  Every box side must be >= rc + skin.
+ * Only LJG_KERNEL_CLUSTER_MIXED runs on an ensemble.  Each box keeps the tower grid, cluster layout, list
+ * order and arithmetic it would have alone.  Lists are rebuilt for all boxes together: every
+ * rebuild_frequency steps, or once a particle of any box moved skin/2.  ljg_set_box returns the context
+ * to one box; the particles are dropped either way. */
+ljg_status ljg_set_boxes(ljg_ctx *ctx, int32_t nboxes, const double *box_min, const double *box_max,
+                         const int64_t *count);
+/* Per-box results of the last ljg_force with calc_globals on an ensemble: potential energy and virial of
+ * each box (LJFunctor::getPotentialEnergy / getVirial, LJFunctor.h:691-720) and its unique pairs within
+ * rc.  Any output pointer may be NULL. */
+ljg_status ljg_get_box_globals(ljg_ctx *ctx, int32_t nboxes, double *potential_energy, double *virial,
+                               int64_t *pair_count);
+
 /* Particle types: ParticlePropertiesLibrary::addSiteType + calculateMixingCoefficients
  * (ParticlePropertiesLibrary.h:443-489): Lorentz-Berthelot mixing is done here, in FP64. */
 ljg_status ljg_set_types(ljg_ctx *ctx, int32_t ntypes, const double *epsilon, const double *sigma,
