@@ -149,6 +149,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->rebuild_frequency < 1) return LJG_ERR_ARG;
   if (p->tile_groups < 0 || p->tile_groups > 64) return LJG_ERR_ARG;
   if (p->build_groups < 0 || p->build_groups > 64) return LJG_ERR_ARG;
+  if (p->list_mode < 0 || p->list_mode > 2) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;
@@ -190,10 +191,12 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   for (auto &s : c.st) s.release();
   c.cell_of.release(); c.cell_count.release(); c.cell_start.release(); c.rank.release(); c.perm.release();
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
-  c.tower_items.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
+  c.tower_items.release(); c.tower_z.release(); c.tower_id.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
   c.pos32.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
+  c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
+  c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev_drift) (void)hipEventDestroy(c.ev_drift);
@@ -295,16 +298,15 @@ ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, cons
   c.box_lo.assign(box_min, box_min + 3 * (size_t)nboxes);
   c.box_hi.assign(box_max, box_max + 3 * (size_t)nboxes);
   c.box_count.assign(count, count + nboxes);
-  std::vector<int64_t> pstart((size_t)nboxes + 1);
+  std::vector<int32_t> box_of((size_t)total);
   std::vector<WrapArgs> wr((size_t)nboxes);
-  pstart[0] = 0;
-  for (int32_t b = 0; b < nboxes; ++b) {
-    pstart[b + 1] = pstart[b] + count[b];
+  for (int32_t b = 0, o = 0; b < nboxes; ++b) {
+    for (int64_t k = 0; k < count[b]; ++k) box_of[o++] = b;
     wr[b] = wrap_args(&box_min[3 * b], &box_max[3 * b]);
   }
-  c.box_pstart.ensure(pstart.size());
+  c.box_of.ensure(box_of.size());
   c.box_wrap.ensure(wr.size());
-  LJG_HIP(hipMemcpyAsync(c.box_pstart.p, pstart.data(), pstart.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+  LJG_HIP(hipMemcpyAsync(c.box_of.p, box_of.data(), box_of.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                          c.stream));
   LJG_HIP(hipMemcpyAsync(c.box_wrap.p, wr.data(), wr.size() * sizeof(WrapArgs), hipMemcpyHostToDevice, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));

@@ -18,22 +18,11 @@
 
 namespace ljg {
 
-// Box of upload index o: the last b with pstart[b] <= o.
-__device__ inline int32_t box_of(int64_t o, int32_t nboxes, const int64_t *__restrict__ pstart) {
-  int32_t lo = 0, hi = nboxes - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (pstart[mid] <= o) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // Tower histogram over the ensemble: wrap into the particle's own box, then that box's tower index
 // offset by the box's first tower.
 __global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, double *__restrict__ y,
-                                   double *__restrict__ z, const int32_t *__restrict__ orig, int32_t nboxes,
-                                   const int64_t *__restrict__ pstart, const TowerGeom *__restrict__ tg,
+                                   double *__restrict__ z, const int32_t *__restrict__ orig,
+                                   const int32_t *__restrict__ box_of, const TowerGeom *__restrict__ tg,
                                    const WrapArgs *__restrict__ wr, const int64_t *__restrict__ tbase,
                                    int32_t *__restrict__ tower_of, int32_t *__restrict__ rank,
                                    int32_t *__restrict__ count) {
@@ -41,7 +30,7 @@ __global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, doubl
   const bool active = i < nslots && orig[i] >= 0;
   int32_t t = -1;
   if (active) {
-    const int32_t b = box_of(orig[i], nboxes, pstart);
+    const int32_t b = box_of[orig[i]];
     const WrapArgs w = wr[b];
     const double ox = x[i], oy = y[i], oz = z[i];
     const double px = wrap_coord(ox, 0, w), py = wrap_coord(oy, 1, w), pz = wrap_coord(oz, 2, w);
@@ -57,11 +46,11 @@ __global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, doubl
   }
 }
 
-__global__ void k_wrap_boxes(int64_t nslots, StateView s, int32_t nboxes, const int64_t *__restrict__ pstart,
+__global__ void k_wrap_boxes(int64_t nslots, StateView s, const int32_t *__restrict__ box_of,
                              const WrapArgs *__restrict__ wr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots || s.orig[i] < 0) return;
-  const WrapArgs w = wr[box_of(s.orig[i], nboxes, pstart)];
+  const WrapArgs w = wr[box_of[s.orig[i]]];
   double *c[3] = {s.x, s.y, s.z};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -135,14 +124,14 @@ void ensemble_tower_hist(Ctx &c) {
   State &s = c.st[c.cur];
   const int64_t ns = c.nslots;
   hipLaunchKernelGGL(k_tower_hist_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
-                     s.orig.p, c.nboxes, c.box_pstart.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, c.tower_of.p,
+                     s.orig.p, c.box_of.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, c.tower_of.p,
                      c.rank.p, c.tower_count.p);
   LJG_HIP(hipGetLastError());
 }
 
 void ensemble_wrap(Ctx &c) {
   hipLaunchKernelGGL(k_wrap_boxes, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), c.nboxes, c.box_pstart.p, c.box_wrap.p);
+                     view(c.st[c.cur]), c.box_of.p, c.box_wrap.p);
   LJG_HIP(hipGetLastError());
 }
 

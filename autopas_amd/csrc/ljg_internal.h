@@ -228,6 +228,8 @@ struct Ctx {
   DevBuf<int32_t> tower_count;  // ntowers
   DevBuf<int64_t> tower_start;  // ntowers + 1 (particle offsets)
   DevBuf<int32_t> tower_items;  // sorted old slots per tower
+  DevBuf<double> tower_z;       // z of each tower item
+  DevBuf<int64_t> tower_id;     // id of each tower item
   DevBuf<int32_t> ctow_count;   // clusters per tower
   DevBuf<int64_t> ctow_start;   // ntowers + 1 (cluster offsets)
   DevBuf<int32_t> cluster_tower;  // nclusters
@@ -295,7 +297,7 @@ struct Ctx {
   int64_t box_ntowers = 0;
   DevBuf<TowerGeom> box_tg;            // per box
   DevBuf<WrapArgs> box_wrap;           // per box
-  DevBuf<int64_t> box_pstart;          // nboxes + 1: upload-index range of each box
+  DevBuf<int32_t> box_of;              // box of each particle, upload order
   DevBuf<int64_t> box_tbase;           // nboxes + 1: first tower of each box
   DevBuf<int32_t> tower_box;           // box of each tower
   DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch

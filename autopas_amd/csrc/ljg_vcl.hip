@@ -57,13 +57,19 @@ __global__ void k_tower_hist(int64_t nslots, double *__restrict__ x, double *__r
   }
 }
 
+// Also copies each particle's z and id next to its slot, so that the register sorts read them contiguously.
 __global__ void k_tower_scatter(int64_t nslots, const int32_t *__restrict__ tower_of, const int32_t *__restrict__ rank,
-                                const int64_t *__restrict__ start, int32_t *__restrict__ items) {
+                                const int64_t *__restrict__ start, const double *__restrict__ z,
+                                const int64_t *__restrict__ id, int32_t *__restrict__ items,
+                                double *__restrict__ items_z, int64_t *__restrict__ items_id) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
   const int32_t t = tower_of[i];
   if (t < 0) return;
-  items[start[t] + rank[i]] = (int32_t)i;
+  const int64_t k = start[t] + rank[i];
+  items[k] = (int32_t)i;
+  items_z[k] = z[i];
+  items_id[k] = id[i];
 }
 
 // One workgroup: exclusive scans of the tower sizes (particle offsets) and of the clusters per tower,
@@ -124,6 +130,137 @@ __global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers
   if (tid == 0) stats[1] = (unsigned long long)sm[0];
 }
 
+// The same layout for grids of many towers (ensembles of boxes), in three passes so that every thread
+// handles kLayoutPer towers: per-block totals, a scan of the block totals (one workgroup), and the
+// per-tower offsets.  The totals and statistics are those of k_tower_layout.
+constexpr int kLayoutBlock = 256;
+constexpr int kLayoutPer = 16;
+constexpr int64_t kLayoutChunk = (int64_t)kLayoutBlock * kLayoutPer;
+constexpr int64_t kLayoutOneBlockMax = 16384;  // towers up to which the single-workgroup kernel is used
+
+__global__ __launch_bounds__(kLayoutBlock) void k_layout_sums(int64_t ntowers, const int32_t *__restrict__ count,
+                                                              int32_t M, int64_t *__restrict__ bsum,
+                                                              int32_t *__restrict__ bmax) {
+  __shared__ int64_t sp[kLayoutBlock], sc[kLayoutBlock];
+  __shared__ int32_t sm[kLayoutBlock];
+  const int tid = threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * kLayoutChunk + (int64_t)tid * kLayoutPer;
+  const int64_t e = b + kLayoutPer < ntowers ? b + kLayoutPer : ntowers;
+  int64_t np = 0, nc = 0;
+  int32_t mx = 0;
+  for (int64_t t = b; t < e; ++t) {
+    const int32_t k = count[t];
+    np += k;
+    nc += (k + M - 1) / M;
+    mx = k > mx ? k : mx;
+  }
+  sp[tid] = np;
+  sc[tid] = nc;
+  sm[tid] = mx;
+  __syncthreads();
+  for (int s = kLayoutBlock / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      sp[tid] += sp[tid + s];
+      sc[tid] += sc[tid + s];
+      sm[tid] = sm[tid + s] > sm[tid] ? sm[tid + s] : sm[tid];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    bsum[2 * blockIdx.x] = sp[0];
+    bsum[2 * blockIdx.x + 1] = sc[0];
+    bmax[blockIdx.x] = sm[0];
+  }
+}
+
+// Exclusive scan of the block totals in place; totals at index ntowers of start/cstart and the stats.
+__global__ __launch_bounds__(kLayoutThreads) void k_layout_top(int64_t nblocks, int64_t ntowers,
+                                                               int64_t *__restrict__ bsum,
+                                                               const int32_t *__restrict__ bmax,
+                                                               int64_t *__restrict__ start,
+                                                               int64_t *__restrict__ cstart,
+                                                               unsigned long long *__restrict__ stats) {
+  __shared__ int64_t sp[kLayoutThreads], sc[kLayoutThreads];
+  __shared__ int32_t sm[kLayoutThreads];
+  const int tid = threadIdx.x;
+  const int64_t per = (nblocks + kLayoutThreads - 1) / kLayoutThreads;
+  const int64_t b = tid * per, e = b + per < nblocks ? b + per : nblocks;
+  int64_t np = 0, nc = 0;
+  int32_t mx = 0;
+  for (int64_t k = b; k < e; ++k) {
+    np += bsum[2 * k];
+    nc += bsum[2 * k + 1];
+    mx = bmax[k] > mx ? bmax[k] : mx;
+  }
+  sp[tid] = np;
+  sc[tid] = nc;
+  sm[tid] = mx;
+  __syncthreads();
+  for (int off = 1; off < kLayoutThreads; off <<= 1) {
+    const int64_t ap = tid >= off ? sp[tid - off] : 0, ac = tid >= off ? sc[tid - off] : 0;
+    __syncthreads();
+    sp[tid] += ap;
+    sc[tid] += ac;
+    __syncthreads();
+  }
+  for (int s = kLayoutThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) sm[tid] = sm[tid + s] > sm[tid] ? sm[tid + s] : sm[tid];
+    __syncthreads();
+  }
+  int64_t op = sp[tid] - np, oc = sc[tid] - nc;
+  for (int64_t k = b; k < e; ++k) {
+    const int64_t kp = bsum[2 * k], kc = bsum[2 * k + 1];
+    bsum[2 * k] = op;
+    bsum[2 * k + 1] = oc;
+    op += kp;
+    oc += kc;
+  }
+  if (tid == kLayoutThreads - 1) {
+    start[ntowers] = sp[tid];
+    cstart[ntowers] = sc[tid];
+    stats[0] = (unsigned long long)sp[tid];
+    stats[2] = (unsigned long long)sc[tid];
+  }
+  if (tid == 0) stats[1] = (unsigned long long)sm[0];
+}
+
+__global__ __launch_bounds__(kLayoutBlock) void k_layout_apply(int64_t ntowers, const int32_t *__restrict__ count,
+                                                               int32_t M, const int64_t *__restrict__ boff,
+                                                               int64_t *__restrict__ start,
+                                                               int32_t *__restrict__ ccount,
+                                                               int64_t *__restrict__ cstart) {
+  __shared__ int64_t sp[kLayoutBlock], sc[kLayoutBlock];
+  const int tid = threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * kLayoutChunk + (int64_t)tid * kLayoutPer;
+  const int64_t e = b + kLayoutPer < ntowers ? b + kLayoutPer : ntowers;
+  int64_t np = 0, nc = 0;
+  for (int64_t t = b; t < e; ++t) {
+    const int32_t k = count[t];
+    np += k;
+    nc += (k + M - 1) / M;
+  }
+  sp[tid] = np;
+  sc[tid] = nc;
+  __syncthreads();
+  for (int off = 1; off < kLayoutBlock; off <<= 1) {
+    const int64_t ap = tid >= off ? sp[tid - off] : 0, ac = tid >= off ? sc[tid - off] : 0;
+    __syncthreads();
+    sp[tid] += ap;
+    sc[tid] += ac;
+    __syncthreads();
+  }
+  int64_t op = boff[2 * blockIdx.x] + sp[tid] - np, oc = boff[2 * blockIdx.x + 1] + sc[tid] - nc;
+  for (int64_t t = b; t < e; ++t) {
+    const int32_t k = count[t];
+    const int32_t kc = (k + M - 1) / M;
+    start[t] = op;
+    cstart[t] = oc;
+    ccount[t] = kc;
+    op += k;
+    oc += kc;
+  }
+}
+
 // One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
 // are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.
 // Two instances: CAP = kSortSmall takes the towers up to that size (12 KB of LDS, so many workgroups
@@ -139,17 +276,20 @@ __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, co
   return id[sa] < id[sb];
 }
 
+// Towers of up to min_n particles are left to k_tower_sort_wave (min_n = 0 when it does not run).
 template <int CAP>
 __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
     int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
     const double *__restrict__ z, const int64_t *__restrict__ id, const int64_t *__restrict__ cstart, int32_t M,
-    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err) {
+    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err,
+    int32_t min_n) {
   __shared__ double sz[CAP];
   __shared__ int32_t ss[CAP];
   const int64_t t = blockIdx.x;
   if (t >= ntowers) return;
   const int64_t b = start[t];
   const int n = (int)(start[t + 1] - b);
+  if (n <= min_n) return;                           // sorted in registers by k_tower_sort_wave
   if (CAP > kSortSmall && n <= kSortSmall) return;  // sorted by the small instance
   if (CAP == kSortSmall && n > kSortSmall) return;  // left to the large instance
   if (n > CAP) {
@@ -192,30 +332,133 @@ __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
   for (int64_t c = c0 + threadIdx.x; c < c1; c += kSortThreads) cluster_tower[c] = (int32_t)t;
 }
 
-// Cluster bounding box (Cluster.h:135-149: xy min/max over the members, z from first and last) over the
-// real members, and the reference point c0 = box centre with the number of real members in c0.w.
-__global__ void k_cluster_bbox(int64_t nclusters, int32_t M, const double *__restrict__ x,
+// Short towers (small boxes): one wavefront per tower sorts (z, id) in registers with a bitonic network
+// over 64 E elements, element e * 64 + lane in slot e of a lane (padding sorts last), and writes the same
+// cluster layout as k_tower_sort.  Keys are distinct (ids are), so every sorter produces the one sorted
+// order.  The E = 1 instance takes towers of up to 64 particles, E = 4 those of 65 to 256.
+constexpr int kSortWaveThreads = 256;
+
+// zid_less on values held in registers, without branches (padding sa < 0 sorts last).
+__device__ inline bool zid_less_v(double za, int32_t sa, int64_t ia, double zb, int32_t sb, int64_t ib) {
+  return (sa >= 0) & ((sb < 0) | (za < zb) | ((za == zb) & (ia < ib)));
+}
+
+// One compare-exchange stage of the register bitonic network (size, stride known at compile time).
+template <int E, int SIZE, int STRIDE>
+__device__ inline void bitonic_stage(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
+  if constexpr (STRIDE >= kWave) {
+    // both elements of a pair sit in this lane: slots e and e + STRIDE / 64
+    constexpr int ks = STRIDE / kWave;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((e & ks) != 0) continue;
+      const int e2 = e + ks;
+      const bool up = ((e * kWave + lane) & SIZE) == 0;
+      const bool swap = up ? zid_less_v(zv[e2], sv[e2], iv[e2], zv[e], sv[e], iv[e])
+                           : zid_less_v(zv[e], sv[e], iv[e], zv[e2], sv[e2], iv[e2]);
+      const double za = zv[e], zb = zv[e2];
+      const int32_t sa = sv[e], sb = sv[e2];
+      const int64_t ia = iv[e], ib = iv[e2];
+      zv[e] = swap ? zb : za; zv[e2] = swap ? za : zb;
+      sv[e] = swap ? sb : sa; sv[e2] = swap ? sa : sb;
+      iv[e] = swap ? ib : ia; iv[e2] = swap ? ia : ib;
+    }
+  } else {
+    const bool lower = (lane & STRIDE) == 0;  // this lane holds the lower index of its pairs
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double zo = __shfl_xor(zv[e], STRIDE, kWave);
+      const int32_t so = __shfl_xor(sv[e], STRIDE, kWave);
+      const int64_t io = __shfl_xor(iv[e], STRIDE, kWave);
+      const bool up = ((e * kWave + lane) & SIZE) == 0;  // this element's block sorts ascending
+      // the lower element of an ascending pair keeps the smaller key, and so on
+      const bool take = (lower == up) ? zid_less_v(zo, so, io, zv[e], sv[e], iv[e])
+                                      : zid_less_v(zv[e], sv[e], iv[e], zo, so, io);
+      zv[e] = take ? zo : zv[e];
+      sv[e] = take ? so : sv[e];
+      iv[e] = take ? io : iv[e];
+    }
+  }
+}
+
+template <int E, int SIZE, int STRIDE>
+__device__ inline void bitonic_strides(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
+  bitonic_stage<E, SIZE, STRIDE>(zv, sv, iv, lane);
+  if constexpr (STRIDE > 1) bitonic_strides<E, SIZE, STRIDE / 2>(zv, sv, iv, lane);
+}
+
+template <int E, int SIZE>
+__device__ inline void bitonic_sizes(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
+  bitonic_strides<E, SIZE, SIZE / 2>(zv, sv, iv, lane);
+  if constexpr (SIZE < kWave * E) bitonic_sizes<E, SIZE * 2>(zv, sv, iv, lane);
+}
+
+template <int E>
+__global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
+    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
+    const double *__restrict__ items_z, const int64_t *__restrict__ items_id, const int64_t *__restrict__ cstart,
+    int32_t M, int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower) {
+  constexpr int P = kWave * E;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t t = (int64_t)blockIdx.x * (kSortWaveThreads / kWave) + threadIdx.x / kWave;
+  if (t >= ntowers) return;
+  const int64_t b = start[t];
+  const int n = (int)(start[t + 1] - b);
+  if (n > P || (E > 1 && n <= kWave)) return;  // another instance sorts this tower
+  int32_t sv[E];
+  double zv[E];
+  int64_t iv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = e * kWave + lane;
+    sv[e] = k < n ? items[b + k] : -1;
+    zv[e] = k < n ? items_z[b + k] : 0.;
+    iv[e] = k < n ? items_id[b + k] : 0;
+  }
+  bitonic_sizes<E, 2>(zv, sv, iv, lane);
+  const int64_t c0 = cstart[t], c1 = cstart[t + 1];
+  const int nslot = (int)((c1 - c0) * M);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = e * kWave + lane;
+    if (k < nslot) src[c0 * M + k] = k < n ? sv[e] : -1;
+  }
+  if (P + lane < nslot) src[c0 * M + P + lane] = -1;
+  for (int64_t c = c0 + lane; c < c1; c += kWave) cluster_tower[c] = (int32_t)t;
+}
+
+// One lane per slot, the M lanes of a cluster reduce with shuffles (M divides the wavefront, and the
+// cluster's slots are consecutive, so a cluster never straddles two wavefronts).  The members are sorted
+// by z, so the first and last real members' z are the minimum and maximum.
+__global__ void k_cluster_bbox(int64_t nslots, int32_t M, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z,
                                const int32_t *__restrict__ orig, double *__restrict__ bbox,
                                double4 *__restrict__ c0) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nclusters) return;
-  const int64_t s0 = c * M;
-  double xl = x[s0], xh = x[s0], yl = y[s0], yh = y[s0];
-  const double zl = z[s0];
-  double zh = zl;
-  int nreal = 1;
-  for (int m = 1; m < M; ++m) {
-    const int64_t s = s0 + m;
-    if (orig[s] < 0) break;
-    xl = fmin(xl, x[s]); xh = fmax(xh, x[s]);
-    yl = fmin(yl, y[s]); yh = fmax(yh, y[s]);
-    zh = z[s];
-    ++nreal;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool real = s < nslots && orig[s] >= 0;
+  const double inf = __builtin_inf();
+  double xl = inf, xh = -inf, yl = inf, yh = -inf, zl = inf, zh = -inf;
+  int nreal = real ? 1 : 0;
+  if (real) {
+    xl = xh = x[s];
+    yl = yh = y[s];
+    zl = zh = z[s];
   }
-  double *bb = bbox + 6 * c;
-  bb[0] = xl; bb[1] = yl; bb[2] = zl; bb[3] = xh; bb[4] = yh; bb[5] = zh;
-  c0[c] = make_double4((xl + xh) * 0.5, (yl + yh) * 0.5, (zl + zh) * 0.5, (double)nreal);
+  for (int off = M >> 1; off > 0; off >>= 1) {
+    xl = fmin(xl, __shfl_xor(xl, off, kWave));
+    xh = fmax(xh, __shfl_xor(xh, off, kWave));
+    yl = fmin(yl, __shfl_xor(yl, off, kWave));
+    yh = fmax(yh, __shfl_xor(yh, off, kWave));
+    zl = fmin(zl, __shfl_xor(zl, off, kWave));
+    zh = fmax(zh, __shfl_xor(zh, off, kWave));
+    nreal += __shfl_xor(nreal, off, kWave);
+  }
+  if (s < nslots && s % M == 0) {
+    const int64_t c = s / M;
+    double *bb = bbox + 6 * c;
+    bb[0] = xl; bb[1] = yl; bb[2] = zl; bb[3] = xh; bb[4] = yh; bb[5] = zh;
+    c0[c] = make_double4((xl + xh) * 0.5, (yl + yh) * 0.5, (zl + zh) * 0.5, (double)nreal);
+  }
 }
 
 __device__ inline double box_dist2(const double *a, const double *b, double shx, double shy, double shz) {
@@ -397,6 +640,87 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
   }
 }
 
+// Cluster-pair lists with one lane per i-cluster, for grids whose towers hold few clusters (small boxes,
+// ensembles of them), where the wavefront per tower of k_cluster_lists_tower leaves most lanes idle.
+// Each lane walks the traversal order of k_cluster_lists_tower (oy, ox, sz, then the j-tower's clusters
+// from the first whose z-range can reach the i-cluster) with the same tests, so the rows are identical
+// entry by entry; the j-tower's boxes are read from memory (lanes of one tower read the same lines).
+constexpr int kListLaneThreads = 256;
+
+template <bool MB>
+__global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
+    int64_t nclusters, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
+    const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart,
+    const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
+    int4 *__restrict__ list) {
+  const int64_t ci = (int64_t)blockIdx.x * kListLaneThreads + threadIdx.x;
+  if (ci >= nclusters) return;
+  const int64_t t = cluster_tower[ci];
+  TowerGeom g = g1;
+  int64_t tb = 0;
+  if constexpr (MB) {
+    const int32_t b = tbox[t];
+    g = tgs[b];
+    tb = tbase[b];
+  }
+  const int64_t tl = t - tb;
+  const int tx = (int)(tl % g.nt[0]), ty = (int)(tl / g.nt[0]);
+  const double il2 = g.ilen * g.ilen;
+  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+  double bi[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
+  const double4 ai = c0[ci];
+  const int side = 2 * g.R + 1, nvis = side * side;
+  const int64_t w0 = ci * (int64_t)cap;
+  int32_t cnt = 0;
+  for (int v = 0; v < nvis; ++v) {
+    const int oy = v / side - g.R, ox = v % side - g.R;
+    const int aoy = oy < 0 ? -oy : oy;
+    const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+    const int aox = ox < 0 ? -ox : ox;
+    const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+    if (dtx * dtx + dty * dty > il2) continue;
+    int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+    while (jx < 0) { jx += g.nt[0]; --sx; }
+    while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+    while (jy < 0) { jy += g.nt[1]; --sy; }
+    while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+    const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
+    const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+    for (int sz = -1; sz <= 1; ++sz) {
+      const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
+      const double zmin = bi[2] - zwin - shz;
+      const double zmax = bi[5] + zwin - shz;
+      // every cluster's z-range lies in [box_min, box_min + L): a window outside it finds nothing
+      if (zmax < g.box_min[2] || zmin > g.box_min[2] + g.L[2]) continue;
+      int64_t lo = cb, hi = ce;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (bbox[6 * mid + 5] < zmin) lo = mid + 1;
+        else hi = mid;
+      }
+      for (int64_t k = lo; k < ce; ++k) {
+        const double *bj = bbox + 6 * k;
+        if (bj[2] > zmax) break;
+        if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+          if (cnt < cap) {
+            const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+            const double4 b = c0[k];
+            const float dx = (float)((b.x + shx) - ai.x);
+            const float dy = (float)((b.y + shy) - ai.y);
+            const float dz = (float)((b.z + shz) - ai.z);
+            list[w0 + cnt] = make_int4((int32_t)k | (sid << kShiftBits), __float_as_int(dx), __float_as_int(dy),
+                                       __float_as_int(dz));
+          }
+          ++cnt;
+        }
+      }
+    }
+  }
+  count[ci] = cnt;
+}
+
 // Second pass of the j-particle row build: one wavefront per i-cluster streams the members of its listed
 // j-clusters (64 at a time) and keeps the real j within ilen of at least one real i-member, by the FP64
 // length of the exact minimum-image fixed-point separation (the arithmetic of the Verlet-list build; it
@@ -560,6 +884,11 @@ static void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
   LJG_HIP(hipGetLastError());
 }
 
+// Average clusters per tower below which the cluster-entry lists use a lane per i-cluster, and average
+// particles per tower up to which the towers of at most 256 particles are sorted in registers.
+constexpr int kLaneListsBelow = 24;
+constexpr int kWaveSortBelow = 192;
+
 void vcl_build(Ctx &c) {
   const int M = c.M;
   const double ilen = c.prm.cutoff + c.prm.skin;
@@ -583,6 +912,8 @@ void vcl_build(Ctx &c) {
   c.tower_count.ensure(ntow + 1);
   c.tower_start.ensure(ntow + 1);
   c.tower_items.ensure(c.n + 1);
+  c.tower_z.ensure(c.n + 1);
+  c.tower_id.ensure(c.n + 1);
   c.ctow_count.ensure(ntow + 1);
   c.ctow_start.ensure(ntow + 1);
   c.flag.ensure(4);
@@ -596,11 +927,27 @@ void vcl_build(Ctx &c) {
   }
   // tower offsets, clusters per tower and their offsets, and the sizes for the one synchronisation
   c.tstats.ensure(4);
-  hipLaunchKernelGGL(k_tower_layout, dim3(1), dim3(kLayoutThreads), 0, c.stream, ntow, c.tower_count.p, M,
-                     c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
-  LJG_HIP(hipGetLastError());
+  if (ntow <= kLayoutOneBlockMax) {
+    hipLaunchKernelGGL(k_tower_layout, dim3(1), dim3(kLayoutThreads), 0, c.stream, ntow, c.tower_count.p, M,
+                       c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
+    LJG_HIP(hipGetLastError());
+  } else {
+    const int64_t nblk = (ntow + kLayoutChunk - 1) / kLayoutChunk;
+    c.scan_tmp.ensure(2 * (size_t)nblk + (size_t)nblk / 2 + 1);
+    int64_t *bsum = c.scan_tmp.p;
+    int32_t *bmax = reinterpret_cast<int32_t *>(c.scan_tmp.p + 2 * nblk);
+    hipLaunchKernelGGL(k_layout_sums, dim3((unsigned)nblk), dim3(kLayoutBlock), 0, c.stream, ntow, c.tower_count.p, M,
+                       bsum, bmax);
+    LJG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_layout_top, dim3(1), dim3(kLayoutThreads), 0, c.stream, nblk, ntow, bsum, bmax,
+                       c.tower_start.p, c.ctow_start.p, c.tstats.p);
+    LJG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_layout_apply, dim3((unsigned)nblk), dim3(kLayoutBlock), 0, c.stream, ntow, c.tower_count.p,
+                       M, bsum, c.tower_start.p, c.ctow_count.p, c.ctow_start.p);
+    LJG_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.tower_of.p, c.rank.p,
-                     c.tower_start.p, c.tower_items.p);
+                     c.tower_start.p, s.z.p, s.id.p, c.tower_items.p, c.tower_z.p, c.tower_id.p);
   LJG_HIP(hipGetLastError());
   unsigned long long ts[3] = {0, 0, 0};
   LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
@@ -617,14 +964,32 @@ void vcl_build(Ctx &c) {
   c.list_stats.ensure(6);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)6, c.list_stats.p);
   LJG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
-                     c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p,
-                     c.list_stats.p + 5);
-  LJG_HIP(hipGetLastError());
+  // short towers (small boxes) sort in registers, a wavefront each; the rest in LDS
+  const bool wave_sort = (int64_t)ts[0] <= (int64_t)kWaveSortBelow * ntow;
+  const int32_t min_n = wave_sort ? 4 * kWave : 0;
+  if (wave_sort) {
+    constexpr int per_block = kSortWaveThreads / kWave;
+    hipLaunchKernelGGL(k_tower_sort_wave<1>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream,
+                       ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M,
+                       c.perm.p, c.cluster_tower.p);
+    LJG_HIP(hipGetLastError());
+    if (ts[1] > (unsigned long long)kWave) {
+      hipLaunchKernelGGL(k_tower_sort_wave<4>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream,
+                         ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M,
+                         c.perm.p, c.cluster_tower.p);
+      LJG_HIP(hipGetLastError());
+    }
+  }
+  if (ts[1] > (unsigned long long)min_n) {
+    hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
+                       c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p,
+                       c.list_stats.p + 5, min_n);
+    LJG_HIP(hipGetLastError());
+  }
   if (ts[1] > (unsigned long long)kSortSmall) {
     hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
                        c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p,
-                       c.cluster_tower.p, c.list_stats.p + 5);
+                       c.cluster_tower.p, c.list_stats.p + 5, min_n);
     LJG_HIP(hipGetLastError());
   }
   const bool jrows = c.want_jrows;
@@ -639,7 +1004,8 @@ void vcl_build(Ctx &c) {
   State &t = c.st[c.cur];
   c.bbox.ensure(6 * ncl);
   c.c0.ensure(ncl);
-  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(ncl, 256)), dim3(256), 0, c.stream, ncl, M, t.x.p, t.y.p, t.z.p,
+  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, M, t.x.p,
+                     t.y.p, t.z.p,
                      t.orig.p, c.bbox.p, c.c0.p);
   LJG_HIP(hipGetLastError());
   c.list_count.ensure(ncl * kJclSplit + 1);
@@ -647,6 +1013,8 @@ void vcl_build(Ctx &c) {
   if (jrows) c.jcount.ensure(ncl + 1);
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
   // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
+  // cluster-entry lists: a lane per i-cluster when the towers hold few clusters, unless the caller chose
+  const bool lane_lists = c.prm.list_mode == 2 || (c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow);
   int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24) : (c.list_cap > 0 ? c.list_cap : 64);
   int32_t jcap = c.jcap > 0 ? c.jcap : 160;
   for (;;) {
@@ -663,6 +1031,15 @@ void vcl_build(Ctx &c) {
                          dim3(kListThreads * kJclSplit), 0, c.stream, ntow, g, (const TowerGeom *)nullptr,
                          (const int64_t *)nullptr, (const int32_t *)nullptr, c.ctow_start.p, c.bbox.p, c.c0.p, cap,
                          c.list_count.p, (int4 *)nullptr, c.jcl.p);
+    else if (lane_lists && ens)
+      hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(ncl, kListLaneThreads)), dim3(kListLaneThreads),
+                         0, c.stream, ncl, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_tower.p,
+                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
+    else if (lane_lists)
+      hipLaunchKernelGGL((k_cluster_lists_lane<false>), dim3(blocks_for(ncl, kListLaneThreads)),
+                         dim3(kListLaneThreads), 0, c.stream, ncl, g, (const TowerGeom *)nullptr,
+                         (const int64_t *)nullptr, (const int32_t *)nullptr, c.cluster_tower.p, c.ctow_start.p,
+                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
     else if (ens)
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
                          ntow, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.ctow_start.p, c.bbox.p, c.c0.p, cap,

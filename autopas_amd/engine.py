@@ -55,7 +55,8 @@ class Params(ctypes.Structure):
         ("build_groups", _i32),
         ("tuning_samples", _i32),
         ("tuning_interval", _i32),
-        ("reserved", _i32 * 3),
+        ("list_mode", _i32),
+        ("reserved", _i32 * 2),
     ]
 
 
@@ -222,7 +223,7 @@ class Engine:
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
-                 tuning_interval=0):
+                 tuning_interval=0, list_mode=0):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -238,6 +239,7 @@ class Engine:
         p.build_groups = build_groups
         p.tuning_samples = tuning_samples
         p.tuning_interval = tuning_interval
+        p.list_mode = list_mode
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))

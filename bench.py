@@ -45,6 +45,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--serve-jobs", type=int, default=200, help="jobs in the serving measurement (0: skip)")
     p.add_argument("--serve-workers", type=int, default=4)
+    p.add_argument("--ensemble-boxes", type=int, default=4096,
+                   help="boxes in the batched-ensemble measurement (0: skip)")
+    p.add_argument("--ensemble-steps", type=int, default=200)
+    p.add_argument("--ensemble-only", action="store_true",
+                   help="only the batched-ensemble measurement, printed as one JSON object (for rocprof)")
     p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="with --profile-only: finish with a 256 MiB device-to-device copy, the known-size "
@@ -172,6 +177,63 @@ def serving(jobs, workers):
     return out
 
 
+ENSEMBLE_CANDIDATES = ((8, 4), (8, 2), (4, 8))  # (cluster size, i-clusters per wavefront)
+
+
+def ensemble_bench(nboxes, steps, warmup=10, sample=64):
+    """The batched ensemble: `nboxes` independent periodic boxes (N log-uniform in [64, 8192], rho in
+    [0.3, 0.9], T in [0.7, 1.5], seeds 4000 + b) advanced together by one context with the FP32 cluster
+    kernel, `steps` steps at dt 0.002.  The tile shape is picked by a short pass over
+    ENSEMBLE_CANDIDATES.  Beside it, the first `sample` boxes one at a time, each in its own context."""
+    import autopas_amd as ap
+    from autopas_amd import ensemble as en
+
+    boxes = en.make_boxes(nboxes, seed=4)
+    arrays = en.ensemble_arrays(boxes)
+    lo, hi, counts, st = arrays
+    kernel = ap.KERNEL_CLUSTER_MIXED
+    table = {}
+    for M, G in ENSEMBLE_CANDIDATES:
+        with en.ensemble_engine(boxes, cluster_size=M, tile_groups=G, arrays=arrays) as e:
+            e.step(5, en.DT, kernel)
+            t0 = time.perf_counter()
+            rs = e.step(20, en.DT, kernel)
+            table[f"cluster_M{M}_G{G}"] = (rs.pair_interactions / (time.perf_counter() - t0), M, G)
+    best = max(table, key=lambda k: table[k][0])
+    _, M, G = table[best]
+    with en.ensemble_engine(boxes, cluster_size=M, tile_groups=G, arrays=arrays) as e:
+        e.step(warmup, en.DT, kernel)
+        t0 = time.perf_counter()
+        rs = e.step(steps, en.DT, kernel)
+        wall = time.perf_counter() - t0
+    pairs = 0
+    t0 = time.perf_counter()
+    for b, sl in list(zip(boxes, en.box_slices(counts)))[:sample]:
+        with ap.Engine((0.0, 0.0, 0.0), (b.length,) * 3, en.RC, en.SKIN, cluster_size=M, tile_groups=G) as a:
+            a.upload(st["x"][sl], st["y"][sl], st["z"][sl], st["vx"][sl], st["vy"][sl], st["vz"][sl])
+            pairs += a.step(steps, en.DT, kernel).pair_interactions
+    one_wall = time.perf_counter() - t0
+    value = rs.pair_interactions / wall
+    one = pairs / one_wall
+    return {
+        "workload": f"{nboxes} independent periodic LJ boxes, N log-uniform in [64, 8192], rho U(0.3, 0.9), "
+                    f"T U(0.7, 1.5), rc 2.5, skin 0.3, {steps} steps at dt 0.002, one context for all boxes",
+        "boxes": nboxes,
+        "particles": int(counts.sum()),
+        "layout": best,
+        "kernel": ap.KERNEL_NAMES[kernel],
+        "value": value,
+        "unit": "pair-interactions/s",
+        "ms_per_step": wall / steps * 1e3,
+        "box_steps_per_s": nboxes * steps / wall,
+        "rebuilds": rs.rebuilds,
+        "tuning_pairs_per_s": {k: v[0] for k, v in table.items()},
+        "one_box_at_a_time": {"sample_boxes": sample, "pairs_per_s": one,
+                              "note": "each box in its own context, run to completion one after another"},
+        "speedup_vs_one_at_a_time": value / one if one else None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -180,6 +242,9 @@ def main():
         sys.exit(2)
     import autopas_amd as ap
 
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(args.ensemble_boxes, args.ensemble_steps)}))
+        return
     s = make_system(args.n_side)
     if args.layout:
         kind, m, g = args.layout.split("_")
@@ -262,6 +327,8 @@ def main():
     }
     if args.serve_jobs > 0:
         out["serving"] = serving(args.serve_jobs, args.serve_workers)
+    if args.ensemble_boxes > 0:
+        out["ensemble"] = ensemble_bench(args.ensemble_boxes, args.ensemble_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

@@ -64,7 +64,10 @@ typedef struct {
                              0 = max(3, rebuild_frequency) */
   int32_t tuning_interval; /* LJG_KERNEL_AUTO: steps between tuning phases (AutoPas tuningInterval); 0 = tune
                               once */
-  int32_t reserved[3];
+  int32_t list_mode;      /* cluster-entry list pass: 0 = by grid (a lane per i-cluster when towers hold few
+                           * clusters, else a wavefront per tower), 1 = wavefront per tower, 2 = lane per
+                           * i-cluster; all give the same lists */
+  int32_t reserved[2];
 } ljg_params;
 
 typedef struct {

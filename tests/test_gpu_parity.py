@@ -62,13 +62,16 @@ def test_binning_bit_exact(oracle):
     assert np.array_equal(cells, ref)
 
 
+@pytest.mark.parametrize("list_mode", [1, 2])
 @pytest.mark.parametrize("M", [1, 2, 4, 8])
-def test_cluster_lists_bit_exact(oracle, M):
-    announce(f"cluster build bit-exact M={M}: 10^3 lattice rho 0.75")
+def test_cluster_lists_bit_exact(oracle, M, list_mode):
+    # list_mode 1: a wavefront per tower, 2: a lane per i-cluster (towers of ~20 particles here, so the
+    # towers are sorted in registers)
+    announce(f"cluster build bit-exact M={M} list_mode={list_mode}: 10^3 lattice rho 0.75")
     s = lattice_box(10, 0.75, seed=7)
     L = s["L"]
     ids = np.arange(s["n"], dtype=np.int64)[::-1].copy()  # ids differ from upload index
-    with make_engine(L, M=M) as e:
+    with make_engine(L, M=M, list_mode=list_mode) as e:
         e.upload(s["x"], s["y"], s["z"], id=ids)
         e.build_pairs()
         g = e.clusters()
@@ -99,6 +102,28 @@ def test_verlet_lists_exact(oracle):
     assert np.array_equal(off, roff)
     for p in range(s["n"]):
         assert np.array_equal(np.sort(nb[off[p]:off[p + 1]]), rnb[roff[p]:roff[p + 1]]), p
+
+
+@pytest.mark.parametrize("list_mode", [1, 2])
+@pytest.mark.parametrize("n_side", [28, 50])
+def test_cluster_lists_bit_exact_long_towers(oracle, n_side, list_mode):
+    # M = 8 at rho 0.75: ~4 n_side particles per tower.  n_side 28 (~112 per tower): towers up to 64
+    # particles and those of 65-256 are sorted in registers by two kernels; n_side 50 (~200 per tower):
+    # every tower is sorted in LDS.
+    announce(f"cluster build bit-exact M=8 list_mode={list_mode}: {n_side}^3 lattice rho 0.75")
+    s = lattice_box(n_side, 0.75, seed=17)
+    L = s["L"]
+    ids = np.arange(s["n"], dtype=np.int64)
+    with make_engine(L, M=8, list_mode=list_mode) as e:
+        e.upload(s["x"], s["y"], s["z"], id=ids)
+        e.build_pairs()
+        g = e.clusters()
+    per_tower = s["n"] / (g["towers_per_dim"][0] * g["towers_per_dim"][1])
+    assert (64 < per_tower <= 192) if n_side == 28 else per_tower > 192
+    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), (L, L, L), RC + SKIN, 8)
+    for k in ("slot_particle", "cluster_tower", "list_offsets", "list_j", "list_shift", "bbox"):
+        assert np.array_equal(g[k], r[k]), k
 
 
 @pytest.mark.parametrize("M,n_side", [(1, 10), (2, 10), (4, 10), (8, 10), (4, 6), (8, 6)])
