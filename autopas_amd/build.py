@@ -58,8 +58,24 @@ def _compile(src, extra):
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "autopas_api_test.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "autopas_api_test")
+
+
+def build_cpp_test():
+    """Compile the driver of the C++ host API tests (include/autopas_gpu.hpp) against libljgpu.so, with the
+    host compiler and no FP contraction (its integrator rounds like TimeDiscretization.cpp)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+           CPP_TEST_SRC, "-L", HERE, "-lljgpu", "-Wl,-rpath,$ORIGIN/../../autopas_amd", "-o", CPP_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST_BIN
+
+
 def build(verbose=False):
-    """Compile all sources and link libljgpu.so (always rebuilds)."""
+    """Compile all sources and link libljgpu.so (always rebuilds), then the C++ API test driver."""
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
@@ -69,8 +85,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_test()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CPP_TEST_BIN}")
     return LIB
 
 

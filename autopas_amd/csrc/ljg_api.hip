@@ -216,10 +216,31 @@ ljg_status ljg_device_name(const ljg_ctx *h, char *buf, int32_t buflen) {
   return LJG_OK;
 }
 
+static void mix_types(Ctx &c, int32_t ntypes, const double *eps, const double *sig, const double *mass);
+
 ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const double *sig, const double *mass) {
   if (!h || ntypes < 1 || ntypes > kMaxTypes || !eps || !sig) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
+  mix_types(c, ntypes, eps, sig, mass);
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_shift(ljg_ctx *h, int32_t apply_shift) {
+  LJG_API_BEGIN(h)
+  c.prm.apply_shift = apply_shift != 0;
+  const LJParams old = c.lj;
+  mix_types(c, old.ntypes, old.eps, old.sig, old.mass);
+  return LJG_OK;
+  LJG_API_END
+}
+
+static void mix_types(Ctx &c, int32_t ntypes, const double *eps, const double *sig, const double *mass) {
   LJParams &lj = c.lj;
+  for (int a = 0; a < ntypes; ++a) {
+    lj.eps[a] = eps[a];
+    lj.sig[a] = sig[a];
+  }
   lj.ntypes = ntypes;
   const double rc2 = c.prm.cutoff * c.prm.cutoff;
   lj.rc2 = rc2;
@@ -255,8 +276,6 @@ ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const do
   LJG_HIP(hipMemcpyAsync(c.tab.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
   c.types_set = true;
-  return LJG_OK;
-  LJG_API_END
 }
 
 ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max[3]) {
@@ -412,6 +431,42 @@ ljg_status ljg_set_forces(ljg_ctx *h, const double *fx, const double *fy, const 
                        s.orig.p, c.part_e.p, dst[d]);
     LJG_HIP(hipGetLastError());
   }
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_update_positions(ljg_ctx *h, const double *x, const double *y, const double *z, const double *vx,
+                                const double *vy, const double *vz, int32_t *moved) {
+  if (!h || !x || !y || !z) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  const bool vel = vx && vy && vz;
+  const size_t n = (size_t)c.n;
+  c.part_v.ensure((vel ? 6 : 3) * n);
+  double *buf = c.part_v.p;
+  const double *src[6] = {x, y, z, vx, vy, vz};
+  for (int k = 0; k < (vel ? 6 : 3); ++k)
+    LJG_HIP(hipMemcpyAsync(buf + k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)1, c.flag.p + 2);
+  LJG_HIP(hipGetLastError());
+  md_set_positions(c, buf, vel ? buf + 3 * n : nullptr);
+  int32_t mv = 0;
+  LJG_HIP(hipMemcpyAsync(&mv, c.flag.p + 2, sizeof mv, hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  c.have_forces = false;
+  c.box_glob_valid = false;
+  if (moved) *moved = mv != 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_rebuild(ljg_ctx *h, int32_t kernel) {
+  if (!h || kernel < 0 || kernel > LJG_KERNEL_LAST) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
+  rebuild_for(c, kernel);
   LJG_HIP(hipStreamSynchronize(c.stream));
   return LJG_OK;
   LJG_API_END

@@ -176,6 +176,7 @@ struct LJParams {
   double eps24[kMaxTypes * kMaxTypes], sig2[kMaxTypes * kMaxTypes], shift6[kMaxTypes * kMaxTypes];
   double inv2m[kMaxTypes];  // 1/(2m) per type
   double mass[kMaxTypes];
+  double eps[kMaxTypes], sig[kMaxTypes];  // per-type parameters the tables were mixed from
 };
 
 enum class Layout { None, Upload, Cells, Clusters, Verlet };
@@ -325,6 +326,10 @@ void wrap_positions(Ctx &c);               // ljg_md.hip
 // Rebuild permutation (also sets the rebuild reference positions; writes posfx when given).
 void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx = nullptr);  // ljg_md.hip
 void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  // ljg_md.hip
+// Positions (3n: x, y, z in upload order) into the current layout, velocities too when vel != nullptr,
+// the derived FP32 / fixed-point positions refreshed; flag[2] = 1 if a particle moved skin/2 since the
+// last rebuild.  ljg_md.hip
+void md_set_positions(Ctx &c, const double *pos, const double *vel);
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
 // Ensemble (ljg_ensemble.hip): per-box tower grids for cluster size c.M, the tower histogram that wraps

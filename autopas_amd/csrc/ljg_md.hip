@@ -203,6 +203,48 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   LJG_HIP(hipGetLastError());
 }
 
+// Positions written by the caller between two force evaluations (upload order in, storage order out),
+// with the writes of k_kick_drift's drift: the displacement check against the rebuild positions and the
+// FP32 cluster-relative / fixed-point copies of the layout.
+__global__ void k_set_positions(int64_t nslots, StateView s, int64_t n, const double *__restrict__ pos,
+                                const double *__restrict__ vel, MdArgs a, const double4 *__restrict__ c0,
+                                float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ moved) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const int32_t o = s.orig[i];
+  if (o < 0) return;
+  const double x = pos[o], y = pos[n + o], z = pos[2 * n + o];
+  s.x[i] = x; s.y[i] = y; s.z[i] = z;
+  if (vel) {
+    s.vx[i] = vel[o]; s.vy[i] = vel[n + o]; s.vz[i] = vel[2 * n + o];
+  }
+  const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
+  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) moved[0] = 1;
+  const int t = s.type[i];
+  if (a.write_pos32) {
+    const double4 r = c0[i / a.M];
+    pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
+  }
+  if (a.write_fixed) {
+    posfx[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - a.fg.min[0]) * a.fg.inv_scale[0]),
+                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - a.fg.min[1]) * a.fg.inv_scale[1]),
+                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - a.fg.min[2]) * a.fg.inv_scale[2]), t);
+  }
+}
+
+void md_set_positions(Ctx &c, const double *pos, const double *vel) {
+  MdArgs a{};
+  const double hs = c.prm.skin * 0.5;
+  a.half_skin2 = hs * hs;
+  a.write_pos32 = c.layout == Layout::Clusters && !c.jrows_valid;
+  a.write_fixed = c.layout == Layout::Verlet || (c.layout == Layout::Clusters && c.jrows_valid);
+  a.fg = c.fg;
+  a.M = c.M;
+  hipLaunchKernelGGL(k_set_positions, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                     view(c.st[c.cur]), c.n, pos, vel, a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p + 2);
+  LJG_HIP(hipGetLastError());
+}
+
 IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   State &s = c.st[c.cur];
   IntegArgs a{};

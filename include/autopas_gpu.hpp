@@ -1,0 +1,577 @@
+/*
+ * autopas_gpu.hpp -- C++ host API of the MI355X engine: the part of AutoPas's interface that the
+ * Lennard-Jones short-range path uses, over the C ABI of ljgpu.h.  Header-only; link libljgpu.so.
+ *
+ * A program written against AutoPas for this path,
+ *
+ *   autopas::AutoPas<mdLib::MoleculeLJ> autoPas;            AutoPasDecl.h:46-110
+ *   autoPas.setBoxMin(..); autoPas.setBoxMax(..); autoPas.setCutoff(..); autoPas.setVerletSkin(..);
+ *   autoPas.setVerletRebuildFrequency(..); autoPas.init();
+ *   autoPas.addParticle(p);                                 AutoPasDecl.h:172
+ *   mdLib::LJFunctor<mdLib::MoleculeLJ, true, true, autopas::FunctorN3Modes::Both, true> f(cutoff, ppl);
+ *   autoPas.computeInteractions(&f);                        AutoPasDecl.h:271, AutoPasImpl.h:116-135
+ *   for (auto &p : autoPas) { ... }                         AutoPasDecl.h:280
+ *
+ * compiles against the namespace autopas_amd with the same calls.  What differs:
+ *   - the box is periodic inside the engine: updateContainer() returns no leaving particles, and the
+ *     positions come back wrapped into the box after each list rebuild (md-flexible's periodic wrap,
+ *     RegularGridDecomposition.cpp:563-587, done on the device);
+ *   - the functor's per-pair and per-SoA entry points are not provided (the GPU evaluates the whole
+ *     container); they throw "not implemented" like PairwiseFunctor's defaults (PairwiseFunctor.h:56-104);
+ *   - step(n, dt) runs n velocity-Verlet steps on the device (the engine's own loop);
+ *   - setKernel() picks the GPU layout (AutoPas picks containers and traversals through its tuner).
+ * Every error is thrown as autopas_amd::AutoPasException, as ExceptionHandler does by default
+ * (ExceptionHandler.cpp:12).
+ */
+#ifndef AUTOPAS_GPU_HPP
+#define AUTOPAS_GPU_HPP
+
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ljgpu.h"
+
+namespace autopas_amd {
+
+/* autopas::utils::ExceptionHandler::AutoPasException (ExceptionHandler.h:116-140). */
+class AutoPasException : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
+namespace detail {
+inline void check(const ljg_ctx *ctx, ljg_status st, const char *what) {
+  if (st == LJG_OK) return;
+  std::string msg = std::string(what) + " failed (status " + std::to_string(st) + ")";
+  if (st == LJG_ERR_NODEVICE) msg += ": no HIP device available";
+  if (ctx) {
+    const char *e = ljg_last_error(ctx);
+    if (e && *e) msg += ": " + std::string(e);
+  }
+  throw AutoPasException(msg);
+}
+}  // namespace detail
+
+/* autopas::OwnershipState (OwnershipState.h:20-29). */
+enum class OwnershipState : int64_t { dummy = 0, owned = 1, halo = 2 };
+
+/* autopas::FunctorN3Modes (Functor.h). */
+enum class FunctorN3Modes { Newton3Only, Newton3Off, Both };
+
+/* mdLib::MoleculeLJ (MoleculeLJ.h:20-212 over ParticleBase.h:32-445): position, velocity, force, old
+ * force, id, type and ownership. */
+class MoleculeLJ {
+ public:
+  MoleculeLJ() = default;
+  MoleculeLJ(const std::array<double, 3> &pos, const std::array<double, 3> &v, unsigned long moleculeId,
+             unsigned long typeId = 0)
+      : _r(pos), _v(v), _id(moleculeId), _typeId(typeId) {}
+
+  const std::array<double, 3> &getR() const { return _r; }
+  void setR(const std::array<double, 3> &r) { _r = r; }
+  void addR(const std::array<double, 3> &r) {
+    for (int d = 0; d < 3; ++d) _r[d] += r[d];
+  }
+  const std::array<double, 3> &getV() const { return _v; }
+  void setV(const std::array<double, 3> &v) { _v = v; }
+  void addV(const std::array<double, 3> &v) {
+    for (int d = 0; d < 3; ++d) _v[d] += v[d];
+  }
+  const std::array<double, 3> &getF() const { return _f; }
+  void setF(const std::array<double, 3> &f) { _f = f; }
+  void addF(const std::array<double, 3> &f) {
+    for (int d = 0; d < 3; ++d) _f[d] += f[d];
+  }
+  void subF(const std::array<double, 3> &f) {
+    for (int d = 0; d < 3; ++d) _f[d] -= f[d];
+  }
+  const std::array<double, 3> &getOldF() const { return _oldF; }
+  void setOldF(const std::array<double, 3> &f) { _oldF = f; }
+  size_t getID() const { return _id; }
+  void setID(size_t id) { _id = id; }
+  size_t getTypeId() const { return _typeId; }
+  void setTypeId(size_t t) { _typeId = t; }
+  OwnershipState getOwnershipState() const { return _ownership; }
+  void setOwnershipState(OwnershipState s) { _ownership = s; }
+  bool isOwned() const { return _ownership == OwnershipState::owned; }
+  bool isHalo() const { return _ownership == OwnershipState::halo; }
+  bool isDummy() const { return _ownership == OwnershipState::dummy; }
+
+ private:
+  std::array<double, 3> _r{}, _v{}, _f{}, _oldF{};
+  size_t _id = 0;
+  size_t _typeId = 0;
+  OwnershipState _ownership = OwnershipState::owned;
+};
+
+/* ParticlePropertiesLibrary<double, size_t> (ParticlePropertiesLibrary.h:30-489): per-site mass, epsilon
+ * and sigma, Lorentz-Berthelot mixing (:443-489) and the shift (calcShift6, :575-582).  The engine mixes
+ * the same way on its own from the per-site values; the tables here serve the getters. */
+class ParticlePropertiesLibrary {
+ public:
+  explicit ParticlePropertiesLibrary(double cutoff) : _cutoff(cutoff) {}
+
+  void addSiteType(size_t siteId, double mass) {
+    if (siteId != _mass.size())
+      throw AutoPasException("ParticlePropertiesLibrary::addSiteType(): trying to register a site type with id " +
+                             std::to_string(siteId) +
+                             ". Please register types consecutively, starting at id 0. Currently there are " +
+                             std::to_string(_mass.size()) + " registered types.");
+    _mass.push_back(mass);
+    _eps.push_back(0.);
+    _sig.push_back(0.);
+  }
+  void addLJParametersToSite(size_t siteId, double epsilon, double sigma) {
+    if (siteId >= _mass.size())
+      throw AutoPasException("ParticlePropertiesLibrary::addLJParametersToSite(): site id " +
+                             std::to_string(siteId) + " is not registered.");
+    _eps[siteId] = epsilon;
+    _sig[siteId] = sigma;
+  }
+  void calculateMixingCoefficients() {
+    const size_t n = _mass.size();
+    if (n == 0) throw AutoPasException("ParticlePropertiesLibrary::calculateMixingCoefficients(): no site types.");
+    const double rc2 = _cutoff * _cutoff;
+    _mix.assign(3 * n * n, 0.);
+    for (size_t a = 0; a < n; ++a) {
+      for (size_t b = 0; b < n; ++b) {
+        const double e24 = 24 * std::sqrt(_eps[a] * _eps[b]);
+        const double s = (_sig[a] + _sig[b]) / 2.0;
+        const double s2 = s * s;
+        const double q2 = s2 / rc2, q6 = q2 * q2 * q2;
+        double *m = &_mix[3 * (n * a + b)];
+        m[0] = e24;
+        m[1] = s2;
+        m[2] = e24 * (q6 - q6 * q6);
+      }
+    }
+  }
+  int getNumberRegisteredSiteTypes() const { return (int)_mass.size(); }
+  double getSiteMass(size_t i) const { return _mass.at(i); }
+  double getEpsilon(size_t i) const { return _eps.at(i); }
+  double getSigma(size_t i) const { return _sig.at(i); }
+  double getMixing24Epsilon(size_t i, size_t j) const { return mixed(i, j, 0); }
+  double getMixingSigmaSquared(size_t i, size_t j) const { return mixed(i, j, 1); }
+  double getMixingShift6(size_t i, size_t j) const { return mixed(i, j, 2); }
+  double getCutoff() const { return _cutoff; }
+
+  const std::vector<double> &masses() const { return _mass; }
+  const std::vector<double> &epsilons() const { return _eps; }
+  const std::vector<double> &sigmas() const { return _sig; }
+
+ private:
+  double mixed(size_t i, size_t j, int k) const {
+    const size_t n = _mass.size();
+    if (_mix.size() != 3 * n * n)
+      throw AutoPasException("ParticlePropertiesLibrary: calculateMixingCoefficients() was not called.");
+    return _mix.at(3 * (n * i + j) + k);
+  }
+  double _cutoff;
+  std::vector<double> _mass, _eps, _sig, _mix;
+};
+
+/* mdLib::LJFunctor<Particle_T, applyShift, useMixing, useNewton3, calculateGlobals, countFLOPs,
+ * relevantForTuning> (LJFunctor.h:39-120).  The object passed to AutoPas::computeInteractions; the engine
+ * evaluates the whole container with it and fills in what endTraversal would (LJFunctor.h:641-720). */
+template <class Particle_T, bool applyShift = false, bool useMixing = false,
+          FunctorN3Modes useNewton3 = FunctorN3Modes::Both, bool calculateGlobals = false, bool countFLOPs = false,
+          bool relevantForTuning = true>
+class LJFunctor {
+ public:
+  static constexpr bool kApplyShift = applyShift;
+  static constexpr bool kUseMixing = useMixing;
+  static constexpr bool kCalculateGlobals = calculateGlobals;
+  static constexpr bool kCountFLOPs = countFLOPs;
+
+  LJFunctor() = delete;
+  explicit LJFunctor(double cutoff) : _cutoff(cutoff) {
+    static_assert(not useMixing,
+                  "Mixing without a ParticlePropertiesLibrary is not possible! Use a different constructor or set "
+                  "mixing to false.");
+  }
+  LJFunctor(double cutoff, ParticlePropertiesLibrary &particlePropertiesLibrary)
+      : _cutoff(cutoff), _ppl(&particlePropertiesLibrary) {
+    static_assert(useMixing,
+                  "Not using Mixing but using a ParticlePropertiesLibrary is not allowed! Use a different constructor "
+                  "or set mixing to true.");
+  }
+
+  std::string getName() { return "LJFunctorGPU"; }
+  bool isRelevantForTuning() { return relevantForTuning; }
+  bool allowsNewton3() { return useNewton3 == FunctorN3Modes::Newton3Only or useNewton3 == FunctorN3Modes::Both; }
+  bool allowsNonNewton3() { return useNewton3 == FunctorN3Modes::Newton3Off or useNewton3 == FunctorN3Modes::Both; }
+  double getCutoff() const { return _cutoff; }
+
+  /* Single-type constants (LJFunctor.h:588-596): 24 epsilon and sigma^2. */
+  void setParticleProperties(double epsilon24, double sigmaSquared) {
+    _epsilon24 = epsilon24;
+    _sigmaSquared = sigmaSquared;
+  }
+  double getEpsilon24() const { return _epsilon24; }
+  double getSigmaSquared() const { return _sigmaSquared; }
+  ParticlePropertiesLibrary *getParticlePropertiesLibrary() const { return _ppl; }
+
+  void initTraversal() {
+    _postProcessed = false;
+    _potentialEnergy = 0.;
+    _virial = 0.;
+  }
+
+  /* LJFunctor.h:691-720, same messages. */
+  double getPotentialEnergy() {
+    if (not calculateGlobals)
+      throw AutoPasException(
+          "Trying to get potential energy even though calculateGlobals is false. If you want this functor to "
+          "calculate global values, please specify calculateGlobals to be true.");
+    if (not _postProcessed) throw AutoPasException("Cannot get potential energy, because endTraversal was not called.");
+    return _potentialEnergy;
+  }
+  double getVirial() {
+    if (not calculateGlobals)
+      throw AutoPasException(
+          "Trying to get virial even though calculateGlobals is false. If you want this functor to calculate "
+          "global values, please specify calculateGlobals to be true.");
+    if (not _postProcessed) throw AutoPasException("Cannot get virial, because endTraversal was not called.");
+    return _virial;
+  }
+
+  /* FLOP model of LJFunctor.h:722-789: 8 per distance, 15 per non-Newton-3 kernel call (the engine's lists
+   * are full, so every pair within rc is two kernel calls), 8 or 9 per non-Newton-3 globals calculation. */
+  size_t getNumFLOPs() const {
+    if constexpr (countFLOPs) {
+      const size_t calls = 2 * _pairs;
+      return 8 * _distanceCalls + 15 * calls + (calculateGlobals ? (applyShift ? 9 : 8) * calls : 0);
+    } else {
+      return std::numeric_limits<size_t>::max();
+    }
+  }
+  double getHitRate() const {
+    if constexpr (countFLOPs) {
+      return _distanceCalls ? (double)(2 * _pairs) / (double)_distanceCalls : 0.;
+    } else {
+      return std::numeric_limits<double>::quiet_NaN();
+    }
+  }
+
+  void AoSFunctor(Particle_T &, Particle_T &, bool) { throw AutoPasException(getName() + "::AoSFunctor: not implemented"); }
+  template <class SoAView>
+  void SoAFunctorSingle(SoAView, bool) {
+    throw AutoPasException(getName() + "::SoAFunctorSingle: not implemented");
+  }
+  template <class SoAView>
+  void SoAFunctorPair(SoAView, SoAView, bool) {
+    throw AutoPasException(getName() + "::SoAFunctorPair: not implemented");
+  }
+  template <class SoAView, class NeighborList>
+  void SoAFunctorVerlet(SoAView, size_t, const NeighborList &, bool) {
+    throw AutoPasException(getName() + "::SoAFunctorVerlet: not implemented");
+  }
+
+  /* Called by AutoPas::computeInteractions with the engine's results (endTraversal's role). */
+  void setTraversalResults(const ljg_globals &g) {
+    _potentialEnergy = g.potential_energy;
+    _virial = g.virial;
+    _pairs += (size_t)g.pair_count;
+    _distanceCalls += g.candidate_evals > 0 ? (size_t)g.candidate_evals : 0;
+    _postProcessed = true;
+  }
+
+ private:
+  double _cutoff;
+  ParticlePropertiesLibrary *_ppl = nullptr;
+  double _epsilon24 = 24.;
+  double _sigmaSquared = 1.;
+  bool _postProcessed = false;
+  double _potentialEnergy = 0., _virial = 0.;
+  size_t _pairs = 0, _distanceCalls = 0;
+};
+
+/* autopas::AutoPas<Particle_T> (AutoPasDecl.h) for the Lennard-Jones path: owned particles in a periodic
+ * box, kept host-side in insertion order and mirrored on the device. */
+template <class Particle_T>
+class AutoPas {
+ public:
+  using ParticleType = Particle_T;
+  using iterator = typename std::vector<Particle_T>::iterator;
+  using const_iterator = typename std::vector<Particle_T>::const_iterator;
+
+  AutoPas() = default;
+  AutoPas(const AutoPas &) = delete;
+  AutoPas &operator=(const AutoPas &) = delete;
+  ~AutoPas() {
+    if (_ctx) ljg_destroy(_ctx);
+  }
+
+  void setBoxMin(const std::array<double, 3> &boxMin) { _boxMin = boxMin; }
+  void setBoxMax(const std::array<double, 3> &boxMax) { _boxMax = boxMax; }
+  const std::array<double, 3> &getBoxMin() const { return _boxMin; }
+  const std::array<double, 3> &getBoxMax() const { return _boxMax; }
+  void setCutoff(double cutoff) {
+    if (cutoff <= 0.) throw AutoPasException("Cutoff <= 0.0: " + std::to_string(cutoff));
+    _cutoff = cutoff;
+  }
+  double getCutoff() const { return _cutoff; }
+  void setVerletSkin(double skin) { _skin = skin; }
+  double getVerletSkin() const { return _skin; }
+  void setVerletRebuildFrequency(unsigned int f) { _rebuildFrequency = f; }
+  unsigned int getVerletRebuildFrequency() const { return _rebuildFrequency; }
+  void setVerletClusterSize(unsigned int m) { _clusterSize = m; }
+  unsigned int getVerletClusterSize() const { return _clusterSize; }
+
+  /* GPU layout (LJG_KERNEL_*); -1 (default): the j-row kernel when every box side exceeds 2 (rc + skin),
+   * else the FP32 cluster kernel.  LJG_KERNEL_AUTO applies to step() only. */
+  void setKernel(int32_t kernel) { _kernel = kernel; }
+  int32_t getKernel() const { return _kernel; }
+  void setDevice(int32_t device) { _device = device; }
+
+  /* AutoPas::init (AutoPasImpl.h:55-113): creates the engine context on the device. */
+  void init() {
+    if (_ctx) {
+      ljg_destroy(_ctx);
+      _ctx = nullptr;
+    }
+    ljg_params p{};
+    for (int d = 0; d < 3; ++d) {
+      p.box_min[d] = _boxMin[d];
+      p.box_max[d] = _boxMax[d];
+    }
+    p.cutoff = _cutoff;
+    p.skin = _skin;
+    p.cluster_size = (int32_t)_clusterSize;
+    p.rebuild_frequency = (int32_t)_rebuildFrequency;
+    p.dynamic_rebuild = 1;
+    p.apply_shift = 0;
+    p.device = _device;
+    ljg_ctx *ctx = nullptr;
+    detail::check(nullptr, ljg_create(&p, &ctx), "AutoPas::init: ljg_create");
+    _ctx = ctx;
+    _shift = 0;
+    _structureChanged = true;
+  }
+
+  /* AutoPas::resizeBox (AutoPasDecl.h:116): particles outside the new box are removed and returned. */
+  std::vector<Particle_T> resizeBox(const std::array<double, 3> &boxMin, const std::array<double, 3> &boxMax) {
+    requireInit("resizeBox");
+    detail::check(_ctx, ljg_set_box(_ctx, boxMin.data(), boxMax.data()), "AutoPas::resizeBox");
+    _boxMin = boxMin;
+    _boxMax = boxMax;
+    std::vector<Particle_T> kept, left;
+    for (auto &p : _particles) (inBox(p.getR()) ? kept : left).push_back(p);
+    _particles = std::move(kept);
+    _structureChanged = true;
+    return left;
+  }
+
+  /* AutoPas::addParticle (AutoPasDecl.h:172, LogicHandler.h:339-360). */
+  void addParticle(const Particle_T &p) {
+    if (!inBox(p.getR()))
+      throw AutoPasException("LogicHandler: Trying to add a particle that is not in the bounding box.");
+    _particles.push_back(p);
+    _structureChanged = true;
+  }
+  void deleteAllParticles() {
+    _particles.clear();
+    _structureChanged = true;
+  }
+  size_t getNumberOfParticles() const { return _particles.size(); }
+
+  /* AutoPas::updateContainer (AutoPasDecl.h:140).  The engine's box is periodic: no particle leaves, and
+   * positions are wrapped into the box on the device at the next list rebuild. */
+  std::vector<Particle_T> updateContainer() { return {}; }
+
+  /* AutoPas::computeInteractions (AutoPasImpl.h:116-135, LogicHandler.h:1066-1143): the functor's pair
+   * forces are added to the particles' forces, and the functor's globals are set.  Lists are rebuilt as
+   * LogicHandler does (LogicHandler.h:987-1015): on the first call after a change of the particle set,
+   * every getVerletRebuildFrequency() calls, and when a particle moved skin/2.  Returns false (no tuning
+   * phase in progress). */
+  template <class Functor>
+  bool computeInteractions(Functor *f) {
+    requireInit("computeInteractions");
+    if (!f) throw AutoPasException("AutoPas::computeInteractions: null functor");
+    if (f->getCutoff() != _cutoff)
+      throw AutoPasException("AutoPas::computeInteractions: the functor's cutoff differs from AutoPas's cutoff");
+    applyFunctor(*f);
+    const int32_t kernel = forceKernel();
+    const size_t n = _particles.size();
+    if (n == 0) {
+      f->initTraversal();
+      f->setTraversalResults(ljg_globals{});
+      return false;
+    }
+    pushState();
+    if (_rebuildDue || _stepsSinceRebuild >= _rebuildFrequency) {
+      detail::check(_ctx, ljg_rebuild(_ctx, kernel), "AutoPas::computeInteractions: rebuild");
+      _stepsSinceRebuild = 0;
+    }
+    f->initTraversal();
+    ljg_globals g{};
+    detail::check(_ctx, ljg_force(_ctx, kernel, Functor::kCalculateGlobals ? 1 : 0, &g),
+                  "AutoPas::computeInteractions: ljg_force");
+    f->setTraversalResults(g);
+    pullState(true);
+    ++_stepsSinceRebuild;
+    return false;
+  }
+
+  /* The engine's own loop: nsteps velocity-Verlet steps on the device (TimeDiscretization.cpp:17-165 with
+   * the rebuild policy of LogicHandler.h:987-1015), then the particles are brought back. */
+  ljg_run_stats step(int32_t nsteps, double dt) {
+    requireInit("step");
+    ljg_run_stats rs{};
+    if (_particles.empty() || nsteps <= 0) return rs;
+    pushState();
+    const int32_t kernel = _kernel >= 0 ? _kernel : defaultKernel();
+    detail::check(_ctx, ljg_step(_ctx, nsteps, dt, kernel, &rs), "AutoPas::step");
+    pullState(false);
+    _stepsSinceRebuild = 0;
+    _rebuildDue = false;
+    return rs;
+  }
+
+  iterator begin() { return _particles.begin(); }
+  iterator end() { return _particles.end(); }
+  const_iterator begin() const { return _particles.begin(); }
+  const_iterator end() const { return _particles.end(); }
+  template <class Lambda>
+  void forEach(Lambda &&fn) {
+    for (auto &p : _particles) fn(p);
+  }
+
+  ljg_ctx *context() { return _ctx; }
+
+ private:
+  void requireInit(const char *what) const {
+    if (!_ctx) throw AutoPasException(std::string("AutoPas::") + what + ": init() was not called");
+  }
+  bool inBox(const std::array<double, 3> &r) const {
+    for (int d = 0; d < 3; ++d)
+      if (!(r[d] >= _boxMin[d] && r[d] < _boxMax[d])) return false;
+    return true;
+  }
+  int32_t defaultKernel() const {
+    for (int d = 0; d < 3; ++d)
+      if (!(_boxMax[d] - _boxMin[d] > 2. * (_cutoff + _skin))) return LJG_KERNEL_CLUSTER_MIXED;
+    return LJG_KERNEL_CLUSTER_JROWS_MIXED;
+  }
+  int32_t forceKernel() const {
+    if (_kernel == LJG_KERNEL_AUTO || _kernel < 0) return defaultKernel();
+    return _kernel;
+  }
+
+  template <class Functor>
+  void applyFunctor(Functor &f) {
+    std::vector<double> eps, sig, mass;
+    if constexpr (Functor::kUseMixing) {
+      const ParticlePropertiesLibrary *ppl = f.getParticlePropertiesLibrary();
+      eps = ppl->epsilons();
+      sig = ppl->sigmas();
+      mass = ppl->masses();
+      for (const auto &p : _particles)
+        if (p.getTypeId() >= eps.size())
+          throw AutoPasException("AutoPas::computeInteractions: particle type " + std::to_string(p.getTypeId()) +
+                                 " is not registered in the ParticlePropertiesLibrary");
+    } else {
+      eps = {f.getEpsilon24() / 24.};
+      sig = {std::sqrt(f.getSigmaSquared())};
+      mass = {1.};
+      for (const auto &p : _particles)
+        if (p.getTypeId() != 0)
+          throw AutoPasException("AutoPas::computeInteractions: a functor without mixing needs type 0 particles");
+    }
+    if (eps != _eps || sig != _sig || mass != _mass) {
+      detail::check(_ctx, ljg_set_types(_ctx, (int32_t)eps.size(), eps.data(), sig.data(), mass.data()),
+                    "AutoPas::computeInteractions: ljg_set_types");
+      _eps = eps;
+      _sig = sig;
+      _mass = mass;
+    }
+    const int32_t shift = Functor::kApplyShift ? 1 : 0;
+    if (shift != _shift) {
+      detail::check(_ctx, ljg_set_shift(_ctx, shift), "AutoPas::computeInteractions: ljg_set_shift");
+      _shift = shift;
+    }
+  }
+
+  /* Host particles -> device: a full upload after a change of the particle set, else positions and
+   * velocities into the current layout (which also tells whether a particle moved skin/2). */
+  void pushState() {
+    const size_t n = _particles.size();
+    std::vector<double> a(6 * n);
+    for (size_t i = 0; i < n; ++i) {
+      const auto &p = _particles[i];
+      for (int d = 0; d < 3; ++d) {
+        a[d * n + i] = p.getR()[d];
+        a[(3 + d) * n + i] = p.getV()[d];
+      }
+    }
+    if (_structureChanged) {
+      std::vector<int32_t> type(n);
+      std::vector<int64_t> id(n);
+      for (size_t i = 0; i < n; ++i) {
+        type[i] = (int32_t)_particles[i].getTypeId();
+        id[i] = (int64_t)_particles[i].getID();
+      }
+      detail::check(_ctx,
+                    ljg_upload(_ctx, (int64_t)n, &a[0], &a[n], &a[2 * n], &a[3 * n], &a[4 * n], &a[5 * n],
+                               type.data(), id.data()),
+                    "AutoPas: ljg_upload");
+      _structureChanged = false;
+      _rebuildDue = true;
+    } else {
+      int32_t moved = 0;
+      detail::check(_ctx,
+                    ljg_update_positions(_ctx, &a[0], &a[n], &a[2 * n], &a[3 * n], &a[4 * n], &a[5 * n], &moved),
+                    "AutoPas: ljg_update_positions");
+      _rebuildDue = _rebuildDue || moved != 0;
+    }
+  }
+
+  /* Device -> host particles: positions (wrapped at rebuilds), and either the pair forces added to the
+   * particles' forces (computeInteractions) or the whole state (step). */
+  void pullState(bool addForces) {
+    const size_t n = _particles.size();
+    std::vector<double> a(9 * n);
+    detail::check(_ctx,
+                  ljg_download(_ctx, &a[0], &a[n], &a[2 * n], addForces ? nullptr : &a[3 * n],
+                               addForces ? nullptr : &a[4 * n], addForces ? nullptr : &a[5 * n], &a[6 * n],
+                               &a[7 * n], &a[8 * n]),
+                  "AutoPas: ljg_download");
+    for (size_t i = 0; i < n; ++i) {
+      auto &p = _particles[i];
+      p.setR({a[i], a[n + i], a[2 * n + i]});
+      const std::array<double, 3> f{a[6 * n + i], a[7 * n + i], a[8 * n + i]};
+      if (addForces) {
+        p.addF(f);
+      } else {
+        p.setV({a[3 * n + i], a[4 * n + i], a[5 * n + i]});
+        p.setF(f);
+      }
+    }
+  }
+
+  std::array<double, 3> _boxMin{0., 0., 0.}, _boxMax{10., 10., 10.};
+  double _cutoff = 1.;
+  double _skin = 0.2;
+  unsigned int _rebuildFrequency = 10;
+  unsigned int _clusterSize = 4;
+  int32_t _kernel = -1;
+  int32_t _device = 0;
+  ljg_ctx *_ctx = nullptr;
+  std::vector<Particle_T> _particles;
+  bool _structureChanged = true;
+  bool _rebuildDue = true;
+  unsigned int _stepsSinceRebuild = 0;
+  int32_t _shift = 0;
+  std::vector<double> _eps, _sig, _mass;
+};
+
+}  // namespace autopas_amd
+
+#endif

@@ -144,6 +144,19 @@ ljg_status ljg_upload(ljg_ctx *ctx, int64_t n, const double *x, const double *y,
                       const double *vx, const double *vy, const double *vz, const int32_t *type, const int64_t *id);
 /* Set the current force (e.g. F0 restored from a checkpoint, MDFlexConfig.cpp:173). Upload order. */
 ljg_status ljg_set_forces(ljg_ctx *ctx, const double *fx, const double *fy, const double *fz);
+/* Overwrite the positions (and the velocities, if vx/vy/vz are given) of the uploaded particles, in
+ * upload order, keeping the current layout and neighbour structure: the caller moved its particles between
+ * two force evaluations, as md-flexible's TimeDiscretization does through AutoPas's iterators
+ * (TimeDiscretization.cpp:17-69).  *moved (if given) is set to 1 when a particle is now skin/2 or more
+ * from where it was at the last rebuild (LogicHandler.h:1000-1015), else 0. */
+ljg_status ljg_update_positions(ljg_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                                const double *vy, const double *vz, int32_t *moved);
+/* Rebuild the neighbour structure the given kernel runs on, now (LogicHandler::rebuildNeighborLists,
+ * LogicHandler.h:1082-1090): wrap into the box, bin or cluster, build lists. */
+ljg_status ljg_rebuild(ljg_ctx *ctx, int32_t kernel);
+/* Truncated-shifted energy on or off (LJFunctor's applyShift template argument, LJFunctor.h:39-44); the
+ * mixing table is recomputed from the last ljg_set_types parameters. */
+ljg_status ljg_set_shift(ljg_ctx *ctx, int32_t apply_shift);
 
 /* LinkedCells binning (counting sort into CellBlock3D cells; CellBlock3D.h:320-425, LinkedCells.h:147-203).
  * Leaves the particle state in cell order.  Used by the plain FP64 kernel. */

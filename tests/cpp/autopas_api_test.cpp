@@ -1,0 +1,237 @@
+// Driver for the C++ host API (include/autopas_gpu.hpp), run by tests/test_cpp_api.py.
+//
+//   autopas_api_test errors            API misuse raises AutoPasException (no GPU needed for most)
+//   autopas_api_test nodevice          init() without a GPU throws AutoPasException
+//   autopas_api_test forces <file>     computeInteractions on a seeded lattice; writes x y z fx fy fz per
+//                                      particle and the globals, for comparison with the oracle
+//   autopas_api_test loop <file>       md-flexible's step sequence through the API (host integrator,
+//                                      TimeDiscretization.cpp:17-165) next to AutoPas::step on the device;
+//                                      writes both end states
+//
+// Every mode prints "ok <mode>" on success and exits 0; a failed check prints the reason and exits 1.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "autopas_gpu.hpp"
+
+using autopas_amd::AutoPas;
+using autopas_amd::AutoPasException;
+using autopas_amd::FunctorN3Modes;
+using autopas_amd::MoleculeLJ;
+using autopas_amd::ParticlePropertiesLibrary;
+
+namespace {
+
+constexpr double kCutoff = 2.5, kSkin = 0.3;
+
+int fail(const std::string &why) {
+  std::printf("FAIL %s\n", why.c_str());
+  return 1;
+}
+
+template <class F>
+bool throws(F &&fn, const char *needle) {
+  try {
+    fn();
+  } catch (const AutoPasException &e) {
+    return needle == nullptr || std::strstr(e.what(), needle) != nullptr;
+  }
+  return false;
+}
+
+struct Lattice {
+  double L;
+  std::vector<double> x, y, z, vx, vy, vz;
+  std::vector<int32_t> type;
+};
+
+Lattice lattice(int n_side, double density, uint64_t seed, int ntypes) {
+  Lattice s;
+  const double a = std::pow(density, -1.0 / 3.0);
+  s.L = n_side * a;
+  const int64_t n = (int64_t)n_side * n_side * n_side;
+  for (auto *v : {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz}) v->resize((size_t)n);
+  s.type.resize((size_t)n);
+  const double lo[3] = {0., 0., 0.};
+  if (ljg_generate_lattice(seed, n, n_side, n_side, n_side, a, lo, 0.05, 0.9, ntypes, s.x.data(), s.y.data(),
+                           s.z.data(), s.vx.data(), s.vy.data(), s.vz.data(), s.type.data()) != LJG_OK)
+    throw AutoPasException("ljg_generate_lattice failed");
+  return s;
+}
+
+void setup(AutoPas<MoleculeLJ> &ap, double L, int32_t kernel) {
+  ap.setBoxMin({0., 0., 0.});
+  ap.setBoxMax({L, L, L});
+  ap.setCutoff(kCutoff);
+  ap.setVerletSkin(kSkin);
+  ap.setVerletRebuildFrequency(10);
+  ap.setVerletClusterSize(4);
+  ap.setKernel(kernel);
+  ap.init();
+}
+
+void add_all(AutoPas<MoleculeLJ> &ap, const Lattice &s, bool types) {
+  for (size_t i = 0; i < s.x.size(); ++i)
+    ap.addParticle(MoleculeLJ({s.x[i], s.y[i], s.z[i]}, {s.vx[i], s.vy[i], s.vz[i]}, i,
+                              types ? (unsigned long)s.type[i] : 0ul));
+}
+
+void write_state(FILE *f, const char *tag, AutoPas<MoleculeLJ> &ap) {
+  std::fprintf(f, "%s %zu\n", tag, ap.getNumberOfParticles());
+  for (auto &p : ap)
+    std::fprintf(f, "%zu %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p.getID(), p.getR()[0],
+                 p.getR()[1], p.getR()[2], p.getV()[0], p.getV()[1], p.getV()[2], p.getF()[0], p.getF()[1],
+                 p.getF()[2]);
+}
+
+int mode_errors() {
+  AutoPas<MoleculeLJ> ap;
+  autopas_amd::LJFunctor<MoleculeLJ> f(kCutoff);
+  if (!throws([&] { ap.computeInteractions(&f); }, "init() was not called"))
+    return fail("computeInteractions before init");
+  if (!throws([&] { f.getPotentialEnergy(); }, "calculateGlobals is false")) return fail("globals off");
+  autopas_amd::LJFunctor<MoleculeLJ, true, false, FunctorN3Modes::Both, true> fg(kCutoff);
+  if (!throws([&] { fg.getVirial(); }, "endTraversal was not called")) return fail("virial before traversal");
+  MoleculeLJ a({0., 0., 0.}, {0., 0., 0.}, 0), b({1., 0., 0.}, {0., 0., 0.}, 1);
+  if (!throws([&] { f.AoSFunctor(a, b, true); }, "not implemented")) return fail("AoSFunctor");
+  if (!throws([&] { f.SoAFunctorPair(0, 0, true); }, "not implemented")) return fail("SoAFunctorPair");
+  ap.setBoxMin({0., 0., 0.});
+  ap.setBoxMax({10., 10., 10.});
+  if (!throws([&] { ap.addParticle(MoleculeLJ({10., 1., 1.}, {0., 0., 0.}, 0)); }, "not in the bounding box"))
+    return fail("particle outside the box");
+  ap.addParticle(MoleculeLJ({1., 1., 1.}, {0., 0., 0.}, 0));
+  if (ap.getNumberOfParticles() != 1) return fail("particle count");
+  ParticlePropertiesLibrary ppl(kCutoff);
+  if (!throws([&] { ppl.addSiteType(1, 1.); }, "consecutively")) return fail("site ids must be consecutive");
+  ppl.addSiteType(0, 1.);
+  ppl.addLJParametersToSite(0, 1., 1.);
+  ppl.addSiteType(1, 2.);
+  ppl.addLJParametersToSite(1, 1.21, 1.1);
+  if (!throws([&] { ppl.getMixing24Epsilon(0, 1); }, "calculateMixingCoefficients")) return fail("mixing before calc");
+  ppl.calculateMixingCoefficients();
+  const double e01 = 24 * std::sqrt(1.21), s01 = 1.05 * 1.05;
+  if (std::fabs(ppl.getMixing24Epsilon(0, 1) - e01) > 1e-15 * e01) return fail("mixed epsilon");
+  if (std::fabs(ppl.getMixingSigmaSquared(1, 0) - s01) > 1e-15 * s01) return fail("mixed sigma");
+  std::printf("ok errors\n");
+  return 0;
+}
+
+int mode_nodevice() {
+  AutoPas<MoleculeLJ> ap;
+  ap.setBoxMin({0., 0., 0.});
+  ap.setBoxMax({10., 10., 10.});
+  ap.setCutoff(kCutoff);
+  ap.setVerletSkin(kSkin);
+  try {
+    ap.init();
+  } catch (const AutoPasException &e) {
+    if (std::strstr(e.what(), "no HIP device") == nullptr) return fail(std::string("unexpected message: ") + e.what());
+    std::printf("ok nodevice\n");
+    return 0;
+  }
+  std::printf("ok nodevice (a device is present)\n");
+  return 0;
+}
+
+// Forces and globals through computeInteractions: a single-type functor (setParticleProperties) on the
+// j-row kernel and a two-type mixing functor on the cluster kernel, on the same lattice.
+int mode_forces(const char *path) {
+  FILE *out = std::fopen(path, "w");
+  if (!out) return fail("cannot open output");
+  const Lattice s = lattice(8, 0.75, 3, 2);
+  {
+    AutoPas<MoleculeLJ> ap;
+    setup(ap, s.L, -1);  // default kernel (the j-row kernel for this box)
+    add_all(ap, s, false);
+    autopas_amd::LJFunctor<MoleculeLJ, true, false, FunctorN3Modes::Both, true, true> f(kCutoff);
+    f.setParticleProperties(24., 1.);
+    ap.computeInteractions(&f);
+    std::fprintf(out, "globals %.17g %.17g %.6f %zu\n", f.getPotentialEnergy(), f.getVirial(), f.getHitRate(),
+                 f.getNumFLOPs());
+    write_state(out, "single", ap);
+  }
+  {
+    AutoPas<MoleculeLJ> ap;
+    setup(ap, s.L, LJG_KERNEL_CLUSTER_MIXED);
+    add_all(ap, s, true);
+    ParticlePropertiesLibrary ppl(kCutoff);
+    ppl.addSiteType(0, 1.);
+    ppl.addLJParametersToSite(0, 1., 1.);
+    ppl.addSiteType(1, 1.);
+    ppl.addLJParametersToSite(1, 1.2, 1.05);
+    ppl.calculateMixingCoefficients();
+    autopas_amd::LJFunctor<MoleculeLJ, true, true, FunctorN3Modes::Both, true> f(kCutoff, ppl);
+    ap.computeInteractions(&f);
+    std::fprintf(out, "globals %.17g %.17g 0 0\n", f.getPotentialEnergy(), f.getVirial());
+    write_state(out, "mixing", ap);
+  }
+  std::fclose(out);
+  std::printf("ok forces\n");
+  return 0;
+}
+
+// md-flexible's step: calculatePositionsAndResetForces, updateContainer, computeInteractions,
+// calculateVelocities (TimeDiscretization.cpp:17-69,152-165), with the factors formed as there.
+template <class Functor>
+void host_step(AutoPas<MoleculeLJ> &ap, Functor &f, double dt) {
+  for (auto &p : ap) {
+    const double ff = dt * dt / (2 * 1.0);
+    const auto &v = p.getV();
+    const auto &F = p.getF();
+    p.addR({v[0] * dt + F[0] * ff, v[1] * dt + F[1] * ff, v[2] * dt + F[2] * ff});
+    p.setOldF(F);
+    p.setF({0., 0., 0.});
+  }
+  (void)ap.updateContainer();
+  ap.computeInteractions(&f);
+  for (auto &p : ap) {
+    const double kf = dt / (2 * 1.0);
+    const auto &F = p.getF();
+    const auto &oF = p.getOldF();
+    p.addV({(F[0] + oF[0]) * kf, (F[1] + oF[1]) * kf, (F[2] + oF[2]) * kf});
+  }
+}
+
+int mode_loop(const char *path, int32_t kernel) {
+  FILE *out = std::fopen(path, "w");
+  if (!out) return fail("cannot open output");
+  const Lattice s = lattice(8, 0.75, 5, 1);
+  const double dt = 0.002;
+  const int steps = 25;
+  // the host loop starts from zero force, as the device loop does after the upload
+  AutoPas<MoleculeLJ> host;
+  setup(host, s.L, kernel);
+  add_all(host, s, false);
+  autopas_amd::LJFunctor<MoleculeLJ, true> f(kCutoff);
+  f.setParticleProperties(24., 1.);
+  for (int k = 0; k < steps; ++k) host_step(host, f, dt);
+  write_state(out, "host", host);
+  AutoPas<MoleculeLJ> dev;
+  setup(dev, s.L, kernel);
+  add_all(dev, s, false);
+  const ljg_run_stats rs = dev.step(steps, dt);
+  std::fprintf(out, "stats %lld %lld\n", (long long)rs.steps, (long long)rs.rebuilds);
+  write_state(out, "device", dev);
+  std::fclose(out);
+  std::printf("ok loop\n");
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) return fail("usage: autopas_api_test errors|nodevice|forces <file>|loop <file> [kernel]");
+  const std::string mode = argv[1];
+  try {
+    if (mode == "errors") return mode_errors();
+    if (mode == "nodevice") return mode_nodevice();
+    if (mode == "forces" && argc >= 3) return mode_forces(argv[2]);
+    if (mode == "loop" && argc >= 3) return mode_loop(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
+  } catch (const AutoPasException &e) {
+    return fail(std::string("AutoPasException: ") + e.what());
+  }
+  return fail("unknown mode " + mode);
+}

@@ -1,0 +1,116 @@
+"""C++ host API (include/autopas_gpu.hpp) through its test driver tests/cpp/autopas_api_test.
+
+CPU: misuse raises AutoPasException with the reference's messages; init() without a device throws.
+GPU: computeInteractions forces and globals against the FP64 direct sum (RMS relative force error <= 1e-5,
+energy and virial 1e-5 relative, for the FP32-pair kernels); md-flexible's step sequence through the API
+(host integrator, device forces) against AutoPas::step on the device.  The two loops build their lists
+at different points (the host loop after the first drift, the device loop before it), so they agree to
+the rounding of the FP32 pair term: positions within 1e-6, velocities within 1e-4 after 25 steps; with
+the FP64 cluster kernel within 1e-10 and 1e-8.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import announce
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "autopas_api_test")
+RC = 2.5
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(BIN):
+        from autopas_amd import build
+
+        build.build_cpp_test()
+    return BIN
+
+
+def run(driver, *args, timeout=120):
+    r = subprocess.run([driver, *args], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def read_states(path):
+    out, cur = {}, None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if w[0] in ("globals", "stats"):
+                out.setdefault(w[0], []).append([float(v) for v in w[1:]])
+            elif len(w) == 2:
+                cur = w[0]
+                out[cur] = []
+            else:
+                out[cur].append([float(v) for v in w])
+    for k in list(out):
+        out[k] = np.array(out[k])
+    return out
+
+
+def test_api_misuse_raises(driver):
+    assert "ok errors" in run(driver, "errors")
+
+
+def test_init_without_device_throws(driver):
+    assert "ok nodevice" in run(driver, "nodevice")
+
+
+def rms_rel(f, fr):
+    return float(np.sqrt(np.sum((f - fr) ** 2) / np.sum(fr ** 2)))
+
+
+@pytest.mark.gpu
+def test_compute_interactions_matches_direct_sum(driver, oracle, tmp_path):
+    announce("C++ API computeInteractions: 8^3 lattice, single type (j-row kernel) and two types (cluster kernel)")
+    path = str(tmp_path / "forces.txt")
+    run(driver, "forces", path)
+    st = read_states(path)
+    n_side, rho = 8, 0.75
+    L = n_side * rho ** (-1.0 / 3.0)
+    for k, tag in enumerate(("single", "mixing")):
+        s = st[tag]
+        order = np.argsort(s[:, 0])
+        s = s[order]
+        x, y, z, f = s[:, 1], s[:, 2], s[:, 3], s[:, 7:10]
+        if tag == "single":
+            ref = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L))
+        else:
+            import autopas_amd as ap
+
+            _, _, _, _, _, _, types = ap.generate_lattice(3, n_side ** 3, (n_side,) * 3, rho ** (-1.0 / 3.0),
+                                                          (0.0, 0.0, 0.0), 0.05, 0.9, 2)
+            ref = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L), types=types, eps=(1.0, 1.2), sig=(1.0, 1.05))
+        fr = np.stack([ref["fx"], ref["fy"], ref["fz"]], axis=1)
+        assert rms_rel(f, fr) <= 1e-5, tag
+        e, v = st["globals"][k][0], st["globals"][k][1]
+        assert abs(e - ref["epot"]) <= 1e-5 * abs(ref["epot"]), tag
+        assert abs(v - ref["virial"]) <= 1e-5 * abs(ref["virial"]), tag
+    hit_rate, flops = st["globals"][0][2], st["globals"][0][3]
+    assert 0.05 < hit_rate < 1.0
+    assert flops > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,tol_x,tol_v", [(4, 1e-6, 1e-4), (2, 1e-6, 1e-4), (1, 1e-10, 1e-8)])
+def test_md_loop_through_api_matches_device_steps(driver, tmp_path, kernel, tol_x, tol_v):
+    announce(f"C++ API md loop vs AutoPas::step: kernel {kernel}, 8^3 lattice, 25 steps")
+    path = str(tmp_path / "loop.txt")
+    run(driver, "loop", path, str(kernel))
+    st = read_states(path)
+    h = st["host"][np.argsort(st["host"][:, 0])]
+    d = st["device"][np.argsort(st["device"][:, 0])]
+    assert st["stats"][0][0] == 25 and st["stats"][0][1] >= 3
+    n_side, rho = 8, 0.75
+    L = n_side * rho ** (-1.0 / 3.0)
+    dx = h[:, 1:4] - d[:, 1:4]
+    dx -= L * np.round(dx / L)  # one side may have wrapped a particle the other has not yet
+    assert np.max(np.abs(dx)) <= tol_x
+    assert np.max(np.abs(h[:, 4:7] - d[:, 4:7])) <= tol_v
+    # the loop moved the particles: not a trivially equal pair of initial states
+    assert np.max(np.abs(h[:, 4:7])) > 0.1
