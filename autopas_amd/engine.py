@@ -94,6 +94,9 @@ SIGNATURES = {
     "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "ljg_set_forces": (_i32, [_p, _p, _p, _p]),
+    "ljg_update_positions": (_i32, [_p] + [_p] * 7),
+    "ljg_rebuild": (_i32, [_p, _i32]),
+    "ljg_set_shift": (_i32, [_p, _i32]),
     "ljg_bin": (_i32, [_p]),
     "ljg_build_pairs": (_i32, [_p]),
     "ljg_force": (_i32, [_p, _i32, _i32, ctypes.POINTER(Globals)]),
@@ -333,6 +336,21 @@ class Engine:
     def set_forces(self, fx, fy, fz):
         f = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz)]
         self._check(self.lib.ljg_set_forces(self.h, *[_ptr(a) for a in f]), "ljg_set_forces")
+
+    def update_positions(self, x, y, z, vx=None, vy=None, vz=None):
+        """Overwrite positions (and velocities) in upload order, keeping the lists; returns whether a
+        particle is now skin/2 or more from its position at the last rebuild."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, vx, vy, vz)]
+        moved = ctypes.c_int32(0)
+        self._check(self.lib.ljg_update_positions(self.h, *[_ptr(a) for a in arrs], ctypes.byref(moved)),
+                    "ljg_update_positions")
+        return bool(moved.value)
+
+    def rebuild(self, kernel=KERNEL_CLUSTER_MIXED):
+        self._check(self.lib.ljg_rebuild(self.h, kernel), "ljg_rebuild")
+
+    def set_shift(self, apply_shift):
+        self._check(self.lib.ljg_set_shift(self.h, int(bool(apply_shift))), "ljg_set_shift")
 
     def bin(self):
         self._check(self.lib.ljg_bin(self.h), "ljg_bin")

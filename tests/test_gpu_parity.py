@@ -471,3 +471,35 @@ def test_auto_layout_tuning():
     d = np.stack([st["x"] - ref["x"], st["y"] - ref["y"], st["z"] - ref["z"]])
     d -= L * np.round(d / L)
     assert np.max(np.abs(d)) < 2e-5
+
+
+@pytest.mark.parametrize("kernel", [ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED,
+                                    ap.KERNEL_LC_FP64])
+def test_update_positions_keeps_lists(oracle, kernel):
+    # positions moved by the caller between two force evaluations (< skin/2): the kept lists still give
+    # the forces of the moved system; a move of skin/2 is reported; the shift switch changes the energy
+    # by the pair count times the shift (LJFunctor.h:174-197)
+    announce(f"update positions, kernel={ap.KERNEL_NAMES[kernel]}: 8^3 lattice")
+    s = lattice_box(8, 0.75, seed=21)
+    L = s["L"]
+    rng = np.random.default_rng(4)
+    with make_engine(L) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        e.rebuild(kernel)
+        moved = {k: s[k] + rng.uniform(-0.05, 0.05, s["n"]) for k in ("x", "y", "z")}
+        assert not e.update_positions(moved["x"], moved["y"], moved["z"])
+        g = e.force(kernel, globals=True)
+        f = e.download()
+        e.set_shift(False)
+        g0 = e.force(kernel, globals=True)
+        far = dict(moved)
+        far["x"] = far["x"].copy()
+        far["x"][7] += 0.3  # at least 0.25 from its rebuild position, beyond skin/2 = 0.15
+        assert e.update_positions(far["x"], far["y"], far["z"])
+    ref = oracle.direct_sum(moved["x"], moved["y"], moved["z"], RC, box_len=(L, L, L))
+    tol = 1e-10 if kernel in (ap.KERNEL_CLUSTER_FP64, ap.KERNEL_LC_FP64) else 1e-5
+    assert rms_rel(f, ref) <= tol
+    assert abs(g.potential_energy - ref["epot"]) <= tol * abs(ref["epot"])
+    rc2 = RC * RC
+    shift = 4.0 * ((1.0 / rc2) ** 3 - (1.0 / rc2) ** 6)  # shift6 / 6 for eps = sigma = 1
+    assert abs((g.potential_energy - g0.potential_energy) - g.pair_count * shift) <= 1e-6 * abs(g.potential_energy)
