@@ -218,6 +218,16 @@ ljg_status ljg_device_name(const ljg_ctx *h, char *buf, int32_t buflen) {
 
 static void mix_types(Ctx &c, int32_t ntypes, const double *eps, const double *sig, const double *mass);
 
+ljg_status ljg_memory_info(const ljg_ctx *h, uint64_t *free_bytes, uint64_t *total_bytes) {
+  if (!h || !free_bytes || !total_bytes) return LJG_ERR_ARG;
+  if (hipSetDevice(h->c.device) != hipSuccess) return LJG_ERR_HIP;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return LJG_ERR_HIP;
+  *free_bytes = f;
+  *total_bytes = t;
+  return LJG_OK;
+}
+
 ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const double *sig, const double *mass) {
   if (!h || ntypes < 1 || ntypes > kMaxTypes || !eps || !sig) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
@@ -300,28 +310,32 @@ ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max
   LJG_API_END
 }
 
-ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, const double *box_max,
-                         const int64_t *count) {
-  if (!h || nboxes < 1 || !box_min || !box_max || !count) return LJG_ERR_ARG;
-  LJG_API_BEGIN(h)
+// Box list checks shared by ljg_set_boxes and ljg_edit_boxes; returns the reason or nullptr.
+static const char *check_boxes(const Ctx &c, int32_t nboxes, const double *box_min, const double *box_max,
+                               const int64_t *count, int64_t *total) {
   const double ilen = c.prm.cutoff + c.prm.skin;
-  int64_t total = 0;
+  int64_t t = 0;
   for (int32_t b = 0; b < nboxes; ++b) {
-    if (count[b] < 1) return fail(c, LJG_ERR_ARG, "a box without particles");
+    if (count[b] < 1) return "a box without particles";
     for (int d = 0; d < 3; ++d)
-      if (!(box_max[3 * b + d] - box_min[3 * b + d] >= ilen)) return fail(c, LJG_ERR_ARG, "box shorter than rc + skin");
-    total += count[b];
+      if (!(box_max[3 * b + d] - box_min[3 * b + d] >= ilen)) return "box shorter than rc + skin";
+    t += count[b];
   }
-  if (total > (int64_t)INT32_MAX / 2) return fail(c, LJG_ERR_ARG, "too many particles");
-  LJG_HIP(hipStreamSynchronize(c.stream));
-  c.box_lo.assign(box_min, box_min + 3 * (size_t)nboxes);
-  c.box_hi.assign(box_max, box_max + 3 * (size_t)nboxes);
-  c.box_count.assign(count, count + nboxes);
+  if (t > (int64_t)INT32_MAX / 2) return "too many particles";
+  *total = t;
+  return nullptr;
+}
+
+// Host and device tables of an ensemble (bounds, counts, box of each particle, wrap arguments).
+static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const double *box_max,
+                           const int64_t *count, int64_t total) {
+  std::vector<double> lo(box_min, box_min + 3 * (size_t)nboxes), hi(box_max, box_max + 3 * (size_t)nboxes);
+  std::vector<int64_t> cnt(count, count + nboxes);
   std::vector<int32_t> box_of((size_t)total);
   std::vector<WrapArgs> wr((size_t)nboxes);
   for (int32_t b = 0, o = 0; b < nboxes; ++b) {
-    for (int64_t k = 0; k < count[b]; ++k) box_of[o++] = b;
-    wr[b] = wrap_args(&box_min[3 * b], &box_max[3 * b]);
+    for (int64_t k = 0; k < cnt[b]; ++k) box_of[o++] = b;
+    wr[b] = wrap_args(&lo[3 * b], &hi[3 * b]);
   }
   c.box_of.ensure(box_of.size());
   c.box_wrap.ensure(wr.size());
@@ -329,19 +343,108 @@ ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, cons
                          c.stream));
   LJG_HIP(hipMemcpyAsync(c.box_wrap.p, wr.data(), wr.size() * sizeof(WrapArgs), hipMemcpyHostToDevice, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
+  c.box_lo = std::move(lo);
+  c.box_hi = std::move(hi);
+  c.box_count = std::move(cnt);
   // the single-box parameters keep the first box (the fixed-point and cell geometry never run here)
   for (int d = 0; d < 3; ++d) {
-    c.prm.box_min[d] = box_min[d];
-    c.prm.box_max[d] = box_max[d];
+    c.prm.box_min[d] = c.box_lo[d];
+    c.prm.box_max[d] = c.box_hi[d];
   }
   c.nboxes = nboxes;
   c.box_geom_M = 0;
   c.box_glob_valid = false;
+}
+
+ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, const double *box_max,
+                         const int64_t *count) {
+  if (!h || nboxes < 1 || !box_min || !box_max || !count) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  int64_t total = 0;
+  if (const char *why = check_boxes(c, nboxes, box_min, box_max, count, &total)) return fail(c, LJG_ERR_ARG, why);
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  set_box_tables(c, nboxes, box_min, box_max, count, total);
   c.n = 0;
   c.nslots = 0;
   c.layout = Layout::None;
   c.have_forces = false;
   c.steps_since_rebuild = 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_edit_boxes(ljg_ctx *h, int32_t nkeep, const int32_t *keep, int32_t nadd, const double *box_min,
+                          const double *box_max, const int64_t *count, const double *x, const double *y,
+                          const double *z, const double *vx, const double *vy, const double *vz, const int32_t *type,
+                          const int64_t *id) {
+  if (!h || nkeep < 0 || nadd < 0 || nkeep + nadd < 1 || (nkeep > 0 && !keep)) return LJG_ERR_ARG;
+  if (nadd > 0 && (!box_min || !box_max || !count || !x || !y || !z)) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (nkeep > 0 && (c.nboxes == 0 || c.n == 0)) return fail(c, LJG_ERR_STATE, "no ensemble to keep boxes from");
+  const int32_t old_nb = c.nboxes;
+  std::vector<int64_t> old_pstart((size_t)old_nb + 1, 0), new_base((size_t)old_nb, -1);
+  for (int32_t b = 0; b < old_nb; ++b) old_pstart[b + 1] = old_pstart[b] + c.box_count[b];
+  const int32_t nb = nkeep + nadd;
+  std::vector<double> lo(3 * (size_t)nb), hi(3 * (size_t)nb);
+  std::vector<int64_t> cnt((size_t)nb);
+  int64_t kept = 0;
+  for (int32_t k = 0; k < nkeep; ++k) {
+    const int32_t b = keep[k];
+    if (b < 0 || b >= old_nb || new_base[b] >= 0) return fail(c, LJG_ERR_ARG, "keep lists a box that does not exist or twice");
+    new_base[b] = kept;
+    for (int d = 0; d < 3; ++d) {
+      lo[3 * k + d] = c.box_lo[3 * b + d];
+      hi[3 * k + d] = c.box_hi[3 * b + d];
+    }
+    cnt[k] = c.box_count[b];
+    kept += cnt[k];
+  }
+  int64_t added = 0;
+  for (int32_t a = 0; a < nadd; ++a) {
+    for (int d = 0; d < 3; ++d) {
+      lo[3 * (nkeep + a) + d] = box_min[3 * a + d];
+      hi[3 * (nkeep + a) + d] = box_max[3 * a + d];
+    }
+    cnt[nkeep + a] = count[a];
+    added += count[a];
+  }
+  if (type) {
+    for (int64_t i = 0; i < added; ++i)
+      if (type[i] < 0 || type[i] >= c.lj.ntypes) return fail(c, LJG_ERR_ARG, "particle type not set");
+  }
+  int64_t total = 0;
+  if (const char *why = check_boxes(c, nb, lo.data(), hi.data(), cnt.data(), &total)) return fail(c, LJG_ERR_ARG, why);
+  ensemble_edit(c, old_pstart, new_base, kept, added, x, y, z, vx, vy, vz, type, id);
+  set_box_tables(c, nb, lo.data(), hi.data(), cnt.data(), total);
+  c.n = total;
+  c.nslots = total;
+  c.layout = Layout::Upload;
+  c.have_forces = false;
+  c.steps_since_rebuild = 0;
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_download_range(ljg_ctx *h, int64_t first, int64_t count, double *x, double *y, double *z, double *vx,
+                              double *vy, double *vz, double *fx, double *fy, double *fz) {
+  if (!h || first < 0 || count < 0) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (first + count > c.n) return fail(c, LJG_ERR_ARG, "range beyond the uploaded particles");
+  if (count == 0) return LJG_OK;
+  State &s = c.st[c.cur];
+  double *outs[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
+  const double *ins[9] = {s.x.p, s.y.p, s.z.p, s.vx.p, s.vy.p, s.vz.p, s.fx.p, s.fy.p, s.fz.p};
+  c.part_v.ensure((size_t)c.n);
+  for (int k = 0; k < 9; ++k) {
+    if (!outs[k]) continue;
+    hipLaunchKernelGGL(k_to_upload_order, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.orig.p,
+                       ins[k], c.part_v.p);
+    LJG_HIP(hipGetLastError());
+    LJG_HIP(hipMemcpyAsync(outs[k], c.part_v.p + first, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
+                           c.stream));
+    LJG_HIP(hipStreamSynchronize(c.stream));
+  }
   return LJG_OK;
   LJG_API_END
 }

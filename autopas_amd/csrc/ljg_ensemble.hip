@@ -91,6 +91,93 @@ __global__ __launch_bounds__(kBoxReduceThreads) void k_box_reduce(int32_t nboxes
   }
 }
 
+// Particles of kept boxes from the current layout to their new upload index (the rebuild reference is
+// reset to the position: the edit is followed by a rebuild).
+__global__ void k_keep_boxes(int64_t nslots, StateView o, StateView n, const int32_t *__restrict__ box_of,
+                             const int64_t *__restrict__ old_pstart, const int64_t *__restrict__ new_base) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int32_t orig = o.orig[s];
+  if (orig < 0) return;
+  const int32_t b = box_of[orig];
+  const int64_t nb = new_base[b];
+  if (nb < 0) return;
+  const int64_t i = nb + (orig - old_pstart[b]);
+  const double x = o.x[s], y = o.y[s], z = o.z[s];
+  n.x[i] = x; n.y[i] = y; n.z[i] = z;
+  n.rx[i] = x; n.ry[i] = y; n.rz[i] = z;
+  n.vx[i] = o.vx[s]; n.vy[i] = o.vy[s]; n.vz[i] = o.vz[s];
+  n.fx[i] = o.fx[s]; n.fy[i] = o.fy[s]; n.fz[i] = o.fz[s];
+  n.ofx[i] = o.ofx[s]; n.ofy[i] = o.ofy[s]; n.ofz[i] = o.ofz[s];
+  n.type[i] = o.type[s];
+  n.orig[i] = (int32_t)i;
+  n.id[i] = o.id[s];
+}
+
+// New particles (6 x count doubles: x, y, z, vx, vy, vz) at upload indices first .. first + count.
+__global__ void k_add_particles(int64_t first, int64_t count, StateView n, const double *__restrict__ in,
+                                int32_t have_v, const int32_t *__restrict__ type, const int64_t *__restrict__ id) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int64_t i = first + k;
+  const double x = in[k], y = in[count + k], z = in[2 * count + k];
+  n.x[i] = x; n.y[i] = y; n.z[i] = z;
+  n.rx[i] = x; n.ry[i] = y; n.rz[i] = z;
+  n.vx[i] = have_v ? in[3 * count + k] : 0.;
+  n.vy[i] = have_v ? in[4 * count + k] : 0.;
+  n.vz[i] = have_v ? in[5 * count + k] : 0.;
+  n.fx[i] = 0.; n.fy[i] = 0.; n.fz[i] = 0.;
+  n.ofx[i] = 0.; n.ofy[i] = 0.; n.ofz[i] = 0.;
+  n.type[i] = type ? type[k] : 0;
+  n.orig[i] = (int32_t)i;
+  n.id[i] = id ? id[k] : i;
+}
+
+void ensemble_edit(Ctx &c, const std::vector<int64_t> &old_pstart, const std::vector<int64_t> &new_base, int64_t kept,
+                   int64_t added, const double *x, const double *y, const double *z, const double *vx,
+                   const double *vy, const double *vz, const int32_t *type, const int64_t *id) {
+  const int64_t total = kept + added;
+  const int other = 1 - c.cur;
+  c.st[other].ensure((size_t)total);
+  if (kept > 0) {
+    c.scan_tmp.ensure(old_pstart.size() + new_base.size());
+    int64_t *d_pstart = c.scan_tmp.p, *d_base = c.scan_tmp.p + old_pstart.size();
+    LJG_HIP(hipMemcpyAsync(d_pstart, old_pstart.data(), old_pstart.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           c.stream));
+    LJG_HIP(hipMemcpyAsync(d_base, new_base.data(), new_base.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           c.stream));
+    hipLaunchKernelGGL(k_keep_boxes, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
+                       view(c.st[c.cur]), view(c.st[other]), c.box_of.p, d_pstart, d_base);
+    LJG_HIP(hipGetLastError());
+    LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
+  }
+  if (added > 0) {
+    const bool have_v = vx && vy && vz;
+    const size_t n = (size_t)added;
+    c.part_v.ensure(6 * n);
+    const double *src[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < (have_v ? 6 : 3); ++k)
+      LJG_HIP(hipMemcpyAsync(c.part_v.p + k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    int32_t *d_type = nullptr;
+    int64_t *d_id = nullptr;
+    if (type) {
+      c.rank.ensure(n);
+      d_type = c.rank.p;
+      LJG_HIP(hipMemcpyAsync(d_type, type, n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    }
+    if (id) {
+      c.scan_tmp.ensure(n);
+      d_id = c.scan_tmp.p;
+      LJG_HIP(hipMemcpyAsync(d_id, id, n * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    }
+    hipLaunchKernelGGL(k_add_particles, dim3(blocks_for(added, 256)), dim3(256), 0, c.stream, kept, added,
+                       view(c.st[other]), c.part_v.p, (int32_t)have_v, d_type, d_id);
+    LJG_HIP(hipGetLastError());
+    LJG_HIP(hipStreamSynchronize(c.stream));
+  }
+  c.cur = other;
+}
+
 void ensemble_geometry(Ctx &c) {
   if (c.box_geom_M == c.M) return;
   const double ilen = c.prm.cutoff + c.prm.skin;

@@ -338,6 +338,12 @@ void ensemble_geometry(Ctx &c);
 void ensemble_tower_hist(Ctx &c);
 void ensemble_wrap(Ctx &c);
 void ensemble_box_globals(Ctx &c);
+// Keep the boxes with new_base[b] >= 0 (their particles move to upload indices new_base[b] + local index,
+// with positions, velocities, forces, types and ids), then append `added` new particles from the host
+// after the `kept` ones (zero force).  Leaves the state in upload order in the other buffer, made current.
+void ensemble_edit(Ctx &c, const std::vector<int64_t> &old_pstart, const std::vector<int64_t> &new_base, int64_t kept,
+                   int64_t added, const double *x, const double *y, const double *z, const double *vx,
+                   const double *vy, const double *vz, const int32_t *type, const int64_t *id);
 
 constexpr int kShiftBits = 27;
 constexpr int32_t kClusterMask = (1 << kShiftBits) - 1;

@@ -87,9 +87,12 @@ SIGNATURES = {
     "ljg_destroy": (_i32, [_p]),
     "ljg_last_error": (ctypes.c_char_p, [_p]),
     "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
+    "ljg_memory_info": (_i32, [_p, _p, _p]),
     "ljg_set_box": (_i32, [_p, _p, _p]),
     "ljg_set_boxes": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_get_box_globals": (_i32, [_p, _i32, _p, _p, _p]),
+    "ljg_edit_boxes": (_i32, [_p, _i32, _p, _i32, _p, _p, _p] + [_p] * 8),
+    "ljg_download_range": (_i32, [_p, _i64, _i64] + [_p] * 9),
     "ljg_get_tuning": (_i32, [_p, _p]),
     "ljg_set_types": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_upload": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -278,6 +281,12 @@ class Engine:
         self._check(self.lib.ljg_device_name(self.h, buf, 256), "ljg_device_name")
         return buf.value.decode()
 
+    def memory_info(self):
+        """(free, total) device memory in bytes."""
+        f, t = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(self.lib.ljg_memory_info(self.h, ctypes.byref(f), ctypes.byref(t)), "ljg_memory_info")
+        return f.value, t.value
+
     def tuning(self):
         """State of the layout tuning (kernel=KERNEL_AUTO): candidates with their scores and the choice."""
         t = Tuning()
@@ -306,7 +315,43 @@ class Engine:
             raise ValueError("box_min, box_max and counts describe different numbers of boxes")
         self._check(self.lib.ljg_set_boxes(self.h, len(cnt), _ptr(lo), _ptr(hi), _ptr(cnt)), "ljg_set_boxes")
         self.nboxes = len(cnt)
+        self._counts = cnt.copy()
         self.n = 0
+
+    def edit_boxes(self, keep, box_min=None, box_max=None, counts=None, x=None, y=None, z=None, vx=None, vy=None,
+                   vz=None, type=None, id=None):
+        """Keep the boxes ``keep`` (indices into the current ensemble, in their new order) with their state,
+        then append new boxes (bounds of shape (nadd, 3), counts and their particles concatenated)."""
+        k = np.ascontiguousarray(keep, dtype=np.int32)
+        nadd = 0 if counts is None else len(counts)
+        lo = hi = cnt = None
+        arrs = [None] * 6
+        t = i = None
+        if nadd:
+            lo = np.ascontiguousarray(box_min, dtype=np.float64).reshape(-1, 3)
+            hi = np.ascontiguousarray(box_max, dtype=np.float64).reshape(-1, 3)
+            cnt = np.ascontiguousarray(counts, dtype=np.int64)
+            arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, vx, vy, vz)]
+            t = None if type is None else np.ascontiguousarray(type, dtype=np.int32)
+            i = None if id is None else np.ascontiguousarray(id, dtype=np.int64)
+        self._check(self.lib.ljg_edit_boxes(self.h, len(k), _ptr(k) if len(k) else None, nadd, _ptr(lo), _ptr(hi),
+                                            _ptr(cnt), *[_ptr(a) for a in arrs], _ptr(t), _ptr(i)), "ljg_edit_boxes")
+        self.nboxes = len(k) + nadd
+        self.n = int(self._box_counts(k, cnt).sum())
+
+    def _box_counts(self, keep, added):
+        old = getattr(self, "_counts", np.zeros(0, dtype=np.int64))
+        new = np.concatenate([old[np.asarray(keep, dtype=np.int64)] if len(keep) else np.zeros(0, dtype=np.int64),
+                              added if added is not None else np.zeros(0, dtype=np.int64)])
+        self._counts = new
+        return new
+
+    def download_range(self, first, count):
+        """State of particles [first, first + count) of the upload order."""
+        out = [np.empty(count) for _ in range(9)]
+        self._check(self.lib.ljg_download_range(self.h, first, count, *[_ptr(a) for a in out]), "ljg_download_range")
+        keys = ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz")
+        return dict(zip(keys, out))
 
     def box_globals(self):
         """Per-box potential energy, virial and pair count of the last force(globals=True) on an ensemble."""

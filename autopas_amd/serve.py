@@ -169,6 +169,118 @@ class Server:
         return max(j.finish for j in jobs) if jobs else 0.0
 
 
+class BatchedServer:
+    """Continuous batching on one engine context: every running job is a box of one ensemble, so each
+    step advances all running jobs in the same launches.  Between epochs of at most ``epoch`` steps the
+    finished jobs are downloaded and retired and the waiting jobs that have arrived are admitted, first come
+    first served, while the particles in flight stay within ``particle_budget`` (the HBM budget) and the
+    batch holds at most ``max_jobs`` jobs (fewer jobs in a batch finish sooner each); the running
+    jobs' state stays resident on the device (``Engine.edit_boxes``).  An epoch ends early when a running job
+    reaches its step count, so every job runs exactly its own number of steps.
+
+    Each job's trajectory is the one it has alone up to the rounding of the FP32 pair term: the lists of a
+    running job are rebuilt when the batch changes, so its pair sums may be taken in another order."""
+
+    def __init__(self, particle_budget=None, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=8, tile_groups=4,
+                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None):
+        self.kernel = kernel
+        self.epoch = epoch
+        self.max_jobs = max_jobs  # jobs in one batch at most (None: only the particle budget limits)
+        self.keep_results = keep_results
+        self.engine = engine.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), RC, SKIN, cluster_size=cluster_size,
+                                    rebuild_frequency=10, dynamic_rebuild=True, tile_groups=tile_groups)
+        if particle_budget is None:
+            free, _ = self.engine.memory_info()
+            particle_budget = int(0.8 * free / bytes_per_particle)
+        self.particle_budget = particle_budget
+        self.pairs = 0
+        self.epochs = 0
+        self.edits = 0
+
+    def close(self):
+        self.engine.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _edit(self, keep, admit):
+        lo = hi = cnt = None
+        parts = {}
+        if admit:
+            systems = [job_system(j) for j in admit]
+            lo = np.zeros((len(admit), 3))
+            hi = np.array([[L] * 3 for L, _ in systems])
+            cnt = np.array([j.n for j in admit], dtype=np.int64)
+            for k, key in enumerate(("x", "y", "z", "vx", "vy", "vz")):
+                parts[key] = np.concatenate([st[k] for _, st in systems])
+        self.engine.edit_boxes(keep, lo, hi, cnt, **parts)
+        self.edits += 1
+
+    def run(self, jobs):
+        """Serve ``jobs`` (arrival times relative to now); returns the wall time to the last completion."""
+        t0 = time.perf_counter()
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        head = 0
+        active = []  # [job, steps left], in box order
+        while head < len(pending) or active:
+            now = time.perf_counter() - t0
+            done = [k for k, (_, left) in enumerate(active) if left == 0]
+            if done:
+                offsets = np.concatenate([[0], np.cumsum([j.n for j, _ in active])])
+                for k in done:
+                    job = active[k][0]
+                    if self.keep_results:
+                        job.result = self.engine.download_range(int(offsets[k]), job.n)
+                    job.finish = time.perf_counter() - t0
+            keep = [k for k, (_, left) in enumerate(active) if left > 0]
+            inflight = sum(active[k][0].n for k in keep)
+            admit = []
+            while head < len(pending) and pending[head].arrival <= now and (
+                    self.max_jobs is None or len(keep) + len(admit) < self.max_jobs) and (
+                    inflight + pending[head].n <= self.particle_budget or not (keep or admit)):
+                job = pending[head]
+                head += 1
+                job.start = now
+                job.worker = 0
+                admit.append(job)
+                inflight += job.n
+            if admit or done:
+                if keep or admit:
+                    self._edit(keep, admit)
+                active = [active[k] for k in keep] + [[j, j.steps] for j in admit]
+            if not active:
+                if head < len(pending):
+                    time.sleep(max(0.0, pending[head].arrival - (time.perf_counter() - t0)))
+                continue
+            steps = min(self.epoch, min(left for _, left in active))
+            rs = self.engine.step(steps, DT, self.kernel)
+            self.pairs += rs.pair_interactions
+            self.epochs += 1
+            for a in active:
+                a[1] -= steps
+        return max(j.finish for j in jobs) if jobs else 0.0
+
+
+def serve_open_loop_batched(count, load=0.8, seed=5, calibration_jobs=None, **server_kw):
+    """The serving measurement with continuous batching: calibrate the batched server's throughput on a
+    closed-loop batch, then serve ``count`` jobs arriving as a Poisson process at ``load`` times it."""
+    with BatchedServer(**server_kw) as srv:
+        calib = make_jobs(calibration_jobs or max(8, count // 4), None, seed=seed + 1)
+        wall = srv.run(calib)
+        capacity = len(calib) / wall
+        jobs = make_jobs(count, load * capacity, seed=seed)
+        srv.pairs = srv.epochs = srv.edits = 0
+        wall = srv.run(jobs)
+        out = latency_summary(jobs, wall)
+        out.update(pairs=int(srv.pairs), pairs_per_s=srv.pairs / wall if wall else 0.0, load=load,
+                   capacity_jobs_per_s=capacity, kernel=engine.KERNEL_NAMES[srv.kernel], epochs=srv.epochs,
+                   batch_edits=srv.edits, particle_budget=srv.particle_budget, epoch_steps=srv.epoch)
+    return out, jobs
+
+
 def latency_summary(jobs, wall):
     lat = np.array([j.latency for j in jobs])
     pairs = sum(j.pairs for j in jobs)

@@ -110,6 +110,9 @@ ljg_status ljg_destroy(ljg_ctx *ctx);
 const char *ljg_last_error(const ljg_ctx *ctx);
 /* Human-readable name of the device the context runs on (for logs). */
 ljg_status ljg_device_name(const ljg_ctx *ctx, char *buf, int32_t buflen);
+/* Free and total device memory in bytes (hipMemGetInfo on the context's device): the HBM budget a
+ * serving loop admits jobs against. */
+ljg_status ljg_memory_info(const ljg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Change the periodic box of a context (AutoPas::setBoxMin/setBoxMax, AutoPasDecl.h:589-595, or
  * resizeBox, :116).  The particles are dropped: upload the next system afterwards.  Device buffers are
@@ -126,6 +129,17 @@ ljg_status ljg_set_box(ljg_ctx *ctx, const double box_min[3], const double box_m
  * to one box; the particles are dropped either way. */
 ljg_status ljg_set_boxes(ljg_ctx *ctx, int32_t nboxes, const double *box_min, const double *box_max,
                          const int64_t *count);
+/* Edit an ensemble in place, keeping running boxes resident on the device.  The new ensemble is the boxes
+ * keep[0..nkeep) of the current one (indices into it, in their new order; positions, velocities,
+ * forces, types and ids kept) followed by nadd new boxes given as for ljg_set_boxes + ljg_upload
+ * (box_min/box_max 3 per box, count, and their particles concatenated; vx/vy/vz/type/id may be NULL).
+ * Upload indices are renumbered box by box; the lists are rebuilt at the next force evaluation.  A
+ * serving loop uses this to retire finished jobs and admit new ones between batched steps (md-flexible
+ * starts and ends one simulation per AutoPas instance, AutoPasImpl.h:55-113). */
+ljg_status ljg_edit_boxes(ljg_ctx *ctx, int32_t nkeep, const int32_t *keep, int32_t nadd, const double *box_min,
+                          const double *box_max, const int64_t *count, const double *x, const double *y,
+                          const double *z, const double *vx, const double *vy, const double *vz, const int32_t *type,
+                          const int64_t *id);
 /* Per-box results of the last ljg_force with calc_globals on an ensemble: potential energy and virial of
  * each box (LJFunctor::getPotentialEnergy / getVirial, LJFunctor.h:691-720) and its unique pairs within
  * rc.  Any output pointer may be NULL. */
@@ -184,6 +198,9 @@ ljg_status ljg_set_timing(ljg_ctx *ctx, int32_t enabled);
 /* Download state in upload order.  Any pointer may be NULL. */
 ljg_status ljg_download(ljg_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                         double *fx, double *fy, double *fz);
+/* Download particles [first, first + count) of the upload order (one box of an ensemble, for example). */
+ljg_status ljg_download_range(ljg_ctx *ctx, int64_t first, int64_t count, double *x, double *y, double *z,
+                              double *vx, double *vy, double *vz, double *fx, double *fy, double *fz);
 
 /* ---- inspection (parity tests) -------------------------------------------------------------- */
 /* Cell of each particle (upload order) after ljg_bin: CellBlock3D::get1DIndexOfPosition of the
