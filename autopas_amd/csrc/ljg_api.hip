@@ -161,7 +161,6 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
     LJG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     LJG_HIP(hipEventCreate(&c.ev0));
     LJG_HIP(hipEventCreate(&c.ev1));
-    LJG_HIP(hipEventCreateWithFlags(&c.ev_drift, hipEventDisableTiming));
     // coherent, device-mapped: the drift kernel writes the displacement stamp here directly
     LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(c.host_flag, 0, 4 * sizeof(int32_t));
@@ -198,8 +197,8 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
+  for (auto e : c.event_pool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
-  if (c.ev_drift) (void)hipEventDestroy(c.ev_drift);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
@@ -624,100 +623,151 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 
 // nsteps velocity-Verlet steps with one kernel (LogicHandler.h:987-1015 rebuild policy); the statistics
 // are added to rs.  HIP-event timing of force launches and rebuilds when `timing`.
+//
+// Step s is: drift of s (fused with the kick of s - 1), a rebuild if one is due, force of s.  The steps
+// between two scheduled rebuilds are queued as one chunk (at most kChunkSteps) and the host reads the
+// displacement flag once at the chunk's end.  A drift that moves a particle skin/2 since the last rebuild
+// raises the flag with its step's stamp; every later launch of the chunk exits at once (SkipIf), so the
+// state stops right after that drift.  The host then rebuilds, clears the flag and resumes with that
+// step's force, as LogicHandler.h:1000-1015 rebuilds before the force of the step that moved too far.
+// The j-row kernel advances the particles in its epilogue (kick of step s, drift of step s + 1), so only
+// the very first drift runs as a kernel of its own.
 constexpr int32_t kTimingStride = 8;
+constexpr int32_t kChunkSteps = 32;
+
+__global__ void k_clear_flag(int32_t *flag) { flag[0] = 0; }
+
+static hipEvent_t take_event(Ctx &c) {
+  if (!c.event_pool.empty()) {
+    hipEvent_t e = c.event_pool.back();
+    c.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  LJG_HIP(hipEventCreate(&e));
+  return e;
+}
 
 static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool timing, ljg_run_stats &rs) {
-  std::vector<hipEvent_t> fev, rev;  // pairs (start, stop) per force launch / rebuild
-  auto new_event = [](std::vector<hipEvent_t> &v) {
-    hipEvent_t e;
-    LJG_HIP(hipEventCreate(&e));
-    v.push_back(e);
-    return e;
-  };
+  std::vector<hipEvent_t> rev;                           // pairs (start, stop) per rebuild
+  std::vector<std::pair<int32_t, hipEvent_t>> fev;       // (step, event): start, stop per timed force launch
   c.hits_slots.ensure(kAccSlots * kAccStride);
   hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kAccSlots * kAccStride), 0, c.stream, (int64_t)(kAccSlots * kAccStride),
                      reinterpret_cast<int64_t *>(c.hits_slots.p));
   LJG_HIP(hipGetLastError());
   const bool saved_timing = c.timing;
   c.timing = false;  // per-launch events are recorded here instead
+  // clears both copies of the flag; every queued launch (a drift that may still raise it) has run first
+  auto clear_flag = [&]() {
+    LJG_HIP(hipStreamSynchronize(c.stream));
+    static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+    LJG_HIP(hipGetLastError());
+  };
+  clear_flag();
+  auto record_rebuild = [&]() {
+    if (!timing) return;
+    rev.push_back(take_event(c));
+    LJG_HIP(hipEventRecord(rev.back(), c.stream));
+  };
   if (!layout_matches(c, kernel)) {
-    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    record_rebuild();
     rebuild_for(c, kernel);
-    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    record_rebuild();
     ++rs.rebuilds;
   }
   c.hits_acc = c.hits_slots.p;
   auto rebuild_now = [&]() {
-    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    record_rebuild();
     c.carry_force = false;  // the force of this step is computed right after the rebuild
     rebuild_for(c, kernel);
     c.carry_force = true;
-    if (timing) LJG_HIP(hipEventRecord(new_event(rev), c.stream));
+    record_rebuild();
     ++rs.rebuilds;
   };
-  // The j-row kernel advances the particles in its epilogue (kick of step s, drift of step s + 1), so
-  // only the first drift runs as a kernel of its own.
   const bool fused = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
-  int32_t stamp = 0;
-  if (fused && nsteps > 0) {
-    stamp = ++c.step_stamp;
-    md_kick_drift(c, false, true, dt, stamp);
-    LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
-  }
-  for (int32_t s = 0; s < nsteps; ++s) {
-    const bool last = s + 1 == nsteps;
-    if (!fused) {
-      // kick of the previous step (if any) fused with this step's drift; a particle that moved skin/2
-      // since the last rebuild stamps flag[0] with this step's number
-      stamp = ++c.step_stamp;
-      md_kick_drift(c, s > 0, true, dt, stamp);
-      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
+  const bool dyn = c.prm.dynamic_rebuild != 0;
+  // stamp of step s (s = 0 .. nsteps - 1): base + s + 1, never 0
+  if (c.step_stamp > INT32_MAX - nsteps - 2) c.step_stamp = 0;  // both flags were just cleared
+  const int32_t base = c.step_stamp;
+  c.step_stamp += nsteps + 1;
+  auto stamp_of = [&](int32_t step) { return base + step + 1; };
+  // the force of step s; with `speculative`, it exits if the flag was raised at or before step s
+  auto force_step = [&](int32_t step, bool speculative) {
+    const bool time_force = timing && step % kTimingStride == 0;
+    if (time_force) {
+      fev.emplace_back(step, take_event(c));
+      LJG_HIP(hipEventRecord(fev.back().second, c.stream));
     }
-    const int32_t next_stamp = fused && !last ? ++c.step_stamp : 0;
-    // Each event is a marker the GPU processes between kernels (~5 us), so force launches are timed on
-    // every kTimingStride-th step only; the statistics report the mean over the timed launches.
-    const bool time_force = timing && s % kTimingStride == 0;
-    auto force_now = [&]() {
-      if (time_force) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
-      if (fused) {
-        const IntegArgs ia = integ_args(c, true, !last, dt, next_stamp);
-        cluster_jrows_force(c, false, &ia);
-      } else {
-        run_force(c, kernel, false);
-      }
-      if (time_force) LJG_HIP(hipEventRecord(new_event(fev), c.stream));
-    };
-    if (c.steps_since_rebuild >= c.prm.rebuild_frequency) {
-      rebuild_now();
-      force_now();
-    } else if (c.prm.dynamic_rebuild) {
-      // LogicHandler.h:1000-1015 rebuilds when a particle moved skin/2.  The drift wrote its stamp to
-      // pinned memory; the force is queued before the host reads it, and exits at once on the device if
-      // the stamp is set, so the GPU never waits for the host's decision.
-      c.skip = SkipIf{c.flag.p, stamp};
-      force_now();
-      c.skip = SkipIf{nullptr, 0};
-      LJG_HIP(hipEventSynchronize(c.ev_drift));
-      if (static_cast<volatile int32_t *>(c.host_flag)[0] == stamp) {
-        if (time_force) {  // the skipped launch is not a force evaluation
-          for (int k = 0; k < 2; ++k) {
-            LJG_HIP(hipEventDestroy(fev.back()));
-            fev.pop_back();
-          }
-        }
-        rebuild_now();
-        force_now();
-      }
+    c.skip = speculative && dyn ? SkipIf{c.flag.p, stamp_of(step)} : SkipIf{nullptr, 0};
+    if (fused) {
+      const bool last = step + 1 == nsteps;
+      const IntegArgs ia = integ_args(c, true, !last, dt, last ? 0 : stamp_of(step + 1));
+      cluster_jrows_force(c, false, &ia);
+      if (!last) std::swap(c.posfx, c.posfx_next);  // the launch wrote the next step's positions there
     } else {
-      force_now();
+      run_force(c, kernel, false);
     }
-    if (fused && !last) {
-      // the launch wrote the next step's positions into the second buffer
-      std::swap(c.posfx, c.posfx_next);
-      LJG_HIP(hipEventRecord(c.ev_drift, c.stream));
-      stamp = next_stamp;
+    c.skip = SkipIf{nullptr, 0};
+    if (time_force) {
+      fev.emplace_back(step, take_event(c));
+      LJG_HIP(hipEventRecord(fev.back().second, c.stream));
     }
-    c.steps_since_rebuild++;
+  };
+  // the drift of step s (fused with the kick of s - 1); for the j-row kernel only that of step 0
+  auto drift_step = [&](int32_t step, bool speculative) {
+    if (fused && step > 0) return;
+    const SkipIf sk = speculative && dyn ? SkipIf{c.flag.p, stamp_of(step) - 1} : SkipIf{nullptr, 0};
+    md_kick_drift(c, !fused && step > 0, true, dt, stamp_of(step), sk);
+  };
+  int32_t s = 0;
+  while (s < nsteps) {
+    // steps on the current lists before the next scheduled rebuild, queued as one chunk
+    int32_t k = c.prm.rebuild_frequency - (int32_t)c.steps_since_rebuild;
+    if (k < 0) k = 0;
+    if (k > nsteps - s) k = nsteps - s;
+    if (k > kChunkSteps) k = kChunkSteps;
+    for (int32_t j = 0; j < k; ++j) {
+      drift_step(s + j, true);
+      force_step(s + j, true);
+    }
+    // A scheduled rebuild before the force of step s + k is queued at once, after that step's drift: it
+    // works on the state where the queued steps stopped, which is also the state a raised flag asks to
+    // rebuild at.  Otherwise the host reads the flag now.
+    const bool rebuild_next = s + k < nsteps && (int64_t)c.steps_since_rebuild + k >= c.prm.rebuild_frequency;
+    if (rebuild_next) {
+      drift_step(s + k, true);
+      rebuild_now();
+      LJG_HIP(hipStreamSynchronize(c.stream));
+    } else if (dyn) {
+      LJG_HIP(hipStreamSynchronize(c.stream));
+    }
+    const int32_t raised = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
+    if (raised == 0 && !rebuild_next) {
+      s += k;
+      c.steps_since_rebuild += k;
+      continue;
+    }
+    // step t is next: steps s .. t - 1 are complete and step t's drift is done; a raised flag names t
+    // (the drift that moved a particle skin/2), and nothing queued after that drift ran
+    const int32_t t = raised != 0 ? raised - base - 1 : s + k;
+    if (t < s || t > s + k || (!fused && !rebuild_next && t == s + k))
+      throw HipError{hipErrorUnknown, "displacement flag outside the queued steps", __LINE__};
+    // (the rebuild recomputes the fixed-point positions from the FP64 state, so the position buffers the
+    // skipped launches swapped need no repair)
+    for (auto it = fev.begin(); it != fev.end();) {  // timed launches of skipped steps
+      if (it->first >= t) {
+        c.event_pool.push_back(it->second);
+        it = fev.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    if (!rebuild_next) rebuild_now();
+    clear_flag();
+    force_step(t, false);
+    c.steps_since_rebuild = 1;
+    s = t + 1;
   }
   c.hits_acc = nullptr;
   if (nsteps > 0 && !fused) md_kick_drift(c, true, false, dt, 0);  // final velocity update
@@ -731,7 +781,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   rs.pair_interactions += hits2 / 2;
   for (size_t k = 0; k + 1 < fev.size(); k += 2) {
     float ms = 0.f;
-    LJG_HIP(hipEventElapsedTime(&ms, fev[k], fev[k + 1]));
+    LJG_HIP(hipEventElapsedTime(&ms, fev[k].second, fev[k + 1].second));
     rs.force_ms += ms;
     rs.force_launches++;
   }
@@ -740,8 +790,8 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     LJG_HIP(hipEventElapsedTime(&ms, rev[k], rev[k + 1]));
     rs.rebuild_ms += ms;
   }
-  for (auto e : fev) (void)hipEventDestroy(e);
-  for (auto e : rev) (void)hipEventDestroy(e);
+  for (auto &e : fev) c.event_pool.push_back(e.second);
+  for (auto e : rev) c.event_pool.push_back(e);
   c.have_forces = true;
 }
 

@@ -127,13 +127,20 @@ struct FixGeom {
   int32_t cubic;
 };
 
-// A force launch that does nothing when *flag == stamp: the step loop launches a step's force before the
-// host has read that step's displacement flag, and a step that needs a rebuild first skips it.
+// A launch that does nothing once an earlier step raised the displacement flag: *flag holds the stamp
+// (running step number) of the first drift that moved a particle skin/2 since the last rebuild, and
+// stays set until the host has rebuilt.  The step loop queues several steps before it reads the flag;
+// every launch of a step at or after the raising one exits at once, so the host resumes from that step
+// after the rebuild.  Writes of the flag by the launch itself carry a larger stamp and do not count.
 struct SkipIf {
   const int32_t *flag;
   int32_t stamp;
 };
-__device__ inline bool skip_launch(const SkipIf &s) { return s.flag != nullptr && *s.flag == s.stamp; }
+__device__ inline bool skip_launch(const SkipIf &s) {
+  if (s.flag == nullptr) return false;
+  const int32_t f = *s.flag;
+  return f != 0 && f <= s.stamp;
+}
 
 // Velocity-Verlet update fused into a force kernel's epilogue (kick of this step, drift of the next):
 // the FP64 state of the current layout, the fixed-point positions to write for the next step, the
@@ -200,7 +207,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_drift = nullptr;  // recorded after a step's drift (host waits on it, not on the stream)
+  std::vector<hipEvent_t> event_pool;  // timing events of the step loop, reused from call to call
   bool timing = false;
 
   int64_t n = 0;       // real particles
@@ -269,7 +276,7 @@ struct Ctx {
   DevBuf<int64_t> part_hits;
   int64_t part_items = 0;         // partial-sum slots written by the last force launch
   unsigned long long *hits_acc = nullptr;  // when set, force kernels add their pair counts here (kAccSlots lines)
-  SkipIf skip{nullptr, 0};                 // force launches exit at once when *skip.flag == skip.stamp
+  SkipIf skip{nullptr, 0};                 // force launches exit at once when skip_launch(skip)
   DevBuf<unsigned long long> hits_slots;   // the step loop's pair-count lines
   DevBuf<double> red_out;         // [0]=e [1]=v
   DevBuf<int64_t> red_hits;       // [0]
@@ -321,7 +328,7 @@ void verlet_geometry(Ctx &c);              // ljg_verlet.hip: fixed-point scales
 void verlet_fixed_slots(Ctx &c, int64_t nslots);  // ljg_verlet.hip: posfx for slots [0, nslots)
 void cluster_jrows_force(Ctx &c, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
 IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);    // ljg_md.hip
-void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp);  // ljg_md.hip
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp, SkipIf skip = SkipIf{nullptr, 0});  // ljg_md.hip
 void wrap_positions(Ctx &c);               // ljg_md.hip
 // Rebuild permutation (also sets the rebuild reference positions; writes posfx when given).
 void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx = nullptr);  // ljg_md.hip

@@ -143,7 +143,8 @@ struct MdArgs {
 
 __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double4 *__restrict__ c0,
                              float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ flag,
-                             int32_t *__restrict__ hflag) {
+                             int32_t *__restrict__ hflag, SkipIf sk) {
+  if (skip_launch(sk)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
   if (s.orig[i] < 0) return;
@@ -181,7 +182,7 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
   }
 }
 
-void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
+void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp, SkipIf skip) {
   MdArgs a{};
   a.dt = dt;
   a.stamp = stamp;
@@ -199,7 +200,7 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
     a.drift_fac[t] = dt * dt / (2 * m);
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p, c.host_flag_dev);
+                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p, c.host_flag_dev, skip);
   LJG_HIP(hipGetLastError());
 }
 
