@@ -335,7 +335,8 @@ __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
 // Short towers (small boxes): one wavefront per tower sorts (z, id) in registers with a bitonic network
 // over 64 E elements, element e * 64 + lane in slot e of a lane (padding sorts last), and writes the same
 // cluster layout as k_tower_sort.  Keys are distinct (ids are), so every sorter produces the one sorted
-// order.  The E = 1 instance takes towers of up to 64 particles, E = 4 those of 65 to 256.
+// order.  The E = 1 instance takes towers of up to 64 particles, E = 4 those of 65 to 256, E = 8 those
+// of 257 to 512.
 constexpr int kSortWaveThreads = 256;
 
 // zid_less on values held in registers, without branches (padding sa < 0 sorts last).
@@ -404,7 +405,9 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
   if (t >= ntowers) return;
   const int64_t b = start[t];
   const int n = (int)(start[t + 1] - b);
-  if (n > P || (E > 1 && n <= kWave)) return;  // another instance sorts this tower
+  constexpr int LO = E == 1 ? 0 : E == 4 ? kWave : 4 * kWave;  // the next shorter instance's reach
+  static_assert(E == 1 || E == 4 || E == 8, "instances cover up to 64, 65-256 and 257-512 particles");
+  if (n > P || n <= LO) return;  // another instance sorts this tower
   int32_t sv[E];
   double zv[E];
   int64_t iv[E];
@@ -885,9 +888,9 @@ static void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
 }
 
 // Average clusters per tower below which the cluster-entry lists use a lane per i-cluster, and average
-// particles per tower up to which the towers of at most 256 particles are sorted in registers.
+// particles per tower up to which the towers of at most 512 particles are sorted in registers.
 constexpr int kLaneListsBelow = 24;
-constexpr int kWaveSortBelow = 192;
+constexpr int kWaveSortBelow = 448;
 
 void vcl_build(Ctx &c) {
   const int M = c.M;
@@ -964,21 +967,22 @@ void vcl_build(Ctx &c) {
   c.list_stats.ensure(6);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)6, c.list_stats.p);
   LJG_HIP(hipGetLastError());
-  // short towers (small boxes) sort in registers, a wavefront each; the rest in LDS
+  // towers of up to 512 particles sort in registers, a wavefront each (three instances by length, each
+  // launched only if some tower is in its range); longer ones in LDS
   const bool wave_sort = (int64_t)ts[0] <= (int64_t)kWaveSortBelow * ntow;
-  const int32_t min_n = wave_sort ? 4 * kWave : 0;
+  const int32_t min_n = wave_sort ? 8 * kWave : 0;
   if (wave_sort) {
     constexpr int per_block = kSortWaveThreads / kWave;
-    hipLaunchKernelGGL(k_tower_sort_wave<1>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream,
-                       ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M,
-                       c.perm.p, c.cluster_tower.p);
-    LJG_HIP(hipGetLastError());
-    if (ts[1] > (unsigned long long)kWave) {
-      hipLaunchKernelGGL(k_tower_sort_wave<4>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream,
-                         ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M,
-                         c.perm.p, c.cluster_tower.p);
-      LJG_HIP(hipGetLastError());
-    }
+#define LJG_WAVE_SORT(E)                                                                                        \
+  hipLaunchKernelGGL(k_tower_sort_wave<E>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream, \
+                     ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M, c.perm.p, \
+                     c.cluster_tower.p);                                                                        \
+  LJG_HIP(hipGetLastError());
+    // every tower is sorted by exactly one instance; the longest tower (ts[1]) bounds which ones can have work
+    LJG_WAVE_SORT(1)
+    if (ts[1] > (unsigned long long)kWave) LJG_WAVE_SORT(4)
+    if (ts[1] > (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(8)
+#undef LJG_WAVE_SORT
   }
   if (ts[1] > (unsigned long long)min_n) {
     hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,

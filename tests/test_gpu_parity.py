@@ -105,23 +105,25 @@ def test_verlet_lists_exact(oracle):
 
 
 @pytest.mark.parametrize("list_mode", [1, 2])
-@pytest.mark.parametrize("n_side", [28, 50])
-def test_cluster_lists_bit_exact_long_towers(oracle, n_side, list_mode):
-    # M = 8 at rho 0.75: ~4 n_side particles per tower.  n_side 28 (~112 per tower): towers up to 64
-    # particles and those of 65-256 are sorted in registers by two kernels; n_side 50 (~200 per tower):
-    # every tower is sorted in LDS.
-    announce(f"cluster build bit-exact M=8 list_mode={list_mode}: {n_side}^3 lattice rho 0.75")
-    s = lattice_box(n_side, 0.75, seed=17)
-    L = s["L"]
-    ids = np.arange(s["n"], dtype=np.int64)
-    with make_engine(L, M=8, list_mode=list_mode) as e:
-        e.upload(s["x"], s["y"], s["z"], id=ids)
+@pytest.mark.parametrize("nz,sorter", [(40, "registers, up to 256"), (100, "registers, up to 512"), (150, "LDS")])
+def test_cluster_lists_bit_exact_long_towers(oracle, nz, sorter, list_mode):
+    # 20 x 20 x nz lattice at rho 0.75 with M = 8: 10 x 10 towers of ~4 nz particles, so each tower sorter
+    # (registers for up to 64, 256 and 512 particles, LDS above) gets the towers of one of these boxes
+    announce(f"cluster build bit-exact M=8 list_mode={list_mode}: 20x20x{nz} lattice rho 0.75 ({sorter})")
+    a = 0.75 ** (-1.0 / 3.0)
+    dims = (20, 20, nz)
+    n = 20 * 20 * nz
+    x, y, z, _, _, _, _ = ap.generate_lattice(17, n, dims, a, (0.0, 0.0, 0.0), 0.05, 0.9)
+    box = tuple(d * a for d in dims)
+    ids = np.arange(n, dtype=np.int64)
+    with ap.Engine((0.0, 0.0, 0.0), box, RC, SKIN, cluster_size=8, list_mode=list_mode) as e:
+        e.upload(x, y, z, id=ids)
         e.build_pairs()
         g = e.clusters()
-    per_tower = s["n"] / (g["towers_per_dim"][0] * g["towers_per_dim"][1])
-    assert (64 < per_tower <= 192) if n_side == 28 else per_tower > 192
-    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
-    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), (L, L, L), RC + SKIN, 8)
+    per_tower = n / (g["towers_per_dim"][0] * g["towers_per_dim"][1])
+    assert {40: 64 < per_tower <= 256, 100: 256 < per_tower <= 448, 150: per_tower > 448}[nz], per_tower
+    wx, wy, wz = oracle.wrap(x, y, z, (0, 0, 0), box)
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), box, RC + SKIN, 8)
     for k in ("slot_particle", "cluster_tower", "list_offsets", "list_j", "list_shift", "bbox"):
         assert np.array_equal(g[k], r[k]), k
 
