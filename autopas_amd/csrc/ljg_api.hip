@@ -192,7 +192,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.tower_z.release(); c.tower_id.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.pos32.release(); c.pos32_next.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.cl_glob.release(); c.box_glob.release();
@@ -630,8 +630,8 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 // raises the flag with its step's stamp; every later launch of the chunk exits at once (SkipIf), so the
 // state stops right after that drift.  The host then rebuilds, clears the flag and resumes with that
 // step's force, as LogicHandler.h:1000-1015 rebuilds before the force of the step that moved too far.
-// The j-row kernel advances the particles in its epilogue (kick of step s, drift of step s + 1), so only
-// the very first drift runs as a kernel of its own.
+// The FP32 cluster kernels (j-row and cluster tiles) advance the particles in their epilogue (kick of step
+// s, drift of step s + 1), so with them only the very first drift runs as a kernel of its own.
 constexpr int32_t kTimingStride = 8;
 constexpr int32_t kChunkSteps = 32;
 
@@ -685,7 +685,9 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     record_rebuild();
     ++rs.rebuilds;
   };
-  const bool fused = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
+  // the FP32 cluster kernels advance the particles in their epilogue
+  const bool fused = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED || kernel == LJG_KERNEL_CLUSTER_MIXED;
+  const bool jrows = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
   const bool dyn = c.prm.dynamic_rebuild != 0;
   // stamp of step s (s = 0 .. nsteps - 1): base + s + 1, never 0
   if (c.step_stamp > INT32_MAX - nsteps - 2) c.step_stamp = 0;  // both flags were just cleared
@@ -703,8 +705,13 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     if (fused) {
       const bool last = step + 1 == nsteps;
       const IntegArgs ia = integ_args(c, true, !last, dt, last ? 0 : stamp_of(step + 1));
-      cluster_jrows_force(c, false, &ia);
-      if (!last) std::swap(c.posfx, c.posfx_next);  // the launch wrote the next step's positions there
+      if (jrows) {
+        cluster_jrows_force(c, false, &ia);
+        if (!last) std::swap(c.posfx, c.posfx_next);  // the launch wrote the next step's positions there
+      } else {
+        cluster_force(c, true, false, &ia);
+        if (!last) std::swap(c.pos32, c.pos32_next);
+      }
     } else {
       run_force(c, kernel, false);
     }
@@ -714,7 +721,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
       LJG_HIP(hipEventRecord(fev.back().second, c.stream));
     }
   };
-  // the drift of step s (fused with the kick of s - 1); for the j-row kernel only that of step 0
+  // the drift of step s (fused with the kick of s - 1); with a fused kernel only that of step 0
   auto drift_step = [&](int32_t step, bool speculative) {
     if (fused && step > 0) return;
     const SkipIf sk = speculative && dyn ? SkipIf{c.flag.p, stamp_of(step) - 1} : SkipIf{nullptr, 0};

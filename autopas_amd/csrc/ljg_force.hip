@@ -173,13 +173,16 @@ __device__ inline int group_value_slot(int l) {
 // of the current one.  Padding j-members (and lanes past the end of the list) sit at +3e30 and padding
 // i-members are moved to -3e30, so their r^2 overflows to +inf; the single unsigned compare of the bit
 // pattern, (bits(r^2) - 1) < bits(rc^2), is 0 < r^2 <= rc^2 and rejects them and the i == j self pair.
-template <int M, int G, bool MIX, bool GLOB>
+// INTEG: the velocity-Verlet update in the epilogue, as in k_force_cluster_jrows (kick of this step, drift of
+// the next, FP64 in k_kick_drift's operation order); the next FP32 cluster-relative positions go to
+// ia.pos32_next, because other workgroups still read this step's.
+template <int M, int G, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
-    SkipIf sk, double *__restrict__ clg) {
+    SkipIf sk, double *__restrict__ clg, IntegArgs ia) {
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
@@ -293,11 +296,11 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         const double val = v[r];
         // padding members are never stored
         if (q < M) {
-          if ((real >> q) & 1u) fx[s0 + q] = val;
+          if (!INTEG && ((real >> q) & 1u)) fx[s0 + q] = val;
         } else if (q < 2 * M) {
-          if ((real >> (q - M)) & 1u) fy[s0 + q - M] = val;
+          if (!INTEG && ((real >> (q - M)) & 1u)) fy[s0 + q - M] = val;
         } else if (q < 3 * M) {
-          if ((real >> (q - 2 * M)) & 1u) fz[s0 + q - 2 * M] = val;
+          if (!INTEG && ((real >> (q - 2 * M)) & 1u)) fz[s0 + q - 2 * M] = val;
         } else if (q == 3 * M) {
           h_out = (int64_t)val;
           if (GLOB && clg) clg[3 * (int64_t)ic + 2] = val;  // per-cluster sums for per-box globals
@@ -307,6 +310,55 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         } else if (GLOB && q == 3 * M + 2) {
           v_out = val;
           if (clg) clg[3 * (int64_t)ic + 1] = val;
+        }
+      }
+    }
+    if constexpr (INTEG) {
+      // the group's forces go through LDS so that each lane then advances one whole member
+      __shared__ double sF[kWavesPerBlock * G * 3 * M];
+      double *fsh = sF + ((threadIdx.x / kWave) * G + grp) * (3 * M);
+      if ((l & ((1 << (KL - K)) - 1)) == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int q = q0 + r;
+          if (q < 3 * M) fsh[q] = v[r];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const double4 cc = ia.c0[ic];
+      for (int m = l; m < M; m += LG) {
+#pragma clang fp contract(off)
+        if (!((real >> m) & 1u)) continue;
+        const int64_t i = (int64_t)ic * M + m;
+        const double Fx = fsh[m], Fy = fsh[M + m], Fz = fsh[2 * M + m];
+        const int t = __float_as_int(pos32[i].w);
+        fx[i] = Fx;
+        fy[i] = Fy;
+        fz[i] = Fz;
+        double vx = ia.vx[i], vy = ia.vy[i], vz = ia.vz[i];
+        if (ia.kick) {
+          const double kf = ia.kick_fac[t];
+          vx += (Fx + ia.ofx[i]) * kf;
+          vy += (Fy + ia.ofy[i]) * kf;
+          vz += (Fz + ia.ofz[i]) * kf;
+          ia.vx[i] = vx; ia.vy[i] = vy; ia.vz[i] = vz;
+        }
+        if (ia.drift) {
+          const double ff = ia.drift_fac[t];
+          const double dx = vx * ia.dt + Fx * ff;
+          const double dy = vy * ia.dt + Fy * ff;
+          const double dz = vz * ia.dt + Fz * ff;
+          ia.ofx[i] = Fx; ia.ofy[i] = Fy; ia.ofz[i] = Fz;
+          const double x = ia.x[i] + dx, y = ia.y[i] + dy, z = ia.z[i] + dz;
+          ia.x[i] = x; ia.y[i] = y; ia.z[i] = z;
+          const double ex = x - ia.rx[i], ey = y - ia.ry[i], ez = z - ia.rz[i];
+          if (ex * ex + ey * ey + ez * ez >= ia.half_skin2) {
+            ia.flag[0] = ia.stamp;
+            ia.hflag[0] = ia.stamp;
+          }
+          ia.pos32_next[i] = make_float4((float)(x - cc.x), (float)(y - cc.y), (float)(z - cc.z), __int_as_float(t));
         }
       }
     }
@@ -682,7 +734,7 @@ void lc_force(Ctx &c, bool globals) {
 }
 
 template <int M, int G>
-static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
+static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, const IntegArgs *integ) {
   constexpr int per_block = kWavesPerBlock * G;
   const int nb = blocks_for(c.nclusters, per_block);
   c.part_e.ensure(nb);
@@ -695,15 +747,18 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
     c.cl_glob.ensure(3 * (size_t)c.nclusters);
     clg = c.cl_glob.p;
   }
-#define LJG_LAUNCH(MIXV, GLOBV)                                                                                    \
-  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream,          \
+  const IntegArgs ia = integ ? *integ : IntegArgs{};
+#define LJG_LAUNCH(MIXV, GLOBV, INTV)                                                                              \
+  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
                      c.nclusters, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,           \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
-                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, clg)
-  if (mix) {
-    if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
+                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, clg, ia)
+  if (integ) {
+    if (mix) LJG_LAUNCH(true, false, true); else LJG_LAUNCH(false, false, true);
+  } else if (mix) {
+    if (glob) LJG_LAUNCH(true, true, false); else LJG_LAUNCH(true, false, false);
   } else {
-    if (glob) LJG_LAUNCH(false, true); else LJG_LAUNCH(false, false);
+    if (glob) LJG_LAUNCH(false, true, false); else LJG_LAUNCH(false, false, false);
   }
 #undef LJG_LAUNCH
 }
@@ -768,7 +823,8 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
 // i-clusters per wave for the FP32 kernel: the tuned default per cluster size, or the caller's choice.
 int default_groups(int M) { return M >= 8 ? 1 : M == 4 ? 2 : M == 2 ? 4 : 16; }
 
-void cluster_force(Ctx &c, bool mixed, bool globals) {
+void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ) {
+  if (integ && (!mixed || globals)) throw HipError{hipErrorInvalidValue, "fused update needs the FP32 kernel", -2};
   const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
@@ -784,7 +840,7 @@ void cluster_force(Ctx &c, bool mixed, bool globals) {
     }
   } else {
 #define LJG_CASE(MV, GV) \
-  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab); goto launched; }
+  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ); goto launched; }
     LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4)
     LJG_CASE(4, 1) LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8)
     LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)

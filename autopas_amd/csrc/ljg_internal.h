@@ -148,7 +148,9 @@ __device__ inline bool skip_launch(const SkipIf &s) {
 struct IntegArgs {
   double *vx, *vy, *vz, *ofx, *ofy, *ofz, *x, *y, *z;
   const double *rx, *ry, *rz;
-  int4 *posfx_next;
+  int4 *posfx_next;      // j-row kernel: next fixed-point positions
+  float4 *pos32_next;    // FP32 cluster kernel: next cluster-relative positions (relative to c0)
+  const double4 *c0;
   int32_t *flag, *hflag;
   double dt, half_skin2;
   int32_t kick, drift, stamp;
@@ -252,6 +254,7 @@ struct Ctx {
   // that turns j's cluster-relative FP32 position into i's frame, formed in FP64 and rounded once).
   DevBuf<int4> list;
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
+  DevBuf<float4> pos32_next;    // the fused integrator of the FP32 cluster kernel writes the next step's here
 
   // Verlet lists on the linked cells (fixed-point positions, transposed rows per wavefront)
   FixGeom fg{};
@@ -320,7 +323,7 @@ void lc_bin(Ctx &c);                       // ljg_bin.hip
 void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
 void vcl_pos32(Ctx &c);                    // ljg_vcl.hip
-void cluster_force(Ctx &c, bool mixed, bool globals);  // ljg_force.hip
+void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
 void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
 void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
 void verlet_force(Ctx &c, bool globals);   // ljg_verlet.hip

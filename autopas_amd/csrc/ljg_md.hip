@@ -253,8 +253,14 @@ IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   a.ofx = s.ofx.p; a.ofy = s.ofy.p; a.ofz = s.ofz.p;
   a.x = s.x.p; a.y = s.y.p; a.z = s.z.p;
   a.rx = s.rx.p; a.ry = s.ry.p; a.rz = s.rz.p;
-  c.posfx_next.ensure(c.nslots);
-  a.posfx_next = c.posfx_next.p;
+  if (c.jrows_valid) {
+    c.posfx_next.ensure(c.nslots);
+    a.posfx_next = c.posfx_next.p;
+  } else {
+    c.pos32_next.ensure(c.nslots);
+    a.pos32_next = c.pos32_next.p;
+  }
+  a.c0 = c.c0.p;
   a.flag = c.flag.p;
   a.hflag = c.host_flag_dev;
   a.dt = dt;

@@ -850,24 +850,31 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
 // FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots sit at
 // the sentinel +3e30 in every coordinate: any distance to them squares to +inf in FP32, which fails the
 // cutoff test, while the difference itself stays finite (so masked terms contribute exact zeros).
+// Written to both position buffers: the fused integrator of the FP32 cluster kernel writes the real
+// slots of the second one each step, and the padding slots must hold the sentinel in both.
 __global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
                         const double *__restrict__ z, const int32_t *__restrict__ type,
-                        const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32) {
+                        const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32,
+                        float4 *__restrict__ pos32_next) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
+  float4 p;
   if (orig[s] < 0) {
-    pos32[s] = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
-    return;
+    p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
+  } else {
+    const double4 r = c0[s / M];
+    p = make_float4((float)(x[s] - r.x), (float)(y[s] - r.y), (float)(z[s] - r.z), __int_as_float(type[s]));
   }
-  const double4 r = c0[s / M];
-  pos32[s] = make_float4((float)(x[s] - r.x), (float)(y[s] - r.y), (float)(z[s] - r.z), __int_as_float(type[s]));
+  pos32[s] = p;
+  pos32_next[s] = p;
 }
 
 void vcl_pos32(Ctx &c) {
   c.pos32.ensure(c.nslots);
+  c.pos32_next.ensure(c.nslots);
   State &s = c.st[c.cur];
   hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
-                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p);
+                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p);
   LJG_HIP(hipGetLastError());
 }
 
