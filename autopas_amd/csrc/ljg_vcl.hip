@@ -542,8 +542,11 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
     const int32_t *__restrict__ tbox, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
     const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
     int32_t *__restrict__ jcl) {
-  __shared__ double sbw[NS][kListLds * 6];
-  __shared__ double4 scw[NS][kListLds];
+  // the j-cluster index pass (JCL) needs neither the reference points nor as deep a chunk, and the smaller
+  // LDS footprint lets more workgroups share a CU
+  constexpr int LDSN = JCL ? kListLds / 2 : kListLds;
+  __shared__ double sbw[NS][LDSN * 6];
+  __shared__ double4 scw[NS][JCL ? 1 : LDSN];
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double *sb = sbw[wv];
   double4 *sc = scw[wv];
@@ -592,20 +595,23 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
       const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
       const int64_t cb = cstart[tj], ce = cstart[tj + 1];
       if (cb == ce) continue;
-      const bool single = ce - cb <= kListLds;
+      const bool single = ce - cb <= LDSN;
       for (int sz = -1; sz <= 1; ++sz) {
         const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
         const double zmin = bi[2] - zwin - shz;  // window in the j tower's own frame
         const double zmax = bi[5] + zwin - shz;
-        for (int64_t jb = cb; jb < ce; jb += kListLds) {
-          const int nj = (int)(ce - jb < kListLds ? ce - jb : kListLds);
+        for (int64_t jb = cb; jb < ce; jb += LDSN) {
+          const int nj = (int)(ce - jb < LDSN ? ce - jb : LDSN);
           if (!single || (sz == -1 && jb == cb)) {
             wave_sync();
             for (int k = lane; k < 6 * nj; k += kWave) sb[k] = bbox[6 * jb + k];
-            for (int k = lane; k < nj; k += kWave) sc[k] = c0[jb + k];
+            if constexpr (!JCL)
+              for (int k = lane; k < nj; k += kWave) sc[k] = c0[jb + k];
             wave_sync();
           }
           if (!act) continue;
+          // every cluster's z-range lies in [box_min, box_min + L): a window outside it finds nothing
+          if (zmax < g.box_min[2] || zmin > g.box_min[2] + g.L[2]) continue;
           // lower bound of zhi >= zmin in the chunk
           int lo = 0, hi = nj;
           while (lo < hi) {
