@@ -30,6 +30,11 @@ PEAK_HBM_GBS = 8000.0     # MI355X published HBM3E bandwidth
 # HBM bytes per force launch from the FETCH_SIZE / WRITE_SIZE counter passes (scaled by the calibration
 # copy) recorded in profiles/cluster_jrows_m8g8_bulk_liquid_1M_hbm_traffic.txt, per layout.
 TRAFFIC_BYTES = {"jrows_M8_G8": 422.8e6}
+# Wavefront VALU instructions per force launch (SQ_INSTS_VALU) recorded in
+# profiles/cluster_jrows_m8g8_bulk_liquid_1M_valu_counters.txt, per layout; the issue capacity of a launch is
+# one wavefront instruction per CU per clock.
+VALU_INSTRUCTIONS = {"jrows_M8_G8": 6.311e7}
+MAX_CLOCK_GHZ = 2.4  # MI355X published peak engine clock
 
 
 def parse():
@@ -328,11 +333,16 @@ def main():
             "hbm_frac": (TRAFFIC_BYTES[best] / force_avg_s / 1e9 / PEAK_HBM_GBS) if best in TRAFFIC_BYTES else None,
             "kernel": best,
             "flop_per_pair": FLOP_PER_PAIR,
+            "valu_instructions_per_launch": VALU_INSTRUCTIONS.get(best),
+            "valu_issue_frac": (VALU_INSTRUCTIONS[best] / (256 * MAX_CLOCK_GHZ * 1e9 * force_avg_s))
+                               if best in VALU_INSTRUCTIONS else None,
             "note": "VALU-bound (no MFMA): achieved counts 26 FLOP per unique pair within rc (LJFunctor.h:722-789), "
                     "while the kernel executes ~22 VALU instructions per candidate pair and ~6.5 candidates per "
                     "unique pair (full rows, 31% within rc); peak is the MI355X FP32 vector figure (packed FP32; "
                     "the non-packed issue rate is 78.6). The launch includes the fused velocity-Verlet update; "
-                    "traffic is HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counters",
+                    "traffic is HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counters; valu_issue_frac is "
+                    "the counted VALU instructions over 256 CUs x 2.4 GHz x the launch time (one wavefront "
+                    "instruction per CU per clock): the kernel is bound by VALU issue, not by FLOPs or HBM",
         },
     }
     if args.serve_jobs > 0:
