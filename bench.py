@@ -249,6 +249,31 @@ def ensemble_bench(nboxes, steps, warmup=10, sample=64):
     }
 
 
+def energy_drift(layout_kernel, M, G, n_side=32, steps=1000):
+    """Total-energy drift per particle over `steps` steps (dt 0.002) on the 32,768-particle sample of the
+    liquid, for the FP64 cluster kernel and the chosen FP32-pair layout; energies from the FP64 kernel."""
+    import autopas_amd as ap
+
+    s = make_system(n_side)
+    out = {}
+    for name, kernel, m, g in (("fp64", ap.KERNEL_CLUSTER_FP64, M, 0), ("fp32_pair", layout_kernel, M, G)):
+        with ap.Engine((0.0, 0.0, 0.0), (s["L"],) * 3, 2.5, 0.3, cluster_size=m, tile_groups=g,
+                       rebuild_frequency=10) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+
+            def energy():
+                gl = e.force(ap.KERNEL_CLUSTER_FP64, globals=True)
+                st = e.download()
+                return (0.5 * float(np.sum(st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2)) + gl.potential_energy) / s["n"]
+
+            e0 = energy()
+            e.step(steps, 0.002, kernel)
+            out[name] = energy() - e0
+    return {"particles": s["n"], "steps": steps, "dt": 0.002, "drift_per_particle_fp64": out["fp64"],
+            "drift_per_particle_fp32_pair": out["fp32_pair"],
+            "ratio": abs(out["fp32_pair"]) / abs(out["fp64"]) if out["fp64"] else None}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -345,6 +370,7 @@ def main():
                     "instruction per CU per clock): the kernel is bound by VALU issue, not by FLOPs or HBM",
         },
     }
+    out["energy_drift"] = energy_drift(kernel, M, G)
     if args.serve_jobs > 0:
         out["serving"] = serving(args.serve_jobs, args.serve_workers)
     if args.ensemble_boxes > 0:

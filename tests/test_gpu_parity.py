@@ -505,3 +505,59 @@ def test_update_positions_keeps_lists(oracle, kernel):
     rc2 = RC * RC
     shift = 4.0 * ((1.0 / rc2) ** 3 - (1.0 / rc2) ** 6)  # shift6 / 6 for eps = sigma = 1
     assert abs((g.potential_energy - g0.potential_energy) - g.pair_count * shift) <= 1e-6 * abs(g.potential_energy)
+
+
+def test_c1_plain_method_trajectory(oracle):
+    # The plain-method configuration at full size: 32^3 simple-cubic lattice, spacing 1.1 (periodic 35.2),
+    # jitter +-0.05, T 0.7, rc 2.5, skin 0.3, lists rebuilt every 10 steps, 100 velocity-Verlet steps at
+    # dt 0.001 with the FP64 linked-cells kernel, against the oracle's linked-cells forces and integrator
+    # (FP64 both, summed in other orders: positions within 1e-9, velocities within 1e-8, pair counts equal)
+    announce("C1 plain method: 32^3 lattice a 1.1, 100 steps dt 0.001, LC FP64 vs oracle")
+    n_side, a, dt, steps = 32, 1.1, 0.001, 100
+    L = n_side * a
+    n = n_side ** 3
+    x, y, z, vx, vy, vz, _ = ap.generate_lattice(1, n, (n_side,) * 3, a, (0.0, 0.0, 0.0), 0.05, 0.7)
+    with make_engine(L, rebuild_frequency=10) as e:
+        e.upload(x, y, z, vx, vy, vz)
+        rs = e.step(steps, dt, ap.KERNEL_LC_FP64)
+        g = e.download()
+    x, y, z, vx, vy, vz = (a_.copy() for a_ in (x, y, z, vx, vy, vz))
+    fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+    m = np.array([1.0])
+    pairs = 0
+    for _ in range(steps):
+        ofx, ofy, ofz = oracle.verlet_positions(x, y, z, vx, vy, vz, fx, fy, fz, m, None, dt)
+        x, y, z = oracle.wrap(x, y, z, (0, 0, 0), (L, L, L))
+        r = oracle.lc_forces(x, y, z, RC, SKIN, (0, 0, 0), (L, L, L))
+        fx, fy, fz = r["fx"], r["fy"], r["fz"]
+        pairs += r["pairs"]
+        oracle.verlet_velocities(vx, vy, vz, fx, fy, fz, ofx, ofy, ofz, m, None, dt)
+    assert rs.pair_interactions == pairs
+    d = np.stack([g["x"] - x, g["y"] - y, g["z"] - z])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-9
+    for k, v in (("vx", vx), ("vy", vy), ("vz", vz)):
+        assert np.max(np.abs(g[k] - v)) < 1e-8, k
+
+
+def total_energy_per_particle(e, n):
+    g = e.force(ap.KERNEL_CLUSTER_FP64, globals=True)
+    st = e.download()
+    return (0.5 * np.sum(st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2) + g.potential_energy) / n
+
+
+@pytest.mark.parametrize("kernel,M,G", [(ap.KERNEL_CLUSTER_JROWS_MIXED, 8, 8), (ap.KERNEL_CLUSTER_MIXED, 8, 4)])
+def test_energy_drift_fp32_pair_matches_fp64(kernel, M, G):
+    # energy drift over 1000 steps of the FP32-pair / FP64-accumulate kernels within 2x of the FP64 run's
+    # (20^3 liquid at rho 0.75, T 0.9, dt 0.002; energies evaluated with the FP64 kernel)
+    announce(f"energy drift kernel={ap.KERNEL_NAMES[kernel]} vs FP64: 20^3 liquid, 1000 steps")
+    s = lattice_box(20, 0.75, seed=2, temperature=0.9)
+    drifts = {}
+    for k, m_, g_ in ((ap.KERNEL_CLUSTER_FP64, M, 0), (kernel, M, G)):
+        with make_engine(s["L"], M=m_, tile_groups=g_, rebuild_frequency=10) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            e0 = total_energy_per_particle(e, s["n"])
+            e.step(1000, 0.002, k)
+            drifts[k] = total_energy_per_particle(e, s["n"]) - e0
+    d64, d32 = abs(drifts[ap.KERNEL_CLUSTER_FP64]), abs(drifts[kernel])
+    assert d32 <= 2.0 * d64 + 1e-6, (d32, d64)
