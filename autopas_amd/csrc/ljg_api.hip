@@ -1,10 +1,12 @@
 // ljg_api.hip -- the C ABI (include/ljgpu.h): context lifetime, uploads/downloads in upload order, the
 // rebuild policy and the step loop.  All HIP errors are caught here and turned into status codes.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <new>
 #include <utility>
+#include <vector>
 
 #include "ljg_internal.h"
 
@@ -87,6 +89,34 @@ static void rebuild_for(Ctx &c, int kernel) {
     if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_build(c);
   }
   c.steps_since_rebuild = 0;  // the rebuild's permutation set the reference positions
+  std::fill(c.box_since.begin(), c.box_since.end(), 0);
+}
+
+static bool layout_matches(const Ctx &c, int kernel);
+
+// Rebuild of the ensemble boxes `mark` lists (host, per box), in place in their regions; the other boxes
+// keep their layout and lists.  Every box marked: the whole build.
+static void rebuild_boxes(Ctx &c, int kernel, const std::vector<int32_t> &mark) {
+  bool all = true;
+  for (int32_t m : mark) all = all && m != 0;
+  if (all || !c.regions_valid || !layout_matches(c, kernel)) {
+    rebuild_for(c, kernel);
+    return;
+  }
+  c.box_active_host = mark;
+  LJG_HIP(hipMemcpyAsync(c.box_active.p, c.box_active_host.data(), mark.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, c.stream));
+  c.want_jrows = false;
+  c.build_partial = true;
+  try {
+    vcl_build(c);
+  } catch (...) {
+    c.build_partial = false;
+    throw;
+  }
+  c.build_partial = false;
+  for (size_t b = 0; b < mark.size(); ++b)
+    if (mark[b]) c.box_since[b] = 0;
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {
@@ -195,6 +225,8 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.pos32.release(); c.pos32_next.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
+  c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release(); c.box_active.release();
+  c.box_flag.release(); c.list_tmp.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
@@ -353,6 +385,13 @@ static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const 
   c.nboxes = nboxes;
   c.box_geom_M = 0;
   c.box_glob_valid = false;
+  c.regions_valid = false;
+  c.build_partial = false;
+  c.box_since.assign((size_t)nboxes, 0);
+  c.box_active.ensure((size_t)nboxes);
+  c.box_flag.ensure((size_t)nboxes);
+  LJG_HIP(hipMemsetAsync(c.box_flag.p, 0, (size_t)nboxes * sizeof(int32_t), c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
 }
 
 ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, const double *box_max,
@@ -418,6 +457,7 @@ ljg_status ljg_edit_boxes(ljg_ctx *h, int32_t nkeep, const int32_t *keep, int32_
   c.n = total;
   c.nslots = total;
   c.layout = Layout::Upload;
+  c.regions_valid = false;
   c.have_forces = false;
   c.steps_since_rebuild = 0;
   return LJG_OK;
@@ -460,7 +500,9 @@ ljg_status ljg_get_box_globals(ljg_ctx *h, int32_t nboxes, double *potential_ene
   for (int32_t b = 0; b < nboxes; ++b) {
     if (potential_energy) potential_energy[b] = g[3 * b];
     if (virial) virial[b] = g[3 * b + 1];
-    if (pair_count) pair_count[b] = (int64_t)std::llround(g[3 * b + 2]);
+    // half the box's pair hits, rounded down as a box alone reports them (a pair at the cutoff can be
+    // counted from one side only: the FP32 separations of the two directions round apart)
+    if (pair_count) pair_count[b] = (int64_t)std::floor(g[3 * b + 2]);
   }
   return LJG_OK;
   LJG_API_END
@@ -513,6 +555,7 @@ ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, c
   else wrap_positions(c);
   LJG_HIP(hipStreamSynchronize(c.stream));
   c.layout = Layout::Upload;
+  c.regions_valid = false;
   c.have_forces = false;
   c.steps_since_rebuild = 0;
   return LJG_OK;
@@ -663,6 +706,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
     LJG_HIP(hipGetLastError());
+    if (c.nboxes > 0) fill_i32(c, c.box_flag.p, c.nboxes, 0);
   };
   clear_flag();
   auto record_rebuild = [&]() {
@@ -727,38 +771,53 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     const SkipIf sk = speculative && dyn ? SkipIf{c.flag.p, stamp_of(step) - 1} : SkipIf{nullptr, 0};
     md_kick_drift(c, !fused && step > 0, true, dt, stamp_of(step), sk);
   };
+  // An ensemble keeps a schedule per box, the one the box has alone: box b is rebuilt before the force of
+  // the step at which its steps since its last rebuild reach the frequency, or whose drift moved one of
+  // its particles skin/2 (it raises the box's flag with the shared one).  A raised flag stops every box
+  // after that drift; only the boxes due then are rebuilt, in place, and all go on from that step.
+  const bool per_box = c.nboxes > 0 && (int64_t)c.box_since.size() == c.nboxes;
+  std::vector<int32_t> mark, raised_box;
+  auto steps_to_due = [&]() {
+    int64_t since = c.steps_since_rebuild;
+    if (per_box) since = *std::max_element(c.box_since.begin(), c.box_since.end());
+    return (int64_t)c.prm.rebuild_frequency - since;
+  };
   int32_t s = 0;
   while (s < nsteps) {
     // steps on the current lists before the next scheduled rebuild, queued as one chunk
-    int32_t k = c.prm.rebuild_frequency - (int32_t)c.steps_since_rebuild;
-    if (k < 0) k = 0;
-    if (k > nsteps - s) k = nsteps - s;
-    if (k > kChunkSteps) k = kChunkSteps;
+    int64_t k64 = steps_to_due();
+    if (k64 < 0) k64 = 0;
+    if (k64 > nsteps - s) k64 = nsteps - s;
+    if (k64 > kChunkSteps) k64 = kChunkSteps;
+    const int32_t k = (int32_t)k64;
     for (int32_t j = 0; j < k; ++j) {
       drift_step(s + j, true);
       force_step(s + j, true);
     }
+    const bool due = s + k < nsteps && steps_to_due() - k <= 0;
     // A scheduled rebuild before the force of step s + k is queued at once, after that step's drift: it
     // works on the state where the queued steps stopped, which is also the state a raised flag asks to
-    // rebuild at.  Otherwise the host reads the flag now.
-    const bool rebuild_next = s + k < nsteps && (int64_t)c.steps_since_rebuild + k >= c.prm.rebuild_frequency;
+    // rebuild at.  Otherwise the host reads the flag now.  (An ensemble reads the flags first: which
+    // boxes to rebuild depends on where the steps stopped.)
+    const bool rebuild_next = due && !per_box;
+    if (due) drift_step(s + k, true);
     if (rebuild_next) {
-      drift_step(s + k, true);
       rebuild_now();
       LJG_HIP(hipStreamSynchronize(c.stream));
-    } else if (dyn) {
+    } else if (dyn || due) {
       LJG_HIP(hipStreamSynchronize(c.stream));
     }
     const int32_t raised = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
-    if (raised == 0 && !rebuild_next) {
+    if (raised == 0 && !due) {
       s += k;
       c.steps_since_rebuild += k;
+      for (auto &v : c.box_since) v += k;
       continue;
     }
     // step t is next: steps s .. t - 1 are complete and step t's drift is done; a raised flag names t
     // (the drift that moved a particle skin/2), and nothing queued after that drift ran
     const int32_t t = raised != 0 ? raised - base - 1 : s + k;
-    if (t < s || t > s + k || (!fused && !rebuild_next && t == s + k))
+    if (t < s || t > s + k || (!fused && !due && t == s + k))
       throw HipError{hipErrorUnknown, "displacement flag outside the queued steps", __LINE__};
     // (the rebuild recomputes the fixed-point positions from the FP64 state, so the position buffers the
     // skipped launches swapped need no repair)
@@ -770,10 +829,39 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
         ++it;
       }
     }
-    if (!rebuild_next) rebuild_now();
+    if (per_box) {
+      // The force launches of steps t .. s + k - 1 exited at once, yet each swapped the position buffers
+      // (all but the run's last step): undo an odd count, since the boxes not rebuilt read theirs on.
+      if (fused && !jrows) {
+        int32_t swaps = 0;
+        for (int32_t j = t; j < s + k; ++j) swaps += j + 1 != nsteps;
+        if (swaps & 1) std::swap(c.pos32, c.pos32_next);
+      }
+      // the boxes whose drift of step t raised the flag, and those due at step t
+      mark.assign((size_t)c.nboxes, 0);
+      if (raised != 0) {
+        raised_box.resize((size_t)c.nboxes);
+        LJG_HIP(hipMemcpyAsync(raised_box.data(), c.box_flag.p, raised_box.size() * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, c.stream));
+        LJG_HIP(hipStreamSynchronize(c.stream));
+      }
+      for (int32_t b = 0; b < c.nboxes; ++b) {
+        c.box_since[b] += t - s;
+        mark[b] = (raised != 0 && raised_box[b] != 0) || c.box_since[b] >= c.prm.rebuild_frequency;
+      }
+      record_rebuild();
+      c.carry_force = false;
+      rebuild_boxes(c, kernel, mark);
+      c.carry_force = true;
+      record_rebuild();
+      ++rs.rebuilds;
+    } else if (!rebuild_next) {
+      rebuild_now();
+    }
     clear_flag();
     force_step(t, false);
     c.steps_since_rebuild = 1;
+    for (auto &v : c.box_since) v += 1;
     s = t + 1;
   }
   c.hits_acc = nullptr;

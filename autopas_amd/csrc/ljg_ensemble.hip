@@ -24,13 +24,14 @@ __global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, doubl
                                    double *__restrict__ z, const int32_t *__restrict__ orig,
                                    const int32_t *__restrict__ box_of, const TowerGeom *__restrict__ tg,
                                    const WrapArgs *__restrict__ wr, const int64_t *__restrict__ tbase,
-                                   int32_t *__restrict__ tower_of, int32_t *__restrict__ rank,
-                                   int32_t *__restrict__ count) {
+                                   const int32_t *__restrict__ box_active, int32_t *__restrict__ tower_of,
+                                   int32_t *__restrict__ rank, int32_t *__restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < nslots && orig[i] >= 0;
+  const int32_t o = i < nslots ? orig[i] : -1;
+  const int32_t b = o >= 0 ? box_of[o] : 0;
+  const bool active = o >= 0 && (box_active == nullptr || box_active[b] != 0);
   int32_t t = -1;
   if (active) {
-    const int32_t b = box_of[orig[i]];
     const WrapArgs w = wr[b];
     const double ox = x[i], oy = y[i], oz = z[i];
     const double px = wrap_coord(ox, 0, w), py = wrap_coord(oy, 1, w), pz = wrap_coord(oz, 2, w);
@@ -59,19 +60,89 @@ __global__ void k_wrap_boxes(int64_t nslots, StateView s, const int32_t *__restr
   }
 }
 
-// One wavefront per box: the sums of the box's i-clusters [cstart[tbase[b]], cstart[tbase[b+1]]) in a
-// fixed lane assignment and butterfly, then LJFunctor::endTraversal's scaling for full lists (halve
+// Tower layout of each box inside its regions, one wavefront per box (marked boxes only when box_active
+// is given): particle offsets start[t] from pbase[b], cluster offsets cstart[t] from cbase[b] with
+// ccount[t] = ceil(count[t] / M) clusters, towers in index order; stats[1] = longest tower.
+constexpr int kBoxLayoutThreads = 256;
+
+__global__ __launch_bounds__(kBoxLayoutThreads) void k_box_layout(
+    int32_t nboxes, const int64_t *__restrict__ tbase, const int64_t *__restrict__ pbase,
+    const int64_t *__restrict__ cbase, const int32_t *__restrict__ box_active, const int32_t *__restrict__ count,
+    int32_t M, int64_t *__restrict__ start, int32_t *__restrict__ ccount, int64_t *__restrict__ cstart,
+    unsigned long long *__restrict__ stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int32_t b = (int32_t)(blockIdx.x * (kBoxLayoutThreads / kWave) + threadIdx.x / kWave);
+  if (b >= nboxes) return;
+  if (box_active && box_active[b] == 0) return;
+  const int64_t t0 = tbase[b], t1 = tbase[b + 1];
+  int64_t p = pbase[b], q = cbase[b];
+  int32_t mx = 0;
+  for (int64_t tb = t0; tb < t1; tb += kWave) {
+    const int64_t t = tb + lane;
+    const int32_t k = t < t1 ? count[t] : 0;
+    const int32_t kc = (k + M - 1) / M;
+    int32_t sk = k, sc = kc;  // inclusive scans over the lanes
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int32_t ak = __shfl_up(sk, off, kWave), ac = __shfl_up(sc, off, kWave);
+      if (lane >= off) {
+        sk += ak;
+        sc += ac;
+      }
+    }
+    if (t < t1) {
+      start[t] = p + (sk - k);
+      cstart[t] = q + (sc - kc);
+      ccount[t] = kc;
+    }
+    p += __shfl(sk, kWave - 1, kWave);
+    q += __shfl(sc, kWave - 1, kWave);
+    mx = k > mx ? k : mx;
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const int32_t o = __shfl_xor(mx, off, kWave);
+    mx = o > mx ? o : mx;
+  }
+  if (lane == 0) atomicMax(stats + 1, (unsigned long long)mx);
+}
+
+// src = -1 (padding) over the slots of the marked boxes' regions, and no tower for their clusters.
+__global__ void k_region_prep(int64_t nslots, int32_t M, const int32_t *__restrict__ cluster_box,
+                              const int32_t *__restrict__ box_active, int32_t *__restrict__ src,
+                              int32_t *__restrict__ cluster_tower) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int64_t c = s / M;
+  if (box_active && box_active[cluster_box[c]] == 0) return;
+  src[s] = -1;
+  if (s % M == 0) cluster_tower[c] = -1;
+}
+
+// A wavefront per cluster of an unmarked box copies its row (count[c] <= oldcap entries) to stride newcap.
+constexpr int kRestrideThreads = 256;
+
+__global__ __launch_bounds__(kRestrideThreads) void k_restride_rows(
+    int64_t nclusters, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
+    const int32_t *__restrict__ count, const int4 *__restrict__ in, int32_t oldcap, int4 *__restrict__ out,
+    int32_t newcap) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = (int64_t)blockIdx.x * (kRestrideThreads / kWave) + threadIdx.x / kWave;
+  if (c >= nclusters || box_active[cluster_box[c]] != 0) return;
+  const int32_t n = count[c] < oldcap ? count[c] : oldcap;
+  for (int32_t k = lane; k < n; k += kWave) out[c * (int64_t)newcap + k] = in[c * (int64_t)oldcap + k];
+}
+
+// One wavefront per box: the sums of the box's i-clusters [cbase[b], cbase[b+1]) (the empty ones add 0)
+// in a fixed lane assignment and butterfly, then LJFunctor::endTraversal's scaling for full lists (halve
 // everything, energy / 6; LJFunctor.h:661-685).
 constexpr int kBoxReduceThreads = 256;
 
-__global__ __launch_bounds__(kBoxReduceThreads) void k_box_reduce(int32_t nboxes, const int64_t *__restrict__ tbase,
-                                                                  const int64_t *__restrict__ cstart,
+__global__ __launch_bounds__(kBoxReduceThreads) void k_box_reduce(int32_t nboxes, const int64_t *__restrict__ cbase,
                                                                   const double *__restrict__ cl,
                                                                   double *__restrict__ out) {
   const int lane = threadIdx.x & (kWave - 1);
   const int32_t b = (int32_t)(blockIdx.x * (kBoxReduceThreads / kWave) + threadIdx.x / kWave);
   if (b >= nboxes) return;
-  const int64_t cb = cstart[tbase[b]], ce = cstart[tbase[b + 1]];
+  const int64_t cb = cbase[b], ce = cbase[b + 1];
   double e = 0., v = 0., h = 0.;
   for (int64_t ci = cb + lane; ci < ce; ci += kWave) {
     e += cl[3 * ci];
@@ -191,29 +262,76 @@ void ensemble_geometry(Ctx &c) {
   }
   const int64_t ntow = tbase[nb];
   if (ntow > INT32_MAX / 2) throw HipError{hipErrorInvalidValue, "too many towers in the ensemble", -1};
-  std::vector<int32_t> tbox((size_t)ntow);
-  for (int32_t b = 0; b < nb; ++b)
+  // regions: ceil(count/M) clusters plus one per tower bound sum_t ceil(count_t/M) for any distribution
+  std::vector<int64_t> cbase((size_t)nb + 1), pbase((size_t)nb + 1);
+  cbase[0] = pbase[0] = 0;
+  for (int32_t b = 0; b < nb; ++b) {
+    cbase[b + 1] = cbase[b] + (c.box_count[b] + c.M - 1) / c.M + (tbase[b + 1] - tbase[b]);
+    pbase[b + 1] = pbase[b] + c.box_count[b];
+  }
+  const int64_t ncl = cbase[nb];
+  if (ncl > (int64_t)kClusterMask) throw HipError{hipErrorInvalidValue, "too many clusters in the ensemble", -1};
+  std::vector<int32_t> tbox((size_t)ntow), cbox((size_t)ncl);
+  for (int32_t b = 0; b < nb; ++b) {
     for (int64_t t = tbase[b]; t < tbase[b + 1]; ++t) tbox[t] = b;
+    for (int64_t k = cbase[b]; k < cbase[b + 1]; ++k) cbox[k] = b;
+  }
   c.box_tg.ensure((size_t)nb);
   c.box_tbase.ensure((size_t)nb + 1);
+  c.box_cbase.ensure((size_t)nb + 1);
+  c.box_pbase.ensure((size_t)nb + 1);
   c.tower_box.ensure((size_t)ntow);
+  c.cluster_box.ensure((size_t)ncl);
   LJG_HIP(hipMemcpyAsync(c.box_tg.p, tg.data(), tg.size() * sizeof(TowerGeom), hipMemcpyHostToDevice, c.stream));
   LJG_HIP(hipMemcpyAsync(c.box_tbase.p, tbase.data(), tbase.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                          c.stream));
+  LJG_HIP(hipMemcpyAsync(c.box_cbase.p, cbase.data(), cbase.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                         c.stream));
+  LJG_HIP(hipMemcpyAsync(c.box_pbase.p, pbase.data(), pbase.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                         c.stream));
   LJG_HIP(hipMemcpyAsync(c.tower_box.p, tbox.data(), tbox.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipMemcpyAsync(c.cluster_box.p, cbox.data(), cbox.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                         c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
   c.box_ntowers = ntow;
+  c.box_clusters = ncl;
+  c.regions_valid = false;
   c.tg = tg[0];
   c.box_geom_M = c.M;
 }
 
-void ensemble_tower_hist(Ctx &c) {
+void ensemble_tower_hist(Ctx &c, const int32_t *active) {
   State &s = c.st[c.cur];
   const int64_t ns = c.nslots;
   hipLaunchKernelGGL(k_tower_hist_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
-                     s.orig.p, c.box_of.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, c.tower_of.p,
+                     s.orig.p, c.box_of.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, active, c.tower_of.p,
                      c.rank.p, c.tower_count.p);
   LJG_HIP(hipGetLastError());
+}
+
+void ensemble_layout(Ctx &c, const int32_t *active) {
+  constexpr int per_block = kBoxLayoutThreads / kWave;
+  hipLaunchKernelGGL(k_box_layout, dim3(blocks_for(c.nboxes, per_block)), dim3(kBoxLayoutThreads), 0, c.stream,
+                     c.nboxes, c.box_tbase.p, c.box_pbase.p, c.box_cbase.p, active, c.tower_count.p, c.M,
+                     c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
+  LJG_HIP(hipGetLastError());
+}
+
+void ensemble_region_prep(Ctx &c, const int32_t *active) {
+  const int64_t ns = c.box_clusters * c.M;
+  hipLaunchKernelGGL(k_region_prep, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, c.cluster_box.p,
+                     active, c.perm.p, c.cluster_tower.p);
+  LJG_HIP(hipGetLastError());
+}
+
+void ensemble_restride_lists(Ctx &c, const int32_t *active, int32_t oldcap, int32_t newcap) {
+  const int64_t ncl = c.box_clusters;
+  c.list_tmp.ensure((size_t)ncl * newcap + 1);
+  constexpr int per_block = kRestrideThreads / kWave;
+  hipLaunchKernelGGL(k_restride_rows, dim3(blocks_for(ncl, per_block)), dim3(kRestrideThreads), 0, c.stream, ncl,
+                     c.cluster_box.p, active, c.list_count.p, c.list.p, oldcap, c.list_tmp.p, newcap);
+  LJG_HIP(hipGetLastError());
+  std::swap(c.list, c.list_tmp);
 }
 
 void ensemble_wrap(Ctx &c) {
@@ -226,7 +344,7 @@ void ensemble_box_globals(Ctx &c) {
   c.box_glob.ensure(3 * (size_t)c.nboxes);
   constexpr int per_block = kBoxReduceThreads / kWave;
   hipLaunchKernelGGL(k_box_reduce, dim3(blocks_for(c.nboxes, per_block)), dim3(kBoxReduceThreads), 0, c.stream,
-                     c.nboxes, c.box_tbase.p, c.ctow_start.p, c.cl_glob.p, c.box_glob.p);
+                     c.nboxes, c.box_cbase.p, c.cl_glob.p, c.box_glob.p);
   LJG_HIP(hipGetLastError());
 }
 

@@ -357,6 +357,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
           if (ex * ex + ey * ey + ez * ez >= ia.half_skin2) {
             ia.flag[0] = ia.stamp;
             ia.hflag[0] = ia.stamp;
+            if (ia.box_flag) ia.box_flag[ia.cluster_box[i / M]] = 1;
           }
           ia.pos32_next[i] = make_float4((float)(x - cc.x), (float)(y - cc.y), (float)(z - cc.z), __int_as_float(t));
         }

@@ -152,6 +152,8 @@ struct IntegArgs {
   float4 *pos32_next;    // FP32 cluster kernel: next cluster-relative positions (relative to c0)
   const double4 *c0;
   int32_t *flag, *hflag;
+  int32_t *box_flag;            // ensemble: set to 1 for the box whose particle raised the flag (else null)
+  const int32_t *cluster_box;   // ensemble: box of each cluster
   double dt, half_skin2;
   int32_t kick, drift, stamp;
   double kick_fac[kMaxTypes];   // dt / (2m)
@@ -311,6 +313,21 @@ struct Ctx {
   DevBuf<int32_t> box_of;              // box of each particle, upload order
   DevBuf<int64_t> box_tbase;           // nboxes + 1: first tower of each box
   DevBuf<int32_t> tower_box;           // box of each tower
+  // Fixed regions: box b owns clusters [cbase[b], cbase[b+1]) (its slots are those clusters' M each),
+  // ceil(count/M) + towers of them, enough for any distribution over its towers, so a rebuild of one
+  // box moves no other; the clusters after the box's last tower's are empty (all padding, no list).
+  // Its tower items (the sort input) are [pbase[b], pbase[b+1]).
+  DevBuf<int64_t> box_cbase;           // nboxes + 1
+  DevBuf<int64_t> box_pbase;           // nboxes + 1
+  DevBuf<int32_t> cluster_box;         // box of each cluster
+  int64_t box_clusters = 0;            // cbase[nboxes]
+  bool regions_valid = false;          // the slots are laid out in the regions of the current geometry
+  DevBuf<int32_t> box_active;          // per box: rebuilt by the next build (read when build_partial)
+  bool build_partial = false;          // the next build rebuilds the boxes of box_active only, in place
+  std::vector<int32_t> box_active_host;
+  DevBuf<int32_t> box_flag;            // per box: a particle of the box raised the displacement flag
+  std::vector<int64_t> box_since;      // per box: steps since the box's last rebuild
+  DevBuf<int4> list_tmp;               // rows of the boxes a partial build keeps, when the row length grows
   DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch
   DevBuf<double> box_glob;             // per box (energy, virial, pairs) after endTraversal scaling
   bool box_glob_valid = false;
@@ -322,7 +339,8 @@ void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t 
 void lc_bin(Ctx &c);                       // ljg_bin.hip
 void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
-void vcl_pos32(Ctx &c);                    // ljg_vcl.hip
+void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v);  // ljg_vcl.hip: p[0..n) = v on the stream
+void vcl_pos32(Ctx &c, const int32_t *active = nullptr);  // ljg_vcl.hip (active: ensemble boxes to write)
 void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
 void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
 void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
@@ -342,12 +360,22 @@ void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  /
 void md_set_positions(Ctx &c, const double *pos, const double *vel);
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
-// Ensemble (ljg_ensemble.hip): per-box tower grids for cluster size c.M, the tower histogram that wraps
-// each position into its own box, the wrap alone, and the per-box reduction of cl_glob into box_glob.
+// Ensemble (ljg_ensemble.hip): per-box tower grids and fixed regions for cluster size c.M, the tower
+// histogram that wraps each position into its own box, the per-box tower layout inside the regions, the
+// wrap alone, and the per-box reduction of cl_glob into box_glob.  `active` (device, per box; null: every
+// box) restricts a build to the boxes it marks.
 void ensemble_geometry(Ctx &c);
-void ensemble_tower_hist(Ctx &c);
+void ensemble_tower_hist(Ctx &c, const int32_t *active);
+void ensemble_layout(Ctx &c, const int32_t *active);
+// src = -1 and cluster_tower = -1 over the regions of the active boxes (the sorts then fill their towers)
+void ensemble_region_prep(Ctx &c, const int32_t *active);
+// The cluster-pair rows of the boxes that `active` does not mark, from stride oldcap to newcap.
+void ensemble_restride_lists(Ctx &c, const int32_t *active, int32_t oldcap, int32_t newcap);
 void ensemble_wrap(Ctx &c);
 void ensemble_box_globals(Ctx &c);
+// The slots of the active boxes' regions gathered from src (old slots of the same regions) in place,
+// through the other state buffer; the other boxes' slots are not touched.  ljg_md.hip
+void permute_active(Ctx &c, const int32_t *src, const int32_t *active);
 // Keep the boxes with new_base[b] >= 0 (their particles move to upload indices new_base[b] + local index,
 // with positions, velocities, forces, types and ids), then append `added` new particles from the host
 // after the `kept` ones (zero force).  Leaves the state in upload order in the other buffer, made current.

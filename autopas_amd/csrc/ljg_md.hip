@@ -116,6 +116,58 @@ __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateVi
   }
 }
 
+// The same gather over the slots of the marked ensemble boxes only, into the other buffer at the same
+// slots, then copied back: each box's particles stay in the box's region, so the gather reads no slot
+// another thread writes, and the unmarked boxes' slots are never touched.
+__global__ void k_permute_boxes(int64_t nslots, int32_t M, const int32_t *__restrict__ src,
+                                const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
+                                StateView o, StateView n, int32_t carry_force) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots || box_active[cluster_box[s / M]] == 0) return;
+  const int32_t k = src[s];
+  if (k >= 0) {
+    const double x = o.x[k], y = o.y[k], z = o.z[k];
+    n.x[s] = x;   n.y[s] = y;   n.z[s] = z;
+    n.rx[s] = x;  n.ry[s] = y;  n.rz[s] = z;
+    n.vx[s] = o.vx[k]; n.vy[s] = o.vy[k]; n.vz[s] = o.vz[k];
+    if (carry_force) { n.fx[s] = o.fx[k]; n.fy[s] = o.fy[k]; n.fz[s] = o.fz[k]; }
+    else { n.fx[s] = 0.; n.fy[s] = 0.; n.fz[s] = 0.; }
+    n.ofx[s] = o.ofx[k]; n.ofy[s] = o.ofy[k]; n.ofz[s] = o.ofz[k];
+    n.type[s] = o.type[k]; n.orig[s] = o.orig[k]; n.id[s] = o.id[k];
+  } else {
+    n.x[s] = kDummyPos; n.y[s] = kDummyPos; n.z[s] = kDummyPos;
+    n.vx[s] = 0.; n.vy[s] = 0.; n.vz[s] = 0.;
+    n.fx[s] = 0.; n.fy[s] = 0.; n.fz[s] = 0.;
+    n.ofx[s] = 0.; n.ofy[s] = 0.; n.ofz[s] = 0.;
+    n.rx[s] = kDummyPos; n.ry[s] = kDummyPos; n.rz[s] = kDummyPos;
+    n.type[s] = 0; n.orig[s] = -1; n.id[s] = -1;
+  }
+}
+
+__global__ void k_copy_boxes(int64_t nslots, int32_t M, const int32_t *__restrict__ cluster_box,
+                             const int32_t *__restrict__ box_active, StateView from, StateView to) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots || box_active[cluster_box[s / M]] == 0) return;
+  to.x[s] = from.x[s];   to.y[s] = from.y[s];   to.z[s] = from.z[s];
+  to.vx[s] = from.vx[s]; to.vy[s] = from.vy[s]; to.vz[s] = from.vz[s];
+  to.fx[s] = from.fx[s]; to.fy[s] = from.fy[s]; to.fz[s] = from.fz[s];
+  to.ofx[s] = from.ofx[s]; to.ofy[s] = from.ofy[s]; to.ofz[s] = from.ofz[s];
+  to.rx[s] = from.rx[s]; to.ry[s] = from.ry[s]; to.rz[s] = from.rz[s];
+  to.type[s] = from.type[s]; to.orig[s] = from.orig[s]; to.id[s] = from.id[s];
+}
+
+void permute_active(Ctx &c, const int32_t *src, const int32_t *active) {
+  const int other = 1 - c.cur;
+  const int64_t ns = c.nslots;
+  c.st[other].ensure((size_t)ns);
+  hipLaunchKernelGGL(k_permute_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, src,
+                     c.cluster_box.p, active, view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force);
+  LJG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, c.cluster_box.p,
+                     active, view(c.st[other]), view(c.st[c.cur]));
+  LJG_HIP(hipGetLastError());
+}
+
 void permute_state(Ctx &c, const int32_t *src, int64_t new_slots, int4 *posfx) {
   const int other = 1 - c.cur;
   c.st[other].ensure((size_t)new_slots);
@@ -136,6 +188,8 @@ struct MdArgs {
   double dt;
   double half_skin2;
   int32_t kick, drift, write_pos32, write_fixed, M, stamp;
+  int32_t *box_flag;            // ensemble: set to 1 for the box whose particle raised the flag (else null)
+  const int32_t *cluster_box;   // ensemble: box of each cluster
   FixGeom fg;
   double kick_fac[kMaxTypes];   // dt / (2m), as TimeDiscretization.cpp:160 forms it
   double drift_fac[kMaxTypes];  // dt * dt / (2m), as TimeDiscretization.cpp:40 forms it
@@ -170,6 +224,7 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
   if (ex * ex + ey * ey + ez * ez >= a.half_skin2) {
     flag[0] = a.stamp;
     hflag[0] = a.stamp;  // pinned host copy, read by the step loop without a device-to-host copy
+    if (a.box_flag) a.box_flag[a.cluster_box[i / a.M]] = 1;
   }
   if (a.write_pos32) {
     const double4 r = c0[i / a.M];  // padding slots returned above and keep their sentinel positions
@@ -194,6 +249,10 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp, Skip
   a.write_fixed = c.layout == Layout::Verlet || (c.layout == Layout::Clusters && c.jrows_valid);
   a.fg = c.fg;
   a.M = c.M;
+  if (c.nboxes > 0 && c.regions_valid && c.layout == Layout::Clusters) {
+    a.box_flag = c.box_flag.p;
+    a.cluster_box = c.cluster_box.p;
+  }
   for (int t = 0; t < kMaxTypes; ++t) {
     const double m = c.lj.mass[t] > 0. ? c.lj.mass[t] : 1.;
     a.kick_fac[t] = dt / (2 * m);
@@ -263,6 +322,10 @@ IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   a.c0 = c.c0.p;
   a.flag = c.flag.p;
   a.hflag = c.host_flag_dev;
+  if (c.nboxes > 0 && c.regions_valid) {
+    a.box_flag = c.box_flag.p;
+    a.cluster_box = c.cluster_box.p;
+  }
   a.dt = dt;
   const double hs = c.prm.skin * 0.5;
   a.half_skin2 = hs * hs;

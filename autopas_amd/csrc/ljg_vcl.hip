@@ -262,7 +262,8 @@ __global__ __launch_bounds__(kLayoutBlock) void k_layout_apply(int64_t ntowers, 
 }
 
 // One workgroup per tower: bitonic sort of (z, id) in LDS, then the tower's slots of the cluster layout
-// are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.
+// are written: src[new slot] = old slot, -1 for padding; cluster_tower for the tower's clusters.  A tower
+// holds count[t] particles, its items from start[t], its clusters [cstart[t], cstart[t] + ccount[t]).
 // Two instances: CAP = kSortSmall takes the towers up to that size (12 KB of LDS, so many workgroups
 // share a CU), CAP = kTowerSortCap the rare larger ones; each skips the towers of the other.
 constexpr int kSortThreads = 256;
@@ -279,16 +280,16 @@ __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, co
 // Towers of up to min_n particles are left to k_tower_sort_wave (min_n = 0 when it does not run).
 template <int CAP>
 __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
-    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
-    const double *__restrict__ z, const int64_t *__restrict__ id, const int64_t *__restrict__ cstart, int32_t M,
-    int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err,
-    int32_t min_n) {
+    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
+    const int32_t *__restrict__ items, const double *__restrict__ z, const int64_t *__restrict__ id,
+    const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
+    int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err, int32_t min_n) {
   __shared__ double sz[CAP];
   __shared__ int32_t ss[CAP];
   const int64_t t = blockIdx.x;
   if (t >= ntowers) return;
   const int64_t b = start[t];
-  const int n = (int)(start[t + 1] - b);
+  const int n = count[t];
   if (n <= min_n) return;                           // sorted in registers by k_tower_sort_wave
   if (CAP > kSortSmall && n <= kSortSmall) return;  // sorted by the small instance
   if (CAP == kSortSmall && n > kSortSmall) return;  // left to the large instance
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
       __syncthreads();
     }
   }
-  const int64_t c0 = cstart[t], c1 = cstart[t + 1];
+  const int64_t c0 = cstart[t], c1 = c0 + ccount[t];
   const int64_t nslot = (c1 - c0) * M;
   for (int64_t k = threadIdx.x; k < nslot; k += kSortThreads) src[c0 * M + k] = k < n ? ss[k] : -1;
   for (int64_t c = c0 + threadIdx.x; c < c1; c += kSortThreads) cluster_tower[c] = (int32_t)t;
@@ -396,15 +397,16 @@ __device__ inline void bitonic_sizes(double (&zv)[E], int32_t (&sv)[E], int64_t 
 
 template <int E>
 __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
-    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ items,
-    const double *__restrict__ items_z, const int64_t *__restrict__ items_id, const int64_t *__restrict__ cstart,
-    int32_t M, int32_t *__restrict__ src, int32_t *__restrict__ cluster_tower) {
+    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
+    const int32_t *__restrict__ items, const double *__restrict__ items_z, const int64_t *__restrict__ items_id,
+    const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
+    int32_t *__restrict__ cluster_tower) {
   constexpr int P = kWave * E;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t t = (int64_t)blockIdx.x * (kSortWaveThreads / kWave) + threadIdx.x / kWave;
   if (t >= ntowers) return;
   const int64_t b = start[t];
-  const int n = (int)(start[t + 1] - b);
+  const int n = count[t];
   constexpr int LO = E == 1 ? 0 : E == 4 ? kWave : 4 * kWave;  // the next shorter instance's reach
   static_assert(E == 1 || E == 4 || E == 8, "instances cover up to 64, 65-256 and 257-512 particles");
   if (n > P || n <= LO) return;  // another instance sorts this tower
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
     iv[e] = k < n ? items_id[b + k] : 0;
   }
   bitonic_sizes<E, 2>(zv, sv, iv, lane);
-  const int64_t c0 = cstart[t], c1 = cstart[t + 1];
+  const int64_t c0 = cstart[t], c1 = c0 + ccount[t];
   const int nslot = (int)((c1 - c0) * M);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -433,12 +435,17 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
 // One lane per slot, the M lanes of a cluster reduce with shuffles (M divides the wavefront, and the
 // cluster's slots are consecutive, so a cluster never straddles two wavefronts).  The members are sorted
 // by z, so the first and last real members' z are the minimum and maximum.
+// In an ensemble (cluster_box given) only the clusters of the boxes box_active marks (every box when it
+// is null) are written, and an empty cluster of a region's tail gets an empty row (lcount = 0).
 __global__ void k_cluster_bbox(int64_t nslots, int32_t M, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z,
                                const int32_t *__restrict__ orig, double *__restrict__ bbox,
-                               double4 *__restrict__ c0) {
+                               double4 *__restrict__ c0, const int32_t *__restrict__ cluster_box,
+                               const int32_t *__restrict__ box_active, int32_t *__restrict__ lcount) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool real = s < nslots && orig[s] >= 0;
+  // a cluster's M slots are in one box: the test is uniform over the lanes that exchange values below
+  const bool mine = s < nslots && (box_active == nullptr || box_active[cluster_box[s / M]] != 0);
+  const bool real = mine && orig[s] >= 0;
   const double inf = __builtin_inf();
   double xl = inf, xh = -inf, yl = inf, yh = -inf, zl = inf, zh = -inf;
   int nreal = real ? 1 : 0;
@@ -456,11 +463,12 @@ __global__ void k_cluster_bbox(int64_t nslots, int32_t M, const double *__restri
     zh = fmax(zh, __shfl_xor(zh, off, kWave));
     nreal += __shfl_xor(nreal, off, kWave);
   }
-  if (s < nslots && s % M == 0) {
+  if (mine && s % M == 0) {
     const int64_t c = s / M;
     double *bb = bbox + 6 * c;
     bb[0] = xl; bb[1] = yl; bb[2] = zl; bb[3] = xh; bb[4] = yh; bb[5] = zh;
     c0[c] = make_double4((xl + xh) * 0.5, (yl + yh) * 0.5, (zl + zh) * 0.5, (double)nreal);
+    if (lcount && nreal == 0) lcount[c] = 0;
   }
 }
 
@@ -539,9 +547,9 @@ __device__ inline void wave_sync() {
 template <bool JCL, int NS, bool MB>
 __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
     int64_t ntowers, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
-    const int32_t *__restrict__ tbox, const int64_t *__restrict__ cstart, const double *__restrict__ bbox,
-    const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list,
-    int32_t *__restrict__ jcl) {
+    const int32_t *__restrict__ tbox, const int32_t *__restrict__ box_active, const int64_t *__restrict__ cstart,
+    const int32_t *__restrict__ ccount, const double *__restrict__ bbox, const double4 *__restrict__ c0,
+    int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list, int32_t *__restrict__ jcl) {
   // the j-cluster index pass (JCL) needs neither the reference points nor as deep a chunk, and the smaller
   // LDS footprint lets more workgroups share a CU
   constexpr int LDSN = JCL ? kListLds / 2 : kListLds;
@@ -556,12 +564,13 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
   int64_t tb = 0;
   if constexpr (MB) {
     const int32_t b = tbox[t];
+    if (box_active && box_active[b] == 0) return;  // the whole workgroup: one tower
     g = tgs[b];
     tb = tbase[b];
   }
   const int64_t tl = t - tb;
   const int tx = (int)(tl % g.nt[0]), ty = (int)(tl / g.nt[0]);
-  const int64_t ib = cstart[t], ie = cstart[t + 1];
+  const int64_t ib = cstart[t], ie = ib + ccount[t];
   const double il2 = g.ilen * g.ilen;
   const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
   const bool xy_repeats = g.nt[0] < 2 * g.R + 1 || g.nt[1] < 2 * g.R + 1;
@@ -593,7 +602,7 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
       while (jy < 0) { jy += g.nt[1]; --sy; }
       while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
       const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
-      const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+      const int64_t cb = cstart[tj], ce = cb + ccount[tj];
       if (cb == ce) continue;
       const bool single = ce - cb <= LDSN;
       for (int sz = -1; sz <= 1; ++sz) {
@@ -659,11 +668,20 @@ constexpr int kListLaneThreads = 256;
 template <bool MB>
 __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
     int64_t nclusters, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
-    const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart,
+    const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
+    const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount,
     const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
     int4 *__restrict__ list) {
   const int64_t ci = (int64_t)blockIdx.x * kListLaneThreads + threadIdx.x;
   if (ci >= nclusters) return;
+  if constexpr (MB) {
+    if (box_active && box_active[cluster_box[ci]] == 0) return;
+  }
+  const double4 ai = c0[ci];
+  if (ai.w == 0.) {  // an empty cluster of a region's tail (no tower)
+    count[ci] = 0;
+    return;
+  }
   const int64_t t = cluster_tower[ci];
   TowerGeom g = g1;
   int64_t tb = 0;
@@ -679,7 +697,6 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
   double bi[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
-  const double4 ai = c0[ci];
   const int side = 2 * g.R + 1, nvis = side * side;
   const int64_t w0 = ci * (int64_t)cap;
   int32_t cnt = 0;
@@ -696,7 +713,7 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
     while (jy < 0) { jy += g.nt[1]; --sy; }
     while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
     const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
-    const int64_t cb = cstart[tj], ce = cstart[tj + 1];
+    const int64_t cb = cstart[tj], ce = cb + ccount[tj];
     for (int sz = -1; sz <= 1; ++sz) {
       const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
       const double zmin = bi[2] - zwin - shz;
@@ -861,9 +878,11 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
 __global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
                         const double *__restrict__ z, const int32_t *__restrict__ type,
                         const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32,
-                        float4 *__restrict__ pos32_next) {
+                        float4 *__restrict__ pos32_next, const int32_t *__restrict__ cluster_box,
+                        const int32_t *__restrict__ box_active) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
+  if (box_active && box_active[cluster_box[s / M]] == 0) return;
   float4 p;
   if (orig[s] < 0) {
     p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
@@ -875,12 +894,12 @@ __global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x,
   pos32_next[s] = p;
 }
 
-void vcl_pos32(Ctx &c) {
+void vcl_pos32(Ctx &c, const int32_t *active) {
   c.pos32.ensure(c.nslots);
   c.pos32_next.ensure(c.nslots);
   State &s = c.st[c.cur];
   hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
-                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p);
+                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p, c.cluster_box.p, active);
   LJG_HIP(hipGetLastError());
 }
 
@@ -894,7 +913,7 @@ __global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-static void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
+void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, p, v);
   LJG_HIP(hipGetLastError());
@@ -920,6 +939,10 @@ void vcl_build(Ctx &c) {
   const TowerGeom &g = c.tg;
   const int64_t ntow = ens ? c.box_ntowers : (int64_t)g.nt[0] * g.nt[1];
   c.ntowers = ntow;
+  // An ensemble laid out in its regions may rebuild only the boxes box_active marks, in place: their
+  // particles stay in their own regions and every other box keeps its slots, clusters and lists.
+  const bool partial = ens && c.build_partial && c.regions_valid;
+  const int32_t *active = partial ? c.box_active.p : nullptr;
 
   const int64_t ns = c.nslots;
   State &s = c.st[c.cur];
@@ -934,16 +957,21 @@ void vcl_build(Ctx &c) {
   c.ctow_start.ensure(ntow + 1);
   c.flag.ensure(4);
   fill_i32(c, c.tower_count.p, ntow + 1, 0);
+  c.tstats.ensure(4);
   if (ens) {
-    ensemble_tower_hist(c);
+    // per-box layouts inside the fixed regions; only the longest tower is needed from the stats
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.tstats.p);
+    LJG_HIP(hipGetLastError());
+    ensemble_tower_hist(c, active);
+    ensemble_layout(c, active);
   } else {
     hipLaunchKernelGGL(k_tower_hist, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
                        s.orig.p, g, wrap_args(c.prm), c.tower_of.p, c.rank.p, c.tower_count.p);
     LJG_HIP(hipGetLastError());
   }
   // tower offsets, clusters per tower and their offsets, and the sizes for the one synchronisation
-  c.tstats.ensure(4);
-  if (ntow <= kLayoutOneBlockMax) {
+  if (ens) {
+  } else if (ntow <= kLayoutOneBlockMax) {
     hipLaunchKernelGGL(k_tower_layout, dim3(1), dim3(kLayoutThreads), 0, c.stream, ntow, c.tower_count.p, M,
                        c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
     LJG_HIP(hipGetLastError());
@@ -968,13 +996,14 @@ void vcl_build(Ctx &c) {
   unsigned long long ts[3] = {0, 0, 0};
   LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
-  const int64_t ncl = (int64_t)ts[2];
+  const int64_t ncl = ens ? c.box_clusters : (int64_t)ts[2];
   if (ts[1] > (unsigned long long)kTowerSortCap)
     throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
   c.nclusters = ncl;
   const int64_t new_slots = ncl * M;
   c.perm.ensure(new_slots);
   c.cluster_tower.ensure(ncl);
+  if (ens) ensemble_region_prep(c, active);  // the regions' tails stay padding
   // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
   // pass, [3], [4] total and longest j-particle row, [5] sort tile exceeded; read back once at the end
   c.list_stats.ensure(6);
@@ -982,14 +1011,14 @@ void vcl_build(Ctx &c) {
   LJG_HIP(hipGetLastError());
   // towers of up to 512 particles sort in registers, a wavefront each (three instances by length, each
   // launched only if some tower is in its range); longer ones in LDS
-  const bool wave_sort = (int64_t)ts[0] <= (int64_t)kWaveSortBelow * ntow;
+  const bool wave_sort = ens || (int64_t)ts[0] <= (int64_t)kWaveSortBelow * ntow;
   const int32_t min_n = wave_sort ? 8 * kWave : 0;
   if (wave_sort) {
     constexpr int per_block = kSortWaveThreads / kWave;
 #define LJG_WAVE_SORT(E)                                                                                        \
   hipLaunchKernelGGL(k_tower_sort_wave<E>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream, \
-                     ntow, c.tower_start.p, c.tower_items.p, c.tower_z.p, c.tower_id.p, c.ctow_start.p, M, c.perm.p, \
-                     c.cluster_tower.p);                                                                        \
+                     ntow, c.tower_start.p, c.tower_count.p, c.tower_items.p, c.tower_z.p, c.tower_id.p,          \
+                     c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p);                           \
   LJG_HIP(hipGetLastError());
     // every tower is sorted by exactly one instance; the longest tower (ts[1]) bounds which ones can have work
     LJG_WAVE_SORT(1)
@@ -999,14 +1028,14 @@ void vcl_build(Ctx &c) {
   }
   if (ts[1] > (unsigned long long)min_n) {
     hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
-                       c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p, c.cluster_tower.p,
-                       c.list_stats.p + 5, min_n);
+                       c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p,
+                       c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, c.list_stats.p + 5, min_n);
     LJG_HIP(hipGetLastError());
   }
   if (ts[1] > (unsigned long long)kSortSmall) {
     hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
-                       c.tower_start.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p, M, c.perm.p,
-                       c.cluster_tower.p, c.list_stats.p + 5, min_n);
+                       c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p,
+                       c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, c.list_stats.p + 5, min_n);
     LJG_HIP(hipGetLastError());
   }
   const bool jrows = c.want_jrows;
@@ -1016,23 +1045,27 @@ void vcl_build(Ctx &c) {
     verlet_geometry(c);
     c.posfx.ensure(new_slots);
   }
-  permute_state(c, c.perm.p, new_slots, jrows ? c.posfx.p : nullptr);
+  if (partial) permute_active(c, c.perm.p, active);
+  else permute_state(c, c.perm.p, new_slots, jrows ? c.posfx.p : nullptr);
+  c.regions_valid = ens;
   c.M = M;
   State &t = c.st[c.cur];
   c.bbox.ensure(6 * ncl);
   c.c0.ensure(ncl);
-  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, M, t.x.p,
-                     t.y.p, t.z.p,
-                     t.orig.p, c.bbox.p, c.c0.p);
-  LJG_HIP(hipGetLastError());
   c.list_count.ensure(ncl * kJclSplit + 1);
+  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, M, t.x.p,
+                     t.y.p, t.z.p, t.orig.p, c.bbox.p, c.c0.p, ens ? c.cluster_box.p : (const int32_t *)nullptr,
+                     active, ens ? c.list_count.p : (int32_t *)nullptr);
+  LJG_HIP(hipGetLastError());
   if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
   if (jrows) c.jcount.ensure(ncl + 1);
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
   // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
   // cluster-entry lists: a lane per i-cluster when the towers hold few clusters, unless the caller chose
   const bool lane_lists = c.prm.list_mode == 2 || (c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow);
-  int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24) : (c.list_cap > 0 ? c.list_cap : 64);
+  // (a partial build starts from the stride the other boxes' rows are stored with)
+  int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24)
+                      : partial ? c.list_row : (c.list_cap > 0 ? c.list_cap : 64);
   int32_t jcap = c.jcap > 0 ? c.jcap : 160;
   for (;;) {
     if (jrows) {
@@ -1043,28 +1076,32 @@ void vcl_build(Ctx &c) {
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)5, c.list_stats.p);
     LJG_HIP(hipGetLastError());
+    const TowerGeom *no_tg = nullptr;
+    const int64_t *no_i64 = nullptr;
+    const int32_t *no_i32 = nullptr;
     if (jrows)
       hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit, false>), dim3((unsigned)ntow),
-                         dim3(kListThreads * kJclSplit), 0, c.stream, ntow, g, (const TowerGeom *)nullptr,
-                         (const int64_t *)nullptr, (const int32_t *)nullptr, c.ctow_start.p, c.bbox.p, c.c0.p, cap,
-                         c.list_count.p, (int4 *)nullptr, c.jcl.p);
+                         dim3(kListThreads * kJclSplit), 0, c.stream, ntow, g, no_tg, no_i64, no_i32, no_i32,
+                         c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
+                         c.jcl.p);
     else if (lane_lists && ens)
       hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(ncl, kListLaneThreads)), dim3(kListLaneThreads),
-                         0, c.stream, ncl, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_tower.p,
-                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
+                         0, c.stream, ncl, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_box.p, active,
+                         c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
+                         c.list.p);
     else if (lane_lists)
       hipLaunchKernelGGL((k_cluster_lists_lane<false>), dim3(blocks_for(ncl, kListLaneThreads)),
-                         dim3(kListLaneThreads), 0, c.stream, ncl, g, (const TowerGeom *)nullptr,
-                         (const int64_t *)nullptr, (const int32_t *)nullptr, c.cluster_tower.p, c.ctow_start.p,
-                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
+                         dim3(kListLaneThreads), 0, c.stream, ncl, g, no_tg, no_i64, no_i32, no_i32, no_i32,
+                         c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
+                         c.list.p);
     else if (ens)
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
-                         ntow, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.ctow_start.p, c.bbox.p, c.c0.p, cap,
-                         c.list_count.p, c.list.p, (int32_t *)nullptr);
+                         ntow, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, active, c.ctow_start.p, c.ctow_count.p,
+                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     else
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, false>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
-                         ntow, g, (const TowerGeom *)nullptr, (const int64_t *)nullptr, (const int32_t *)nullptr,
-                         c.ctow_start.p, c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
+                         ntow, g, no_tg, no_i64, no_i32, no_i32, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p,
+                         cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     LJG_HIP(hipGetLastError());
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
     if (jrows) {
@@ -1093,13 +1130,20 @@ void vcl_build(Ctx &c) {
     const int32_t maxc = (int32_t)st[1], maxj = (int32_t)st[4];
     const int32_t split = jrows ? kJclSplit : 1;  // first-pass rows hold `split` sub-rows of cap/split
     const bool ok1 = maxc <= cap / split, ok2 = !jrows || maxj <= jcap;
-    if (!ok1) cap = split * (maxc + maxc / 8 + 8);
+    if (!ok1) {
+      const int32_t grown = split * (maxc + maxc / 8 + 8);
+      // the rows a partial build keeps move to the longer stride; the marked boxes' rows are made again
+      if (partial) ensemble_restride_lists(c, active, cap, grown);
+      cap = grown;
+    }
     if (jrows && ok1 && !ok2) jcap = maxj + maxj / 8 + 8;
     if (!(ok1 && ok2)) continue;
     // keep the row capacities for the next build, with headroom for fluctuations
     // (a melting lattice lengthens its rows by ~10% between rebuilds early on: a quarter of headroom keeps
     // the two passes from running twice)
-    auto keep = [](int32_t used, int32_t had) {
+    // (a partial build keeps the stride the other boxes' rows are stored with)
+    auto keep = [partial](int32_t used, int32_t had) {
+      if (partial) return had;
       const int32_t want = used + used / 4 + 8;
       return want > had ? want : (had > 2 * want ? want : had);
     };
@@ -1119,7 +1163,7 @@ void vcl_build(Ctx &c) {
   c.jrows_valid = jrows;
   c.built_M = M;
   c.layout = Layout::Clusters;
-  if (!jrows) vcl_pos32(c);
+  if (!jrows) vcl_pos32(c, active);
 }
 
 }  // namespace ljg

@@ -142,7 +142,8 @@ ljg_status ljg_edit_boxes(ljg_ctx *ctx, int32_t nkeep, const int32_t *keep, int3
                           const int64_t *id);
 /* Per-box results of the last ljg_force with calc_globals on an ensemble: potential energy and virial of
  * each box (LJFunctor::getPotentialEnergy / getVirial, LJFunctor.h:691-720) and its unique pairs within
- * rc.  Any output pointer may be NULL. */
+ * rc (half its pair hits, rounded down, as ljg_force reports them for a box alone).  Any output pointer
+ * may be NULL. */
 ljg_status ljg_get_box_globals(ljg_ctx *ctx, int32_t nboxes, double *potential_energy, double *virial,
                                int64_t *pair_count);
 
