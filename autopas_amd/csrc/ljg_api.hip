@@ -90,6 +90,7 @@ static void rebuild_for(Ctx &c, int kernel) {
   }
   c.steps_since_rebuild = 0;  // the rebuild's permutation set the reference positions
   std::fill(c.box_since.begin(), c.box_since.end(), 0);
+  c.box_pending.clear();
 }
 
 static bool layout_matches(const Ctx &c, int kernel);
@@ -117,6 +118,22 @@ static void rebuild_boxes(Ctx &c, int kernel, const std::vector<int32_t> &mark) 
   c.build_partial = false;
   for (size_t b = 0; b < mark.size(); ++b)
     if (mark[b]) c.box_since[b] = 0;
+}
+
+// The layout a kernel needs: the whole build when the layout is another, else the boxes an edit added.
+// Returns whether anything was built.
+static bool prepare_layout(Ctx &c, int kernel) {
+  if (!layout_matches(c, kernel)) {
+    rebuild_for(c, kernel);
+    return true;
+  }
+  bool pending = false;
+  for (int32_t p : c.box_pending) pending = pending || p != 0;
+  if (!pending) return false;
+  const std::vector<int32_t> mark = c.box_pending;
+  rebuild_boxes(c, kernel, mark);
+  c.box_pending.clear();
+  return true;
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {
@@ -452,6 +469,23 @@ ljg_status ljg_edit_boxes(ljg_ctx *h, int32_t nkeep, const int32_t *keep, int32_
   }
   int64_t total = 0;
   if (const char *why = check_boxes(c, nb, lo.data(), hi.data(), cnt.data(), &total)) return fail(c, LJG_ERR_ARG, why);
+  if (c.nboxes > 0 && c.regions_valid && c.layout == Layout::Clusters && !c.jrows_valid && c.built_M == c.M) {
+    // the running boxes keep their layouts, lists and rebuild schedules; only the added boxes are built
+    // (before the next force), so every box runs as it would alone
+    const std::vector<int32_t> src(keep, keep + nkeep);
+    const std::vector<int64_t> since = c.box_since;
+    const std::vector<int32_t> pending = c.box_pending;
+    set_box_tables(c, nb, lo.data(), hi.data(), cnt.data(), total);
+    ensemble_edit_regions(c, src, old_pstart, kept, added, x, y, z, vx, vy, vz, type, id);
+    c.box_pending.assign((size_t)nb, 1);
+    for (int32_t k = 0; k < nkeep; ++k) {
+      c.box_since[k] = since[src[k]];
+      c.box_pending[k] = pending.empty() ? 0 : pending[src[k]];
+    }
+    c.n = total;
+    c.have_forces = false;
+    return LJG_OK;
+  }
   ensemble_edit(c, old_pstart, new_base, kept, added, x, y, z, vx, vy, vz, type, id);
   set_box_tables(c, nb, lo.data(), hi.data(), cnt.data(), total);
   c.n = total;
@@ -641,7 +675,7 @@ ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globa
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
   if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
-  if (!layout_matches(c, kernel)) rebuild_for(c, kernel);
+  prepare_layout(c, kernel);
   run_force(c, kernel, calc_globals != 0);
   if (c.nboxes > 0 && calc_globals) ensemble_box_globals(c);
   c.box_glob_valid = c.nboxes > 0 && calc_globals;
@@ -714,11 +748,13 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     rev.push_back(take_event(c));
     LJG_HIP(hipEventRecord(rev.back(), c.stream));
   };
-  if (!layout_matches(c, kernel)) {
-    record_rebuild();
-    rebuild_for(c, kernel);
+  record_rebuild();
+  if (prepare_layout(c, kernel)) {
     record_rebuild();
     ++rs.rebuilds;
+  } else if (timing) {
+    c.event_pool.push_back(rev.back());
+    rev.pop_back();
   }
   c.hits_acc = c.hits_slots.p;
   auto rebuild_now = [&]() {

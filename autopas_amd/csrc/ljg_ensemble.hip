@@ -249,6 +249,206 @@ void ensemble_edit(Ctx &c, const std::vector<int64_t> &old_pstart, const std::ve
   c.cur = other;
 }
 
+// ---- edits that keep the running boxes' layouts ----------------------------------------------------
+// New slot s of new box k = cluster_box[s / M] comes from old slot s + (src_cbase[k] - cbase[k]) M when
+// the box is kept (upload index shifted by orig_delta[k]); the slots of added boxes start as padding.
+__global__ void k_move_slots(int64_t nslots, int32_t M, const int32_t *__restrict__ cluster_box,
+                             const int64_t *__restrict__ cbase, const int64_t *__restrict__ src_cbase,
+                             const int64_t *__restrict__ orig_delta, StateView o, StateView n,
+                             const float4 *__restrict__ pos32_old, float4 *__restrict__ pos32,
+                             float4 *__restrict__ pos32_next) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int32_t k = cluster_box[s / M];
+  const int64_t sc = src_cbase[k];
+  if (sc < 0) {
+    n.x[s] = kDummyPos; n.y[s] = kDummyPos; n.z[s] = kDummyPos;
+    n.vx[s] = 0.; n.vy[s] = 0.; n.vz[s] = 0.;
+    n.fx[s] = 0.; n.fy[s] = 0.; n.fz[s] = 0.;
+    n.ofx[s] = 0.; n.ofy[s] = 0.; n.ofz[s] = 0.;
+    n.rx[s] = kDummyPos; n.ry[s] = kDummyPos; n.rz[s] = kDummyPos;
+    n.type[s] = 0; n.orig[s] = -1; n.id[s] = -1;
+    const float4 p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
+    pos32[s] = p;
+    pos32_next[s] = p;
+    return;
+  }
+  const int64_t f = s + (sc - cbase[k]) * M;
+  n.x[s] = o.x[f];   n.y[s] = o.y[f];   n.z[s] = o.z[f];
+  n.vx[s] = o.vx[f]; n.vy[s] = o.vy[f]; n.vz[s] = o.vz[f];
+  n.fx[s] = o.fx[f]; n.fy[s] = o.fy[f]; n.fz[s] = o.fz[f];
+  n.ofx[s] = o.ofx[f]; n.ofy[s] = o.ofy[f]; n.ofz[s] = o.ofz[f];
+  n.rx[s] = o.rx[f]; n.ry[s] = o.ry[f]; n.rz[s] = o.rz[f];
+  n.type[s] = o.type[f];
+  const int32_t og = o.orig[f];
+  n.orig[s] = og >= 0 ? (int32_t)(og + orig_delta[k]) : -1;
+  n.id[s] = o.id[f];
+  const float4 p = pos32_old[f];
+  pos32[s] = p;
+  pos32_next[s] = p;
+}
+
+// A wavefront per new cluster of a kept box: its box, centre, tower and cluster-pair row from the old
+// cluster, the j-cluster indices of the row shifted with the region; an added box's clusters get empty
+// rows until its build.
+constexpr int kMoveThreads = 256;
+
+__global__ __launch_bounds__(kMoveThreads) void k_move_clusters(
+    int64_t nclusters, const int32_t *__restrict__ cluster_box, const int64_t *__restrict__ cbase,
+    const int64_t *__restrict__ src_cbase, const int64_t *__restrict__ tower_delta, const double *__restrict__ bbox_old,
+    const double4 *__restrict__ c0_old, const int32_t *__restrict__ count_old, const int32_t *__restrict__ ctower_old,
+    const int4 *__restrict__ list_old, int32_t cap, double *__restrict__ bbox, double4 *__restrict__ c0,
+    int32_t *__restrict__ count, int32_t *__restrict__ ctower, int4 *__restrict__ list) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = (int64_t)blockIdx.x * (kMoveThreads / kWave) + threadIdx.x / kWave;
+  if (c >= nclusters) return;
+  const int32_t k = cluster_box[c];
+  const int64_t sc = src_cbase[k];
+  if (sc < 0) {
+    if (lane == 0) count[c] = 0;
+    return;
+  }
+  const int64_t f = c + (sc - cbase[k]);
+  const int32_t dc = (int32_t)(cbase[k] - sc);
+  const int32_t n = count_old[f] < cap ? count_old[f] : cap;
+  if (lane < 6) bbox[6 * c + lane] = bbox_old[6 * f + lane];
+  if (lane == 0) {
+    c0[c] = c0_old[f];
+    count[c] = n;
+    const int32_t t = ctower_old[f];
+    ctower[c] = t >= 0 ? (int32_t)(t + tower_delta[k]) : -1;
+  }
+  for (int32_t e = lane; e < n; e += kWave) {
+    int4 v = list_old[f * (int64_t)cap + e];
+    v.x = (v.x & ~kClusterMask) | ((v.x & kClusterMask) + dc);
+    list[c * (int64_t)cap + e] = v;
+  }
+}
+
+// Added particle k (upload index first + k) into its box's region: box b by its upload index
+// (pbase[b0 ..] searched), slot cbase[b] M + (index - pbase[b]).
+__global__ void k_add_to_regions(int64_t first, int64_t count, int32_t b0, int32_t nboxes, int32_t M,
+                                 const int64_t *__restrict__ pbase, const int64_t *__restrict__ cbase, StateView n,
+                                 const double *__restrict__ in, int32_t have_v, const int32_t *__restrict__ type,
+                                 const int64_t *__restrict__ id) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int64_t i = first + k;
+  int32_t lo = b0, hi = nboxes - 1;  // last box with pbase[b] <= i
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (pbase[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t s = cbase[lo] * M + (i - pbase[lo]);
+  const double x = in[k], y = in[count + k], z = in[2 * count + k];
+  n.x[s] = x; n.y[s] = y; n.z[s] = z;
+  n.rx[s] = x; n.ry[s] = y; n.rz[s] = z;
+  n.vx[s] = have_v ? in[3 * count + k] : 0.;
+  n.vy[s] = have_v ? in[4 * count + k] : 0.;
+  n.vz[s] = have_v ? in[5 * count + k] : 0.;
+  n.fx[s] = 0.; n.fy[s] = 0.; n.fz[s] = 0.;
+  n.ofx[s] = 0.; n.ofy[s] = 0.; n.ofz[s] = 0.;
+  n.type[s] = type ? type[k] : 0;
+  n.orig[s] = (int32_t)i;
+  n.id[s] = id ? id[k] : i;
+}
+
+void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const std::vector<int64_t> &old_pstart,
+                           int64_t kept, int64_t added, const double *x, const double *y, const double *z,
+                           const double *vx, const double *vy, const double *vz, const int32_t *type,
+                           const int64_t *id) {
+  // the old layout's tables, before ensemble_geometry replaces them
+  const std::vector<int64_t> old_cbase = c.box_cbase_h, old_tbase = c.box_tbase_h;
+  const int32_t M = c.M, nb = c.nboxes, nkeep = (int32_t)src_box.size();
+  c.box_geom_M = 0;
+  ensemble_geometry(c);
+  const std::vector<int64_t> &cbase = c.box_cbase_h, &tbase = c.box_tbase_h;
+  std::vector<int64_t> tab(3 * (size_t)nb, -1);  // src_cbase, tower_delta, orig_delta per new box
+  std::vector<int64_t> pbase((size_t)nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) pbase[b + 1] = pbase[b] + c.box_count[b];
+  for (int32_t k = 0; k < nkeep; ++k) {
+    const int32_t b = src_box[k];
+    // a kept box has the grid and region it had: same bounds, count and cluster size
+    if (cbase[k + 1] - cbase[k] != old_cbase[b + 1] - old_cbase[b] || tbase[k + 1] - tbase[k] != old_tbase[b + 1] - old_tbase[b])
+      throw HipError{hipErrorUnknown, "a kept box changed its region", __LINE__};
+    tab[k] = old_cbase[b];
+    tab[nb + k] = tbase[k] - old_tbase[b];
+    tab[2 * (size_t)nb + k] = pbase[k] - old_pstart[b];
+  }
+  for (int32_t k = nkeep; k < nb; ++k) tab[nb + k] = tab[2 * (size_t)nb + k] = 0;
+  c.scan_tmp.ensure(tab.size() + pbase.size());
+  int64_t *d_tab = c.scan_tmp.p, *d_pbase = c.scan_tmp.p + tab.size();
+  LJG_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+  LJG_HIP(hipMemcpyAsync(d_pbase, pbase.data(), pbase.size() * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+  const int64_t ncl = c.box_clusters, ns = ncl * M;
+  const int other = 1 - c.cur;
+  c.st[other].ensure((size_t)ns);
+  DevBuf<float4> p32, p32n;
+  DevBuf<double> bb;
+  DevBuf<double4> cc;
+  DevBuf<int32_t> cnt, ctw;
+  DevBuf<int4> rows;
+  const int32_t cap = c.list_row;
+  p32.ensure((size_t)ns);
+  p32n.ensure((size_t)ns);
+  bb.ensure(6 * (size_t)ncl);
+  cc.ensure((size_t)ncl);
+  cnt.ensure((size_t)ncl * kJclSplit + 1);  // the size every build asks for, so a partial one keeps it
+  ctw.ensure((size_t)ncl);
+  rows.ensure((size_t)ncl * cap + 1);
+  hipLaunchKernelGGL(k_move_slots, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, M, c.cluster_box.p,
+                     c.box_cbase.p, d_tab, d_tab + 2 * nb, view(c.st[c.cur]), view(c.st[other]), c.pos32.p, p32.p,
+                     p32n.p);
+  LJG_HIP(hipGetLastError());
+  constexpr int per_block = kMoveThreads / kWave;
+  hipLaunchKernelGGL(k_move_clusters, dim3(blocks_for(ncl, per_block)), dim3(kMoveThreads), 0, c.stream, ncl,
+                     c.cluster_box.p, c.box_cbase.p, d_tab, d_tab + nb, c.bbox.p, c.c0.p, c.list_count.p,
+                     c.cluster_tower.p, c.list.p, cap, bb.p, cc.p, cnt.p, ctw.p, rows.p);
+  LJG_HIP(hipGetLastError());
+  if (added > 0) {
+    const bool have_v = vx && vy && vz;
+    const size_t n = (size_t)added;
+    c.part_v.ensure(6 * n);
+    const double *src[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < (have_v ? 6 : 3); ++k)
+      LJG_HIP(hipMemcpyAsync(c.part_v.p + k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    int32_t *d_type = nullptr;
+    int64_t *d_id = nullptr;
+    if (type) {
+      c.rank.ensure(n);
+      d_type = c.rank.p;
+      LJG_HIP(hipMemcpyAsync(d_type, type, n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    }
+    if (id) {
+      c.tower_items.ensure(2 * n);  // scratch: the sort input is rebuilt by the next build
+      d_id = reinterpret_cast<int64_t *>(c.tower_items.p);
+      LJG_HIP(hipMemcpyAsync(d_id, id, n * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    }
+    hipLaunchKernelGGL(k_add_to_regions, dim3(blocks_for(added, 256)), dim3(256), 0, c.stream, kept, added, nkeep,
+                       nb, M, d_pbase, c.box_cbase.p, view(c.st[other]), c.part_v.p, (int32_t)have_v, d_type, d_id);
+    LJG_HIP(hipGetLastError());
+  }
+  LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope, the old buffers are freed
+  std::swap(c.pos32, p32);
+  std::swap(c.pos32_next, p32n);
+  std::swap(c.bbox, bb);
+  std::swap(c.c0, cc);
+  std::swap(c.list_count, cnt);
+  std::swap(c.cluster_tower, ctw);
+  std::swap(c.list, rows);
+  for (auto *b : {&p32, &p32n}) b->release();
+  bb.release();
+  cc.release();
+  cnt.release();
+  ctw.release();
+  rows.release();
+  c.cur = other;
+  c.nslots = ns;
+  c.nclusters = ncl;
+  c.regions_valid = true;
+}
+
 void ensemble_geometry(Ctx &c) {
   if (c.box_geom_M == c.M) return;
   const double ilen = c.prm.cutoff + c.prm.skin;
@@ -295,6 +495,8 @@ void ensemble_geometry(Ctx &c) {
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
   c.box_ntowers = ntow;
   c.box_clusters = ncl;
+  c.box_cbase_h = std::move(cbase);
+  c.box_tbase_h = std::move(tbase);
   c.regions_valid = false;
   c.tg = tg[0];
   c.box_geom_M = c.M;

@@ -328,6 +328,8 @@ struct Ctx {
   DevBuf<int32_t> box_flag;            // per box: a particle of the box raised the displacement flag
   std::vector<int64_t> box_since;      // per box: steps since the box's last rebuild
   DevBuf<int4> list_tmp;               // rows of the boxes a partial build keeps, when the row length grows
+  std::vector<int64_t> box_cbase_h, box_tbase_h;  // host copies of box_cbase, box_tbase
+  std::vector<int32_t> box_pending;    // per box: placed in its region by an edit, not built yet
   DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch
   DevBuf<double> box_glob;             // per box (energy, virial, pairs) after endTraversal scaling
   bool box_glob_valid = false;
@@ -373,6 +375,16 @@ void ensemble_region_prep(Ctx &c, const int32_t *active);
 void ensemble_restride_lists(Ctx &c, const int32_t *active, int32_t oldcap, int32_t newcap);
 void ensemble_wrap(Ctx &c);
 void ensemble_box_globals(Ctx &c);
+// An edit of an ensemble laid out in its regions: the tables of the new box list are set (kept boxes
+// first, in keep order, then the added ones) and each kept box's slots, clusters, cluster-pair rows and
+// FP32 positions move to its new region unchanged (cluster indices in the rows shifted with it); the
+// added particles are placed in their boxes' regions, unsorted, for a build of those boxes
+// (box_pending).  src_box[k] is the old index of kept box k; old_pstart the old boxes' first upload
+// indices.
+void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const std::vector<int64_t> &old_pstart,
+                           int64_t kept, int64_t added, const double *x, const double *y, const double *z,
+                           const double *vx, const double *vy, const double *vz, const int32_t *type,
+                           const int64_t *id);
 // The slots of the active boxes' regions gathered from src (old slots of the same regions) in place,
 // through the other state buffer; the other boxes' slots are not touched.  ljg_md.hip
 void permute_active(Ctx &c, const int32_t *src, const int32_t *active);
@@ -383,6 +395,8 @@ void ensemble_edit(Ctx &c, const std::vector<int64_t> &old_pstart, const std::ve
                    int64_t added, const double *x, const double *y, const double *z, const double *vx,
                    const double *vy, const double *vz, const int32_t *type, const int64_t *id);
 
+constexpr int kJclSplit = 4;   // wavefronts per tower in the first pass of the j-row build (list_count holds
+                                // kJclSplit counts per cluster)
 constexpr int kShiftBits = 27;
 constexpr int32_t kClusterMask = (1 << kShiftBits) - 1;
 

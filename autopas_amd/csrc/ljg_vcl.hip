@@ -495,7 +495,6 @@ __device__ inline double box_dist2(const double *a, const double *b, double shx,
 // i-cluster is a binary search on the monotone upper z-extents held in LDS.  count[ci] is the row's full
 // length even when it exceeds cap, so the host can grow cap and rebuild if a row overflowed.
 constexpr int kListThreads = 64;
-constexpr int kJclSplit = 4;   // wavefronts per tower in the first pass of the j-row build
 constexpr int kListLds = 128;  // j-clusters staged per chunk
 
 // JCL = true (the j-particle row build, first pass): entries are the j-cluster index only (int32), and a

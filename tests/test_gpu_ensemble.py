@@ -5,10 +5,9 @@ Tolerances (stated per test):
     tile shape, lists rebuilt at the same steps): bit-identical positions, velocities and forces;
   * per-box forces against the FP64 direct sum over periodic images: RMS relative error <= 1e-5; per-box
     potential energy and virial 1e-5 relative, pair count within 1e-4 relative;
-  * boxes kept through an edit of the ensemble, and jobs served in a batch, against the same box or job
-    alone: positions within 1e-6, velocities within 1e-5 RMS relative (1e-3 absolute), since their lists
-    are rebuilt at other steps and the FP32 pair sums run in another order; boxes added by an edit
-    bit-identical.
+  * with dynamic rebuild on, every box keeps its own rebuild steps: bit-identical to the box alone;
+  * boxes kept through an edit of the ensemble (their layouts, lists and schedules move with them), boxes
+    added by an edit, and jobs served in a batch, against the same box or job alone: bit-identical.
 """
 import numpy as np
 import pytest
@@ -181,26 +180,10 @@ def test_ensemble_workload_sample_steps():
     assert 0 <= g.pair_count - int(bg["pair_count"].sum()) <= len(boxes) // 2
 
 
-def min_image(d, L):
-    return d - L * np.round(d / L)
-
-
-def assert_same_trajectory(a, b, L, what):
-    # equal up to the rounding of the FP32 pair term taken in another order: positions within 1e-6 (minimum
-    # image), velocities within 1e-5 RMS relative and 1e-3 absolute (a pair in a hard collision amplifies
-    # a 1e-7 force difference into ~1e-4 of a velocity of ~20)
-    for k in ("x", "y", "z"):
-        assert np.max(np.abs(min_image(a[k] - b[k], L))) <= 1e-6, (what, k)
-    dv = np.concatenate([a[k] - b[k] for k in ("vx", "vy", "vz")])
-    v = np.concatenate([b[k] for k in ("vx", "vy", "vz")])
-    assert np.max(np.abs(dv)) <= 1e-3, what
-    assert np.sqrt(np.sum(dv ** 2) / np.sum(v ** 2)) <= 1e-5, what
-
-
 def test_edit_boxes_keeps_running_boxes():
     # an ensemble retires boxes and admits new ones between steps with the running boxes' state kept on the
-    # device: the kept boxes continue their trajectories (lists rebuilt at the edit, so to FP32 rounding),
-    # the new boxes run exactly as alone (bit-identical: same rebuild steps, dynamic rebuild off)
+    # device: the kept boxes continue their trajectories with their layouts and lists, the new boxes are
+    # built as they would be alone; both bit-identical to the box alone (same rebuild steps)
     boxes = small_boxes()[:6]
     extra = en.make_boxes(2, seed=12, n_range=(100, 800), first_seed=7000)
     announce(f"ensemble edit: {len(boxes)} boxes 15 steps, keep 3 + add 2, 15 more steps")
@@ -226,7 +209,8 @@ def test_edit_boxes_keeps_running_boxes():
             a.upload(st["x"][s0], st["y"][s0], st["z"][s0], st["vx"][s0], st["vy"][s0], st["vz"][s0])
             a.step(30, en.DT, ap.KERNEL_CLUSTER_MIXED)
             alone = a.download()
-        assert_same_trajectory({k: out[k][sl[pos]] for k in out}, alone, box.length, b)
+        for k in alone:
+            assert np.array_equal(out[k][sl[pos]], alone[k]), (b, k)
     for j, box in enumerate(extra):
         s0 = en.box_slices(ecounts)[j]
         with ap.Engine((0.0, 0.0, 0.0), (box.length,) * 3, en.RC, en.SKIN, cluster_size=4,
@@ -240,8 +224,8 @@ def test_edit_boxes_keeps_running_boxes():
 
 def test_batched_server_results_match_jobs_alone():
     # continuous batching: jobs arrive over time, are admitted under a particle budget that holds only a
-    # few of them at once, and each job's end state matches the job run alone in a context of its own, to
-    # the rounding of the FP32 pair term (the batch rebuilds a job's lists whenever the batch changes)
+    # few of them at once, and each job's end state is bit-identical to the job run alone in a context of
+    # its own (dynamic rebuild on: each job keeps its own rebuild steps through every change of the batch)
     from autopas_amd import serve
 
     jobs = serve.make_jobs(12, rate=400.0, seed=7, n_range=(256, 3000), steps_range=(10, 60))
@@ -254,4 +238,5 @@ def test_batched_server_results_match_jobs_alone():
         with ap.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), serve.RC, serve.SKIN, cluster_size=8, tile_groups=4,
                        rebuild_frequency=10, dynamic_rebuild=True) as a:
             _, _, alone = serve.run_job(a, job, ap.KERNEL_CLUSTER_MIXED)
-        assert_same_trajectory(job.result, alone, job.box_length, job.index)
+        for k in alone:
+            assert np.array_equal(job.result[k], alone[k]), (job.index, k)
