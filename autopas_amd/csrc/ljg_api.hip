@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <new>
 #include <utility>
 #include <vector>
@@ -97,16 +98,85 @@ static bool layout_matches(const Ctx &c, int kernel);
 
 // Rebuild of the ensemble boxes `mark` lists (host, per box), in place in their regions; the other boxes
 // keep their layout and lists.  Every box marked: the whole build.
-static void rebuild_boxes(Ctx &c, int kernel, const std::vector<int32_t> &mark) {
+// The marked boxes' slots, towers and clusters as segments and their list, the clusters of the other
+// boxes as segments (runs of consecutive unmarked boxes), and the mark itself, uploaded on the stream for
+// a partial build and the force launches of the two sets.  False when no box is marked.
+static bool prepare_segments(Ctx &c, const std::vector<int32_t> &mark) {
+  std::vector<int32_t> act;
+  for (int32_t b = 0; b < (int32_t)mark.size(); ++b)
+    if (mark[b]) act.push_back(b);
+  if (act.empty()) return false;
+  const int32_t n = (int32_t)act.size(), M = c.M, nb = (int32_t)mark.size();
+  const std::vector<int64_t> &cb = c.box_cbase_h, &tb = c.box_tbase_h;
+  int32_t runs = 0;
+  for (int32_t b = 0; b < nb; ++b) runs += !mark[b] && (b == 0 || mark[b - 1]);
+  // [prefix n+1, base n] for slots, towers, clusters; the boxes; [prefix runs+1, base runs] of the others
+  std::vector<int64_t> &h = c.seg_host;
+  h.assign(7 * (size_t)n + 3 + 2 * (size_t)runs + 1, 0);
+  int64_t *sp = h.data(), *sb = sp + n + 1, *tp = sb + n, *tbs = tp + n + 1, *cp = tbs + n, *cbs = cp + n + 1;
+  int64_t *bx = cbs + n, *op = bx + n, *ob = op + runs + 1;
+  for (int32_t j = 0; j < n; ++j) {
+    const int32_t b = act[j];
+    sb[j] = cb[b] * M;
+    sp[j + 1] = sp[j] + (cb[b + 1] - cb[b]) * M;
+    tbs[j] = tb[b];
+    tp[j + 1] = tp[j] + (tb[b + 1] - tb[b]);
+    cbs[j] = cb[b];
+    cp[j + 1] = cp[j] + (cb[b + 1] - cb[b]);
+    bx[j] = b;
+  }
+  for (int32_t b = 0, r = 0; b < nb;) {
+    if (mark[b]) {
+      ++b;
+      continue;
+    }
+    int32_t e = b;
+    while (e < nb && !mark[e]) ++e;
+    ob[r] = cb[b];
+    op[r + 1] = op[r] + (cb[e] - cb[b]);
+    ++r;
+    b = e;
+  }
+  // one copy from pinned memory: the segments, then the mark (int32 per box)
+  const size_t bytes = h.size() * sizeof(int64_t) + (size_t)nb * sizeof(int32_t);
+  if (c.pin_seg_cap < bytes) {
+    if (c.pin_seg) LJG_HIP(hipHostFree(c.pin_seg));
+    c.pin_seg = nullptr;
+    c.pin_seg_cap = 0;
+    LJG_HIP(hipHostMalloc((void **)&c.pin_seg, bytes + bytes / 2, hipHostMallocDefault));
+    c.pin_seg_cap = bytes + bytes / 2;
+  }
+  std::memcpy(c.pin_seg, h.data(), h.size() * sizeof(int64_t));
+  std::memcpy(c.pin_seg + h.size() * sizeof(int64_t), mark.data(), (size_t)nb * sizeof(int32_t));
+  c.seg_buf.ensure((bytes + sizeof(int64_t) - 1) / sizeof(int64_t));
+  const int64_t *d = c.seg_buf.p;
+  LJG_HIP(hipMemcpyAsync(c.seg_buf.p, c.pin_seg, bytes, hipMemcpyHostToDevice, c.stream));
+  c.seg_active = reinterpret_cast<const int32_t *>(d + h.size());
+  auto dev = [&](const int64_t *p) { return d + (p - h.data()); };
+  c.seg_slots = Segs{dev(sp), dev(sb), n};
+  c.seg_towers = Segs{dev(tp), dev(tbs), n};
+  c.seg_clusters = Segs{dev(cp), dev(cbs), n};
+  c.seg_nslots = sp[n];
+  c.seg_ntowers = tp[n];
+  c.seg_nclusters = cp[n];
+  c.seg_boxes = dev(bx);
+  c.seg_nboxes = n;
+  c.seg_others = Segs{dev(op), dev(ob), runs};
+  c.seg_nothers = op[runs];
+  return true;
+}
+
+// Rebuild of the ensemble boxes `mark` lists (host, per box), in place in their regions; the other boxes
+// keep their layout and lists.  Every box marked: the whole build.  segments_ready: prepare_segments ran
+// for this mark already.
+static void rebuild_boxes(Ctx &c, int kernel, const std::vector<int32_t> &mark, bool segments_ready = false) {
   bool all = true;
   for (int32_t m : mark) all = all && m != 0;
   if (all || !c.regions_valid || !layout_matches(c, kernel)) {
     rebuild_for(c, kernel);
     return;
   }
-  c.box_active_host = mark;
-  LJG_HIP(hipMemcpyAsync(c.box_active.p, c.box_active_host.data(), mark.size() * sizeof(int32_t),
-                         hipMemcpyHostToDevice, c.stream));
+  if (!segments_ready && !prepare_segments(c, mark)) return;
   c.want_jrows = false;
   c.build_partial = true;
   try {
@@ -234,6 +304,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   Ctx &c = h->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
+  if (c.side) (void)hipStreamSynchronize(c.side);
   for (auto &s : c.st) s.release();
   c.cell_of.release(); c.cell_count.release(); c.cell_start.release(); c.rank.release(); c.perm.release();
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
@@ -242,13 +313,18 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.pos32.release(); c.pos32_next.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
-  c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release(); c.box_active.release();
-  c.box_flag.release(); c.list_tmp.release();
+  c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release();
+  c.box_flag.release(); c.list_tmp.release(); c.seg_buf.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
+  if (c.pin_seg) (void)hipHostFree(c.pin_seg);
+  if (c.pin_flags) (void)hipHostFree(c.pin_flags);
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  if (c.side) (void)hipStreamDestroy(c.side);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return LJG_OK;
@@ -405,7 +481,6 @@ static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const 
   c.regions_valid = false;
   c.build_partial = false;
   c.box_since.assign((size_t)nboxes, 0);
-  c.box_active.ensure((size_t)nboxes);
   c.box_flag.ensure((size_t)nboxes);
   LJG_HIP(hipMemsetAsync(c.box_flag.p, 0, (size_t)nboxes * sizeof(int32_t), c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
@@ -812,7 +887,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   // its particles skin/2 (it raises the box's flag with the shared one).  A raised flag stops every box
   // after that drift; only the boxes due then are rebuilt, in place, and all go on from that step.
   const bool per_box = c.nboxes > 0 && (int64_t)c.box_since.size() == c.nboxes;
-  std::vector<int32_t> mark, raised_box;
+  std::vector<int32_t> mark;
   auto steps_to_due = [&]() {
     int64_t since = c.steps_since_rebuild;
     if (per_box) since = *std::max_element(c.box_since.begin(), c.box_since.end());
@@ -837,6 +912,17 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     // boxes to rebuild depends on where the steps stopped.)
     const bool rebuild_next = due && !per_box;
     if (due) drift_step(s + k, true);
+    if (per_box && dyn) {  // the boxes' flags come back with the same synchronisation
+      if (c.pin_flags_cap < (size_t)c.nboxes) {
+        if (c.pin_flags) LJG_HIP(hipHostFree(c.pin_flags));
+        c.pin_flags = nullptr;
+        c.pin_flags_cap = 0;
+        LJG_HIP(hipHostMalloc((void **)&c.pin_flags, (size_t)c.nboxes * sizeof(int32_t), hipHostMallocDefault));
+        c.pin_flags_cap = (size_t)c.nboxes;
+      }
+      LJG_HIP(hipMemcpyAsync(c.pin_flags, c.box_flag.p, (size_t)c.nboxes * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c.stream));
+    }
     if (rebuild_next) {
       rebuild_now();
       LJG_HIP(hipStreamSynchronize(c.stream));
@@ -875,15 +961,59 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
       }
       // the boxes whose drift of step t raised the flag, and those due at step t
       mark.assign((size_t)c.nboxes, 0);
-      if (raised != 0) {
-        raised_box.resize((size_t)c.nboxes);
-        LJG_HIP(hipMemcpyAsync(raised_box.data(), c.box_flag.p, raised_box.size() * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, c.stream));
-        LJG_HIP(hipStreamSynchronize(c.stream));
-      }
+      bool all = true;
       for (int32_t b = 0; b < c.nboxes; ++b) {
         c.box_since[b] += t - s;
-        mark[b] = (raised != 0 && raised_box[b] != 0) || c.box_since[b] >= c.prm.rebuild_frequency;
+        mark[b] = (raised != 0 && c.pin_flags[b] != 0) || c.box_since[b] >= c.prm.rebuild_frequency;
+        all = all && mark[b];
+      }
+      if (fused && !jrows && !all && c.regions_valid && layout_matches(c, kernel) && prepare_segments(c, mark)) {
+        // The marked boxes are rebuilt on the second stream while the force of step t runs for all other
+        // boxes; then the force of the marked boxes.  The two sets share no cluster, slot or list row.
+        // Everything queued has run (the chunk's synchronisation), so the flags clear without waiting.
+        static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+        LJG_HIP(hipGetLastError());
+        fill_i32(c, c.box_flag.p, c.nboxes, 0);
+        const bool last = t + 1 == nsteps;
+        const IntegArgs ia = integ_args(c, true, !last, dt, last ? 0 : stamp_of(t + 1));
+        const int64_t nb_o = cluster_force_blocks(c, c.seg_nothers), nb_m = cluster_force_blocks(c, c.seg_nclusters);
+        c.part_e.ensure((size_t)(nb_o + nb_m));
+        c.part_v.ensure((size_t)(nb_o + nb_m));
+        c.part_hits.ensure((size_t)(nb_o + nb_m));
+        if (!c.side) LJG_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+        if (!c.ev_fork) LJG_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+        if (!c.ev_join) LJG_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+        LJG_HIP(hipEventRecord(c.ev_fork, c.stream));
+        LJG_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+        c.skip = SkipIf{nullptr, 0};
+        const ClusterSubset others{c.seg_others, c.seg_nothers, 0};
+        cluster_force(c, true, false, &ia, &others);
+        const hipStream_t main_stream = c.stream;
+        c.stream = c.side;
+        try {
+          record_rebuild();
+          c.carry_force = false;
+          rebuild_boxes(c, kernel, mark, true);
+          c.carry_force = true;
+          record_rebuild();
+          const ClusterSubset marked{c.seg_clusters, c.seg_nclusters, nb_o};
+          cluster_force(c, true, false, &ia, &marked);
+          LJG_HIP(hipEventRecord(c.ev_join, c.side));
+        } catch (...) {
+          c.stream = main_stream;
+          c.carry_force = true;
+          (void)hipStreamSynchronize(c.side);
+          throw;
+        }
+        c.stream = main_stream;
+        LJG_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+        if (!last) std::swap(c.pos32, c.pos32_next);
+        ++rs.rebuilds;
+        c.steps_since_rebuild = 1;
+        for (auto &v : c.box_since) v += 1;
+        s = t + 1;
+        continue;
       }
       record_rebuild();
       c.carry_force = false;

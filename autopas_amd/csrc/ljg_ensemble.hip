@@ -20,14 +20,15 @@ namespace ljg {
 
 // Tower histogram over the ensemble: wrap into the particle's own box, then that box's tower index
 // offset by the box's first tower.
-__global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, double *__restrict__ y,
+__global__ void k_tower_hist_boxes(int64_t nitems, Segs sg, double *__restrict__ x, double *__restrict__ y,
                                    double *__restrict__ z, const int32_t *__restrict__ orig,
                                    const int32_t *__restrict__ box_of, const TowerGeom *__restrict__ tg,
                                    const WrapArgs *__restrict__ wr, const int64_t *__restrict__ tbase,
                                    const int32_t *__restrict__ box_active, int32_t *__restrict__ tower_of,
                                    int32_t *__restrict__ rank, int32_t *__restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int32_t o = i < nslots ? orig[i] : -1;
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = it < nitems ? seg_index(sg, it) : -1;
+  const int32_t o = i >= 0 ? orig[i] : -1;
   const int32_t b = o >= 0 ? box_of[o] : 0;
   const bool active = o >= 0 && (box_active == nullptr || box_active[b] != 0);
   int32_t t = -1;
@@ -41,7 +42,7 @@ __global__ void k_tower_hist_boxes(int64_t nslots, double *__restrict__ x, doubl
     t = (int32_t)(tbase[b] + tower_index(tg[b], px, py));
   }
   const int32_t r = wave_aggregated_rank(t, active, count);
-  if (i < nslots) {
+  if (i >= 0) {
     tower_of[i] = t;
     rank[i] = r;
   }
@@ -66,13 +67,14 @@ __global__ void k_wrap_boxes(int64_t nslots, StateView s, const int32_t *__restr
 constexpr int kBoxLayoutThreads = 256;
 
 __global__ __launch_bounds__(kBoxLayoutThreads) void k_box_layout(
-    int32_t nboxes, const int64_t *__restrict__ tbase, const int64_t *__restrict__ pbase,
-    const int64_t *__restrict__ cbase, const int32_t *__restrict__ box_active, const int32_t *__restrict__ count,
-    int32_t M, int64_t *__restrict__ start, int32_t *__restrict__ ccount, int64_t *__restrict__ cstart,
-    unsigned long long *__restrict__ stats) {
+    int32_t nboxes, const int64_t *__restrict__ boxes, const int64_t *__restrict__ tbase,
+    const int64_t *__restrict__ pbase, const int64_t *__restrict__ cbase, const int32_t *__restrict__ box_active,
+    const int32_t *__restrict__ count, int32_t M, int64_t *__restrict__ start, int32_t *__restrict__ ccount,
+    int64_t *__restrict__ cstart, unsigned long long *__restrict__ stats) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int32_t b = (int32_t)(blockIdx.x * (kBoxLayoutThreads / kWave) + threadIdx.x / kWave);
-  if (b >= nboxes) return;
+  const int32_t w = (int32_t)(blockIdx.x * (kBoxLayoutThreads / kWave) + threadIdx.x / kWave);
+  if (w >= nboxes) return;
+  const int32_t b = boxes ? (int32_t)boxes[w] : w;  // the marked boxes' list, or every box
   if (box_active && box_active[b] == 0) return;
   const int64_t t0 = tbase[b], t1 = tbase[b + 1];
   int64_t p = pbase[b], q = cbase[b];
@@ -106,11 +108,12 @@ __global__ __launch_bounds__(kBoxLayoutThreads) void k_box_layout(
 }
 
 // src = -1 (padding) over the slots of the marked boxes' regions, and no tower for their clusters.
-__global__ void k_region_prep(int64_t nslots, int32_t M, const int32_t *__restrict__ cluster_box,
+__global__ void k_region_prep(int64_t nitems, Segs sg, int32_t M, const int32_t *__restrict__ cluster_box,
                               const int32_t *__restrict__ box_active, int32_t *__restrict__ src,
                               int32_t *__restrict__ cluster_tower) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  const int64_t s = seg_index(sg, it);
   const int64_t c = s / M;
   if (box_active && box_active[cluster_box[c]] == 0) return;
   src[s] = -1;
@@ -504,24 +507,30 @@ void ensemble_geometry(Ctx &c) {
 
 void ensemble_tower_hist(Ctx &c, const int32_t *active) {
   State &s = c.st[c.cur];
-  const int64_t ns = c.nslots;
-  hipLaunchKernelGGL(k_tower_hist_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.x.p, s.y.p, s.z.p,
-                     s.orig.p, c.box_of.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, active, c.tower_of.p,
+  const int64_t ni = active ? c.seg_nslots : c.nslots;
+  const Segs sg = active ? c.seg_slots : Segs{nullptr, nullptr, 0};
+  if (ni == 0) return;
+  hipLaunchKernelGGL(k_tower_hist_boxes, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, sg, s.x.p, s.y.p,
+                     s.z.p, s.orig.p, c.box_of.p, c.box_tg.p, c.box_wrap.p, c.box_tbase.p, active, c.tower_of.p,
                      c.rank.p, c.tower_count.p);
   LJG_HIP(hipGetLastError());
 }
 
 void ensemble_layout(Ctx &c, const int32_t *active) {
   constexpr int per_block = kBoxLayoutThreads / kWave;
-  hipLaunchKernelGGL(k_box_layout, dim3(blocks_for(c.nboxes, per_block)), dim3(kBoxLayoutThreads), 0, c.stream,
-                     c.nboxes, c.box_tbase.p, c.box_pbase.p, c.box_cbase.p, active, c.tower_count.p, c.M,
-                     c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
+  const int32_t nb = active ? c.seg_nboxes : c.nboxes;
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_box_layout, dim3(blocks_for(nb, per_block)), dim3(kBoxLayoutThreads), 0, c.stream, nb,
+                     active ? c.seg_boxes : (const int64_t *)nullptr, c.box_tbase.p, c.box_pbase.p, c.box_cbase.p,
+                     active, c.tower_count.p, c.M, c.tower_start.p, c.ctow_count.p, c.ctow_start.p, c.tstats.p);
   LJG_HIP(hipGetLastError());
 }
 
 void ensemble_region_prep(Ctx &c, const int32_t *active) {
-  const int64_t ns = c.box_clusters * c.M;
-  hipLaunchKernelGGL(k_region_prep, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, c.cluster_box.p,
+  const int64_t ni = active ? c.seg_nslots : c.box_clusters * c.M;
+  const Segs sg = active ? c.seg_slots : Segs{nullptr, nullptr, 0};
+  if (ni == 0) return;
+  hipLaunchKernelGGL(k_region_prep, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, sg, c.M, c.cluster_box.p,
                      active, c.perm.p, c.cluster_tower.p);
   LJG_HIP(hipGetLastError());
 }

@@ -178,7 +178,7 @@ __device__ inline int group_value_slot(int l) {
 // ia.pos32_next, because other workgroups still read this step's.
 template <int M, int G, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
-    int64_t nclusters, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
+    int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
@@ -194,12 +194,14 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
   }
   const int lane = threadIdx.x & (kWave - 1);
   const int grp = lane / LG, l = lane % LG;
-  int ic = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
+  // item -> i-cluster: every cluster, or the clusters of the segments sg (nclusters items)
+  const int item = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
+  int ic = item < nclusters ? (int)seg_index(sg, item) : -1;
   if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
   const uint32_t rc2_bits = __float_as_uint(rc2);
   double e_out = 0., v_out = 0.;
   int64_t h_out = 0;
-  if (ic < nclusters) {
+  if (ic >= 0) {
     float xi[M], yi[M], zi[M];
     int ti[M];
 #pragma unroll
@@ -735,13 +737,25 @@ void lc_force(Ctx &c, bool globals) {
 }
 
 template <int M, int G>
-static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, const IntegArgs *integ) {
+static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, const IntegArgs *integ,
+                         const ClusterSubset *sub) {
   constexpr int per_block = kWavesPerBlock * G;
-  const int nb = blocks_for(c.nclusters, per_block);
-  c.part_e.ensure(nb);
-  c.part_v.ensure(nb);
-  c.part_hits.ensure(nb);
-  c.part_items = nb;
+  const Segs sg = sub ? sub->segs : Segs{nullptr, nullptr, 0};
+  const int64_t items = sub ? sub->nitems : c.nclusters;
+  if (items == 0) return;
+  const int nb = blocks_for(items, per_block);
+  // a subset launch writes its block partials after part_offset (reserved beforehand: launches of two
+  // subsets may run at once)
+  const int64_t off = sub ? sub->part_offset : 0;
+  if (!sub) {
+    c.part_e.ensure(nb);
+    c.part_v.ensure(nb);
+    c.part_hits.ensure(nb);
+    c.part_items = nb;
+  } else if (c.part_e.cap < (size_t)(off + nb) || c.part_v.cap < (size_t)(off + nb) ||
+             c.part_hits.cap < (size_t)(off + nb)) {
+    throw HipError{hipErrorInvalidValue, "force partials not reserved for a subset launch", __LINE__};
+  }
   const float rc2 = (float)a.rc2;
   double *clg = nullptr;
   if (glob && c.nboxes > 0) {
@@ -751,9 +765,10 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   const IntegArgs ia = integ ? *integ : IntegArgs{};
 #define LJG_LAUNCH(MIXV, GLOBV, INTV)                                                                              \
   hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
-                     c.nclusters, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,           \
+                     items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,             \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
-                     c.st[c.cur].fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, clg, ia)
+                     c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off, c.part_hits.p + off, c.hits_acc, c.skip, \
+                     clg, ia)
   if (integ) {
     if (mix) LJG_LAUNCH(true, false, true); else LJG_LAUNCH(false, false, true);
   } else if (mix) {
@@ -824,8 +839,14 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
 // i-clusters per wave for the FP32 kernel: the tuned default per cluster size, or the caller's choice.
 int default_groups(int M) { return M >= 8 ? 1 : M == 4 ? 2 : M == 2 ? 4 : 16; }
 
-void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ) {
+int64_t cluster_force_blocks(const Ctx &c, int64_t nitems) {
+  const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_groups(c.M);
+  return blocks_for(nitems, kWavesPerBlock * G);
+}
+
+void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, const ClusterSubset *sub) {
   if (integ && (!mixed || globals)) throw HipError{hipErrorInvalidValue, "fused update needs the FP32 kernel", -2};
+  if (sub && !mixed) throw HipError{hipErrorInvalidValue, "cluster subsets need the FP32 kernel", -2};
   const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
@@ -841,7 +862,7 @@ void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ) {
     }
   } else {
 #define LJG_CASE(MV, GV) \
-  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ); goto launched; }
+  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ, sub); goto launched; }
     LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4)
     LJG_CASE(4, 1) LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8)
     LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)

@@ -142,6 +142,24 @@ __device__ inline bool skip_launch(const SkipIf &s) {
   return f != 0 && f <= s.stamp;
 }
 
+// The index ranges a partial ensemble build works on (the marked boxes' slots, towers or clusters):
+// segment j covers [base[j], base[j] + prefix[j+1] - prefix[j]), and a launch over prefix[n] items maps
+// item i to its index.  n == 0: the identity over every index (a whole build).
+struct Segs {
+  const int64_t *prefix, *base;
+  int32_t n;
+};
+__device__ inline int64_t seg_index(const Segs &g, int64_t i) {
+  if (g.n == 0) return i;
+  int32_t lo = 0, hi = g.n - 1;  // last segment with prefix <= i
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (g.prefix[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return g.base[lo] + (i - g.prefix[lo]);
+}
+
 // Velocity-Verlet update fused into a force kernel's epilogue (kick of this step, drift of the next):
 // the FP64 state of the current layout, the fixed-point positions to write for the next step, the
 // displacement flag (device and pinned host copy) and the per-type factors.
@@ -322,14 +340,30 @@ struct Ctx {
   DevBuf<int32_t> cluster_box;         // box of each cluster
   int64_t box_clusters = 0;            // cbase[nboxes]
   bool regions_valid = false;          // the slots are laid out in the regions of the current geometry
-  DevBuf<int32_t> box_active;          // per box: rebuilt by the next build (read when build_partial)
-  bool build_partial = false;          // the next build rebuilds the boxes of box_active only, in place
-  std::vector<int32_t> box_active_host;
+  bool build_partial = false;          // the next build rebuilds the boxes seg_active marks only, in place
   DevBuf<int32_t> box_flag;            // per box: a particle of the box raised the displacement flag
   std::vector<int64_t> box_since;      // per box: steps since the box's last rebuild
   DevBuf<int4> list_tmp;               // rows of the boxes a partial build keeps, when the row length grows
   std::vector<int64_t> box_cbase_h, box_tbase_h;  // host copies of box_cbase, box_tbase
   std::vector<int32_t> box_pending;    // per box: placed in its region by an edit, not built yet
+  // the marked boxes of a partial build as segments of slots, towers and clusters, and their list
+  DevBuf<int64_t> seg_buf;
+  std::vector<int64_t> seg_host;
+  Segs seg_slots{nullptr, nullptr, 0}, seg_towers{nullptr, nullptr, 0}, seg_clusters{nullptr, nullptr, 0};
+  int64_t seg_nslots = 0, seg_ntowers = 0, seg_nclusters = 0;
+  const int64_t *seg_boxes = nullptr;
+  int32_t seg_nboxes = 0;
+  Segs seg_others{nullptr, nullptr, 0};  // the clusters of the boxes not marked
+  int64_t seg_nothers = 0;
+  const int32_t *seg_active = nullptr;   // device: the mark per box, uploaded with the segments
+  // pinned host staging: the segment upload, and the boxes' displacement flags read back
+  char *pin_seg = nullptr;
+  size_t pin_seg_cap = 0;
+  int32_t *pin_flags = nullptr;
+  size_t pin_flags_cap = 0;
+  // second stream: a partial rebuild and the force of its boxes run beside the force of the others
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch
   DevBuf<double> box_glob;             // per box (energy, virial, pairs) after endTraversal scaling
   bool box_glob_valid = false;
@@ -343,7 +377,16 @@ void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
 void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v);  // ljg_vcl.hip: p[0..n) = v on the stream
 void vcl_pos32(Ctx &c, const int32_t *active = nullptr);  // ljg_vcl.hip (active: ensemble boxes to write)
-void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ = nullptr);  // ljg_force.hip
+// A launch of the FP32 cluster kernel over some i-clusters only (the segments' nitems clusters), with
+// its block partials from part_offset (reserved by the caller).
+struct ClusterSubset {
+  Segs segs;
+  int64_t nitems;
+  int64_t part_offset;
+};
+void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ = nullptr,
+                   const ClusterSubset *sub = nullptr);  // ljg_force.hip
+int64_t cluster_force_blocks(const Ctx &c, int64_t nitems);  // ljg_force.hip: workgroups of a launch
 void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
 void verlet_fixed(Ctx &c);                 // ljg_verlet.hip
 void verlet_force(Ctx &c, bool globals);   // ljg_verlet.hip

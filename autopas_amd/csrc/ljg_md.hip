@@ -119,11 +119,13 @@ __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateVi
 // The same gather over the slots of the marked ensemble boxes only, into the other buffer at the same
 // slots, then copied back: each box's particles stay in the box's region, so the gather reads no slot
 // another thread writes, and the unmarked boxes' slots are never touched.
-__global__ void k_permute_boxes(int64_t nslots, int32_t M, const int32_t *__restrict__ src,
+__global__ void k_permute_boxes(int64_t nitems, Segs sg, int32_t M, const int32_t *__restrict__ src,
                                 const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
                                 StateView o, StateView n, int32_t carry_force) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots || box_active[cluster_box[s / M]] == 0) return;
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  const int64_t s = seg_index(sg, it);
+  if (box_active[cluster_box[s / M]] == 0) return;
   const int32_t k = src[s];
   if (k >= 0) {
     const double x = o.x[k], y = o.y[k], z = o.z[k];
@@ -144,10 +146,12 @@ __global__ void k_permute_boxes(int64_t nslots, int32_t M, const int32_t *__rest
   }
 }
 
-__global__ void k_copy_boxes(int64_t nslots, int32_t M, const int32_t *__restrict__ cluster_box,
+__global__ void k_copy_boxes(int64_t nitems, Segs sg, int32_t M, const int32_t *__restrict__ cluster_box,
                              const int32_t *__restrict__ box_active, StateView from, StateView to) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots || box_active[cluster_box[s / M]] == 0) return;
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  const int64_t s = seg_index(sg, it);
+  if (box_active[cluster_box[s / M]] == 0) return;
   to.x[s] = from.x[s];   to.y[s] = from.y[s];   to.z[s] = from.z[s];
   to.vx[s] = from.vx[s]; to.vy[s] = from.vy[s]; to.vz[s] = from.vz[s];
   to.fx[s] = from.fx[s]; to.fy[s] = from.fy[s]; to.fz[s] = from.fz[s];
@@ -158,13 +162,14 @@ __global__ void k_copy_boxes(int64_t nslots, int32_t M, const int32_t *__restric
 
 void permute_active(Ctx &c, const int32_t *src, const int32_t *active) {
   const int other = 1 - c.cur;
-  const int64_t ns = c.nslots;
-  c.st[other].ensure((size_t)ns);
-  hipLaunchKernelGGL(k_permute_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, src,
+  c.st[other].ensure((size_t)c.nslots);
+  const int64_t ni = c.seg_nslots;
+  if (ni == 0) return;
+  hipLaunchKernelGGL(k_permute_boxes, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, c.seg_slots, c.M, src,
                      c.cluster_box.p, active, view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force);
   LJG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_copy_boxes, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.M, c.cluster_box.p,
-                     active, view(c.st[other]), view(c.st[c.cur]));
+  hipLaunchKernelGGL(k_copy_boxes, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, c.seg_slots, c.M,
+                     c.cluster_box.p, active, view(c.st[other]), view(c.st[c.cur]));
   LJG_HIP(hipGetLastError());
 }
 

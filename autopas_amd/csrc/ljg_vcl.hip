@@ -58,12 +58,14 @@ __global__ void k_tower_hist(int64_t nslots, double *__restrict__ x, double *__r
 }
 
 // Also copies each particle's z and id next to its slot, so that the register sorts read them contiguously.
-__global__ void k_tower_scatter(int64_t nslots, const int32_t *__restrict__ tower_of, const int32_t *__restrict__ rank,
-                                const int64_t *__restrict__ start, const double *__restrict__ z,
-                                const int64_t *__restrict__ id, int32_t *__restrict__ items,
-                                double *__restrict__ items_z, int64_t *__restrict__ items_id) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
+__global__ void k_tower_scatter(int64_t nitems, Segs sg, const int32_t *__restrict__ tower_of,
+                                const int32_t *__restrict__ rank, const int64_t *__restrict__ start,
+                                const double *__restrict__ z, const int64_t *__restrict__ id,
+                                int32_t *__restrict__ items, double *__restrict__ items_z,
+                                int64_t *__restrict__ items_id) {
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  const int64_t i = seg_index(sg, it);
   const int32_t t = tower_of[i];
   if (t < 0) return;
   const int64_t k = start[t] + rank[i];
@@ -280,14 +282,14 @@ __device__ inline bool zid_less(double za, int32_t sa, double zb, int32_t sb, co
 // Towers of up to min_n particles are left to k_tower_sort_wave (min_n = 0 when it does not run).
 template <int CAP>
 __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
-    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
+    int64_t ntowers, Segs sg, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
     const int32_t *__restrict__ items, const double *__restrict__ z, const int64_t *__restrict__ id,
     const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
     int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ err, int32_t min_n) {
   __shared__ double sz[CAP];
   __shared__ int32_t ss[CAP];
-  const int64_t t = blockIdx.x;
-  if (t >= ntowers) return;
+  if ((int64_t)blockIdx.x >= ntowers) return;
+  const int64_t t = seg_index(sg, blockIdx.x);
   const int64_t b = start[t];
   const int n = count[t];
   if (n <= min_n) return;                           // sorted in registers by k_tower_sort_wave
@@ -397,14 +399,15 @@ __device__ inline void bitonic_sizes(double (&zv)[E], int32_t (&sv)[E], int64_t 
 
 template <int E>
 __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
-    int64_t ntowers, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
+    int64_t ntowers, Segs sg, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
     const int32_t *__restrict__ items, const double *__restrict__ items_z, const int64_t *__restrict__ items_id,
     const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
     int32_t *__restrict__ cluster_tower) {
   constexpr int P = kWave * E;
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t t = (int64_t)blockIdx.x * (kSortWaveThreads / kWave) + threadIdx.x / kWave;
-  if (t >= ntowers) return;
+  const int64_t w = (int64_t)blockIdx.x * (kSortWaveThreads / kWave) + threadIdx.x / kWave;
+  if (w >= ntowers) return;
+  const int64_t t = seg_index(sg, w);
   const int64_t b = start[t];
   const int n = count[t];
   constexpr int LO = E == 1 ? 0 : E == 4 ? kWave : 4 * kWave;  // the next shorter instance's reach
@@ -437,14 +440,16 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
 // by z, so the first and last real members' z are the minimum and maximum.
 // In an ensemble (cluster_box given) only the clusters of the boxes box_active marks (every box when it
 // is null) are written, and an empty cluster of a region's tail gets an empty row (lcount = 0).
-__global__ void k_cluster_bbox(int64_t nslots, int32_t M, const double *__restrict__ x,
+__global__ void k_cluster_bbox(int64_t nitems, Segs sg, int32_t M, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z,
                                const int32_t *__restrict__ orig, double *__restrict__ bbox,
                                double4 *__restrict__ c0, const int32_t *__restrict__ cluster_box,
                                const int32_t *__restrict__ box_active, int32_t *__restrict__ lcount) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // segments are whole clusters, so the M lanes of a cluster stay together in a wavefront
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = it < nitems ? seg_index(sg, it) : -1;
   // a cluster's M slots are in one box: the test is uniform over the lanes that exchange values below
-  const bool mine = s < nslots && (box_active == nullptr || box_active[cluster_box[s / M]] != 0);
+  const bool mine = s >= 0 && (box_active == nullptr || box_active[cluster_box[s / M]] != 0);
   const bool real = mine && orig[s] >= 0;
   const double inf = __builtin_inf();
   double xl = inf, xh = -inf, yl = inf, yh = -inf, zl = inf, zh = -inf;
@@ -545,7 +550,7 @@ __device__ inline void wave_sync() {
 // start at tbase[box]; the traversal inside the box is the single-box one.
 template <bool JCL, int NS, bool MB>
 __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
-    int64_t ntowers, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
+    int64_t ntowers, Segs sg, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ box_active, const int64_t *__restrict__ cstart,
     const int32_t *__restrict__ ccount, const double *__restrict__ bbox, const double4 *__restrict__ c0,
     int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list, int32_t *__restrict__ jcl) {
@@ -557,8 +562,8 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double *sb = sbw[wv];
   double4 *sc = scw[wv];
-  const int64_t t = blockIdx.x;
-  if (t >= ntowers) return;
+  if ((int64_t)blockIdx.x >= ntowers) return;
+  const int64_t t = seg_index(sg, blockIdx.x);
   TowerGeom g = g1;
   int64_t tb = 0;
   if constexpr (MB) {
@@ -666,13 +671,14 @@ constexpr int kListLaneThreads = 256;
 
 template <bool MB>
 __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
-    int64_t nclusters, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
+    int64_t nclusters, Segs sg, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
     const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount,
     const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
     int4 *__restrict__ list) {
-  const int64_t ci = (int64_t)blockIdx.x * kListLaneThreads + threadIdx.x;
-  if (ci >= nclusters) return;
+  const int64_t it = (int64_t)blockIdx.x * kListLaneThreads + threadIdx.x;
+  if (it >= nclusters) return;
+  const int64_t ci = seg_index(sg, it);
   if constexpr (MB) {
     if (box_active && box_active[cluster_box[ci]] == 0) return;
   }
@@ -874,13 +880,14 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
 // cutoff test, while the difference itself stays finite (so masked terms contribute exact zeros).
 // Written to both position buffers: the fused integrator of the FP32 cluster kernel writes the real
 // slots of the second one each step, and the padding slots must hold the sentinel in both.
-__global__ void k_pos32(int64_t nslots, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
+__global__ void k_pos32(int64_t nslots, Segs sg, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
                         const double *__restrict__ z, const int32_t *__restrict__ type,
                         const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32,
                         float4 *__restrict__ pos32_next, const int32_t *__restrict__ cluster_box,
                         const int32_t *__restrict__ box_active) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nslots) return;
+  const int64_t s = seg_index(sg, it);
   if (box_active && box_active[cluster_box[s / M]] == 0) return;
   float4 p;
   if (orig[s] < 0) {
@@ -897,8 +904,11 @@ void vcl_pos32(Ctx &c, const int32_t *active) {
   c.pos32.ensure(c.nslots);
   c.pos32_next.ensure(c.nslots);
   State &s = c.st[c.cur];
-  hipLaunchKernelGGL(k_pos32, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, c.M, s.x.p, s.y.p,
-                     s.z.p, s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p, c.cluster_box.p, active);
+  const int64_t ni = active ? c.seg_nslots : c.nslots;
+  const Segs sg = active ? c.seg_slots : Segs{nullptr, nullptr, 0};
+  if (ni == 0) return;
+  hipLaunchKernelGGL(k_pos32, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, sg, c.M, s.x.p, s.y.p, s.z.p,
+                     s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p, c.cluster_box.p, active);
   LJG_HIP(hipGetLastError());
 }
 
@@ -941,7 +951,7 @@ void vcl_build(Ctx &c) {
   // An ensemble laid out in its regions may rebuild only the boxes box_active marks, in place: their
   // particles stay in their own regions and every other box keeps its slots, clusters and lists.
   const bool partial = ens && c.build_partial && c.regions_valid;
-  const int32_t *active = partial ? c.box_active.p : nullptr;
+  const int32_t *active = partial ? c.seg_active : nullptr;
 
   const int64_t ns = c.nslots;
   State &s = c.st[c.cur];
@@ -989,9 +999,19 @@ void vcl_build(Ctx &c) {
                        M, bsum, c.tower_start.p, c.ctow_count.p, c.ctow_start.p);
     LJG_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.tower_of.p, c.rank.p,
-                     c.tower_start.p, s.z.p, s.id.p, c.tower_items.p, c.tower_z.p, c.tower_id.p);
-  LJG_HIP(hipGetLastError());
+  // a partial build launches over the marked boxes' segments only
+  const Segs none{nullptr, nullptr, 0};
+  const Segs sg_slots = partial ? c.seg_slots : none, sg_towers = partial ? c.seg_towers : none;
+  const Segs sg_clusters = partial ? c.seg_clusters : none;
+  const int64_t n_slots = partial ? c.seg_nslots : ns, n_towers = partial ? c.seg_ntowers : ntow;
+  if (n_slots > 0) {
+    hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(n_slots, 256)), dim3(256), 0, c.stream, n_slots, sg_slots,
+                       c.tower_of.p, c.rank.p, c.tower_start.p, s.z.p, s.id.p, c.tower_items.p, c.tower_z.p,
+                       c.tower_id.p);
+    LJG_HIP(hipGetLastError());
+  }
+  // the sizes: the longest tower picks the sort instances (only those are launched: an instance with a
+  // large LDS tile or many registers waits for room on a GPU busy with other work)
   unsigned long long ts[3] = {0, 0, 0};
   LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
   LJG_HIP(hipStreamSynchronize(c.stream));
@@ -1015,26 +1035,28 @@ void vcl_build(Ctx &c) {
   if (wave_sort) {
     constexpr int per_block = kSortWaveThreads / kWave;
 #define LJG_WAVE_SORT(E)                                                                                        \
-  hipLaunchKernelGGL(k_tower_sort_wave<E>, dim3(blocks_for(ntow, per_block)), dim3(kSortWaveThreads), 0, c.stream, \
-                     ntow, c.tower_start.p, c.tower_count.p, c.tower_items.p, c.tower_z.p, c.tower_id.p,          \
-                     c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p);                           \
-  LJG_HIP(hipGetLastError());
+  if (n_towers > 0) {                                                                                           \
+    hipLaunchKernelGGL(k_tower_sort_wave<E>, dim3(blocks_for(n_towers, per_block)), dim3(kSortWaveThreads), 0,  \
+                       c.stream, n_towers, sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p,        \
+                       c.tower_z.p, c.tower_id.p, c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p); \
+    LJG_HIP(hipGetLastError());                                                                                 \
+  }
     // every tower is sorted by exactly one instance; the longest tower (ts[1]) bounds which ones can have work
     LJG_WAVE_SORT(1)
     if (ts[1] > (unsigned long long)kWave) LJG_WAVE_SORT(4)
     if (ts[1] > (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(8)
 #undef LJG_WAVE_SORT
   }
-  if (ts[1] > (unsigned long long)min_n) {
-    hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
-                       c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p,
+  if (ts[1] > (unsigned long long)min_n && n_towers > 0) {
+    hipLaunchKernelGGL(k_tower_sort<kSortSmall>, dim3((unsigned)n_towers), dim3(kSortThreads), 0, c.stream, n_towers,
+                       sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p,
                        c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, c.list_stats.p + 5, min_n);
     LJG_HIP(hipGetLastError());
   }
-  if (ts[1] > (unsigned long long)kSortSmall) {
-    hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)ntow), dim3(kSortThreads), 0, c.stream, ntow,
-                       c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p, c.ctow_start.p,
-                       c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, c.list_stats.p + 5, min_n);
+  if (ts[1] > (unsigned long long)kSortSmall && n_towers > 0) {
+    hipLaunchKernelGGL(k_tower_sort<kTowerSortCap>, dim3((unsigned)n_towers), dim3(kSortThreads), 0, c.stream,
+                       n_towers, sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p, s.z.p, s.id.p,
+                       c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, c.list_stats.p + 5, min_n);
     LJG_HIP(hipGetLastError());
   }
   const bool jrows = c.want_jrows;
@@ -1052,10 +1074,14 @@ void vcl_build(Ctx &c) {
   c.bbox.ensure(6 * ncl);
   c.c0.ensure(ncl);
   c.list_count.ensure(ncl * kJclSplit + 1);
-  hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(new_slots, 256)), dim3(256), 0, c.stream, new_slots, M, t.x.p,
-                     t.y.p, t.z.p, t.orig.p, c.bbox.p, c.c0.p, ens ? c.cluster_box.p : (const int32_t *)nullptr,
-                     active, ens ? c.list_count.p : (int32_t *)nullptr);
-  LJG_HIP(hipGetLastError());
+  const int64_t n_bbox = partial ? c.seg_nslots : new_slots;
+  if (n_bbox > 0) {
+    hipLaunchKernelGGL(k_cluster_bbox, dim3(blocks_for(n_bbox, 256)), dim3(256), 0, c.stream, n_bbox, sg_slots, M,
+                       t.x.p, t.y.p, t.z.p, t.orig.p, c.bbox.p, c.c0.p,
+                       ens ? c.cluster_box.p : (const int32_t *)nullptr, active,
+                       ens ? c.list_count.p : (int32_t *)nullptr);
+    LJG_HIP(hipGetLastError());
+  }
   if (jrows && M > kMaxM) throw HipError{hipErrorInvalidValue, "cluster size above 8", -2};
   if (jrows) c.jcount.ensure(ncl + 1);
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
@@ -1078,28 +1104,31 @@ void vcl_build(Ctx &c) {
     const TowerGeom *no_tg = nullptr;
     const int64_t *no_i64 = nullptr;
     const int32_t *no_i32 = nullptr;
-    if (jrows)
+    const int64_t n_lane = partial ? c.seg_nclusters : ncl;
+    if (n_towers == 0 || n_lane == 0)
+      ;  // no box marked
+    else if (jrows)
       hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit, false>), dim3((unsigned)ntow),
-                         dim3(kListThreads * kJclSplit), 0, c.stream, ntow, g, no_tg, no_i64, no_i32, no_i32,
+                         dim3(kListThreads * kJclSplit), 0, c.stream, ntow, none, g, no_tg, no_i64, no_i32, no_i32,
                          c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
                          c.jcl.p);
     else if (lane_lists && ens)
-      hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(ncl, kListLaneThreads)), dim3(kListLaneThreads),
-                         0, c.stream, ncl, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_box.p, active,
+      hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(n_lane, kListLaneThreads)),
+                         dim3(kListLaneThreads), 0, c.stream, n_lane, sg_clusters, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_box.p, active,
                          c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
                          c.list.p);
     else if (lane_lists)
       hipLaunchKernelGGL((k_cluster_lists_lane<false>), dim3(blocks_for(ncl, kListLaneThreads)),
-                         dim3(kListLaneThreads), 0, c.stream, ncl, g, no_tg, no_i64, no_i32, no_i32, no_i32,
+                         dim3(kListLaneThreads), 0, c.stream, ncl, none, g, no_tg, no_i64, no_i32, no_i32, no_i32,
                          c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
                          c.list.p);
     else if (ens)
-      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
-                         ntow, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, active, c.ctow_start.p, c.ctow_count.p,
+      hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)n_towers), dim3(kListThreads), 0,
+                         c.stream, n_towers, sg_towers, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, active, c.ctow_start.p, c.ctow_count.p,
                          c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     else
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, false>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
-                         ntow, g, no_tg, no_i64, no_i32, no_i32, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p,
+                         ntow, none, g, no_tg, no_i64, no_i32, no_i32, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p,
                          cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
     LJG_HIP(hipGetLastError());
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
