@@ -2,13 +2,19 @@
 //
 // md-flexible runs one AutoPas instance per simulation (AutoPasImpl.h:55-113), each with its own tower
 // grid (ClusterTowerBlock2D.h:140-168), cluster lists and rebuilds.  Here the boxes' particles are
-// concatenated box by box and every launch covers all boxes:
+// concatenated box by box and the force launch covers all boxes:
 //   * each box gets the tower grid it would have alone (make_tower_geom on its own box and count), and
 //     the boxes' towers are numbered one box after the other, so clusters and slots stay box-major;
+//   * each box owns a fixed region of clusters (and their slots), ceil(count/M) + towers of them, which
+//     holds any distribution of its particles over its towers; the clusters after its last tower's are
+//     empty.  A box is rebuilt in place in its region, so one box's rebuild moves no other box, and
+//     every box keeps the rebuild schedule it has alone (the step loop's per-box flags and counters);
 //   * the tower histogram wraps each particle into its own box (RegularGridDecomposition.cpp:563-587);
 //   * the cluster-pair list pass reads the grid of the tower's box (k_cluster_lists_tower<.., MB>), and
 //     its list entries carry the image offset c0_j + shift - c0_i formed with that box's lengths, so the
 //     FP32 cluster force kernel runs on the ensemble unchanged;
+//   * an edit of the box list moves each kept box's region (slots, clusters, rows, FP32 positions) as
+//     it is, and places the added boxes' particles in their regions for a build of those boxes alone;
 //   * per-box potential energy, virial and pair count are reduced from per-cluster sums in a fixed order.
 // Compiled with -ffp-contract=off like the single-box build: a box's grid, wrap and tower indices round
 // exactly as they do when the box is alone.
