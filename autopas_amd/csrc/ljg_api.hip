@@ -57,6 +57,19 @@ __global__ void k_to_upload_order(int64_t nslots, const int32_t *__restrict__ or
   if (o >= 0) out[o] = in[s];
 }
 
+// Positions, velocities and forces of the particles with upload indices [first, first + count), array by
+// array (x, y, z, vx, vy, vz, fx, fy, fz), count values each.
+__global__ void k_gather_range(int64_t nslots, const int32_t *__restrict__ orig, int64_t first, int64_t count,
+                               StateView s, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const int64_t k = (int64_t)orig[i] - first;
+  if (k < 0 || k >= count) return;
+  const double *in[9] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.fx, s.fy, s.fz};
+#pragma unroll
+  for (int a = 0; a < 9; ++a) out[a * count + k] = in[a][i];
+}
+
 // out[s] = in[orig[s]] (upload order -> storage order).
 __global__ void k_from_upload_order(int64_t nslots, const int32_t *__restrict__ orig, const double *__restrict__ in,
                                     double *__restrict__ out) {
@@ -115,8 +128,14 @@ static bool prepare_segments(Ctx &c, const std::vector<int32_t> &mark) {
   h.assign(7 * (size_t)n + 3 + 2 * (size_t)runs + 1, 0);
   int64_t *sp = h.data(), *sb = sp + n + 1, *tp = sb + n, *tbs = tp + n + 1, *cp = tbs + n, *cbs = cp + n + 1;
   int64_t *bx = cbs + n, *op = bx + n, *ob = op + runs + 1;
+  // longest tower expected: twice a box's mean tower plus 32, at most its particles (a longer tower is
+  // reported by the build, which then reads the longest tower back and runs again)
+  c.seg_max_count = 0;
   for (int32_t j = 0; j < n; ++j) {
     const int32_t b = act[j];
+    const int64_t nt = tb[b + 1] - tb[b], mean = (c.box_count[b] + nt - 1) / nt;
+    const int64_t guess = 2 * mean + 32 < c.box_count[b] ? 2 * mean + 32 : c.box_count[b];
+    c.seg_max_count = guess > c.seg_max_count ? guess : c.seg_max_count;
     sb[j] = cb[b] * M;
     sp[j + 1] = sp[j] + (cb[b + 1] - cb[b]) * M;
     tbs[j] = tb[b];
@@ -267,6 +286,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->tile_groups < 0 || p->tile_groups > 64) return LJG_ERR_ARG;
   if (p->build_groups < 0 || p->build_groups > 64) return LJG_ERR_ARG;
   if (p->list_mode < 0 || p->list_mode > 2) return LJG_ERR_ARG;
+  if (p->shared_rebuilds < 0 || p->shared_rebuilds > 1) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;
@@ -315,9 +335,12 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release();
   c.box_flag.release(); c.list_tmp.release(); c.seg_buf.release();
+  c.pos32_alt.release(); c.pos32n_alt.release(); c.bbox_alt.release(); c.c0_alt.release(); c.lcount_alt.release();
+  c.ctower_alt.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.pin_seg) (void)hipHostFree(c.pin_seg);
+  if (c.pin_dl) (void)hipHostFree(c.pin_dl);
   if (c.pin_flags) (void)hipHostFree(c.pin_flags);
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -454,19 +477,19 @@ static const char *check_boxes(const Ctx &c, int32_t nboxes, const double *box_m
 static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const double *box_max,
                            const int64_t *count, int64_t total) {
   std::vector<double> lo(box_min, box_min + 3 * (size_t)nboxes), hi(box_max, box_max + 3 * (size_t)nboxes);
-  std::vector<int64_t> cnt(count, count + nboxes);
-  std::vector<int32_t> box_of((size_t)total);
+  std::vector<int64_t> cnt(count, count + nboxes), pstart((size_t)nboxes + 1, 0);
   std::vector<WrapArgs> wr((size_t)nboxes);
-  for (int32_t b = 0, o = 0; b < nboxes; ++b) {
-    for (int64_t k = 0; k < cnt[b]; ++k) box_of[o++] = b;
+  for (int32_t b = 0; b < nboxes; ++b) {
+    pstart[b + 1] = pstart[b] + cnt[b];
     wr[b] = wrap_args(&lo[3 * b], &hi[3 * b]);
   }
-  c.box_of.ensure(box_of.size());
+  c.box_of.ensure((size_t)total);
   c.box_wrap.ensure(wr.size());
-  LJG_HIP(hipMemcpyAsync(c.box_of.p, box_of.data(), box_of.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                         c.stream));
+  c.box_pbase.ensure(pstart.size());
   LJG_HIP(hipMemcpyAsync(c.box_wrap.p, wr.data(), wr.size() * sizeof(WrapArgs), hipMemcpyHostToDevice, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
+  LJG_HIP(hipMemcpyAsync(c.box_pbase.p, pstart.data(), pstart.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                         c.stream));
+  fill_owner(c, c.box_of.p, total, c.box_pbase.p, nboxes);  // box of each particle, upload order
   c.box_lo = std::move(lo);
   c.box_hi = std::move(hi);
   c.box_count = std::move(cnt);
@@ -482,8 +505,8 @@ static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const 
   c.build_partial = false;
   c.box_since.assign((size_t)nboxes, 0);
   c.box_flag.ensure((size_t)nboxes);
-  LJG_HIP(hipMemsetAsync(c.box_flag.p, 0, (size_t)nboxes * sizeof(int32_t), c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
+  fill_i32(c, c.box_flag.p, nboxes, 0);
+  LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
 }
 
 ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, const double *box_max,
@@ -580,19 +603,25 @@ ljg_status ljg_download_range(ljg_ctx *h, int64_t first, int64_t count, double *
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
   if (first + count > c.n) return fail(c, LJG_ERR_ARG, "range beyond the uploaded particles");
   if (count == 0) return LJG_OK;
+  // one pass gathers the nine arrays of the range, one copy brings them back through pinned memory
   State &s = c.st[c.cur];
   double *outs[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
-  const double *ins[9] = {s.x.p, s.y.p, s.z.p, s.vx.p, s.vy.p, s.vz.p, s.fx.p, s.fy.p, s.fz.p};
-  c.part_v.ensure((size_t)c.n);
-  for (int k = 0; k < 9; ++k) {
-    if (!outs[k]) continue;
-    hipLaunchKernelGGL(k_to_upload_order, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.orig.p,
-                       ins[k], c.part_v.p);
-    LJG_HIP(hipGetLastError());
-    LJG_HIP(hipMemcpyAsync(outs[k], c.part_v.p + first, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
-                           c.stream));
-    LJG_HIP(hipStreamSynchronize(c.stream));
+  const size_t bytes = 9 * (size_t)count * sizeof(double);
+  c.part_v.ensure(9 * (size_t)count);
+  hipLaunchKernelGGL(k_gather_range, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.orig.p,
+                     first, count, view(s), c.part_v.p);
+  LJG_HIP(hipGetLastError());
+  if (c.pin_dl_cap < bytes) {
+    if (c.pin_dl) LJG_HIP(hipHostFree(c.pin_dl));
+    c.pin_dl = nullptr;
+    c.pin_dl_cap = 0;
+    LJG_HIP(hipHostMalloc((void **)&c.pin_dl, bytes + bytes / 2, hipHostMallocDefault));
+    c.pin_dl_cap = bytes + bytes / 2;
   }
+  LJG_HIP(hipMemcpyAsync(c.pin_dl, c.part_v.p, bytes, hipMemcpyDeviceToHost, c.stream));
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  for (int k = 0; k < 9; ++k)
+    if (outs[k]) std::memcpy(outs[k], c.pin_dl + (size_t)k * count, (size_t)count * sizeof(double));
   return LJG_OK;
   LJG_API_END
 }
@@ -886,7 +915,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   // the step at which its steps since its last rebuild reach the frequency, or whose drift moved one of
   // its particles skin/2 (it raises the box's flag with the shared one).  A raised flag stops every box
   // after that drift; only the boxes due then are rebuilt, in place, and all go on from that step.
-  const bool per_box = c.nboxes > 0 && (int64_t)c.box_since.size() == c.nboxes;
+  const bool per_box = c.nboxes > 0 && (int64_t)c.box_since.size() == c.nboxes && c.prm.shared_rebuilds == 0;
   std::vector<int32_t> mark;
   auto steps_to_due = [&]() {
     int64_t since = c.steps_since_rebuild;

@@ -393,11 +393,12 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   const int64_t ncl = c.box_clusters, ns = ncl * M;
   const int other = 1 - c.cur;
   c.st[other].ensure((size_t)ns);
-  DevBuf<float4> p32, p32n;
-  DevBuf<double> bb;
-  DevBuf<double4> cc;
-  DevBuf<int32_t> cnt, ctw;
-  DevBuf<int4> rows;
+  // the moved layout goes to the second buffers, which are then swapped in (no allocation once they fit)
+  DevBuf<float4> &p32 = c.pos32_alt, &p32n = c.pos32n_alt;
+  DevBuf<double> &bb = c.bbox_alt;
+  DevBuf<double4> &cc = c.c0_alt;
+  DevBuf<int32_t> &cnt = c.lcount_alt, &ctw = c.ctower_alt;
+  DevBuf<int4> &rows = c.list_tmp;
   const int32_t cap = c.list_row;
   p32.ensure((size_t)ns);
   p32n.ensure((size_t)ns);
@@ -438,7 +439,7 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
                        nb, M, d_pbase, c.box_cbase.p, view(c.st[other]), c.part_v.p, (int32_t)have_v, d_type, d_id);
     LJG_HIP(hipGetLastError());
   }
-  LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope, the old buffers are freed
+  LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
   std::swap(c.pos32, p32);
   std::swap(c.pos32_next, p32n);
   std::swap(c.bbox, bb);
@@ -446,16 +447,28 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   std::swap(c.list_count, cnt);
   std::swap(c.cluster_tower, ctw);
   std::swap(c.list, rows);
-  for (auto *b : {&p32, &p32n}) b->release();
-  bb.release();
-  cc.release();
-  cnt.release();
-  ctw.release();
-  rows.release();
   c.cur = other;
   c.nslots = ns;
   c.nclusters = ncl;
   c.regions_valid = true;
+}
+
+__global__ void k_fill_owner(int64_t n, const int64_t *__restrict__ start, int32_t nb, int32_t *__restrict__ owner) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t lo = 0, hi = nb - 1;  // last b with start[b] <= i
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  owner[i] = lo;
+}
+
+void fill_owner(Ctx &c, int32_t *owner, int64_t n, const int64_t *start, int32_t nb) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_owner, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, start, nb, owner);
+  LJG_HIP(hipGetLastError());
 }
 
 void ensemble_geometry(Ctx &c) {
@@ -480,11 +493,6 @@ void ensemble_geometry(Ctx &c) {
   }
   const int64_t ncl = cbase[nb];
   if (ncl > (int64_t)kClusterMask) throw HipError{hipErrorInvalidValue, "too many clusters in the ensemble", -1};
-  std::vector<int32_t> tbox((size_t)ntow), cbox((size_t)ncl);
-  for (int32_t b = 0; b < nb; ++b) {
-    for (int64_t t = tbase[b]; t < tbase[b + 1]; ++t) tbox[t] = b;
-    for (int64_t k = cbase[b]; k < cbase[b + 1]; ++k) cbox[k] = b;
-  }
   c.box_tg.ensure((size_t)nb);
   c.box_tbase.ensure((size_t)nb + 1);
   c.box_cbase.ensure((size_t)nb + 1);
@@ -498,9 +506,8 @@ void ensemble_geometry(Ctx &c) {
                          c.stream));
   LJG_HIP(hipMemcpyAsync(c.box_pbase.p, pbase.data(), pbase.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                          c.stream));
-  LJG_HIP(hipMemcpyAsync(c.tower_box.p, tbox.data(), tbox.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
-  LJG_HIP(hipMemcpyAsync(c.cluster_box.p, cbox.data(), cbox.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                         c.stream));
+  fill_owner(c, c.tower_box.p, ntow, c.box_tbase.p, nb);
+  fill_owner(c, c.cluster_box.p, ncl, c.box_cbase.p, nb);
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
   c.box_ntowers = ntow;
   c.box_clusters = ncl;

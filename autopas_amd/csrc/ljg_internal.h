@@ -355,12 +355,21 @@ struct Ctx {
   int32_t seg_nboxes = 0;
   Segs seg_others{nullptr, nullptr, 0};  // the clusters of the boxes not marked
   int64_t seg_nothers = 0;
+  int64_t seg_max_count = 0;             // longest tower expected among the marked boxes
+  bool partial_tower_sync = false;       // a partial build reads the longest tower back before sorting
   const int32_t *seg_active = nullptr;   // device: the mark per box, uploaded with the segments
   // pinned host staging: the segment upload, and the boxes' displacement flags read back
   char *pin_seg = nullptr;
   size_t pin_seg_cap = 0;
   int32_t *pin_flags = nullptr;
   size_t pin_flags_cap = 0;
+  double *pin_dl = nullptr;           // pinned staging of ljg_download_range
+  size_t pin_dl_cap = 0;
+  // second buffers an edit of the ensemble writes the moved layout into, then swaps in
+  DevBuf<float4> pos32_alt, pos32n_alt;
+  DevBuf<double> bbox_alt;
+  DevBuf<double4> c0_alt;
+  DevBuf<int32_t> lcount_alt, ctower_alt;
   // second stream: a partial rebuild and the force of its boxes run beside the force of the others
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -410,6 +419,8 @@ void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stat
 // wrap alone, and the per-box reduction of cl_glob into box_glob.  `active` (device, per box; null: every
 // box) restricts a build to the boxes it marks.
 void ensemble_geometry(Ctx &c);
+// owner[i] = the b with start[b] <= i < start[b + 1] for i in [0, n) (start: nb + 1 offsets, on the device)
+void fill_owner(Ctx &c, int32_t *owner, int64_t n, const int64_t *start, int32_t nb);
 void ensemble_tower_hist(Ctx &c, const int32_t *active);
 void ensemble_layout(Ctx &c, const int32_t *active);
 // src = -1 and cluster_tower = -1 over the regions of the active boxes (the sorts then fill their towers)
@@ -429,8 +440,9 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
                            const double *vx, const double *vy, const double *vz, const int32_t *type,
                            const int64_t *id);
 // The slots of the active boxes' regions gathered from src (old slots of the same regions) in place,
-// through the other state buffer; the other boxes' slots are not touched.  ljg_md.hip
-void permute_active(Ctx &c, const int32_t *src, const int32_t *active);
+// through the other state buffer; the other boxes' slots are not touched.  Nothing moves when *skip is
+// set on the device (skip may be null).  ljg_md.hip
+void permute_active(Ctx &c, const int32_t *src, const int32_t *active, const unsigned long long *skip = nullptr);
 // Keep the boxes with new_base[b] >= 0 (their particles move to upload indices new_base[b] + local index,
 // with positions, velocities, forces, types and ids), then append `added` new particles from the host
 // after the `kept` ones (zero force).  Leaves the state in upload order in the other buffer, made current.

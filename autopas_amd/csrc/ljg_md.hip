@@ -121,9 +121,10 @@ __global__ void k_permute(int64_t nnew, const int32_t *__restrict__ src, StateVi
 // another thread writes, and the unmarked boxes' slots are never touched.
 __global__ void k_permute_boxes(int64_t nitems, Segs sg, int32_t M, const int32_t *__restrict__ src,
                                 const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
-                                StateView o, StateView n, int32_t carry_force) {
+                                StateView o, StateView n, int32_t carry_force,
+                                const unsigned long long *__restrict__ skip) {
   const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= nitems) return;
+  if (it >= nitems || (skip && *skip)) return;
   const int64_t s = seg_index(sg, it);
   if (box_active[cluster_box[s / M]] == 0) return;
   const int32_t k = src[s];
@@ -147,9 +148,10 @@ __global__ void k_permute_boxes(int64_t nitems, Segs sg, int32_t M, const int32_
 }
 
 __global__ void k_copy_boxes(int64_t nitems, Segs sg, int32_t M, const int32_t *__restrict__ cluster_box,
-                             const int32_t *__restrict__ box_active, StateView from, StateView to) {
+                             const int32_t *__restrict__ box_active, StateView from, StateView to,
+                             const unsigned long long *__restrict__ skip) {
   const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= nitems) return;
+  if (it >= nitems || (skip && *skip)) return;
   const int64_t s = seg_index(sg, it);
   if (box_active[cluster_box[s / M]] == 0) return;
   to.x[s] = from.x[s];   to.y[s] = from.y[s];   to.z[s] = from.z[s];
@@ -160,16 +162,16 @@ __global__ void k_copy_boxes(int64_t nitems, Segs sg, int32_t M, const int32_t *
   to.type[s] = from.type[s]; to.orig[s] = from.orig[s]; to.id[s] = from.id[s];
 }
 
-void permute_active(Ctx &c, const int32_t *src, const int32_t *active) {
+void permute_active(Ctx &c, const int32_t *src, const int32_t *active, const unsigned long long *skip) {
   const int other = 1 - c.cur;
   c.st[other].ensure((size_t)c.nslots);
   const int64_t ni = c.seg_nslots;
   if (ni == 0) return;
   hipLaunchKernelGGL(k_permute_boxes, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, c.seg_slots, c.M, src,
-                     c.cluster_box.p, active, view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force);
+                     c.cluster_box.p, active, view(c.st[c.cur]), view(c.st[other]), (int32_t)c.carry_force, skip);
   LJG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_copy_boxes, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, c.seg_slots, c.M,
-                     c.cluster_box.p, active, view(c.st[other]), view(c.st[c.cur]));
+                     c.cluster_box.p, active, view(c.st[other]), view(c.st[c.cur]), skip);
   LJG_HIP(hipGetLastError());
 }
 

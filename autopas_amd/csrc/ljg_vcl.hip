@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
     int64_t ntowers, Segs sg, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
     const int32_t *__restrict__ items, const double *__restrict__ items_z, const int64_t *__restrict__ items_id,
     const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
-    int32_t *__restrict__ cluster_tower) {
+    int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ over) {
   constexpr int P = kWave * E;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t w = (int64_t)blockIdx.x * (kSortWaveThreads / kWave) + threadIdx.x / kWave;
@@ -412,6 +412,8 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
   const int n = count[t];
   constexpr int LO = E == 1 ? 0 : E == 4 ? kWave : 4 * kWave;  // the next shorter instance's reach
   static_assert(E == 1 || E == 4 || E == 8, "instances cover up to 64, 65-256 and 257-512 particles");
+  // (over: the longest instance launched reports a tower beyond its reach, which no launched sort takes)
+  if (n > P && over && lane == 0) over[0] = 1ull;
   if (n > P || n <= LO) return;  // another instance sorts this tower
   int32_t sv[E];
   double zv[E];
@@ -929,9 +931,13 @@ void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
 }
 
 // Average clusters per tower below which the cluster-entry lists use a lane per i-cluster, and average
-// particles per tower up to which the towers of at most 512 particles are sorted in registers.
+// particles per tower up to which the towers of at most 512 particles are sorted in registers.  A partial
+// ensemble build of fewer clusters than kPartialTowerListsBelow uses the wavefront per tower: with few
+// boxes marked the pass is bound by the latency of dependent loads (~850 per lane in the lane kernel,
+// ~20 staged rounds per wavefront in the tower kernel), not by idle lanes.
 constexpr int kLaneListsBelow = 24;
 constexpr int kWaveSortBelow = 448;
+constexpr int64_t kPartialTowerListsBelow = 1 << 16;
 
 void vcl_build(Ctx &c) {
   const int M = c.M;
@@ -1010,11 +1016,19 @@ void vcl_build(Ctx &c) {
                        c.tower_id.p);
     LJG_HIP(hipGetLastError());
   }
-  // the sizes: the longest tower picks the sort instances (only those are launched: an instance with a
-  // large LDS tile or many registers waits for room on a GPU busy with other work)
+  // The sizes: the longest tower picks the sort instances (only those are launched: an instance with a
+  // large LDS tile or many registers waits for room on a GPU busy with other work).  A partial build uses
+  // the expected longest tower of the marked boxes instead of waiting for the readback, and launches the
+  // register sorts up to it: a longer tower (rare) is reported by the longest instance launched, the
+  // permutation then skips, and the build runs again with the readback.
   unsigned long long ts[3] = {0, 0, 0};
-  LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
+  const bool guess = partial && !c.partial_tower_sync;
+  if (guess) {
+    ts[1] = (unsigned long long)(c.seg_max_count < 8 * kWave ? c.seg_max_count : 8 * kWave);
+  } else {
+    LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
+    LJG_HIP(hipStreamSynchronize(c.stream));
+  }
   const int64_t ncl = ens ? c.box_clusters : (int64_t)ts[2];
   if (ts[1] > (unsigned long long)kTowerSortCap)
     throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
@@ -1034,17 +1048,20 @@ void vcl_build(Ctx &c) {
   const int32_t min_n = wave_sort ? 8 * kWave : 0;
   if (wave_sort) {
     constexpr int per_block = kSortWaveThreads / kWave;
-#define LJG_WAVE_SORT(E)                                                                                        \
+#define LJG_WAVE_SORT(E, OVER)                                                                                  \
   if (n_towers > 0) {                                                                                           \
     hipLaunchKernelGGL(k_tower_sort_wave<E>, dim3(blocks_for(n_towers, per_block)), dim3(kSortWaveThreads), 0,  \
                        c.stream, n_towers, sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p,        \
-                       c.tower_z.p, c.tower_id.p, c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p); \
+                       c.tower_z.p, c.tower_id.p, c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, \
+                       OVER);                                                                                   \
     LJG_HIP(hipGetLastError());                                                                                 \
   }
     // every tower is sorted by exactly one instance; the longest tower (ts[1]) bounds which ones can have work
-    LJG_WAVE_SORT(1)
-    if (ts[1] > (unsigned long long)kWave) LJG_WAVE_SORT(4)
-    if (ts[1] > (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(8)
+    unsigned long long *over = guess ? c.list_stats.p + 5 : nullptr;
+    const int top = ts[1] > (unsigned long long)(4 * kWave) ? 8 : ts[1] > (unsigned long long)kWave ? 4 : 1;
+    LJG_WAVE_SORT(1, top == 1 ? over : nullptr)
+    if (top >= 4) LJG_WAVE_SORT(4, top == 4 ? over : nullptr)
+    if (top == 8) LJG_WAVE_SORT(8, over)
 #undef LJG_WAVE_SORT
   }
   if (ts[1] > (unsigned long long)min_n && n_towers > 0) {
@@ -1066,7 +1083,7 @@ void vcl_build(Ctx &c) {
     verlet_geometry(c);
     c.posfx.ensure(new_slots);
   }
-  if (partial) permute_active(c, c.perm.p, active);
+  if (partial) permute_active(c, c.perm.p, active, guess ? c.list_stats.p + 5 : nullptr);
   else permute_state(c, c.perm.p, new_slots, jrows ? c.posfx.p : nullptr);
   c.regions_valid = ens;
   c.M = M;
@@ -1087,7 +1104,9 @@ void vcl_build(Ctx &c) {
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
   // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
   // cluster-entry lists: a lane per i-cluster when the towers hold few clusters, unless the caller chose
-  const bool lane_lists = c.prm.list_mode == 2 || (c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow);
+  const bool lane_lists =
+      c.prm.list_mode == 2 || (c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow &&
+                               !(partial && c.seg_nclusters < kPartialTowerListsBelow));
   // (a partial build starts from the stride the other boxes' rows are stored with)
   int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24)
                       : partial ? c.list_row : (c.list_cap > 0 ? c.list_cap : 64);
@@ -1154,6 +1173,19 @@ void vcl_build(Ctx &c) {
     unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
     LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
     LJG_HIP(hipStreamSynchronize(c.stream));
+    if (st[5] && guess) {
+      // a marked tower was longer than the register sorts take: nothing was permuted, so build again
+      // with the longest tower read back first
+      c.partial_tower_sync = true;
+      try {
+        vcl_build(c);
+      } catch (...) {
+        c.partial_tower_sync = false;
+        throw;
+      }
+      c.partial_tower_sync = false;
+      return;
+    }
     if (st[5]) throw HipError{hipErrorInvalidValue, "tower exceeds the 4096-particle sort tile", -1};
     const int32_t maxc = (int32_t)st[1], maxj = (int32_t)st[4];
     const int32_t split = jrows ? kJclSplit : 1;  // first-pass rows hold `split` sub-rows of cap/split

@@ -56,7 +56,8 @@ class Params(ctypes.Structure):
         ("tuning_samples", _i32),
         ("tuning_interval", _i32),
         ("list_mode", _i32),
-        ("reserved", _i32 * 2),
+        ("shared_rebuilds", _i32),
+        ("reserved", _i32),
     ]
 
 
@@ -229,7 +230,7 @@ class Engine:
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
-                 tuning_interval=0, list_mode=0):
+                 tuning_interval=0, list_mode=0, shared_rebuilds=False):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -246,6 +247,7 @@ class Engine:
         p.tuning_samples = tuning_samples
         p.tuning_interval = tuning_interval
         p.list_mode = list_mode
+        p.shared_rebuilds = int(bool(shared_rebuilds))
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))

@@ -178,17 +178,22 @@ class BatchedServer:
     jobs' state stays resident on the device (``Engine.edit_boxes``).  An epoch ends early when a running job
     reaches its step count, so every job runs exactly its own number of steps.
 
-    Each job's trajectory is the one it has alone up to the rounding of the FP32 pair term: the lists of a
-    running job are rebuilt when the batch changes, so its pair sums may be taken in another order."""
+    Each job keeps the rebuild schedule it has alone (its own frequency and displacement trigger, its
+    layout and lists kept through every change of the batch), so its trajectory is bit-identical to the job
+    run alone.  With ``shared_rebuilds`` the batch has one schedule (a trigger in any job rebuilds all), which
+    takes fewer, larger rebuilds; a job's trajectory then equals its run alone up to the rounding of the
+    FP32 pair sums, which its rebuilds at other steps take in another order."""
 
     def __init__(self, particle_budget=None, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=8, tile_groups=4,
-                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None):
+                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None, shared_rebuilds=False):
         self.kernel = kernel
         self.epoch = epoch
         self.max_jobs = max_jobs  # jobs in one batch at most (None: only the particle budget limits)
         self.keep_results = keep_results
+        self.shared_rebuilds = shared_rebuilds
         self.engine = engine.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), RC, SKIN, cluster_size=cluster_size,
-                                    rebuild_frequency=10, dynamic_rebuild=True, tile_groups=tile_groups)
+                                    rebuild_frequency=10, dynamic_rebuild=True, tile_groups=tile_groups,
+                                    shared_rebuilds=shared_rebuilds)
         if particle_budget is None:
             free, _ = self.engine.memory_info()
             particle_budget = int(0.8 * free / bytes_per_particle)
@@ -264,20 +269,24 @@ class BatchedServer:
         return max(j.finish for j in jobs) if jobs else 0.0
 
 
-def serve_open_loop_batched(count, load=0.8, seed=5, calibration_jobs=None, **server_kw):
+def serve_open_loop_batched(count, load=0.8, seed=5, calibration_jobs=None, rate=None, **server_kw):
     """The serving measurement with continuous batching: calibrate the batched server's throughput on a
-    closed-loop batch, then serve ``count`` jobs arriving as a Poisson process at ``load`` times it."""
+    closed-loop batch, then serve ``count`` jobs arriving as a Poisson process at ``load`` times it, or at
+    ``rate`` jobs/s when given (the same arrival stream another server was measured on)."""
     with BatchedServer(**server_kw) as srv:
         calib = make_jobs(calibration_jobs or max(8, count // 4), None, seed=seed + 1)
         wall = srv.run(calib)
         capacity = len(calib) / wall
-        jobs = make_jobs(count, load * capacity, seed=seed)
+        arrival_rate = rate if rate is not None else load * capacity
+        jobs = make_jobs(count, arrival_rate, seed=seed)
         srv.pairs = srv.epochs = srv.edits = 0
         wall = srv.run(jobs)
         out = latency_summary(jobs, wall)
-        out.update(pairs=int(srv.pairs), pairs_per_s=srv.pairs / wall if wall else 0.0, load=load,
-                   capacity_jobs_per_s=capacity, kernel=engine.KERNEL_NAMES[srv.kernel], epochs=srv.epochs,
-                   batch_edits=srv.edits, particle_budget=srv.particle_budget, epoch_steps=srv.epoch)
+        out.update(pairs=int(srv.pairs), pairs_per_s=srv.pairs / wall if wall else 0.0,
+                   load=arrival_rate / capacity, arrival_rate_jobs_per_s=arrival_rate, capacity_jobs_per_s=capacity,
+                   kernel=engine.KERNEL_NAMES[srv.kernel], epochs=srv.epochs, batch_edits=srv.edits,
+                   particle_budget=srv.particle_budget, epoch_steps=srv.epoch,
+                   schedule="shared" if srv.shared_rebuilds else "per job")
     return out, jobs
 
 

@@ -67,7 +67,11 @@ typedef struct {
   int32_t list_mode;      /* cluster-entry list pass: 0 = by grid (a lane per i-cluster when towers hold few
                            * clusters, else a wavefront per tower), 1 = wavefront per tower, 2 = lane per
                            * i-cluster; all give the same lists */
-  int32_t reserved[2];
+  int32_t shared_rebuilds; /* ensemble: 0 = every box keeps the rebuild schedule it has alone (frequency and
+                            * displacement trigger per box, results bit-identical to the box alone); 1 = one
+                            * schedule for all boxes (a trigger in any box rebuilds every box; results equal to
+                            * the box alone up to the FP32 rounding of the pair sums), fewer and larger rebuilds */
+  int32_t reserved;
 } ljg_params;
 
 typedef struct {

@@ -7,7 +7,10 @@ Tolerances (stated per test):
     potential energy and virial 1e-5 relative, pair count within 1e-4 relative;
   * with dynamic rebuild on, every box keeps its own rebuild steps: bit-identical to the box alone;
   * boxes kept through an edit of the ensemble (their layouts, lists and schedules move with them), boxes
-    added by an edit, and jobs served in a batch, against the same box or job alone: bit-identical.
+    added by an edit, and jobs served in a batch, against the same box or job alone: bit-identical;
+  * jobs served in a batch with one shared rebuild schedule, against the job alone: positions within 1e-6
+    (minimum image), velocities within 1e-5 RMS relative and 1e-3 absolute (their lists are rebuilt at
+    other steps, so the FP32 pair sums run in another order).
 """
 import numpy as np
 import pytest
@@ -222,6 +225,41 @@ def test_edit_boxes_keeps_running_boxes():
             assert np.array_equal(out[k][sl[3 + j]], alone[k]), (j, k)
 
 
+def tall_box(seed=21):
+    # a 5 x 5 x 180 jittered lattice in a 5.5 x 5.5 x 198 box: its 2 x 2 towers hold ~1100 particles, longer
+    # than the register sorts of a partial build take
+    rng = np.random.default_rng(seed)
+    a = 1.1
+    g = np.stack(np.meshgrid(np.arange(5), np.arange(5), np.arange(180), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + 0.5) * a + rng.uniform(-0.05, 0.05, g.shape)
+    vel = rng.normal(0.0, 0.9, g.shape)
+    vel -= vel.mean(axis=0)
+    return np.array([5 * a, 5 * a, 180 * a]), pos, vel
+
+
+def test_edit_adds_a_box_with_long_towers():
+    # the added box is built alone (a partial build beside the running boxes); its towers exceed the register
+    # sorts, so the build is repeated with the longest tower read back; the box runs as it does alone
+    boxes = small_boxes()[:3]
+    lo, hi, counts, st = en.ensemble_arrays(boxes)
+    L, pos, vel = tall_box()
+    announce(f"ensemble edit adding a {len(pos)}-particle box with long towers")
+    with en.ensemble_engine(boxes, cluster_size=8, dynamic_rebuild=False, arrays=(lo, hi, counts, st)) as e:
+        e.step(5, en.DT, ap.KERNEL_CLUSTER_MIXED)
+        e.edit_boxes([0, 1, 2], np.zeros((1, 3)), L[None, :], np.array([len(pos)], dtype=np.int64),
+                     pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(), vel[:, 0].copy(), vel[:, 1].copy(),
+                     vel[:, 2].copy())
+        e.step(12, en.DT, ap.KERNEL_CLUSTER_MIXED)
+        first = int(counts.sum())
+        part = e.download_range(first, len(pos))
+    with ap.Engine((0.0, 0.0, 0.0), tuple(L), en.RC, en.SKIN, cluster_size=8, dynamic_rebuild=False) as a:
+        a.upload(pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2])
+        a.step(12, en.DT, ap.KERNEL_CLUSTER_MIXED)
+        alone = a.download()
+    for k in part:
+        assert np.array_equal(part[k], alone[k]), k
+
+
 def test_batched_server_results_match_jobs_alone():
     # continuous batching: jobs arrive over time, are admitted under a particle budget that holds only a
     # few of them at once, and each job's end state is bit-identical to the job run alone in a context of
@@ -240,3 +278,31 @@ def test_batched_server_results_match_jobs_alone():
             _, _, alone = serve.run_job(a, job, ap.KERNEL_CLUSTER_MIXED)
         for k in alone:
             assert np.array_equal(job.result[k], alone[k]), (job.index, k)
+
+
+def min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+def test_batched_server_shared_schedule_within_tolerance():
+    # one rebuild schedule for the whole batch: each job equals its run alone up to the FP32 rounding of
+    # the pair sums (a pair in a hard collision amplifies a 1e-7 force difference into ~1e-4 of a velocity
+    # of ~20, hence the absolute velocity bound)
+    from autopas_amd import serve
+
+    jobs = serve.make_jobs(10, rate=400.0, seed=8, n_range=(256, 3000), steps_range=(10, 60))
+    announce(f"batched serving, shared schedule: {len(jobs)} jobs, budget 6000 particles")
+    with serve.BatchedServer(particle_budget=6000, keep_results=True, epoch=10, shared_rebuilds=True) as srv:
+        srv.run(jobs)
+        assert srv.edits >= 3
+    for job in jobs:
+        with ap.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), serve.RC, serve.SKIN, cluster_size=8, tile_groups=4,
+                       rebuild_frequency=10, dynamic_rebuild=True) as a:
+            _, _, alone = serve.run_job(a, job, ap.KERNEL_CLUSTER_MIXED)
+        a_, b_ = job.result, alone
+        for k in ("x", "y", "z"):
+            assert np.max(np.abs(min_image(a_[k] - b_[k], job.box_length))) <= 1e-6, (job.index, k)
+        dv = np.concatenate([a_[k] - b_[k] for k in ("vx", "vy", "vz")])
+        v = np.concatenate([b_[k] for k in ("vx", "vy", "vz")])
+        assert np.max(np.abs(dv)) <= 1e-3, job.index
+        assert np.sqrt(np.sum(dv ** 2) / np.sum(v ** 2)) <= 1e-5, job.index
