@@ -285,7 +285,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->rebuild_frequency < 1) return LJG_ERR_ARG;
   if (p->tile_groups < 0 || p->tile_groups > 64) return LJG_ERR_ARG;
   if (p->build_groups < 0 || p->build_groups > 64) return LJG_ERR_ARG;
-  if (p->list_mode < 0 || p->list_mode > 2) return LJG_ERR_ARG;
+  if (p->list_mode < 0 || p->list_mode > 3) return LJG_ERR_ARG;
   if (p->shared_rebuilds < 0 || p->shared_rebuilds > 1) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;

@@ -664,11 +664,68 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
   }
 }
 
+// One neighbour-tower visit v of an i-cluster's traversal (k_cluster_lists_tower's order: oy, ox, then sz,
+// then the j-tower's clusters from the first whose z-range can reach the i-cluster), with the same tests:
+// the number of entries it finds, which WRITE stores from row position pos on (below cap).
+template <bool WRITE>
+__device__ inline int32_t visit_entries(int v, int side, const TowerGeom &g, int64_t tb, int tx, int ty,
+                                        const double *bi, const double4 &ai, const int64_t *__restrict__ cstart,
+                                        const int32_t *__restrict__ ccount, const double *__restrict__ bbox,
+                                        const double4 *__restrict__ c0, int32_t cap, int32_t pos,
+                                        int4 *__restrict__ row) {
+  const double il2 = g.ilen * g.ilen;
+  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
+  const int oy = v / side - g.R, ox = v % side - g.R;
+  const int aoy = oy < 0 ? -oy : oy;
+  const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+  const int aox = ox < 0 ? -ox : ox;
+  const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+  if (dtx * dtx + dty * dty > il2) return 0;
+  int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+  while (jx < 0) { jx += g.nt[0]; --sx; }
+  while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+  while (jy < 0) { jy += g.nt[1]; --sy; }
+  while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+  const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
+  const int64_t cb = cstart[tj], ce = cb + ccount[tj];
+  int32_t n = 0;
+  for (int sz = -1; sz <= 1; ++sz) {
+    const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
+    const double zmin = bi[2] - zwin - shz;
+    const double zmax = bi[5] + zwin - shz;
+    // every cluster's z-range lies in [box_min, box_min + L): a window outside it finds nothing
+    if (zmax < g.box_min[2] || zmin > g.box_min[2] + g.L[2]) continue;
+    int64_t lo = cb, hi = ce;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (bbox[6 * mid + 5] < zmin) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int64_t k = lo; k < ce; ++k) {
+      const double *bj = bbox + 6 * k;
+      if (bj[2] > zmax) break;
+      if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+        if (WRITE && pos + n < cap) {
+          const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+          const double4 b = c0[k];
+          const float dx = (float)((b.x + shx) - ai.x);
+          const float dy = (float)((b.y + shy) - ai.y);
+          const float dz = (float)((b.z + shz) - ai.z);
+          row[pos + n] = make_int4((int32_t)k | (sid << kShiftBits), __float_as_int(dx), __float_as_int(dy),
+                                   __float_as_int(dz));
+        }
+        ++n;
+      }
+    }
+  }
+  return n;
+}
+
 // Cluster-pair lists with one lane per i-cluster, for grids whose towers hold few clusters (small boxes,
 // ensembles of them), where the wavefront per tower of k_cluster_lists_tower leaves most lanes idle.
-// Each lane walks the traversal order of k_cluster_lists_tower (oy, ox, sz, then the j-tower's clusters
-// from the first whose z-range can reach the i-cluster) with the same tests, so the rows are identical
-// entry by entry; the j-tower's boxes are read from memory (lanes of one tower read the same lines).
+// Each lane walks the traversal order of k_cluster_lists_tower visit by visit with the same tests, so the
+// rows are identical entry by entry; the j-tower's boxes are read from memory (lanes of one tower read the
+// same lines).
 constexpr int kListLaneThreads = 256;
 
 template <bool MB>
@@ -685,11 +742,13 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
     if (box_active && box_active[cluster_box[ci]] == 0) return;
   }
   const double4 ai = c0[ci];
-  if (ai.w == 0.) {  // an empty cluster of a region's tail (no tower)
+  const int64_t t = cluster_tower[ci];
+  // an empty cluster of a region's tail, or one whose tower a partial build's register sorts left to its
+  // repeat (no tower until then): an empty row
+  if (ai.w == 0. || t < 0) {
     count[ci] = 0;
     return;
   }
-  const int64_t t = cluster_tower[ci];
   TowerGeom g = g1;
   int64_t tb = 0;
   if constexpr (MB) {
@@ -699,59 +758,89 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
   }
   const int64_t tl = t - tb;
   const int tx = (int)(tl % g.nt[0]), ty = (int)(tl / g.nt[0]);
-  const double il2 = g.ilen * g.ilen;
-  const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
   double bi[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
   const int side = 2 * g.R + 1, nvis = side * side;
-  const int64_t w0 = ci * (int64_t)cap;
+  int4 *row = list + ci * (int64_t)cap;
   int32_t cnt = 0;
-  for (int v = 0; v < nvis; ++v) {
-    const int oy = v / side - g.R, ox = v % side - g.R;
-    const int aoy = oy < 0 ? -oy : oy;
-    const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
-    const int aox = ox < 0 ? -ox : ox;
-    const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
-    if (dtx * dtx + dty * dty > il2) continue;
-    int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
-    while (jx < 0) { jx += g.nt[0]; --sx; }
-    while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
-    while (jy < 0) { jy += g.nt[1]; --sy; }
-    while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
-    const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
-    const int64_t cb = cstart[tj], ce = cb + ccount[tj];
-    for (int sz = -1; sz <= 1; ++sz) {
-      const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
-      const double zmin = bi[2] - zwin - shz;
-      const double zmax = bi[5] + zwin - shz;
-      // every cluster's z-range lies in [box_min, box_min + L): a window outside it finds nothing
-      if (zmax < g.box_min[2] || zmin > g.box_min[2] + g.L[2]) continue;
-      int64_t lo = cb, hi = ce;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (bbox[6 * mid + 5] < zmin) lo = mid + 1;
-        else hi = mid;
-      }
-      for (int64_t k = lo; k < ce; ++k) {
-        const double *bj = bbox + 6 * k;
-        if (bj[2] > zmax) break;
-        if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
-          if (cnt < cap) {
-            const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-            const double4 b = c0[k];
-            const float dx = (float)((b.x + shx) - ai.x);
-            const float dy = (float)((b.y + shy) - ai.y);
-            const float dz = (float)((b.z + shz) - ai.z);
-            list[w0 + cnt] = make_int4((int32_t)k | (sid << kShiftBits), __float_as_int(dx), __float_as_int(dy),
-                                       __float_as_int(dz));
-          }
-          ++cnt;
-        }
-      }
-    }
-  }
+  for (int v = 0; v < nvis; ++v)
+    cnt += visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, cnt, row);
   count[ci] = cnt;
+}
+
+// The same rows with a group of kSplitLanes lanes per i-cluster: lane l of the group takes the visits
+// l, l + kSplitLanes, ...; each lane counts its visit's entries, an inclusive scan over the group gives
+// every visit its place in the row, and each lane then writes its visit's entries there.  The rows are
+// those of the lane kernel entry by entry, while the dependent loads of one i-cluster's traversal (the
+// bound of the lane kernel: ~850 per lane on the ensemble) are split over the group.
+constexpr int kSplitLanes = 32;
+constexpr int kListSplitThreads = 256;
+
+template <bool MB>
+__global__ __launch_bounds__(kListSplitThreads) void k_cluster_lists_split(
+    int64_t nclusters, Segs sg, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
+    const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
+    const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount,
+    const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
+    int4 *__restrict__ list) {
+  const int sl = threadIdx.x & (kSplitLanes - 1);
+  const int64_t it = ((int64_t)blockIdx.x * kListSplitThreads + threadIdx.x) / kSplitLanes;
+  // every lane of a group holds the same i-cluster and the same decisions; no lane leaves before the scans
+  const int64_t ci = it < nclusters ? seg_index(sg, it) : -1;
+  bool act = ci >= 0;
+  if constexpr (MB) act = act && (box_active == nullptr || box_active[cluster_box[ci]] != 0);
+  double4 ai = make_double4(0., 0., 0., 0.);
+  int64_t t = -1;
+  if (act) {
+    ai = c0[ci];
+    t = cluster_tower[ci];
+  }
+  // an empty cluster of a region's tail, or one whose tower a partial build's register sorts left to its
+  // repeat (no tower until then), gets an empty row
+  const bool work = act && ai.w != 0. && t >= 0;
+  TowerGeom g = g1;
+  int64_t tb = 0;
+  int tx = 0, ty = 0, side = 1, nvis = 0;
+  double bi[6] = {0., 0., 0., 0., 0., 0.};
+  if (work) {
+    if constexpr (MB) {
+      const int32_t b = tbox[t];
+      g = tgs[b];
+      tb = tbase[b];
+    }
+    const int64_t tl = t - tb;
+    tx = (int)(tl % g.nt[0]);
+    ty = (int)(tl / g.nt[0]);
+    side = 2 * g.R + 1;
+    nvis = side * side;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bi[k] = bbox[6 * ci + k];
+  }
+  // rounds of kSplitLanes visits; the count is made uniform over the wavefront for the scans
+  int rounds = (nvis + kSplitLanes - 1) / kSplitLanes;
+  for (int off = kSplitLanes; off < kWave; off <<= 1) {
+    const int o = __shfl_xor(rounds, off, kWave);
+    rounds = o > rounds ? o : rounds;
+  }
+  int4 *row = work ? list + ci * (int64_t)cap : nullptr;
+  int32_t base = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const int v = r * kSplitLanes + sl;
+    const bool mine = work && v < nvis;
+    const int32_t n =
+        mine ? visit_entries<false>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, 0, row) : 0;
+    int32_t inc = n;  // inclusive scan over the group
+    for (int off = 1; off < kSplitLanes; off <<= 1) {
+      const int32_t y = __shfl_up(inc, off, kSplitLanes);
+      if (sl >= off) inc += y;
+    }
+    const int32_t total = __shfl(inc, kSplitLanes - 1, kSplitLanes);
+    if (mine && n > 0)
+      (void)visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, base + inc - n, row);
+    base += total;
+  }
+  if (act && sl == 0) count[ci] = base;
 }
 
 // Second pass of the j-particle row build: one wavefront per i-cluster streams the members of its listed
@@ -930,14 +1019,16 @@ void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v) {
   LJG_HIP(hipGetLastError());
 }
 
-// Average clusters per tower below which the cluster-entry lists use a lane per i-cluster, and average
-// particles per tower up to which the towers of at most 512 particles are sorted in registers.  A partial
-// ensemble build of fewer clusters than kPartialTowerListsBelow uses the wavefront per tower: with few
-// boxes marked the pass is bound by the latency of dependent loads (~850 per lane in the lane kernel,
-// ~20 staged rounds per wavefront in the tower kernel), not by idle lanes.
+// Average clusters per tower below which the cluster-entry lists use a lane (or a group of lanes) per
+// i-cluster, and average particles per tower up to which the towers of at most 512 particles are sorted in
+// registers.
 constexpr int kLaneListsBelow = 24;
 constexpr int kWaveSortBelow = 448;
-constexpr int64_t kPartialTowerListsBelow = 1 << 16;
+// i-clusters below which those rows are made by a group of lanes per i-cluster: such a build (a few boxes
+// of an ensemble, a small box) is bound by the latency of its dependent loads, ~850 per lane in the lane
+// kernel, ~60 per lane in the group form; above it, by a lane each (the group form scans every visit
+// twice, and a build this large fills the GPU with lanes anyway)
+constexpr int64_t kSplitListsBelow = 1 << 18;
 
 void vcl_build(Ctx &c) {
   const int M = c.M;
@@ -1104,9 +1195,12 @@ void vcl_build(Ctx &c) {
   // Both passes are queued back to back and checked with one synchronisation; a row that overflowed
   // its capacity (the expansion pass reads at most ccap entries) makes both run again with larger rows.
   // cluster-entry lists: a lane per i-cluster when the towers hold few clusters, unless the caller chose
-  const bool lane_lists =
-      c.prm.list_mode == 2 || (c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow &&
-                               !(partial && c.seg_nclusters < kPartialTowerListsBelow));
+  // (list_mode 3: the lane rows with a group of lanes per i-cluster; the automatic choice takes it for the
+  // lane rows of builds of fewer than kSplitListsBelow i-clusters)
+  const bool auto_lane = c.prm.list_mode == 0 && ncl < (int64_t)kLaneListsBelow * ntow;
+  const bool split_lists =
+      c.prm.list_mode == 3 || (auto_lane && (partial ? c.seg_nclusters : ncl) < kSplitListsBelow);
+  const bool lane_lists = c.prm.list_mode == 2 || auto_lane || split_lists;
   // (a partial build starts from the stride the other boxes' rows are stored with)
   int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24)
                       : partial ? c.list_row : (c.list_cap > 0 ? c.list_cap : 64);
@@ -1131,6 +1225,16 @@ void vcl_build(Ctx &c) {
                          dim3(kListThreads * kJclSplit), 0, c.stream, ntow, none, g, no_tg, no_i64, no_i32, no_i32,
                          c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
                          c.jcl.p);
+    else if (split_lists && ens)
+      hipLaunchKernelGGL((k_cluster_lists_split<true>), dim3(blocks_for(n_lane * kSplitLanes, kListSplitThreads)),
+                         dim3(kListSplitThreads), 0, c.stream, n_lane, sg_clusters, g, c.box_tg.p, c.box_tbase.p,
+                         c.tower_box.p, c.cluster_box.p, active, c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p,
+                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
+    else if (split_lists)
+      hipLaunchKernelGGL((k_cluster_lists_split<false>), dim3(blocks_for(ncl * kSplitLanes, kListSplitThreads)),
+                         dim3(kListSplitThreads), 0, c.stream, ncl, none, g, no_tg, no_i64, no_i32, no_i32, no_i32,
+                         c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
+                         c.list.p);
     else if (lane_lists && ens)
       hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(n_lane, kListLaneThreads)),
                          dim3(kListLaneThreads), 0, c.stream, n_lane, sg_clusters, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_box.p, active,

@@ -64,9 +64,9 @@ typedef struct {
                              0 = max(3, rebuild_frequency) */
   int32_t tuning_interval; /* LJG_KERNEL_AUTO: steps between tuning phases (AutoPas tuningInterval); 0 = tune
                               once */
-  int32_t list_mode;      /* cluster-entry list pass: 0 = by grid (a lane per i-cluster when towers hold few
-                           * clusters, else a wavefront per tower), 1 = wavefront per tower, 2 = lane per
-                           * i-cluster; all give the same lists */
+  int32_t list_mode;      /* cluster-entry list pass: 0 = by grid (a group of lanes per i-cluster when towers
+                           * hold few clusters, else a wavefront per tower), 1 = wavefront per tower, 2 = lane
+                           * per i-cluster, 3 = a group of 32 lanes per i-cluster; all give the same lists */
   int32_t shared_rebuilds; /* ensemble: 0 = every box keeps the rebuild schedule it has alone (frequency and
                             * displacement trigger per box, results bit-identical to the box alone); 1 = one
                             * schedule for all boxes (a trigger in any box rebuilds every box; results equal to

@@ -62,10 +62,11 @@ def test_binning_bit_exact(oracle):
     assert np.array_equal(cells, ref)
 
 
-@pytest.mark.parametrize("list_mode", [1, 2])
+@pytest.mark.parametrize("list_mode", [1, 2, 3])
 @pytest.mark.parametrize("M", [1, 2, 4, 8])
 def test_cluster_lists_bit_exact(oracle, M, list_mode):
-    # list_mode 1: a wavefront per tower, 2: a lane per i-cluster (towers of ~20 particles here, so the
+    # list_mode 1: a wavefront per tower, 2: a lane per i-cluster, 3: 32 lanes per i-cluster (one per
+    # neighbour-tower visit; towers of ~20 particles here, so the
     # towers are sorted in registers)
     announce(f"cluster build bit-exact M={M} list_mode={list_mode}: 10^3 lattice rho 0.75")
     s = lattice_box(10, 0.75, seed=7)
@@ -104,7 +105,7 @@ def test_verlet_lists_exact(oracle):
         assert np.array_equal(np.sort(nb[off[p]:off[p + 1]]), rnb[roff[p]:roff[p + 1]]), p
 
 
-@pytest.mark.parametrize("list_mode", [1, 2])
+@pytest.mark.parametrize("list_mode", [1, 2, 3])
 @pytest.mark.parametrize("nz,sorter", [(40, "registers, up to 256"), (100, "registers, up to 512"), (150, "LDS")])
 def test_cluster_lists_bit_exact_long_towers(oracle, nz, sorter, list_mode):
     # 20 x 20 x nz lattice at rho 0.75 with M = 8: 10 x 10 towers of ~4 nz particles, so each tower sorter
