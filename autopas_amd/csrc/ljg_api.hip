@@ -504,6 +504,7 @@ static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const 
   c.regions_valid = false;
   c.build_partial = false;
   c.box_since.assign((size_t)nboxes, 0);
+  c.box_pending.clear();
   c.box_flag.ensure((size_t)nboxes);
   fill_i32(c, c.box_flag.p, nboxes, 0);
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
