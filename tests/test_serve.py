@@ -1,5 +1,6 @@
-"""CPU checks of the serving loop: the job distribution, the systems it generates, and the scheduling and
-latency accounting of the worker pool (with a stand-in for the engine, so no GPU is needed)."""
+"""CPU checks of the serving loop: the job distribution, the systems it generates, the scheduling and
+latency accounting of the worker pool, and the admission, stepping and retirement of the continuous-batching
+server (with stand-ins for the engine, so no GPU is needed)."""
 import math
 import threading
 import time
@@ -88,3 +89,70 @@ def test_server_reports_failures():
     with serve.Server(workers=2, make_engine=_FakeEngine) as srv:
         with pytest.raises(RuntimeError, match="job 2 failed"):
             srv.run(jobs, run=run)
+
+
+class _FakeEnsemble:
+    """Stands in for an ensemble context: tracks the box list through edits (each box tagged by the first x
+    of its particles, which identifies the job), the steps every box has run, and the particles in flight."""
+
+    def __init__(self, *args, **kwargs):
+        self.boxes = []  # [tag, count, steps run]
+        self.peak = 0
+        self.most = 0
+        self.kwargs = kwargs
+
+    def memory_info(self):
+        return 1 << 40, 1 << 41
+
+    def edit_boxes(self, keep, box_min=None, box_max=None, counts=None, x=None, **parts):
+        assert len(set(keep)) == len(keep) and all(0 <= k < len(self.boxes) for k in keep)
+        kept = [self.boxes[k] for k in keep]
+        added = []
+        if counts is not None:
+            ends = np.cumsum(counts)
+            for n, e in zip(counts, ends):
+                added.append([float(x[e - n]), int(n), 0])
+        self.boxes = kept + added
+        self.peak = max(self.peak, sum(b[1] for b in self.boxes))
+        self.most = max(self.most, len(self.boxes))
+
+    def step(self, steps, dt, kernel):
+        assert steps >= 1
+        for b in self.boxes:
+            b[2] += steps
+
+        class R:
+            pair_interactions = 0
+        return R()
+
+    def download_range(self, first, count):
+        offsets = np.cumsum([0] + [b[1] for b in self.boxes])
+        k = int(np.searchsorted(offsets, first))
+        assert offsets[k] == first and self.boxes[k][1] == count
+        return dict(tag=self.boxes[k][0], steps=self.boxes[k][2])
+
+    def close(self):
+        pass
+
+
+@pytest.mark.parametrize("max_jobs", [None, 3])
+def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs):
+    # jobs arrive over ~20 ms; the budget holds a few at once; every job must run exactly its own steps,
+    # start no earlier than it arrived, and be read back from its own place in the batch
+    monkeypatch.setattr(serve.engine, "Engine", _FakeEnsemble)
+    jobs = serve.make_jobs(16, rate=800.0, seed=9, n_range=(256, 4000), steps_range=(3, 40))
+    budget = 6000
+    with serve.BatchedServer(particle_budget=budget, keep_results=True, epoch=10, max_jobs=max_jobs,
+                             shared_rebuilds=True) as srv:
+        assert srv.engine.kwargs["shared_rebuilds"] is True
+        srv.run(jobs)
+        peak, most = srv.engine.peak, srv.engine.most
+        assert srv.edits >= 3 and srv.epochs >= 4
+    assert peak <= max(budget, max(j.n for j in jobs))
+    if max_jobs is not None:
+        assert most <= max_jobs
+    for j in jobs:
+        tag = float(serve.job_system(j)[1][0][0])
+        assert j.result["tag"] == tag, j.index
+        assert j.result["steps"] == j.steps, j.index
+        assert j.start >= j.arrival - 1e-9 and j.finish >= j.start
