@@ -880,13 +880,15 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   c.step_stamp += nsteps + 1;
   auto stamp_of = [&](int32_t step) { return base + step + 1; };
   // the force of step s; with `speculative`, it exits if the flag was raised at or before step s
-  auto force_step = [&](int32_t step, bool speculative) {
+  // (skip_after: with `speculative`, the stamp at or below which a raised flag makes it exit; default its
+  // own step's)
+  auto force_step = [&](int32_t step, bool speculative, int32_t skip_after = 0) {
     const bool time_force = timing && step % kTimingStride == 0;
     if (time_force) {
       fev.emplace_back(step, take_event(c));
       LJG_HIP(hipEventRecord(fev.back().second, c.stream));
     }
-    c.skip = speculative && dyn ? SkipIf{c.flag.p, stamp_of(step)} : SkipIf{nullptr, 0};
+    c.skip = speculative && dyn ? SkipIf{c.flag.p, skip_after ? skip_after : stamp_of(step)} : SkipIf{nullptr, 0};
     if (fused) {
       const bool last = step + 1 == nsteps;
       const IntegArgs ia = integ_args(c, true, !last, dt, last ? 0 : stamp_of(step + 1));
@@ -941,6 +943,9 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     // rebuild at.  Otherwise the host reads the flag now.  (An ensemble reads the flags first: which
     // boxes to rebuild depends on where the steps stopped.)
     const bool rebuild_next = due && !per_box;
+    // after a scheduled rebuild the force of step s + k is queued too, before the host waits: it exits if
+    // the flag was raised at an earlier step, when the host rebuilds at that step instead
+    const bool force_next = rebuild_next && fused;
     if (due) drift_step(s + k, true);
     if (per_box && dyn) {  // the boxes' flags come back with the same synchronisation
       if (c.pin_flags_cap < (size_t)c.nboxes) {
@@ -955,11 +960,31 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     }
     if (rebuild_next) {
       rebuild_now();
+      if (force_next) force_step(s + k, true, stamp_of(s + k) - 1);
       LJG_HIP(hipStreamSynchronize(c.stream));
     } else if (dyn || due) {
       LJG_HIP(hipStreamSynchronize(c.stream));
     }
     const int32_t raised = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
+    if (force_next && (raised == 0 || raised >= stamp_of(s + k))) {
+      // The queued force of step s + k ran.  A flag raised by the drift of step s + k asked for the
+      // rebuild that preceded it: cleared (nothing is queued, so no wait).  One raised by that force's own
+      // drift of step s + k + 1 stays for the next chunk, whose launches then exit until it is handled.
+      if (raised == stamp_of(s + k)) {
+        static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+        LJG_HIP(hipGetLastError());
+      }
+      c.steps_since_rebuild = 1;
+      for (auto &v : c.box_since) v = 1;
+      s += k + 1;
+      continue;
+    }
+    if (force_next && s + k + 1 != nsteps) {
+      // the queued force exited without writing the next positions: undo its swap of the buffers
+      if (jrows) std::swap(c.posfx, c.posfx_next);
+      else std::swap(c.pos32, c.pos32_next);
+    }
     if (raised == 0 && !due) {
       s += k;
       c.steps_since_rebuild += k;

@@ -75,8 +75,8 @@ __global__ void k_tower_scatter(int64_t nitems, Segs sg, const int32_t *__restri
 }
 
 // One workgroup: exclusive scans of the tower sizes (particle offsets) and of the clusters per tower,
-// ceil(count / M), with the totals at index ntowers; stats = {particles, longest tower, clusters}.  Each thread scans a contiguous run of towers, then the run totals are
-// scanned in LDS.
+// ceil(count / M), with the totals at index ntowers; stats = {particles, longest tower, clusters, shortest
+// non-empty tower}.  Each thread scans a contiguous run of towers, then the run totals are scanned in LDS.
 constexpr int kLayoutThreads = 1024;
 
 __global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers, const int32_t *__restrict__ count,
@@ -85,21 +85,23 @@ __global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers
                                                                  int64_t *__restrict__ cstart,
                                                                  unsigned long long *__restrict__ stats) {
   __shared__ int64_t sp[kLayoutThreads], sc[kLayoutThreads];
-  __shared__ int32_t sm[kLayoutThreads];
+  __shared__ int32_t sm[kLayoutThreads], sn[kLayoutThreads];
   const int tid = threadIdx.x;
   const int64_t per = (ntowers + kLayoutThreads - 1) / kLayoutThreads;
   const int64_t b = tid * per, e = b + per < ntowers ? b + per : ntowers;
   int64_t np = 0, nc = 0;
-  int32_t mx = 0;
+  int32_t mx = 0, mn = INT32_MAX;
   for (int64_t t = b; t < e; ++t) {
     const int32_t k = count[t];
     np += k;
     nc += (k + M - 1) / M;
     mx = k > mx ? k : mx;
+    if (k > 0) mn = k < mn ? k : mn;
   }
   sp[tid] = np;
   sc[tid] = nc;
   sm[tid] = mx;
+  sn[tid] = mn;
   __syncthreads();
   // inclusive scan of the run totals (Hillis-Steele), maximum by tree
   for (int off = 1; off < kLayoutThreads; off <<= 1) {
@@ -110,7 +112,10 @@ __global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers
     __syncthreads();
   }
   for (int s = kLayoutThreads / 2; s > 0; s >>= 1) {
-    if (tid < s) sm[tid] = sm[tid + s] > sm[tid] ? sm[tid + s] : sm[tid];
+    if (tid < s) {
+      sm[tid] = sm[tid + s] > sm[tid] ? sm[tid + s] : sm[tid];
+      sn[tid] = sn[tid + s] < sn[tid] ? sn[tid + s] : sn[tid];
+    }
     __syncthreads();
   }
   int64_t op = sp[tid] - np, oc = sc[tid] - nc;
@@ -129,7 +134,10 @@ __global__ __launch_bounds__(kLayoutThreads) void k_tower_layout(int64_t ntowers
     stats[0] = (unsigned long long)sp[tid];
     stats[2] = (unsigned long long)sc[tid];
   }
-  if (tid == 0) stats[1] = (unsigned long long)sm[0];
+  if (tid == 0) {
+    stats[1] = (unsigned long long)sm[0];
+    stats[3] = (unsigned long long)(sn[0] == INT32_MAX ? 0 : sn[0]);
+  }
 }
 
 // The same layout for grids of many towers (ensembles of boxes), in three passes so that every thread
@@ -223,7 +231,10 @@ __global__ __launch_bounds__(kLayoutThreads) void k_layout_top(int64_t nblocks, 
     stats[0] = (unsigned long long)sp[tid];
     stats[2] = (unsigned long long)sc[tid];
   }
-  if (tid == 0) stats[1] = (unsigned long long)sm[0];
+  if (tid == 0) {
+    stats[1] = (unsigned long long)sm[0];
+    stats[3] = 0ull;  // the shortest tower is not tracked here
+  }
 }
 
 __global__ __launch_bounds__(kLayoutBlock) void k_layout_apply(int64_t ntowers, const int32_t *__restrict__ count,
@@ -1112,7 +1123,7 @@ void vcl_build(Ctx &c) {
   // the expected longest tower of the marked boxes instead of waiting for the readback, and launches the
   // register sorts up to it: a longer tower (rare) is reported by the longest instance launched, the
   // permutation then skips, and the build runs again with the readback.
-  unsigned long long ts[3] = {0, 0, 0};
+  unsigned long long ts[4] = {0, 0, 0, 0};
   const bool guess = partial && !c.partial_tower_sync;
   if (guess) {
     ts[1] = (unsigned long long)(c.seg_max_count < 8 * kWave ? c.seg_max_count : 8 * kWave);
@@ -1147,11 +1158,13 @@ void vcl_build(Ctx &c) {
                        OVER);                                                                                   \
     LJG_HIP(hipGetLastError());                                                                                 \
   }
-    // every tower is sorted by exactly one instance; the longest tower (ts[1]) bounds which ones can have work
+    // every tower is sorted by exactly one instance; the longest tower (ts[1]) and the shortest non-empty
+    // one (ts[3], 0 when not known) bound which ones can have work
     unsigned long long *over = guess ? c.list_stats.p + 5 : nullptr;
     const int top = ts[1] > (unsigned long long)(4 * kWave) ? 8 : ts[1] > (unsigned long long)kWave ? 4 : 1;
-    LJG_WAVE_SORT(1, top == 1 ? over : nullptr)
-    if (top >= 4) LJG_WAVE_SORT(4, top == 4 ? over : nullptr)
+    const unsigned long long shortest = ens ? 0ull : ts[3];
+    if (shortest <= (unsigned long long)kWave) LJG_WAVE_SORT(1, top == 1 ? over : nullptr)
+    if (top >= 4 && shortest <= (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(4, top == 4 ? over : nullptr)
     if (top == 8) LJG_WAVE_SORT(8, over)
 #undef LJG_WAVE_SORT
   }
