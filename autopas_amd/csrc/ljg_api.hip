@@ -302,6 +302,10 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
     LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(c.host_flag, 0, 4 * sizeof(int32_t));
     LJG_HIP(hipHostGetDevicePointer((void **)&c.host_flag_dev, c.host_flag, 0));
+    LJG_HIP(hipHostMalloc((void **)&c.pin_mark, 16 * sizeof(unsigned long long),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c.pin_mark, 0, 16 * sizeof(unsigned long long));
+    LJG_HIP(hipHostGetDevicePointer((void **)&c.pin_mark_dev, c.pin_mark, 0));
     c.flag.ensure(4);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.flag.p);
     LJG_HIP(hipGetLastError());
@@ -339,6 +343,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.ctower_alt.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
+  if (c.pin_mark) (void)hipHostFree(c.pin_mark);
   if (c.pin_seg) (void)hipHostFree(c.pin_seg);
   if (c.pin_dl) (void)hipHostFree(c.pin_dl);
   if (c.pin_flags) (void)hipHostFree(c.pin_flags);
@@ -961,9 +966,9 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     if (rebuild_next) {
       rebuild_now();
       if (force_next) force_step(s + k, true, stamp_of(s + k) - 1);
-      LJG_HIP(hipStreamSynchronize(c.stream));
+      stream_wait(c);
     } else if (dyn || due) {
-      LJG_HIP(hipStreamSynchronize(c.stream));
+      stream_wait(c);
     }
     const int32_t raised = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
     if (force_next && (raised == 0 || raised >= stamp_of(s + k))) {

@@ -306,6 +306,9 @@ struct Ctx {
   DevBuf<int32_t> flag;           // [0] displacement, [1] capacity error
   int32_t *host_flag = nullptr;   // pinned, device-mapped: [0] the drift's displacement stamp
   int32_t *host_flag_dev = nullptr;  // device address of host_flag
+  unsigned long long *pin_mark = nullptr;      // pinned, device-mapped: [0] marker, [1..] words read back
+  unsigned long long *pin_mark_dev = nullptr;  // device address of pin_mark
+  unsigned long long mark_seq = 0;
 
   // Scan scratch
   DevBuf<int64_t> scan_tmp;
@@ -412,6 +415,11 @@ void reduce_partials(Ctx &c, int64_t nitems, ljg_globals *out, bool globals);  /
 // the derived FP32 / fixed-point positions refreshed; flag[2] = 1 if a particle moved skin/2 since the
 // last rebuild.  ljg_md.hip
 void md_set_positions(Ctx &c, const double *pos, const double *vel);
+// Waits until the stream has run everything queued, and reads n (<= 15) words of device memory back:
+// a one-thread kernel copies them into pinned mapped memory and then writes a fresh marker there, on
+// which the host spins (a stream synchronisation wakes the host tens of microseconds after the work
+// ends).  After 2 s of spinning it synchronises the stream instead.  ljg_md.hip
+void stream_wait(Ctx &c, const unsigned long long *src = nullptr, int n = 0, unsigned long long *out = nullptr);
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
 // Ensemble (ljg_ensemble.hip): per-box tower grids and fixed regions for cluster size c.M, the tower

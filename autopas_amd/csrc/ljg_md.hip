@@ -2,6 +2,8 @@
 // deterministic reductions of per-work-item partial sums.
 #include <cmath>
 
+#include <chrono>
+
 #include "ljg_internal.h"
 
 namespace ljg {
@@ -443,6 +445,34 @@ __global__ __launch_bounds__(kStatThreads) void k_count_stats(int64_t n, const i
     atomicAdd(&stats[0], ss[0]);
     atomicMax(&stats[1], (unsigned long long)sm[0]);
   }
+}
+
+__global__ void k_mark(unsigned long long *__restrict__ dst, const unsigned long long *__restrict__ src, int32_t n,
+                       unsigned long long seq) {
+  if ((int32_t)threadIdx.x < n) dst[1 + threadIdx.x] = src[threadIdx.x];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    dst[0] = seq;
+  }
+}
+
+void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long long *out) {
+  if (n < 0 || n > 15) throw HipError{hipErrorInvalidValue, "stream_wait reads at most 15 words", __LINE__};
+  const unsigned long long seq = ++c.mark_seq;
+  hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, c.stream, c.pin_mark_dev, src, (int32_t)n, seq);
+  LJG_HIP(hipGetLastError());
+  volatile unsigned long long *m = c.pin_mark;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t spins = 0; m[0] != seq; ++spins) {
+    if ((spins & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+      LJG_HIP(hipStreamSynchronize(c.stream));
+      if (m[0] != seq) throw HipError{hipErrorUnknown, "stream marker not written", __LINE__};
+      break;
+    }
+  }
+  for (int i = 0; i < n; ++i) out[i] = m[1 + i];
 }
 
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats) {

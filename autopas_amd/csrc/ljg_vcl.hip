@@ -1128,8 +1128,7 @@ void vcl_build(Ctx &c) {
   if (guess) {
     ts[1] = (unsigned long long)(c.seg_max_count < 8 * kWave ? c.seg_max_count : 8 * kWave);
   } else {
-    LJG_HIP(hipMemcpyAsync(ts, c.tstats.p, sizeof ts, hipMemcpyDeviceToHost, c.stream));
-    LJG_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c, c.tstats.p, 4, ts);
   }
   const int64_t ncl = ens ? c.box_clusters : (int64_t)ts[2];
   if (ts[1] > (unsigned long long)kTowerSortCap)
@@ -1288,8 +1287,7 @@ void vcl_build(Ctx &c) {
       count_stats(c, c.jcount.p, ncl, c.list_stats.p + 3);
     }
     unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
-    LJG_HIP(hipMemcpyAsync(st, c.list_stats.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
-    LJG_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c, c.list_stats.p, 6, st);
     if (st[5] && guess) {
       // a marked tower was longer than the register sorts take: nothing was permuted, so build again
       // with the longest tower read back first
