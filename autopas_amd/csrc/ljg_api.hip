@@ -48,15 +48,6 @@ __global__ void k_init_upload(int64_t n, StateView s, const int64_t *__restrict_
   s.id[i] = have_id ? id_in[i] : i;
 }
 
-// out[orig[s]] = in[s] for the real slots (storage order -> upload order).
-__global__ void k_to_upload_order(int64_t nslots, const int32_t *__restrict__ orig, const double *__restrict__ in,
-                                  double *__restrict__ out) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  const int32_t o = orig[s];
-  if (o >= 0) out[o] = in[s];
-}
-
 // Positions, velocities and forces of the particles with upload indices [first, first + count), array by
 // array (x, y, z, vx, vy, vz, fx, fy, fz), count values each.
 __global__ void k_gather_range(int64_t nslots, const int32_t *__restrict__ orig, int64_t first, int64_t count,
@@ -602,14 +593,10 @@ ljg_status ljg_edit_boxes(ljg_ctx *h, int32_t nkeep, const int32_t *keep, int32_
   LJG_API_END
 }
 
-ljg_status ljg_download_range(ljg_ctx *h, int64_t first, int64_t count, double *x, double *y, double *z, double *vx,
-                              double *vy, double *vz, double *fx, double *fy, double *fz) {
-  if (!h || first < 0 || count < 0) return LJG_ERR_ARG;
-  LJG_API_BEGIN(h)
-  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
-  if (first + count > c.n) return fail(c, LJG_ERR_ARG, "range beyond the uploaded particles");
-  if (count == 0) return LJG_OK;
-  // one pass gathers the nine arrays of the range, one copy brings them back through pinned memory
+// Upload indices [first, first + count): one pass gathers the nine arrays (null outputs are skipped on the
+// host), one copy brings them back through pinned memory.
+static void gather_range(Ctx &c, int64_t first, int64_t count, double *x, double *y, double *z, double *vx,
+                         double *vy, double *vz, double *fx, double *fy, double *fz) {
   State &s = c.st[c.cur];
   double *outs[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
   const size_t bytes = 9 * (size_t)count * sizeof(double);
@@ -625,9 +612,19 @@ ljg_status ljg_download_range(ljg_ctx *h, int64_t first, int64_t count, double *
     c.pin_dl_cap = bytes + bytes / 2;
   }
   LJG_HIP(hipMemcpyAsync(c.pin_dl, c.part_v.p, bytes, hipMemcpyDeviceToHost, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c);
   for (int k = 0; k < 9; ++k)
     if (outs[k]) std::memcpy(outs[k], c.pin_dl + (size_t)k * count, (size_t)count * sizeof(double));
+}
+
+ljg_status ljg_download_range(ljg_ctx *h, int64_t first, int64_t count, double *x, double *y, double *z, double *vx,
+                              double *vy, double *vz, double *fx, double *fy, double *fz) {
+  if (!h || first < 0 || count < 0) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (first + count > c.n) return fail(c, LJG_ERR_ARG, "range beyond the uploaded particles");
+  if (count == 0) return LJG_OK;
+  gather_range(c, first, count, x, y, z, vx, vy, vz, fx, fy, fz);
   return LJG_OK;
   LJG_API_END
 }
@@ -697,7 +694,7 @@ ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, c
   LJG_HIP(hipGetLastError());
   if (c.nboxes > 0) ensemble_wrap(c);
   else wrap_positions(c);
-  LJG_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c);  // the caller's arrays are free to change once this returns
   c.layout = Layout::Upload;
   c.regions_valid = false;
   c.have_forces = false;
@@ -824,6 +821,14 @@ constexpr int32_t kChunkSteps = 32;
 
 __global__ void k_clear_flag(int32_t *flag) { flag[0] = 0; }
 
+// The pair counts of the step loop's kAccSlots lines, summed into *sum.
+__global__ void k_sum_slots(const unsigned long long *__restrict__ slots, unsigned long long *__restrict__ sum) {
+  if (threadIdx.x != 0) return;
+  unsigned long long t = 0;
+  for (int k = 0; k < kAccSlots; ++k) t += slots[k * kAccStride];
+  sum[0] = t;
+}
+
 static hipEvent_t take_event(Ctx &c) {
   if (!c.event_pool.empty()) {
     hipEvent_t e = c.event_pool.back();
@@ -838,7 +843,7 @@ static hipEvent_t take_event(Ctx &c) {
 static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool timing, ljg_run_stats &rs) {
   std::vector<hipEvent_t> rev;                           // pairs (start, stop) per rebuild
   std::vector<std::pair<int32_t, hipEvent_t>> fev;       // (step, event): start, stop per timed force launch
-  c.hits_slots.ensure(kAccSlots * kAccStride);
+  c.hits_slots.ensure(kAccSlots * kAccStride + 1);  // (+1: their sum, read back at the end)
   hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kAccSlots * kAccStride), 0, c.stream, (int64_t)(kAccSlots * kAccStride),
                      reinterpret_cast<int64_t *>(c.hits_slots.p));
   LJG_HIP(hipGetLastError());
@@ -846,7 +851,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   c.timing = false;  // per-launch events are recorded here instead
   // clears both copies of the flag; every queued launch (a drift that may still raise it) has run first
   auto clear_flag = [&]() {
-    LJG_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c);
     static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
     LJG_HIP(hipGetLastError());
@@ -1093,11 +1098,12 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   c.hits_acc = nullptr;
   if (nsteps > 0 && !fused) md_kick_drift(c, true, false, dt, 0);  // final velocity update
   c.timing = saved_timing;
-  unsigned long long slots[kAccSlots * kAccStride];
-  LJG_HIP(hipMemcpyAsync(slots, c.hits_slots.p, sizeof slots, hipMemcpyDeviceToHost, c.stream));
-  LJG_HIP(hipStreamSynchronize(c.stream));
-  int64_t hits2 = 0;
-  for (int k = 0; k < kAccSlots; ++k) hits2 += (int64_t)slots[k * kAccStride];
+  unsigned long long *sum = c.hits_slots.p + kAccSlots * kAccStride;
+  hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, c.stream, c.hits_slots.p, sum);
+  LJG_HIP(hipGetLastError());
+  unsigned long long total = 0;
+  stream_wait(c, sum, 1, &total);
+  const int64_t hits2 = (int64_t)total;
   rs.steps += nsteps;
   rs.pair_interactions += hits2 / 2;
   for (size_t k = 0; k + 1 < fev.size(); k += 2) {
@@ -1258,18 +1264,7 @@ ljg_status ljg_download(ljg_ctx *h, double *x, double *y, double *z, double *vx,
                         double *fy, double *fz) {
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
-  State &s = c.st[c.cur];
-  double *outs[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
-  const double *ins[9] = {s.x.p, s.y.p, s.z.p, s.vx.p, s.vy.p, s.vz.p, s.fx.p, s.fy.p, s.fz.p};
-  c.part_v.ensure((size_t)c.n);
-  for (int k = 0; k < 9; ++k) {
-    if (!outs[k]) continue;
-    hipLaunchKernelGGL(k_to_upload_order, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.orig.p,
-                       ins[k], c.part_v.p);
-    LJG_HIP(hipGetLastError());
-    LJG_HIP(hipMemcpyAsync(outs[k], c.part_v.p, (size_t)c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    LJG_HIP(hipStreamSynchronize(c.stream));
-  }
+  gather_range(c, 0, c.n, x, y, z, vx, vy, vz, fx, fy, fz);
   return LJG_OK;
   LJG_API_END
 }
