@@ -329,7 +329,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release();
-  c.box_flag.release(); c.list_tmp.release(); c.seg_buf.release();
+  c.box_flag.release(); c.list_tmp.release(); c.seg_buf.release(); c.cell_sorted.release();
   c.pos32_alt.release(); c.pos32n_alt.release(); c.bbox_alt.release(); c.c0_alt.release(); c.lcount_alt.release();
   c.ctower_alt.release();
   c.cl_glob.release(); c.box_glob.release();

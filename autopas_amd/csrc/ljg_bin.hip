@@ -56,23 +56,22 @@ __global__ void k_cell_scatter(int64_t nslots, const int32_t *__restrict__ cell_
   items[start[c] + rank[i]] = (int32_t)i;
 }
 
-// The atomic histogram hands out ranks in arbitrary order; sort every cell's members by particle id so
-// the layout (and with it every floating-point summation order downstream) is reproducible.
-__global__ void k_cell_sort_by_id(int64_t ncells, const int64_t *__restrict__ start, int32_t *__restrict__ items,
-                                  const int64_t *__restrict__ id) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+// The atomic histogram hands out ranks in arbitrary order; every cell's members are put in particle-id
+// order so the layout (and with it every floating-point summation order downstream) is reproducible.
+// A thread per member counts the members of its cell with a smaller id (ids are distinct) and writes
+// itself to that place: independent loads of the cell's few entries, instead of a serial sort per cell.
+__global__ void k_cell_order_by_id(int64_t n, const int32_t *__restrict__ items, const int32_t *__restrict__ cell_of,
+                                   const int64_t *__restrict__ start, const int64_t *__restrict__ id,
+                                   int32_t *__restrict__ sorted) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t v = items[k];
+  const int32_t c = cell_of[v];
   const int64_t b = start[c], e = start[c + 1];
-  for (int64_t k = b + 1; k < e; ++k) {
-    const int32_t v = items[k];
-    const int64_t key = id[v];
-    int64_t m = k - 1;
-    while (m >= b && id[items[m]] > key) {
-      items[m + 1] = items[m];
-      --m;
-    }
-    items[m + 1] = v;
-  }
+  const int64_t key = id[v];
+  int64_t r = 0;
+  for (int64_t m = b; m < e; ++m) r += id[items[m]] < key ? 1 : 0;
+  sorted[b + r] = v;
 }
 
 // Cell of each particle in upload order (inspection).
@@ -124,9 +123,11 @@ void lc_bin(Ctx &c) {
   hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.cell_of.p, c.rank.p,
                      c.cell_start.p, c.perm.p);
   LJG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cell_sort_by_id, dim3(blocks_for(ncells, 128)), dim3(128), 0, c.stream, ncells,
-                     c.cell_start.p, c.perm.p, s.id);
+  c.cell_sorted.ensure((size_t)c.n);
+  hipLaunchKernelGGL(k_cell_order_by_id, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.p,
+                     c.cell_of.p, c.cell_start.p, s.id, c.cell_sorted.p);
   LJG_HIP(hipGetLastError());
+  std::swap(c.perm, c.cell_sorted);
   c.cell_of_particle.ensure(c.n);
   hipLaunchKernelGGL(k_cell_of_particle, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, s.orig, c.cell_of.p,
                      c.cell_of_particle.p);

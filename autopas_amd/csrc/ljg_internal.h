@@ -345,6 +345,7 @@ struct Ctx {
   bool regions_valid = false;          // the slots are laid out in the regions of the current geometry
   bool build_partial = false;          // the next build rebuilds the boxes seg_active marks only, in place
   DevBuf<int32_t> box_flag;            // per box: a particle of the box raised the displacement flag
+  DevBuf<int32_t> cell_sorted;         // LinkedCells binning: cell members in id order
   std::vector<int64_t> box_since;      // per box: steps since the box's last rebuild
   DevBuf<int4> list_tmp;               // rows of the boxes a partial build keeps, when the row length grows
   std::vector<int64_t> box_cbase_h, box_tbase_h;  // host copies of box_cbase, box_tbase

@@ -56,6 +56,9 @@ def parse():
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the batched-ensemble measurement, printed as one JSON object (for rocprof)")
     p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
+    p.add_argument("--build-profile", action="store_true",
+                   help="only the binning and list builds of the liquid (LinkedCells binning, cluster-pair build, "
+                        "j-row build; 5 each from the upload order), then the calibration copy (for rocprof)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="with --profile-only: finish with a 256 MiB device-to-device copy, the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -82,6 +85,28 @@ def copy_calibration(nbytes=COPY_BYTES):
     finally:
         hip.hipFree(src)
         hip.hipFree(dst)
+
+
+def build_profile(sysd, reps=5):
+    """The build kernels of the bulk liquid for the counter and kernel-trace profiles: `reps` times each,
+    from the upload order, the LinkedCells binning (cells of rc + skin), the cluster-pair build (M = 4) and
+    the j-row build (M = 8)."""
+    import autopas_amd as ap
+
+    L = sysd["L"]
+    for kind, M in (("bin", 4), ("pairs", 4), ("jrows", 8)):
+        e = ap.Engine((0.0, 0.0, 0.0), (L, L, L), 2.5, 0.3, cluster_size=M)
+        try:
+            for _ in range(reps):
+                e.upload(sysd["x"], sysd["y"], sysd["z"], sysd["vx"], sysd["vy"], sysd["vz"])
+                if kind == "bin":
+                    e.bin()
+                elif kind == "pairs":
+                    e.build_pairs()
+                else:
+                    e.rebuild(ap.KERNEL_CLUSTER_JROWS_MIXED)
+        finally:
+            e.close()
 
 
 def make_system(n_side, seed=2):
@@ -292,6 +317,11 @@ def main():
         print(json.dumps({"ensemble": ensemble_bench(args.ensemble_boxes, args.ensemble_steps)}))
         return
     s = make_system(args.n_side)
+    if args.build_profile:
+        build_profile(s)
+        copy_calibration()
+        print(json.dumps({"build_profile": True, "particles": s["n"]}))
+        return
     if args.layout:
         kind, m, g = args.layout.split("_")
         best, table = args.layout, {}
