@@ -56,6 +56,9 @@ def parse():
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the batched-ensemble measurement, printed as one JSON object (for rocprof)")
     p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
+    p.add_argument("--slab-steps", type=int, default=100,
+                   help="timed steps of the two-phase four-type slab measurement (0: skip)")
+    p.add_argument("--slab-only", action="store_true", help="only the two-phase slab measurement")
     p.add_argument("--build-profile", action="store_true",
                    help="only the binning and list builds of the liquid (LinkedCells binning, cluster-pair build, "
                         "j-row build; 5 each from the upload order), then the calibration copy (for rocprof)")
@@ -85,6 +88,46 @@ def copy_calibration(nbytes=COPY_BYTES):
     finally:
         hip.hipFree(src)
         hip.hipFree(dst)
+
+
+SLAB_EPS = (0.8, 1.0, 1.1, 1.2)
+SLAB_SIG = (0.9, 1.0, 1.05, 1.1)
+
+
+def slab_bench(steps, warmup=10, plain_steps=20):
+    """The two-phase, four-type system: box 200 x 200 x 400, a liquid slab at rho 0.8 (53.85 thick, centred
+    in z) in gas at rho 0.02, 2.0e6 particles placed by seeded random sequential packing (d_min 0.85, seed 3),
+    four types (sigma 0.9-1.1, epsilon 0.8-1.2) with Lorentz-Berthelot mixing, rc 3, skin 0.3, T 0.9,
+    dt 0.002; the j-row kernel (M = 8) against the plain FP64 linked-cells kernel on the same system."""
+    import autopas_amd as ap
+
+    box = np.array([200.0, 200.0, 400.0])
+    thick = 53.85
+    slab = (box[2] / 2 - thick / 2, box[2] / 2 + thick / 2)
+    n_l = int(round(0.8 * box[0] * box[1] * thick))
+    n_g = int(round(0.02 * (box.prod() - box[0] * box[1] * thick)))
+    x, y, z, vx, vy, vz, t = ap.generate_slab(3, box, slab, n_l, n_g, dmin=0.85, ntypes=4, temperature=0.9)
+    out = {}
+    for key, kernel, M, n in (("jrows", ap.KERNEL_CLUSTER_JROWS_MIXED, 8, steps),
+                              ("plain_fp64", ap.KERNEL_LC_FP64, 4, plain_steps)):
+        e = ap.Engine((0.0, 0.0, 0.0), tuple(box), 3.0, 0.3, cluster_size=M, rebuild_frequency=10)
+        try:
+            e.set_types(SLAB_EPS, SLAB_SIG, [1.0] * 4)
+            e.upload(x, y, z, vx, vy, vz, type=t)
+            e.step(warmup, 0.002, kernel)
+            t0 = time.perf_counter()
+            rs = e.step(n, 0.002, kernel)
+            wall = time.perf_counter() - t0
+        finally:
+            e.close()
+        out[key] = dict(value=rs.pair_interactions / wall, ms_per_step=wall / n * 1e3, steps=n,
+                        rebuilds=rs.rebuilds)
+    return dict(workload=f"two-phase four-type slab: {n_l + n_g} particles (liquid {n_l} at rho 0.8, gas {n_g} "
+                         "at rho 0.02), box 200 x 200 x 400, LB mixing, rc 3, skin 0.3, dt 0.002",
+                unit="pair-interactions/s", value=out["jrows"]["value"], ms_per_step=out["jrows"]["ms_per_step"],
+                steps=steps, rebuilds=out["jrows"]["rebuilds"], kernel="cluster_jrows_fp32pair_fp64acc M8",
+                plain_fp64=out["plain_fp64"],
+                speedup_vs_plain_fp64=out["jrows"]["value"] / out["plain_fp64"]["value"])
 
 
 def build_profile(sysd, reps=5):
@@ -316,6 +359,9 @@ def main():
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(args.ensemble_boxes, args.ensemble_steps)}))
         return
+    if args.slab_only:
+        print(json.dumps({"two_phase_slab": slab_bench(args.slab_steps)}))
+        return
     s = make_system(args.n_side)
     if args.build_profile:
         build_profile(s)
@@ -411,6 +457,8 @@ def main():
         out["serving"] = serving(args.serve_jobs, args.serve_workers)
     if args.ensemble_boxes > 0:
         out["ensemble"] = ensemble_bench(args.ensemble_boxes, args.ensemble_steps)
+    if args.slab_steps > 0:
+        out["two_phase_slab"] = slab_bench(args.slab_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
