@@ -16,6 +16,9 @@
 //    FP64 variant: same tiles, absolute FP64 positions, pair term exactly as LJFunctor.h:288-320.
 // Each kernel also counts the pairs with r^2 <= rc^2 (ballot + popcount on the scalar unit) and, when
 // asked, the potential energy (6x, shift included) and virial per LJFunctor.h:174-197 / :322-345.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "ljg_internal.h"
@@ -385,10 +388,18 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
 //    (kick of this step, drift of the next, same FP64 operations in the same order), writes their
 //    fixed-point positions for the next step to the second buffer (other workgroups still read this
 //    step's), and raises the displacement flag.
+//  * The cutoff decision is the FP64 one.  The FP32 r^2 is compared with two thresholds around rc^2
+//    (cut_lo, cut_hi, bit patterns of positive floats, wider apart than the FP32 error of r^2): below
+//    cut_lo the pair is inside, from cut_hi on outside, and a wave with any pair in between decides those
+//    pairs again in FP64 from the exact fixed-point separation (r^2 <= rc2d).  An FP32 decision that
+//    differs from the FP64 one leaves the force at rc (~0.04 epsilon/sigma) on a particle, and the two
+//    rows of a pair round r^2 differently in the FAST frame: at 10^6 particles a dozen such pairs would
+//    dominate the force error.  The band holds ~1e-6 of the candidate pairs.
 template <int M, int G, bool FAST, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t row,
-    const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, float rc2, float eps24_1,
+    const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, uint32_t cut_lo,
+    uint32_t cut_hi, double rc2d, float rc2, float eps24_1,
     float sig2_1, float shift6_1, int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
     int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, IntegArgs ia) {
@@ -409,7 +420,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
   if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
   const bool live = ic < nclusters;
   const int64_t icl = live ? ic : nclusters - 1;  // dead groups (last wave only) read a real cluster, store nothing
-  const uint32_t rc2_bits = __float_as_uint(rc2);
+  (void)rc2;  // the thresholds cut_lo, cut_hi stand for rc^2 in the pair loop
   const int nreal = (int)c0[icl].w;
   const int4 ref = posfx[icl * M];  // the first member is always real
   float xi[M], yi[M], zi[M];     // FAST: FP32 offsets from the first member (fixed-point units)
@@ -460,18 +471,16 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
       yj = (float)(int32_t)((uint32_t)pj.y - (uint32_t)ref.y);
       zj = (float)(int32_t)((uint32_t)pj.z - (uint32_t)ref.z);
     }
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      float dx, dy, dz;
+    auto separation = [&](int q, float jx, float jy, float jz, const int4 &pp, float &dx, float &dy, float &dz) {
       if constexpr (FAST) {
-        dx = xi[q] - xj; dy = yi[q] - yj; dz = zi[q] - zj;
+        dx = xi[q] - jx; dy = yi[q] - jy; dz = zi[q] - jz;
       } else {
-        dx = (float)(int32_t)(ux[q] - (uint32_t)pj.x) * f.scale32[0];
-        dy = (float)(int32_t)(uy[q] - (uint32_t)pj.y) * f.scale32[1];
-        dz = (float)(int32_t)(uz[q] - (uint32_t)pj.z) * f.scale32[2];
+        dx = (float)(int32_t)(ux[q] - (uint32_t)pp.x) * f.scale32[0];
+        dy = (float)(int32_t)(uy[q] - (uint32_t)pp.y) * f.scale32[1];
+        dz = (float)(int32_t)(uz[q] - (uint32_t)pp.z) * f.scale32[2];
       }
-      const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-      bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+    };
+    auto pair_term = [&](int q, float dx, float dy, float dz, float r2, bool m) {
       if constexpr (!FAST) m = m && valid && q < nreal;
       float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
       if constexpr (MIX) {
@@ -491,7 +500,72 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
         ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
         av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);
       }
+    };
+    // pairs in the band around rc^2 are left out here; anyb collects the lanes that hold one
+    unsigned long long anyb = 0ull;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      float dx, dy, dz;
+      separation(q, xj, yj, zj, pj, dx, dy, dz);
+      const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      const uint32_t ub = __float_as_uint(r2) - 1u;  // r2 = +0 wraps: the pair of a member with itself
+      const bool m = ub < cut_lo;
+      // lane masks of the two compares, combined on the scalar unit
+      anyb |= __builtin_amdgcn_ballot_w64(ub < cut_hi) & ~__builtin_amdgcn_ballot_w64(m);
+      pair_term(q, dx, dy, dz, r2, m);
     }
+    if (anyb != 0ull) {
+      // rare: find the band pairs again (same operands, same r^2), decide them in FP64 from the exact
+      // fixed-point separation and add those inside.  Nothing is kept from the loop above for this: the
+      // j values pass through an empty asm and the positions are loaded again (the member's index also
+      // through the asm, so that the load is not hoisted out of the j loop), which keeps the loop at
+      // the registers it needs without the band.
+      float jx = xj, jy = yj, jz = zj;
+      asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz));
+      int4 pp = make_int4(0, 0, 0, 0);
+      if (nj > 0) pp = posfx[rowp[valid ? base + l : jlast]];
+      // first the decisions, one bit per member, then the pair terms: the FP64 temporaries are dead
+      // before the pair term needs its own
+      uint32_t inside = 0u;
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        float dx, dy, dz;
+        separation(q, jx, jy, jz, pp, dx, dy, dz);
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const uint32_t ub = __float_as_uint(r2) - 1u;
+        const bool inband = ub >= cut_lo && ub < cut_hi;
+        if (__ballot(inband) == 0ull) continue;
+        int32_t qq = q;
+        asm volatile("" : "+v"(qq));
+        const int4 pq = posfx[icl * M + qq];
+        const double ex = (double)(int32_t)((uint32_t)pq.x - (uint32_t)pp.x) * f.scale[0];
+        const double ey = (double)(int32_t)((uint32_t)pq.y - (uint32_t)pp.y) * f.scale[1];
+        const double ez = (double)(int32_t)((uint32_t)pq.z - (uint32_t)pp.z) * f.scale[2];
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        if (inband && d2 > 0.0 && d2 <= rc2d) inside |= 1u << q;
+        // one member at a time: the next one's operands come out of this asm, after this one's result
+        asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(pp.x), "+v"(pp.y), "+v"(pp.z), "+v"(inside));
+      }
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        const bool in = (inside >> q) & 1u;
+        if (__ballot(in) != 0ull) {
+          float dx, dy, dz;
+          separation(q, jx, jy, jz, pp, dx, dy, dz);
+          const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          pair_term(q, dx, dy, dz, r2, in);
+        }
+        asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(pp.x), "+v"(pp.y), "+v"(pp.z), "+v"(ax[q]),
+                     "+v"(ay[q]), "+v"(az[q]));
+      }
+    }
+  }
+  // the sums pass through an empty asm: otherwise the compiler folds them to 0 for lanes that skip the
+  // loop and materialises those zeros before it, 2 NV registers live for the whole loop
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    asm volatile("" : "+v"(ax[q]), "+v"(ay[q]), "+v"(az[q]));
+    if constexpr (GLOB) asm volatile("" : "+v"(ae[q]), "+v"(av[q]));
   }
   double v[NV];
 #pragma unroll
@@ -812,11 +886,25 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   const double u2 = fast ? f.inv_scale[0] * f.inv_scale[0] : 1.0;  // (2^32/L)^2 in the fast variant
   const float rc2 = (float)(a.rc2 * u2);
   const float sig2 = (float)(a.sig2 * u2);
+  // band around rc^2 decided in FP64, at least 2^-12 relative (the exact variant's r^2 is off by a few
+  // FP32 roundings).  FAST offsets (up to 2^31 units from the first member) round by at most 64 units
+  // each, the separation by at most 192, so r^2 is off by at most 2 sqrt(3) 192 / rc_u relative plus a
+  // few roundings; the band is four times that
+  double rel = std::ldexp(1.0, -12);
+  if (fast) {
+    const double rc_u = std::sqrt(a.rc2 * u2);
+    rel = std::max(rel, 4.0 * (2.0 * std::sqrt(3.0) * 192.0 / rc_u + std::ldexp(4.0, -24)));
+  }
+  const uint32_t band = (uint32_t)std::min(std::ceil(rel * std::ldexp(1.0, 23)), 1.0e6);
+  uint32_t rcb;
+  std::memcpy(&rcb, &rc2, sizeof rcb);
+  const uint32_t cut_lo = rcb - band, cut_hi = rcb + band;
   State &s = c.st[c.cur];
   const IntegArgs ia = integ ? *integ : IntegArgs{};
 #define LJG_LAUNCH(CUB, MIXV, GLOBV, INTV)                                                                          \
   hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream, \
-                     c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, rc2,                   \
+                     c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, cut_lo, cut_hi, a.rc2, \
+                     rc2,                                                                                         \
                      (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,    \
                      c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia)
 #define LJG_LAUNCH2(CUB)                                                                                 \

@@ -348,6 +348,29 @@ def energy_drift(layout_kernel, M, G, n_side=32, steps=1000):
             "ratio": abs(out["fp32_pair"]) / abs(out["fp64"]) if out["fp64"] else None}
 
 
+def force_accuracy(sysd, layout_kernel, M, G):
+    """Force RMS relative error of the chosen FP32-pair layout against the FP64 cluster kernel on the full
+    benchmark system (same positions, one force evaluation each): sqrt(sum |dF|^2 / sum |F|^2)."""
+    import autopas_amd as ap
+
+    L = sysd["L"]
+    res = {}
+    for name, kernel, m, g in (("fp32_pair", layout_kernel, M, G), ("fp64", ap.KERNEL_CLUSTER_FP64, 4, 0)):
+        with ap.Engine((0.0, 0.0, 0.0), (L, L, L), 2.5, 0.3, cluster_size=m, tile_groups=g) as e:
+            e.upload(sysd["x"], sysd["y"], sysd["z"])
+            gl = e.force(kernel, globals=True)
+            f = e.download()
+            res[name] = (np.stack([f["fx"], f["fy"], f["fz"]]), gl)
+    (fm, gm), (fr, gr) = res["fp32_pair"], res["fp64"]
+    d = fm - fr
+    return {"particles": sysd["n"],
+            "force_rms_rel_err": float(np.sqrt(np.sum(d * d) / np.sum(fr * fr))),
+            "force_max_abs_err": float(np.max(np.abs(d))),
+            "energy_rel_err": abs(gm.potential_energy - gr.potential_energy) / abs(gr.potential_energy),
+            "virial_rel_err": abs(gm.virial - gr.virial) / abs(gr.virial),
+            "pair_count": gm.pair_count, "pair_count_fp64": gr.pair_count}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -452,6 +475,7 @@ def main():
                     "instruction per CU per clock): the kernel is bound by VALU issue, not by FLOPs or HBM",
         },
     }
+    out["force_accuracy"] = force_accuracy(s, kernel, M, G)
     out["energy_drift"] = energy_drift(kernel, M, G)
     if args.serve_jobs > 0:
         out["serving"] = serving(args.serve_jobs, args.serve_workers)

@@ -416,6 +416,30 @@ def test_mixed_precision_accuracy_c2_subsample(oracle):
     assert res[ap.KERNEL_CLUSTER_FP64][1].pair_count == ref["pairs"]
 
 
+@pytest.mark.parametrize("M,G", [(4, 8), (8, 8)])
+def test_mixed_precision_accuracy_full_liquid(M, G):
+    # 10^6 particles (100^3 lattice, rho 0.75, seed 2): the benchmark system itself.  The FP64 cluster kernel
+    # is the reference here (it matches the oracle to 1e-12 on the smaller boxes above); the j-row FP32-pair
+    # kernel must stay within 1e-5 force RMS relative error, energy and virial within 1e-5.
+    announce(f"mixed precision accuracy: 100^3 lattice rho 0.75 seed 2, j-rows M={M} G={G}")
+    s = lattice_box(100, 0.75, seed=2)
+    L = s["L"]
+    res = {}
+    for kernel, m, g in ((ap.KERNEL_CLUSTER_JROWS_MIXED, M, G), (ap.KERNEL_CLUSTER_FP64, 4, 0)):
+        with make_engine(L, M=m, tile_groups=g) as e:
+            e.upload(s["x"], s["y"], s["z"])
+            gl = e.force(kernel, globals=True)
+            f = e.download()
+            res[kernel] = (f, gl)
+    (f, g), (fr, gr) = res[ap.KERNEL_CLUSTER_JROWS_MIXED], res[ap.KERNEL_CLUSTER_FP64]
+    err = rms_rel(f, fr)
+    print(f"force RMS relative error vs FP64 on 10^6 particles: {err:.3e}")
+    assert err <= 1e-5
+    assert abs(g.potential_energy - gr.potential_energy) <= 1e-5 * abs(gr.potential_energy)
+    assert abs(g.virial - gr.virial) <= 1e-5 * abs(gr.virial)
+    assert abs(g.pair_count - gr.pair_count) <= 1e-4 * gr.pair_count
+
+
 def test_checkpoint_restart_continues_the_trajectory(tmp_path):
     # 30 steps in one go vs 15 steps, checkpoint, a fresh context restored from it, 15 more steps (FP64)
     announce("checkpoint restart: 6^3 lattice, 15 + 15 steps vs 30 steps, LC FP64")
