@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
 //    pairs again in FP64 from the exact fixed-point separation (r^2 <= rc2d).  An FP32 decision that
 //    differs from the FP64 one leaves the force at rc (~0.04 epsilon/sigma) on a particle, and the two
 //    rows of a pair round r^2 differently in the FAST frame: at 10^6 particles a dozen such pairs would
-//    dominate the force error.  The band holds ~1e-6 of the candidate pairs.
+//    dominate the force error.  The band holds ~3e-5 of the candidate pairs (~1% of wave iterations).
 template <int M, int G, bool FAST, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t row,
@@ -886,11 +886,13 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   const double u2 = fast ? f.inv_scale[0] * f.inv_scale[0] : 1.0;  // (2^32/L)^2 in the fast variant
   const float rc2 = (float)(a.rc2 * u2);
   const float sig2 = (float)(a.sig2 * u2);
-  // band around rc^2 decided in FP64, at least 2^-12 relative (the exact variant's r^2 is off by a few
-  // FP32 roundings).  FAST offsets (up to 2^31 units from the first member) round by at most 64 units
-  // each, the separation by at most 192, so r^2 is off by at most 2 sqrt(3) 192 / rc_u relative plus a
-  // few roundings; the band is four times that
-  double rel = std::ldexp(1.0, -12);
+  // band around rc^2 decided in FP64, four times the FP32 error of r^2 and at least 2^-18 relative.  The
+  // exact variant's r^2 is off by ~9 roundings (conversion, scale, three squares and sums: ~5e-7).
+  // FAST offsets (up to 2^31 units from the first member) round by at most 64 units each, the
+  // separation by at most 192, so r^2 is off by at most 2 sqrt(3) 192 / rc_u relative plus a few
+  // roundings.  The band is kept narrow because a wave iteration holds 64 M candidate pairs: at 2^-12
+  // one iteration in ten had a pair in the band and took the FP64 pass.
+  double rel = std::ldexp(1.0, -18);
   if (fast) {
     const double rc_u = std::sqrt(a.rc2 * u2);
     rel = std::max(rel, 4.0 * (2.0 * std::sqrt(3.0) * 192.0 / rc_u + std::ldexp(4.0, -24)));

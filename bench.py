@@ -33,7 +33,7 @@ TRAFFIC_BYTES = {"jrows_M8_G8": 422.8e6}
 # Wavefront VALU instructions per force launch (SQ_INSTS_VALU) recorded in
 # profiles/cluster_jrows_m8g8_bulk_liquid_1M_valu_counters.txt, per layout; the issue capacity of a launch is
 # one wavefront instruction per CU per clock.
-VALU_INSTRUCTIONS = {"jrows_M8_G8": 6.311e7}
+VALU_INSTRUCTIONS = {"jrows_M8_G8": 6.688e7}
 MAX_CLOCK_GHZ = 2.4  # MI355X published peak engine clock
 
 
@@ -467,7 +467,7 @@ def main():
             "valu_issue_frac": (VALU_INSTRUCTIONS[best] / (256 * MAX_CLOCK_GHZ * 1e9 * force_avg_s))
                                if best in VALU_INSTRUCTIONS else None,
             "note": "VALU-bound (no MFMA): achieved counts 26 FLOP per unique pair within rc (LJFunctor.h:722-789), "
-                    "while the kernel executes ~22 VALU instructions per candidate pair and ~6.5 candidates per "
+                    "while the kernel executes ~23 VALU instructions per candidate pair and ~6.5 candidates per "
                     "unique pair (full rows, 31% within rc); peak is the MI355X FP32 vector figure (packed FP32; "
                     "the non-packed issue rate is 78.6). The launch includes the fused velocity-Verlet update; "
                     "traffic is HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counters; valu_issue_frac is "
