@@ -897,10 +897,8 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
     const double rc_u = std::sqrt(a.rc2 * u2);
     rel = std::max(rel, 4.0 * (2.0 * std::sqrt(3.0) * 192.0 / rc_u + std::ldexp(4.0, -24)));
   }
-  const uint32_t band = (uint32_t)std::min(std::ceil(rel * std::ldexp(1.0, 23)), 1.0e6);
-  uint32_t rcb;
-  std::memcpy(&rcb, &rc2, sizeof rcb);
-  const uint32_t cut_lo = rcb - band, cut_hi = rcb + band;
+  uint32_t cut_lo, cut_hi;
+  cutoff_band(rc2, rel, cut_lo, cut_hi);
   State &s = c.st[c.cur];
   const IntegArgs ia = integ ? *integ : IntegArgs{};
 #define LJG_LAUNCH(CUB, MIXV, GLOBV, INTV)                                                                          \

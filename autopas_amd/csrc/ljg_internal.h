@@ -471,6 +471,18 @@ __host__ __device__ inline int shift_component(int sid, int d) {
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// Thresholds on bits(r^2) - 1 (r^2 a positive FP32) around rc2 for the FP32-pair kernels that decide
+// the pairs near the cutoff again in FP64: below lo the pair is inside, from hi on outside.  rel is the
+// relative half-width of the band (1 ulp of rc2 is 2^-23 to 2^-24 of it).
+inline void cutoff_band(float rc2, double rel, uint32_t &lo, uint32_t &hi) {
+  uint32_t b;
+  std::memcpy(&b, &rc2, sizeof b);
+  const double ulps = std::ceil(rel * 8388608.0);
+  const uint32_t w = (uint32_t)(ulps < 1.0e6 ? ulps : 1.0e6);
+  lo = b - w;
+  hi = b + w;
+}
+
 // Same-address device atomics from many workgroups serialize (~12 ns each on MI355X); counters that
 // every workgroup of a kernel updates are spread over kAccSlots lines, one per XCD (workgroups are
 // dealt to the 8 XCDs round-robin), and summed by the host.

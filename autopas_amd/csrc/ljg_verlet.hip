@@ -163,7 +163,8 @@ constexpr int kFlush = 16;
 template <bool CUBIC, bool MIX, bool GLOB>
 __global__ __launch_bounds__(kVThreads) void k_force_verlet(
     int64_t n, const int4 *__restrict__ posfx, int32_t cap, const int32_t *__restrict__ count,
-    const int32_t *__restrict__ list, FixGeom f, float rc2, float eps24_1, float sig2_1, float shift6_1,
+    const int32_t *__restrict__ list, FixGeom f, uint32_t cut_lo, uint32_t cut_hi, double rc2d, float eps24_1,
+    float sig2_1, float shift6_1,
     int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
     unsigned long long *__restrict__ acc, SkipIf sk) {
@@ -194,7 +195,6 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
     nmax = o > nmax ? o : nmax;
   }
   nmax = nmax < cap ? nmax : cap;
-  const uint32_t rc2_bits = __float_as_uint(rc2);
   const int ti = xi.w;
   double dax = 0., day = 0., daz = 0., dae = 0., dav = 0.;
   int hits = 0;
@@ -219,7 +219,17 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
         dz *= f.scale32[2];
       }
       const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-      const bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+      // the cutoff decision is the FP64 one: pairs whose FP32 r^2 lies between cut_lo and cut_hi (rare)
+      // are decided again by the FP64 r^2 of the exact fixed-point separation
+      const uint32_t ub = __float_as_uint(r2) - 1u;  // r2 = +0 (the row's padding is the particle itself) wraps
+      bool m = ub < cut_lo;
+      if ((__builtin_amdgcn_ballot_w64(ub < cut_hi) & ~__builtin_amdgcn_ballot_w64(m)) != 0ull) {
+        const double ex = (double)(int32_t)((uint32_t)xi.x - (uint32_t)xj.x) * f.scale[0];
+        const double ey = (double)(int32_t)((uint32_t)xi.y - (uint32_t)xj.y) * f.scale[1];
+        const double ez = (double)(int32_t)((uint32_t)xi.z - (uint32_t)xj.z) * f.scale[2];
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        if (!m && ub < cut_hi) m = d2 > 0.0 && d2 <= rc2d;
+      }
       float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
       if constexpr (MIX) {
         const int t = 3 * (ti * ntypes + xj.w);
@@ -289,11 +299,15 @@ void verlet_force(Ctx &c, bool globals) {
   const double u2 = f.cubic ? f.inv_scale[0] * f.inv_scale[0] : 1.0;  // (2^32/L)^2 in the cubic variant
   const float rc2 = (float)(c.lj.rc2 * u2);
   const float sig2 = (float)(c.lj.sig2[0] * u2);
+  // r^2 from the exact integer separation: one conversion, one scale and the sums, ~9 roundings
+  uint32_t cut_lo, cut_hi;
+  cutoff_band(rc2, std::ldexp(1.0, -18), cut_lo, cut_hi);
   State &s = c.st[c.cur];
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
 #define LJG_LAUNCH(CUB, MIXV, GLOBV)                                                                             \
   hipLaunchKernelGGL((k_force_verlet<CUB, MIXV, GLOBV>), dim3(nb), dim3(kVThreads), 0, c.stream, c.n, c.posfx.p, \
-                     c.vrow, c.vcount.p, c.vlist.p, f, rc2, (float)c.lj.eps24[0], sig2, (float)c.lj.shift6[0],   \
+                     c.vrow, c.vcount.p, c.vlist.p, f, cut_lo, cut_hi, c.lj.rc2, (float)c.lj.eps24[0], sig2,      \
+                     (float)c.lj.shift6[0],                                                                       \
                      c.lj.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
 #define LJG_LAUNCH2(CUB)                                                 \
   if (mix) {                                                             \
