@@ -399,8 +399,8 @@ template <int M, int G, bool FAST, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t row,
     const int32_t *__restrict__ lcount, const int32_t *__restrict__ jlist, FixGeom f, uint32_t cut_lo,
-    uint32_t cut_hi, double rc2d, float rc2, float eps24_1,
-    float sig2_1, float shift6_1, int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx,
+    uint32_t cut_hi, double rc2d, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
+    const double *__restrict__ tab64, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
     int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, IntegArgs ia) {
   if (skip_launch(sk)) return;
@@ -420,7 +420,6 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
   if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
   const bool live = ic < nclusters;
   const int64_t icl = live ? ic : nclusters - 1;  // dead groups (last wave only) read a real cluster, store nothing
-  (void)rc2;  // the thresholds cut_lo, cut_hi stand for rc^2 in the pair loop
   const int nreal = (int)c0[icl].w;
   const int4 ref = posfx[icl * M];  // the first member is always real
   float xi[M], yi[M], zi[M];     // FAST: FP32 offsets from the first member (fixed-point units)
@@ -904,7 +903,6 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
 #define LJG_LAUNCH(CUB, MIXV, GLOBV, INTV)                                                                          \
   hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream, \
                      c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, cut_lo, cut_hi, a.rc2, \
-                     rc2,                                                                                         \
                      (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,    \
                      c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia)
 #define LJG_LAUNCH2(CUB)                                                                                 \
