@@ -866,11 +866,16 @@ __global__ __launch_bounds__(kListSplitThreads) void k_cluster_lists_split(
 // frame; otherwise it uses exact per-pair differences.
 constexpr int kExpandThreads = 256;
 
-template <int M, int NS, int G>
+// CUB (cubic boxes): the member test runs in FP32 on the separation in fixed-point units, against ilen^2
+// in those units; tests whose FP32 d^2 lies in a band around it (cut_lo, cut_hi on the bit pattern; the
+// FP32 error of d^2 is ~5 roundings, the band 2^-18 relative) are decided again by the FP64 test above,
+// so the rows are the same.  Otherwise every test is the FP64 one.
+template <int M, int NS, int G, bool CUB>
 __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t ccap,
-    const int32_t *__restrict__ ccount, const int32_t *__restrict__ jcl, FixGeom f, double ilen, int32_t cap,
-    int32_t *__restrict__ count, int32_t *__restrict__ jrows, unsigned long long *__restrict__ stats) {
+    const int32_t *__restrict__ ccount, const int32_t *__restrict__ jcl, FixGeom f, double ilen, uint32_t cut_lo,
+    uint32_t cut_hi, int32_t cap, int32_t *__restrict__ count, int32_t *__restrict__ jrows,
+    unsigned long long *__restrict__ stats) {
   constexpr int LG = kWave / G;  // lanes per i-cluster
   const int lane = threadIdx.x & (kWave - 1);
   const int grp = lane / LG, l = lane % LG;
@@ -883,6 +888,10 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
   int4 xi[M];
 #pragma unroll
   for (int q = 0; q < M; ++q) xi[q] = posfx[icl * M + q];
+  // lanes whose cluster has a real member q
+  unsigned long long real_m[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) real_m[q] = __builtin_amdgcn_ballot_w64(q < nri);
   // frame limits for the row entries: L/2 - ilen minus the largest member offset
   int64_t jlim[3];
   bool frame_bad = false;
@@ -954,13 +963,40 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
       }
     }
     bool near = false;
+    // the FP64 member test
+    auto near64 = [&](int q, const int4 &pj) {
+      const double dx = (double)(int32_t)((uint32_t)xi[q].x - (uint32_t)pj.x) * f.scale[0];
+      const double dy = (double)(int32_t)((uint32_t)xi[q].y - (uint32_t)pj.y) * f.scale[1];
+      const double dz = (double)(int32_t)((uint32_t)xi[q].z - (uint32_t)pj.z) * f.scale[2];
+      return dx * dx + dy * dy + dz * dz <= il2;
+    };
     if (inrow && xj.w >= 0) {  // padding slots carry type -1
+      if constexpr (CUB) {
+        // per member, the lanes inside and the lanes in the band, as wave masks (scalar unit)
+        unsigned long long in_m = 0ull, band_m = 0ull;
 #pragma unroll
-      for (int q = 0; q < M; ++q) {
-        const double dx = (double)(int32_t)((uint32_t)xi[q].x - (uint32_t)xj.x) * f.scale[0];
-        const double dy = (double)(int32_t)((uint32_t)xi[q].y - (uint32_t)xj.y) * f.scale[1];
-        const double dz = (double)(int32_t)((uint32_t)xi[q].z - (uint32_t)xj.z) * f.scale[2];
-        near = near || (q < nri && dx * dx + dy * dy + dz * dz <= il2);
+        for (int q = 0; q < M; ++q) {
+          const float dx = (float)(int32_t)((uint32_t)xi[q].x - (uint32_t)xj.x);
+          const float dy = (float)(int32_t)((uint32_t)xi[q].y - (uint32_t)xj.y);
+          const float dz = (float)(int32_t)((uint32_t)xi[q].z - (uint32_t)xj.z);
+          const uint32_t ub = __float_as_uint(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+          const unsigned long long lo = __builtin_amdgcn_ballot_w64(ub < cut_lo) & real_m[q];
+          in_m |= lo;
+          band_m |= __builtin_amdgcn_ballot_w64(ub < cut_hi) & real_m[q] & ~lo;
+        }
+        near = (in_m >> lane) & 1ull;
+        if (band_m != 0ull) {
+          // rare; j passes through an empty asm so that the integer separations above are not kept
+          int4 pj = xj;
+          asm volatile("" : "+v"(pj.x), "+v"(pj.y), "+v"(pj.z));
+          if ((band_m >> lane) & 1ull) {
+#pragma unroll
+            for (int q = 0; q < M; ++q) near = near || (q < nri && near64(q, pj));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < M; ++q) near = near || (q < nri && near64(q, xj));
       }
       if (near) {
         frame_bad = frame_bad || llabs((int64_t)(int32_t)((uint32_t)xj.x - (uint32_t)xi[0].x)) > jlim[0] ||
@@ -1270,12 +1306,20 @@ void vcl_build(Ctx &c) {
     if (jrows) {
       // i-clusters per wavefront of the expansion pass (tuned on the 10^6-particle liquid: 4 for M >= 4)
       const int eg = c.prm.build_groups > 0 ? c.prm.build_groups : (M >= 4 ? 4 : M == 2 ? 4 : 8);
+      // FP32 member test of the cubic variant: ilen^2 in fixed-point units, band 2^-18 relative
+      uint32_t cut_lo = 0, cut_hi = 0;
+      cutoff_band((float)(ilen * ilen * c.fg.inv_scale[0] * c.fg.inv_scale[0]), std::ldexp(1.0, -18), cut_lo, cut_hi);
 #define LJG_EXPAND(MV, GV)                                                                                        \
   if (M == MV && eg == GV) {                                                                                      \
     const int nb = blocks_for(ncl, (kExpandThreads / kWave) * GV);                                               \
-    hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV>), dim3(nb), dim3(kExpandThreads), 0, c.stream, ncl,    \
-                       c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, jcap, c.jcount.p, c.jlist.p, \
-                       c.list_stats.p);                                                                          \
+    if (c.fg.cubic)                                                                                               \
+      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, true>), dim3(nb), dim3(kExpandThreads), 0, c.stream, \
+                         ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, cut_lo, cut_hi, jcap,  \
+                         c.jcount.p, c.jlist.p, c.list_stats.p);                                                 \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, false>), dim3(nb), dim3(kExpandThreads), 0,          \
+                         c.stream, ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, cut_lo,      \
+                         cut_hi, jcap, c.jcount.p, c.jlist.p, c.list_stats.p);                                   \
     goto expanded;                                                                                                \
   }
       LJG_EXPAND(1, 4) LJG_EXPAND(1, 8) LJG_EXPAND(2, 2) LJG_EXPAND(2, 4) LJG_EXPAND(4, 1) LJG_EXPAND(4, 2)
