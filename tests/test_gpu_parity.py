@@ -443,6 +443,46 @@ def test_mixed_precision_accuracy_full_liquid(kernel, M, G):
     assert abs(g.virial - gr.virial) <= 1e-5 * abs(gr.virial)
 
 
+@pytest.mark.parametrize("kernel", [ap.KERNEL_CLUSTER_JROWS_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_FP64])
+@pytest.mark.parametrize("M", [4, 8])
+def test_cutoff_decision_matches_fp64(kernel, M):
+    # Isolated pairs at r = rc (1 + eps), |eps| from 3e-8 to 3e-7: below the FP32 resolution of r^2 (~1.2e-7
+    # relative) at the small end, above the fixed-point position quantum (L / 2^32 = 9.3e-9 sigma here) at
+    # all.  The kernels that decide the pairs near rc in FP64 keep exactly the pairs an FP64 test keeps;
+    # a pair decided the other way would leave the whole force F(rc) ~ 0.039 on both particles.
+    if kernel == ap.KERNEL_VERLET_MIXED and M == 8:
+        pytest.skip("cluster size does not apply to this kernel")
+    announce(f"cutoff decision vs FP64: pairs at rc (1 +- 3e-8 .. 3e-7), {ap.KERNEL_NAMES[kernel]} M={M}")
+    rng = np.random.default_rng(5)
+    L = 40.0
+    eps = np.array([-3e-7, -1e-7, -3e-8, 3e-8, 1e-7, 3e-7] * 4)
+    pos = []
+    for k, e in enumerate(eps):
+        base = np.array([4.0 + 8.0 * (k % 4), 4.0 + 8.0 * ((k // 4) % 4), 6.0 + 12.0 * (k // 16)])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pos += [base, base + RC * (1.0 + e) * u]
+    x, y, z = (np.array([p[d] for p in pos]) for d in range(3))
+    d = np.stack([x[1::2] - x[0::2], y[1::2] - y[0::2], z[1::2] - z[0::2]], 1)
+    expect = int(np.sum(np.sum(d * d, 1) <= RC * RC))
+    assert expect == int(np.sum(eps < 0))
+    with make_engine(L, M=M) as e:
+        e.upload(x, y, z)
+        g = e.force(kernel, globals=True)
+        f = e.download()
+    # FP64 pair forces of the kept pairs (LJFunctor.h:288-320 with sigma = epsilon = 1)
+    r2 = np.sum(d * d, 1)
+    inv = 1.0 / r2
+    l6 = inv ** 3
+    fac = np.where(r2 <= RC * RC, 24.0 * (l6 * l6 * 2.0 - l6) * inv, 0.0)
+    fi = -d * fac[:, None]  # force on the first particle of each pair
+    ref = np.zeros((len(x), 3))
+    ref[0::2], ref[1::2] = fi, -fi
+    assert g.pair_count == expect
+    got = np.stack([f["fx"], f["fy"], f["fz"]], 1)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
 def test_checkpoint_restart_continues_the_trajectory(tmp_path):
     # 30 steps in one go vs 15 steps, checkpoint, a fresh context restored from it, 15 more steps (FP64)
     announce("checkpoint restart: 6^3 lattice, 15 + 15 steps vs 30 steps, LC FP64")
