@@ -325,12 +325,12 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.tower_z.release(); c.tower_id.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.pos32_next.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.pos32.release(); c.pos32_next.release(); c.posd.release(); c.posd_next.release(); c.pos32_ext.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release();
   c.box_flag.release(); c.list_tmp.release(); c.seg_buf.release(); c.cell_sorted.release();
-  c.pos32_alt.release(); c.pos32n_alt.release(); c.bbox_alt.release(); c.c0_alt.release(); c.lcount_alt.release();
+  c.pos32_alt.release(); c.pos32n_alt.release(); c.posd_alt.release(); c.posdn_alt.release(); c.bbox_alt.release(); c.c0_alt.release(); c.lcount_alt.release();
   c.ctower_alt.release();
   c.cl_glob.release(); c.box_glob.release();
   if (c.host_flag) (void)hipHostFree(c.host_flag);
@@ -907,7 +907,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
         if (!last) std::swap(c.posfx, c.posfx_next);  // the launch wrote the next step's positions there
       } else {
         cluster_force(c, true, false, &ia);
-        if (!last) std::swap(c.pos32, c.pos32_next);
+        if (!last) swap_cluster_positions(c);
       }
     } else {
       run_force(c, kernel, false);
@@ -993,7 +993,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     if (force_next && s + k + 1 != nsteps) {
       // the queued force exited without writing the next positions: undo its swap of the buffers
       if (jrows) std::swap(c.posfx, c.posfx_next);
-      else std::swap(c.pos32, c.pos32_next);
+      else swap_cluster_positions(c);
     }
     if (raised == 0 && !due) {
       s += k;
@@ -1022,7 +1022,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
       if (fused && !jrows) {
         int32_t swaps = 0;
         for (int32_t j = t; j < s + k; ++j) swaps += j + 1 != nsteps;
-        if (swaps & 1) std::swap(c.pos32, c.pos32_next);
+        if (swaps & 1) swap_cluster_positions(c);
       }
       // the boxes whose drift of step t raised the flag, and those due at step t
       mark.assign((size_t)c.nboxes, 0);
@@ -1073,7 +1073,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
         }
         c.stream = main_stream;
         LJG_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
-        if (!last) std::swap(c.pos32, c.pos32_next);
+        if (!last) swap_cluster_positions(c);
         ++rs.rebuilds;
         c.steps_since_rebuild = 1;
         for (auto &v : c.box_since) v += 1;

@@ -265,7 +265,8 @@ __global__ void k_move_slots(int64_t nslots, int32_t M, const int32_t *__restric
                              const int64_t *__restrict__ cbase, const int64_t *__restrict__ src_cbase,
                              const int64_t *__restrict__ orig_delta, StateView o, StateView n,
                              const float4 *__restrict__ pos32_old, float4 *__restrict__ pos32,
-                             float4 *__restrict__ pos32_next) {
+                             float4 *__restrict__ pos32_next, const double4 *__restrict__ posd_old,
+                             double4 *__restrict__ posd, double4 *__restrict__ posd_next) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
   const int32_t k = cluster_box[s / M];
@@ -280,6 +281,9 @@ __global__ void k_move_slots(int64_t nslots, int32_t M, const int32_t *__restric
     const float4 p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
     pos32[s] = p;
     pos32_next[s] = p;
+    const double4 d = make_double4(kDummyPos, kDummyPos, kDummyPos, 0.);
+    posd[s] = d;
+    posd_next[s] = d;
     return;
   }
   const int64_t f = s + (sc - cbase[k]) * M;
@@ -295,6 +299,9 @@ __global__ void k_move_slots(int64_t nslots, int32_t M, const int32_t *__restric
   const float4 p = pos32_old[f];
   pos32[s] = p;
   pos32_next[s] = p;
+  const double4 d = posd_old[f];
+  posd[s] = d;
+  posd_next[s] = d;
 }
 
 // A wavefront per new cluster of a kept box: its box, centre, tower and cluster-pair row from the old
@@ -395,6 +402,7 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   c.st[other].ensure((size_t)ns);
   // the moved layout goes to the second buffers, which are then swapped in (no allocation once they fit)
   DevBuf<float4> &p32 = c.pos32_alt, &p32n = c.pos32n_alt;
+  DevBuf<double4> &pd = c.posd_alt, &pdn = c.posdn_alt;
   DevBuf<double> &bb = c.bbox_alt;
   DevBuf<double4> &cc = c.c0_alt;
   DevBuf<int32_t> &cnt = c.lcount_alt, &ctw = c.ctower_alt;
@@ -402,6 +410,8 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   const int32_t cap = c.list_row;
   p32.ensure((size_t)ns);
   p32n.ensure((size_t)ns);
+  pd.ensure((size_t)ns);
+  pdn.ensure((size_t)ns);
   bb.ensure(6 * (size_t)ncl);
   cc.ensure((size_t)ncl);
   cnt.ensure((size_t)ncl * kJclSplit + 1);  // the size every build asks for, so a partial one keeps it
@@ -409,7 +419,7 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   rows.ensure((size_t)ncl * cap + 1);
   hipLaunchKernelGGL(k_move_slots, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, M, c.cluster_box.p,
                      c.box_cbase.p, d_tab, d_tab + 2 * nb, view(c.st[c.cur]), view(c.st[other]), c.pos32.p, p32.p,
-                     p32n.p);
+                     p32n.p, c.posd.p, pd.p, pdn.p);
   LJG_HIP(hipGetLastError());
   constexpr int per_block = kMoveThreads / kWave;
   hipLaunchKernelGGL(k_move_clusters, dim3(blocks_for(ncl, per_block)), dim3(kMoveThreads), 0, c.stream, ncl,
@@ -442,6 +452,8 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
   std::swap(c.pos32, p32);
   std::swap(c.pos32_next, p32n);
+  std::swap(c.posd, pd);
+  std::swap(c.posd_next, pdn);
   std::swap(c.bbox, bb);
   std::swap(c.c0, cc);
   std::swap(c.list_count, cnt);

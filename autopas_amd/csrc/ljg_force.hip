@@ -174,18 +174,37 @@ __device__ inline int group_value_slot(int l) {
 // lanes of a group stream the flat j sequence of their i-cluster's list (entry e, member b) LG at a time
 // and evaluate each j against the M i-particles.  Loads of the next j are issued before the pair math
 // of the current one.  Padding j-members (and lanes past the end of the list) sit at +3e30 and padding
-// i-members are moved to -3e30, so their r^2 overflows to +inf; the single unsigned compare of the bit
-// pattern, (bits(r^2) - 1) < bits(rc^2), is 0 < r^2 <= rc^2 and rejects them and the i == j self pair.
+// i-members are moved to -3e30, so their r^2 overflows to +inf and the bit-pattern test
+// (bits(r^2) - 1) < cut rejects them and the i == j self pair (r^2 = +0 wraps to the largest value).
 // INTEG: the velocity-Verlet update in the epilogue, as in k_force_cluster_jrows (kick of this step, drift of
 // the next, FP64 in k_kick_drift's operation order); the next FP32 cluster-relative positions go to
-// ia.pos32_next, because other workgroups still read this step's.
+// ia.pos32_next and the next FP64 positions to ia.posd_next, because other workgroups still read this
+// step's.
+// The cutoff decision is the FP64 one, as in k_force_cluster_jrows: the FP32 r^2 is compared with two
+// thresholds around rc^2 (cut_lo, cut_hi), and a wave with any pair in between decides those pairs again
+// from the FP64 positions of this step (bd.posd) with the entry's periodic shift, exactly as
+// k_force_cluster_fp64 does (r^2 <= rc^2 with r^2 > 0).  The band is four times a bound on the FP32 error
+// of r^2, taken per wave from the largest cluster-relative offset since the last build (bd.ext) plus the
+// skin/2 a particle may move before the next rebuild: the separation x_q - (p_j + off) rounds each of
+// its four operands once and its two sums once, every operand below 2E + r for members within E of
+// their cluster's centre, so |error| <= 2^-24 (5E + 3r) per component.
+struct MixedBand {
+  const double4 *posd;             // FP64 positions of this step, per slot
+  const unsigned long long *ext;   // kAccSlots lines: the largest |pos32 component| (float bits)
+  double L[3];                     // box of a single-box context
+  const TowerGeom *btg;            // ensemble: per-box geometry (its L), else null
+  const int32_t *cluster_box;      // ensemble: box of each cluster
+  double rc2;
+  float rc, half_skin;
+};
+
 template <int M, int G, bool MIX, bool GLOB, bool INTEG>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
-    SkipIf sk, double *__restrict__ clg, IntegArgs ia) {
+    SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd) {
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
@@ -201,7 +220,20 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
   const int item = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
   int ic = item < nclusters ? (int)seg_index(sg, item) : -1;
   if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
-  const uint32_t rc2_bits = __float_as_uint(rc2);
+  // the FP64 band around rc^2 (uniform over the launch)
+  uint32_t cut_lo, cut_hi;
+  {
+    float E = 0.f;
+#pragma unroll
+    for (int k = 0; k < kAccSlots; ++k) E = fmaxf(E, __uint_as_float((uint32_t)bd.ext[k * kAccStride]));
+    E += bd.half_skin;
+    const float u = 5.9604645e-8f;  // 2^-24
+    const float rel = 4.f * (3.4641016f * u * (5.f * E + 3.f * bd.rc) / bd.rc + 3.f * u);
+    const uint32_t w = (uint32_t)fminf(ceilf(rel * 8388608.f), 1.0e6f);
+    const uint32_t rb = __float_as_uint(rc2);
+    cut_lo = rb - w;
+    cut_hi = rb + w;
+  }
   double e_out = 0., v_out = 0.;
   int64_t h_out = 0;
   if (ic >= 0) {
@@ -219,6 +251,11 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     uint32_t real = 0;  // bit q set for the i-cluster's real (non-padding) members
 #pragma unroll
     for (int q = 0; q < M; ++q) real |= (xi[q] == -kPadPos32 ? 0u : 1u) << q;
+    double Lx = bd.L[0], Ly = bd.L[1], Lz = bd.L[2];
+    if (bd.btg) {
+      const TowerGeom &g = bd.btg[bd.cluster_box[ic]];
+      Lx = g.L[0]; Ly = g.L[1]; Lz = g.L[2];
+    }
     float ax[M], ay[M], az[M], ae[M], av[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
@@ -246,11 +283,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         e = list[beg + (uint32_t)kc / M];
         p = pos32[(int64_t)(e.x & kClusterMask) * M + (kc & (M - 1))];
       }
-#pragma unroll
-      for (int q = 0; q < M; ++q) {
-        const float dx = xi[q] - xj, dy = yi[q] - yj, dz = zi[q] - zj;
-        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        const bool m = (__float_as_uint(r2) - 1u) < rc2_bits;
+      auto pair_term = [&](int q, float dx, float dy, float dz, float r2, bool m) {
         float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
         if constexpr (MIX) {
           const int t = 3 * (ti[q] * ntypes + tj);
@@ -269,6 +302,58 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         if constexpr (GLOB) {
           ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
           av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);  // r2 is +inf for padding
+        }
+      };
+      // pairs in the band around rc^2 are left out here; anyb collects the lanes that hold one
+      unsigned long long anyb = 0ull;
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        const float dx = xi[q] - xj, dy = yi[q] - yj, dz = zi[q] - zj;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const uint32_t ub = __float_as_uint(r2) - 1u;
+        const bool m = ub < cut_lo;
+        anyb |= __builtin_amdgcn_ballot_w64(ub < cut_hi) & ~__builtin_amdgcn_ballot_w64(m);
+        pair_term(q, dx, dy, dz, r2, m);
+      }
+      if (anyb != 0ull) {
+        // rare: find the band pairs again (same operands, same r^2) and decide them from the FP64
+        // positions.  The j values pass through an empty asm and this lane's entry is loaded again, so
+        // that the loop keeps the registers it needs without the band (as in k_force_cluster_jrows).
+        float jx = xj, jy = yj, jz = zj;
+        asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz));
+        const int kc = valid ? base + l : nj - 1;
+        const int4 ec = list[beg + (uint32_t)kc / M];
+        const int64_t sj = (int64_t)(ec.x & kClusterMask) * M + (kc & (M - 1));
+        const int sid = (int)((uint32_t)ec.x >> kShiftBits);
+        const double4 pj = bd.posd[sj];
+        double xjd = pj.x + shift_component(sid, 0) * Lx;
+        double yjd = pj.y + shift_component(sid, 1) * Ly;
+        double zjd = pj.z + shift_component(sid, 2) * Lz;
+        uint32_t inside = 0u;
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
+          const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          const uint32_t ub = __float_as_uint(r2) - 1u;
+          const bool inband = ub >= cut_lo && ub < cut_hi;
+          if (__ballot(inband) == 0ull) continue;
+          int32_t qq = q;
+          asm volatile("" : "+v"(qq));
+          const double4 pi = bd.posd[(int64_t)ic * M + qq];
+          const double drx = pi.x - xjd, dry = pi.y - yjd, drz = pi.z - zjd;
+          const double dr2 = drx * drx + dry * dry + drz * drz;
+          if (inband && dr2 <= bd.rc2 && dr2 > 0.) inside |= 1u << q;
+          asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(xjd), "+v"(yjd), "+v"(zjd), "+v"(inside));
+        }
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const bool in = (inside >> q) & 1u;
+          if (__ballot(in) != 0ull) {
+            const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
+            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            pair_term(q, dx, dy, dz, r2, in);
+          }
+          asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(ax[q]), "+v"(ay[q]), "+v"(az[q]));
         }
       }
     }
@@ -365,6 +450,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
             if (ia.box_flag) ia.box_flag[ia.cluster_box[i / M]] = 1;
           }
           ia.pos32_next[i] = make_float4((float)(x - cc.x), (float)(y - cc.y), (float)(z - cc.z), __int_as_float(t));
+          ia.posd_next[i] = make_double4(x, y, z, 0.);
         }
       }
     }
@@ -836,12 +922,23 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
     clg = c.cl_glob.p;
   }
   const IntegArgs ia = integ ? *integ : IntegArgs{};
+  MixedBand bd{};
+  bd.posd = c.posd.p;
+  bd.ext = c.pos32_ext.p;
+  for (int d = 0; d < 3; ++d) bd.L[d] = a.L[d];
+  if (c.nboxes > 0) {
+    bd.btg = c.box_tg.p;
+    bd.cluster_box = c.cluster_box.p;
+  }
+  bd.rc2 = a.rc2;
+  bd.rc = (float)std::sqrt(a.rc2);
+  bd.half_skin = (float)(0.5 * c.prm.skin);
 #define LJG_LAUNCH(MIXV, GLOBV, INTV)                                                                              \
   hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
                      items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,             \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
                      c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off, c.part_hits.p + off, c.hits_acc, c.skip, \
-                     clg, ia)
+                     clg, ia, bd)
   if (integ) {
     if (mix) LJG_LAUNCH(true, false, true); else LJG_LAUNCH(false, false, true);
   } else if (mix) {

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ljgpu.h"
@@ -168,6 +169,7 @@ struct IntegArgs {
   const double *rx, *ry, *rz;
   int4 *posfx_next;      // j-row kernel: next fixed-point positions
   float4 *pos32_next;    // FP32 cluster kernel: next cluster-relative positions (relative to c0)
+  double4 *posd_next;    // FP32 cluster kernel: next FP64 positions (the copy its cutoff band reads)
   const double4 *c0;
   int32_t *flag, *hflag;
   int32_t *box_flag;            // ensemble: set to 1 for the box whose particle raised the flag (else null)
@@ -275,6 +277,13 @@ struct Ctx {
   DevBuf<int4> list;
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
   DevBuf<float4> pos32_next;    // the fused integrator of the FP32 cluster kernel writes the next step's here
+  // FP64 positions of the same step as pos32 (x, y, z; w unused), kept in step with pos32 and swapped with
+  // it: the FP32 cluster kernel decides pairs near rc from them, while its fused integrator advances the
+  // state's x, y, z in place
+  DevBuf<double4> posd, posd_next;
+  // bound on |pos32| components since the last full build: kAccSlots lines of kAccStride u64, each the
+  // bits of a non-negative float (atomicMax); sets the width of the FP32 cluster kernel's cutoff band
+  DevBuf<unsigned long long> pos32_ext;
 
   // Verlet lists on the linked cells (fixed-point positions, transposed rows per wavefront)
   FixGeom fg{};
@@ -371,6 +380,7 @@ struct Ctx {
   size_t pin_dl_cap = 0;
   // second buffers an edit of the ensemble writes the moved layout into, then swaps in
   DevBuf<float4> pos32_alt, pos32n_alt;
+  DevBuf<double4> posd_alt, posdn_alt;
   DevBuf<double> bbox_alt;
   DevBuf<double4> c0_alt;
   DevBuf<int32_t> lcount_alt, ctower_alt;
@@ -471,6 +481,13 @@ __host__ __device__ inline int shift_component(int sid, int d) {
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// The FP32 cluster kernel's two position buffers change roles (its fused integrator wrote the next step's
+// positions into the second): the FP32 and the FP64 copies always together.
+inline void swap_cluster_positions(Ctx &c) {
+  std::swap(c.pos32, c.pos32_next);
+  std::swap(c.posd, c.posd_next);
+}
+
 // Thresholds on bits(r^2) - 1 (r^2 a positive FP32) around rc2 for the FP32-pair kernels that decide
 // the pairs near the cutoff again in FP64: below lo the pair is inside, from hi on outside.  rel is the
 // relative half-width of the band (1 ulp of rc2 is 2^-23 to 2^-24 of it).
@@ -490,6 +507,25 @@ constexpr int kAccSlots = 8;
 constexpr int kAccStride = 16;  // u64 per slot: one 128-byte line each
 __device__ inline void acc_add(unsigned long long *acc, unsigned long long v) {
   atomicAdd(acc + (blockIdx.x & (kAccSlots - 1)) * kAccStride, v);
+}
+
+// lines[l * kAccStride] = max(lines[...], v) over the workgroup's v >= 0 (as float bits, so the unsigned
+// order is the float order), with one atomic per workgroup at most, on the line of its XCD, and none when
+// the line already holds as much.  Every thread of the workgroup (blockDim.x a multiple of 64, at most
+// 1024) must call this.
+__device__ inline void block_max_to_lines(float v, unsigned long long *lines) {
+  __shared__ float wmax[16];
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+  const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.f;
+    for (int k = 0; k < nw; ++k) m = fmaxf(m, wmax[k]);
+    unsigned long long *line = lines + (blockIdx.x & (kAccSlots - 1)) * kAccStride;
+    const unsigned long long bits = (unsigned long long)__float_as_uint(m);
+    if (bits > *(volatile unsigned long long *)line) atomicMax(line, bits);
+  }
 }
 
 // Histogram rank with one atomic per run of equal keys inside the wavefront.  Particles arrive in the

@@ -205,8 +205,8 @@ struct MdArgs {
 };
 
 __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double4 *__restrict__ c0,
-                             float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ flag,
-                             int32_t *__restrict__ hflag, SkipIf sk) {
+                             float4 *__restrict__ pos32, double4 *__restrict__ posd, int4 *__restrict__ posfx,
+                             int32_t *__restrict__ flag, int32_t *__restrict__ hflag, SkipIf sk) {
   if (skip_launch(sk)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
@@ -238,6 +238,7 @@ __global__ void k_kick_drift(int64_t nslots, StateView s, MdArgs a, const double
   if (a.write_pos32) {
     const double4 r = c0[i / a.M];  // padding slots returned above and keep their sentinel positions
     pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
+    posd[i] = make_double4(x, y, z, 0.);
   }
   if (a.write_fixed) {
     posfx[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - a.fg.min[0]) * a.fg.inv_scale[0]),
@@ -268,7 +269,7 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp, Skip
     a.drift_fac[t] = dt * dt / (2 * m);
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p, c.host_flag_dev, skip);
+                     view(c.st[c.cur]), a, c.c0.p, c.pos32.p, c.posd.p, c.posfx.p, c.flag.p, c.host_flag_dev, skip);
   LJG_HIP(hipGetLastError());
 }
 
@@ -277,28 +278,36 @@ void md_kick_drift(Ctx &c, bool kick, bool drift, double dt, int32_t stamp, Skip
 // FP32 cluster-relative / fixed-point copies of the layout.
 __global__ void k_set_positions(int64_t nslots, StateView s, int64_t n, const double *__restrict__ pos,
                                 const double *__restrict__ vel, MdArgs a, const double4 *__restrict__ c0,
-                                float4 *__restrict__ pos32, int4 *__restrict__ posfx, int32_t *__restrict__ moved) {
+                                float4 *__restrict__ pos32, double4 *__restrict__ posd, unsigned long long *ext,
+                                int4 *__restrict__ posfx, int32_t *__restrict__ moved) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  const int32_t o = s.orig[i];
-  if (o < 0) return;
-  const double x = pos[o], y = pos[n + o], z = pos[2 * n + o];
-  s.x[i] = x; s.y[i] = y; s.z[i] = z;
-  if (vel) {
-    s.vx[i] = vel[o]; s.vy[i] = vel[n + o]; s.vz[i] = vel[2 * n + o];
+  const int32_t o = i < nslots ? s.orig[i] : -1;
+  float e = 0.f;
+  if (o >= 0) {
+    const double x = pos[o], y = pos[n + o], z = pos[2 * n + o];
+    s.x[i] = x; s.y[i] = y; s.z[i] = z;
+    if (vel) {
+      s.vx[i] = vel[o]; s.vy[i] = vel[n + o]; s.vz[i] = vel[2 * n + o];
+    }
+    const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
+    if (ex * ex + ey * ey + ez * ez >= a.half_skin2) moved[0] = 1;
+    const int t = s.type[i];
+    if (a.write_pos32) {
+      const double4 r = c0[i / a.M];
+      const float4 p = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
+      pos32[i] = p;
+      posd[i] = make_double4(x, y, z, 0.);
+      e = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
+    }
+    if (a.write_fixed) {
+      posfx[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - a.fg.min[0]) * a.fg.inv_scale[0]),
+                           (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - a.fg.min[1]) * a.fg.inv_scale[1]),
+                           (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - a.fg.min[2]) * a.fg.inv_scale[2]), t);
+    }
   }
-  const double ex = x - s.rx[i], ey = y - s.ry[i], ez = z - s.rz[i];
-  if (ex * ex + ey * ey + ez * ez >= a.half_skin2) moved[0] = 1;
-  const int t = s.type[i];
-  if (a.write_pos32) {
-    const double4 r = c0[i / a.M];
-    pos32[i] = make_float4((float)(x - r.x), (float)(y - r.y), (float)(z - r.z), __int_as_float(t));
-  }
-  if (a.write_fixed) {
-    posfx[i] = make_int4((int32_t)(uint32_t)(uint64_t)__double2ll_rn((x - a.fg.min[0]) * a.fg.inv_scale[0]),
-                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((y - a.fg.min[1]) * a.fg.inv_scale[1]),
-                         (int32_t)(uint32_t)(uint64_t)__double2ll_rn((z - a.fg.min[2]) * a.fg.inv_scale[2]), t);
-  }
+  // the caller may move particles farther than the step loop does: the cutoff band of the FP32 cluster
+  // kernel follows the largest cluster-relative offset
+  if (a.write_pos32) block_max_to_lines(e, ext);
 }
 
 void md_set_positions(Ctx &c, const double *pos, const double *vel) {
@@ -310,7 +319,8 @@ void md_set_positions(Ctx &c, const double *pos, const double *vel) {
   a.fg = c.fg;
   a.M = c.M;
   hipLaunchKernelGGL(k_set_positions, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots,
-                     view(c.st[c.cur]), c.n, pos, vel, a, c.c0.p, c.pos32.p, c.posfx.p, c.flag.p + 2);
+                     view(c.st[c.cur]), c.n, pos, vel, a, c.c0.p, c.pos32.p, c.posd.p, c.pos32_ext.p, c.posfx.p,
+                     c.flag.p + 2);
   LJG_HIP(hipGetLastError());
 }
 
@@ -327,6 +337,8 @@ IntegArgs integ_args(Ctx &c, bool kick, bool drift, double dt, int32_t stamp) {
   } else {
     c.pos32_next.ensure(c.nslots);
     a.pos32_next = c.pos32_next.p;
+    c.posd_next.ensure(c.nslots);
+    a.posd_next = c.posd_next.p;
   }
   a.c0 = c.c0.p;
   a.flag = c.flag.p;

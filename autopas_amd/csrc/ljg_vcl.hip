@@ -1013,46 +1013,67 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
   if (__ballot(frame_bad) != 0ull && lane == 0) stats[2] = 1ull;  // every writer stores the same value
 }
 
+__global__ void k_zero_u64(int64_t n, unsigned long long *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 // FP32 position relative to the cluster reference point, particle type bits in w.  Padding slots sit at
 // the sentinel +3e30 in every coordinate: any distance to them squares to +inf in FP32, which fails the
 // cutoff test, while the difference itself stays finite (so masked terms contribute exact zeros).
 // Written to both position buffers: the fused integrator of the FP32 cluster kernel writes the real
-// slots of the second one each step, and the padding slots must hold the sentinel in both.
+// slots of the second one each step, and the padding slots must hold the sentinel in both.  The FP64
+// positions go to posd (both buffers too), and the largest |component| of the real slots' FP32 positions
+// into the lines of ext.
 __global__ void k_pos32(int64_t nslots, Segs sg, int32_t M, const double *__restrict__ x, const double *__restrict__ y,
                         const double *__restrict__ z, const int32_t *__restrict__ type,
                         const int32_t *__restrict__ orig, const double4 *__restrict__ c0, float4 *__restrict__ pos32,
-                        float4 *__restrict__ pos32_next, const int32_t *__restrict__ cluster_box,
+                        float4 *__restrict__ pos32_next, double4 *__restrict__ posd, double4 *__restrict__ posd_next,
+                        unsigned long long *__restrict__ ext, const int32_t *__restrict__ cluster_box,
                         const int32_t *__restrict__ box_active) {
   const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= nslots) return;
-  const int64_t s = seg_index(sg, it);
-  if (box_active && box_active[cluster_box[s / M]] == 0) return;
-  float4 p;
-  if (orig[s] < 0) {
-    p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
-  } else {
-    const double4 r = c0[s / M];
-    p = make_float4((float)(x[s] - r.x), (float)(y[s] - r.y), (float)(z[s] - r.z), __int_as_float(type[s]));
+  float e = 0.f;
+  if (it < nslots) {
+    const int64_t s = seg_index(sg, it);
+    if (!box_active || box_active[cluster_box[s / M]] != 0) {
+      float4 p;
+      double4 d;
+      if (orig[s] < 0) {
+        p = make_float4(kPadPos32, kPadPos32, kPadPos32, __int_as_float(0));
+        d = make_double4(kDummyPos, kDummyPos, kDummyPos, 0.);
+      } else {
+        const double4 r = c0[s / M];
+        d = make_double4(x[s], y[s], z[s], 0.);
+        p = make_float4((float)(d.x - r.x), (float)(d.y - r.y), (float)(d.z - r.z), __int_as_float(type[s]));
+        e = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
+      }
+      pos32[s] = p;
+      pos32_next[s] = p;
+      posd[s] = d;
+      posd_next[s] = d;
+    }
   }
-  pos32[s] = p;
-  pos32_next[s] = p;
+  block_max_to_lines(e, ext);
 }
 
 void vcl_pos32(Ctx &c, const int32_t *active) {
   c.pos32.ensure(c.nslots);
   c.pos32_next.ensure(c.nslots);
+  c.posd.ensure(c.nslots);
+  c.posd_next.ensure(c.nslots);
+  c.pos32_ext.ensure(kAccSlots * kAccStride);
   State &s = c.st[c.cur];
   const int64_t ni = active ? c.seg_nslots : c.nslots;
   const Segs sg = active ? c.seg_slots : Segs{nullptr, nullptr, 0};
+  // a whole build starts the bound again; a partial one only raises it (the other boxes keep theirs)
+  if (!active)
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kAccSlots * kAccStride), 0, c.stream,
+                       (int64_t)(kAccSlots * kAccStride), c.pos32_ext.p);
   if (ni == 0) return;
   hipLaunchKernelGGL(k_pos32, dim3(blocks_for(ni, 256)), dim3(256), 0, c.stream, ni, sg, c.M, s.x.p, s.y.p, s.z.p,
-                     s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p, c.cluster_box.p, active);
+                     s.type.p, s.orig.p, c.c0.p, c.pos32.p, c.pos32_next.p, c.posd.p, c.posd_next.p, c.pos32_ext.p,
+                     c.cluster_box.p, active);
   LJG_HIP(hipGetLastError());
-}
-
-__global__ void k_zero_u64(int64_t n, unsigned long long *__restrict__ p) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
 }
 
 __global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {

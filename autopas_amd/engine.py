@@ -305,6 +305,7 @@ class Engine:
         self._check(self.lib.ljg_set_box(self.h, _ptr(lo), _ptr(hi)), "ljg_set_box")
         self.nboxes = 0
         self.n = 0
+        self._types = self._ids = None
 
     def set_boxes(self, box_min, box_max, counts):
         """Turn the context into an ensemble of independent periodic boxes: box b spans
@@ -319,6 +320,7 @@ class Engine:
         self.nboxes = len(cnt)
         self._counts = cnt.copy()
         self.n = 0
+        self._types = self._ids = None
 
     def edit_boxes(self, keep, box_min=None, box_max=None, counts=None, x=None, y=None, z=None, vx=None, vy=None,
                    vz=None, type=None, id=None):
@@ -336,10 +338,29 @@ class Engine:
             arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, vx, vy, vz)]
             t = None if type is None else np.ascontiguousarray(type, dtype=np.int32)
             i = None if id is None else np.ascontiguousarray(id, dtype=np.int64)
+        old_counts = getattr(self, "_counts", np.zeros(0, dtype=np.int64))
         self._check(self.lib.ljg_edit_boxes(self.h, len(k), _ptr(k) if len(k) else None, nadd, _ptr(lo), _ptr(hi),
                                             _ptr(cnt), *[_ptr(a) for a in arrs], _ptr(t), _ptr(i)), "ljg_edit_boxes")
         self.nboxes = len(k) + nadd
         self.n = int(self._box_counts(k, cnt).sum())
+        self._edit_host_types_ids(old_counts, k, cnt, t, i)
+
+    def _edit_host_types_ids(self, old_counts, keep, added, t, i):
+        """Host copies of types and ids in the new upload order, as ljg_edit_boxes arranges them: the kept
+        boxes' particles keep theirs, added particles take the given ones (type 0 and their new upload
+        index by default)."""
+        if getattr(self, "_types", None) is None or len(self._types) != int(np.sum(old_counts)):
+            self._types = self._ids = None
+            return
+        start = np.concatenate([[0], np.cumsum(old_counts)])
+        parts_t = [self._types[start[b]:start[b + 1]] for b in keep]
+        parts_i = [self._ids[start[b]:start[b + 1]] for b in keep]
+        kept = int(sum(len(p) for p in parts_t))
+        nadd = 0 if added is None else int(np.sum(added))
+        parts_t.append(np.zeros(nadd, dtype=np.int32) if t is None else t)
+        parts_i.append(np.arange(kept, kept + nadd, dtype=np.int64) if i is None else i)
+        self._types = np.concatenate(parts_t).astype(np.int32)
+        self._ids = np.concatenate(parts_i).astype(np.int64)
 
     def _box_counts(self, keep, added):
         old = getattr(self, "_counts", np.zeros(0, dtype=np.int64))
@@ -374,7 +395,8 @@ class Engine:
         t = None if type is None else np.ascontiguousarray(type, dtype=np.int32)
         i = None if id is None else np.ascontiguousarray(id, dtype=np.int64)
         n = len(arrs[0])
-        self._keep = arrs + [t, i]
+        self._types = np.zeros(n, dtype=np.int32) if t is None else t.copy()
+        self._ids = np.arange(n, dtype=np.int64) if i is None else i.copy()
         self._check(
             self.lib.ljg_upload(self.h, n, *[_ptr(a) for a in arrs], _ptr(t), _ptr(i)), "ljg_upload"
         )
@@ -420,8 +442,11 @@ class Engine:
 
     def write_checkpoint(self, directory, name, iteration, digits=None):
         """Write the current state (upload order, uploaded types and ids) as an md-flexible checkpoint."""
+        t, i = getattr(self, "_types", None), getattr(self, "_ids", None)
+        if t is None or len(t) != self.n or len(i) != self.n:
+            raise EngineError(3, "write_checkpoint: the host copies of types and ids do not describe the current "
+                                 "particles (upload them again)")
         st = self.download()
-        t, i = self._keep[6], self._keep[7]
         return write_checkpoint(directory, name, iteration, st, type=t, id=i, digits=digits)
 
     def load_checkpoint(self, path):

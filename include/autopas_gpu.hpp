@@ -409,6 +409,8 @@ class AutoPas {
     if (_rebuildDue || _stepsSinceRebuild >= _rebuildFrequency) {
       detail::check(_ctx, ljg_rebuild(_ctx, kernel), "AutoPas::computeInteractions: rebuild");
       _stepsSinceRebuild = 0;
+      _rebuildDue = false;
+      ++_numRebuilds;
     }
     f->initTraversal();
     ljg_globals g{};
@@ -417,8 +419,19 @@ class AutoPas {
     f->setTraversalResults(g);
     pullState(true);
     ++_stepsSinceRebuild;
+    ++_iterations;
     return false;
   }
+
+  /* AutoPas::getMeanRebuildFrequency (AutoPasDecl.h:1111, LogicHandler.h:633-646): iterations per list
+   * rebuild so far (computeInteractions calls and device steps), or the configured frequency before the
+   * first rebuild. */
+  double getMeanRebuildFrequency() const {
+    if (_numRebuilds == 0) return static_cast<double>(_rebuildFrequency);
+    return static_cast<double>(_iterations) / static_cast<double>(_numRebuilds);
+  }
+  /* List rebuilds so far (LogicHandler's _numRebuilds, LogicHandler.h:839,1102). */
+  size_t getNumRebuilds() const { return _numRebuilds; }
 
   /* The engine's own loop: nsteps velocity-Verlet steps on the device (TimeDiscretization.cpp:17-165 with
    * the rebuild policy of LogicHandler.h:987-1015), then the particles are brought back. */
@@ -432,6 +445,8 @@ class AutoPas {
     pullState(false);
     _stepsSinceRebuild = 0;
     _rebuildDue = false;
+    _numRebuilds += static_cast<size_t>(rs.rebuilds);
+    _iterations += static_cast<size_t>(rs.steps);
     return rs;
   }
 
@@ -568,6 +583,8 @@ class AutoPas {
   bool _structureChanged = true;
   bool _rebuildDue = true;
   unsigned int _stepsSinceRebuild = 0;
+  size_t _numRebuilds = 0;
+  size_t _iterations = 0;
   int32_t _shift = 0;
   std::vector<double> _eps, _sig, _mass;
 };

@@ -97,6 +97,12 @@ def test_compute_interactions_matches_direct_sum(driver, oracle, tmp_path):
 
 
 @pytest.mark.gpu
+def test_compute_interactions_rebuild_policy(driver):
+    announce("C++ API computeInteractions: rebuild every 10 calls and on a skin/2 move")
+    assert "ok rebuilds" in run(driver, "rebuilds")
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kernel,tol_x,tol_v", [(4, 1e-6, 1e-4), (2, 1e-6, 1e-4), (1, 1e-10, 1e-8)])
 def test_md_loop_through_api_matches_device_steps(driver, tmp_path, kernel, tol_x, tol_v):
     announce(f"C++ API md loop vs AutoPas::step: kernel {kernel}, 8^3 lattice, 25 steps")
