@@ -37,6 +37,7 @@ SOURCES = {
     "ljg_ensemble.hip": ["-ffp-contract=off"],
     "ljg_generate.cpp": ["-ffp-contract=off"],
     "ljg_checkpoint.cpp": [],
+    "ljg_noalloc.hip": ["-ffp-contract=off"],
 }
 
 

@@ -397,24 +397,7 @@ static void mix_types(Ctx &c, int32_t ntypes, const double *eps, const double *s
   lj.ntypes = ntypes;
   const double rc2 = c.prm.cutoff * c.prm.cutoff;
   lj.rc2 = rc2;
-  // ParticlePropertiesLibrary.h:443-473 and calcShift6 :575-582
-  for (int a = 0; a < ntypes; ++a) {
-    for (int b = 0; b < ntypes; ++b) {
-      const int k = ntypes * a + b;
-      const double e24 = 24 * std::sqrt(eps[a] * eps[b]);
-      const double s = (sig[a] + sig[b]) / 2.0;
-      const double s2 = s * s;
-      lj.eps24[k] = e24;
-      lj.sig2[k] = s2;
-      if (c.prm.apply_shift) {
-        const double q2 = s2 / rc2;
-        const double q6 = q2 * q2 * q2;
-        lj.shift6[k] = e24 * (q6 - q6 * q6);
-      } else {
-        lj.shift6[k] = 0.;
-      }
-    }
-  }
+  mix_table(ntypes, eps, sig, rc2, c.prm.apply_shift != 0, lj.eps24, lj.sig2, lj.shift6);
   for (int t = 0; t < kMaxTypes; ++t) {
     lj.mass[t] = (mass && t < ntypes) ? mass[t] : 1.;
     lj.inv2m[t] = 1. / (2. * lj.mass[t]);

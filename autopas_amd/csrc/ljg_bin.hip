@@ -6,39 +6,17 @@
 #include <cmath>
 #include <utility>
 
+#include "ljg_cells.h"
 #include "ljg_internal.h"
 
 namespace ljg {
-
-// CellBlock3D.h:320-349 and threeToOneD (ThreeDimensionalMapping.h:29-31).
-__device__ inline int32_t cell_index_1d(const CellGeom &g, double px, double py, double pz) {
-  const double p[3] = {px, py, pz};
-  int32_t c3[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const long value = (long)floor((p[d] - g.box_min[d]) * g.recip[d]) + g.cpil;
-    const long nonneg = value > 0 ? value : 0;
-    const long upper = g.nh[d] - 1;
-    long c = nonneg < upper ? nonneg : upper;
-    if (p[d] >= g.box_max[d]) {
-      const long first_upper_halo = g.nh[d] - g.cpil;
-      c = c > first_upper_halo ? c : first_upper_halo;
-    } else if (p[d] < g.box_min[d] && c == g.cpil) {
-      --c;
-    } else if (p[d] < g.box_max[d] && c == g.nh[d] - g.cpil) {
-      --c;
-    }
-    c3[d] = (int32_t)c;
-  }
-  return (c3[2] * g.nh[1] + c3[1]) * g.nh[0] + c3[0];
-}
 
 __global__ void k_cell_hist(int64_t nslots, const double *__restrict__ x, const double *__restrict__ y,
                             const double *__restrict__ z, const int32_t *__restrict__ orig, CellGeom g,
                             int32_t *__restrict__ cell_of, int32_t *__restrict__ rank,
                             int32_t *__restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < nslots && orig[i] >= 0;
+  const bool active = i < nslots && (orig == nullptr || orig[i] >= 0);
   const int32_t c = active ? cell_index_1d(g, x[i], y[i], z[i]) : -1;
   const int32_t r = wave_aggregated_rank(c, active, count);
   if (i < nslots) {
@@ -68,9 +46,9 @@ __global__ void k_cell_order_by_id(int64_t n, const int32_t *__restrict__ items,
   const int32_t v = items[k];
   const int32_t c = cell_of[v];
   const int64_t b = start[c], e = start[c + 1];
-  const int64_t key = id[v];
+  const int64_t key = id ? id[v] : v;  // without ids: the index
   int64_t r = 0;
-  for (int64_t m = b; m < e; ++m) r += id[items[m]] < key ? 1 : 0;
+  for (int64_t m = b; m < e; ++m) r += (id ? id[items[m]] : items[m]) < key ? 1 : 0;
   sorted[b + r] = v;
 }
 
@@ -89,24 +67,31 @@ __global__ void k_gather_cell(int64_t n, const int32_t *__restrict__ src, const 
   if (s < n) cell_new[s] = cell_old[src[s]];
 }
 
-void lc_bin(Ctx &c) {
-  // Geometry: CellBlock3D::rebuild with cellSizeFactor 1 (CellBlock3D.h:359-405).
-  CellGeom &g = c.cg;
-  const double ilen = c.prm.cutoff + c.prm.skin;
+CellGeom make_cell_geom(const double lo[3], const double hi[3], double ilen, int64_t *ncells) {
+  // CellBlock3D::rebuild with cellSizeFactor 1 (CellBlock3D.h:359-405)
+  CellGeom g{};
   g.cpil = 1;
-  int64_t ncells = 1;
+  int64_t nc = 1;
   for (int d = 0; d < 3; ++d) {
-    g.box_min[d] = c.prm.box_min[d];
-    g.box_max[d] = c.prm.box_max[d];
-    const double L = c.prm.box_max[d] - c.prm.box_min[d];
+    g.box_min[d] = lo[d];
+    g.box_max[d] = hi[d];
+    const double L = hi[d] - lo[d];
     g.L[d] = L;
     long cpd = (long)std::floor(L / (ilen * 1.0));
     if (cpd < 1) cpd = 1;
     g.nh[d] = (int32_t)(cpd + 2 * g.cpil);
     g.cell_len[d] = L / (double)cpd;
     g.recip[d] = (double)cpd / L;
-    ncells *= g.nh[d];
+    nc *= g.nh[d];
   }
+  *ncells = nc;
+  return g;
+}
+
+void lc_bin(Ctx &c) {
+  int64_t ncells = 0;
+  c.cg = make_cell_geom(c.prm.box_min, c.prm.box_max, c.prm.cutoff + c.prm.skin, &ncells);
+  CellGeom &g = c.cg;
   c.ncells = ncells;
   const int64_t ns = c.nslots;
   c.cell_of.ensure(ns);
@@ -140,6 +125,20 @@ void lc_bin(Ctx &c) {
   LJG_HIP(hipGetLastError());
   std::swap(c.cell_of, c.rank);
   c.layout = Layout::Cells;
+}
+
+void cell_bin_launch(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const CellGeom &g,
+                     int32_t *cell_of, int32_t *rank, int32_t *count, const int64_t *start, int32_t *items,
+                     const int64_t *id, int32_t *sorted, int phase) {
+  if (phase == 0) {
+    hipLaunchKernelGGL(k_cell_hist, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, x, y, z,
+                       (const int32_t *)nullptr, g, cell_of, rank, count);
+  } else {
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cell_of, rank, start, items);
+    hipLaunchKernelGGL(k_cell_order_by_id, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, items, cell_of, start, id,
+                       sorted);
+  }
+  LJG_HIP(hipGetLastError());
 }
 
 }  // namespace ljg

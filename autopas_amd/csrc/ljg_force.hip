@@ -872,8 +872,29 @@ static ForceArgs make_args(const Ctx &c) {
   return a;
 }
 
+void lc_force_launch(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                     const int32_t *type, const int32_t *cell_of, const int64_t *cell_start, const CellGeom &g,
+                     double rc2, int32_t ntypes, const double *tab, double eps24, double sig2, double shift6,
+                     double *fx, double *fy, double *fz, double *pe, double *pv, int64_t *ph, bool globals,
+                     unsigned long long *acc, SkipIf skip) {
+  ForceArgs a{};
+  for (int d = 0; d < 3; ++d) a.L[d] = g.L[d];
+  a.rc2 = rc2;
+  a.ntypes = ntypes;
+  a.eps24 = eps24;
+  a.sig2 = sig2;
+  a.shift6 = shift6;
+  const int nb = blocks_for(n, kFThreads);
+  if (ntypes > 1)
+    hipLaunchKernelGGL(k_force_lc_fp64<true>, dim3(nb), dim3(kFThreads), 0, st, n, x, y, z, type, cell_of,
+                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip);
+  else
+    hipLaunchKernelGGL(k_force_lc_fp64<false>, dim3(nb), dim3(kFThreads), 0, st, n, x, y, z, type, cell_of,
+                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip);
+  LJG_HIP(hipGetLastError());
+}
+
 void lc_force(Ctx &c, bool globals) {
-  const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
   const int nb = blocks_for(c.n, kFThreads);
@@ -883,15 +904,9 @@ void lc_force(Ctx &c, bool globals) {
   c.part_items = nb;
   State &s = c.st[c.cur];
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
-  if (mix)
-    hipLaunchKernelGGL(k_force_lc_fp64<true>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
-                       s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc, c.skip);
-  else
-    hipLaunchKernelGGL(k_force_lc_fp64<false>, dim3(nb), dim3(kFThreads), 0, c.stream, c.n, s.x.p, s.y.p, s.z.p,
-                       s.type.p, c.cell_of.p, c.cell_start.p, c.cg, a, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                       c.part_v.p, c.part_hits.p, (int32_t)globals, c.hits_acc, c.skip);
-  LJG_HIP(hipGetLastError());
+  lc_force_launch(c.stream, c.n, s.x.p, s.y.p, s.z.p, s.type.p, c.cell_of.p, c.cell_start.p, c.cg, c.lj.rc2,
+                  c.lj.ntypes, tab, c.lj.eps24[0], c.lj.sig2[0], c.lj.shift6[0], s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
+                  c.part_v.p, c.part_hits.p, globals, c.hits_acc, c.skip);
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -198,6 +199,29 @@ __device__ inline double wrap_coord(double p, int d, const WrapArgs &w) {
     return q > w.mn[d] ? q : w.mn[d];
   }
   return p;
+}
+
+// Lorentz-Berthelot mixing of per-type epsilon and sigma into ntypes x ntypes tables (row-major):
+// ParticlePropertiesLibrary.h:443-473 and calcShift6 :575-582.
+inline void mix_table(int32_t ntypes, const double *eps, const double *sig, double rc2, bool apply_shift,
+                      double *eps24, double *sig2, double *shift6) {
+  for (int a = 0; a < ntypes; ++a) {
+    for (int b = 0; b < ntypes; ++b) {
+      const int k = ntypes * a + b;
+      const double e24 = 24 * std::sqrt(eps[a] * eps[b]);
+      const double s = (sig[a] + sig[b]) / 2.0;
+      const double s2 = s * s;
+      eps24[k] = e24;
+      sig2[k] = s2;
+      if (apply_shift) {
+        const double q2 = s2 / rc2;
+        const double q6 = q2 * q2 * q2;
+        shift6[k] = e24 * (q6 - q6 * q6);
+      } else {
+        shift6[k] = 0.;
+      }
+    }
+  }
 }
 
 // LJ parameters (mixing tables are ntypes x ntypes, row-major).
@@ -396,6 +420,28 @@ struct Ctx {
 void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t n);  // out has n+1 entries
 
 void lc_bin(Ctx &c);                       // ljg_bin.hip
+// CellBlock3D geometry of the box [lo, hi) for interaction length ilen, cell size factor 1 (ljg_bin.hip)
+CellGeom make_cell_geom(const double lo[3], const double hi[3], double ilen, int64_t *ncells);
+// LinkedCells binning of n particles on explicit buffers (ljg_bin.hip): phase 0 = cell and rank of every
+// particle (count must be zero), phase 1 = scatter into the cells by rank and order each cell by id (index
+// when id is null) into sorted.
+void cell_bin_launch(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const CellGeom &g,
+                     int32_t *cell_of, int32_t *rank, int32_t *count, const int64_t *start, int32_t *items,
+                     const int64_t *id, int32_t *sorted, int phase);
+// The plain FP64 linked-cells kernel on explicit buffers (ljg_force.hip); block partials into pe/pv/ph
+// (blocks_for(n, 256) of them).
+void lc_force_launch(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                     const int32_t *type, const int32_t *cell_of, const int64_t *cell_start, const CellGeom &g,
+                     double rc2, int32_t ntypes, const double *tab, double eps24, double sig2, double shift6,
+                     double *fx, double *fy, double *fz, double *pe, double *pv, int64_t *ph, bool globals,
+                     unsigned long long *acc, SkipIf skip);
+// Exclusive scan on a given stream with caller-provided temporary storage of scan_tmp_len(n) entries.
+int64_t scan_tmp_len(int64_t n);  // ljg_md.hip
+void exclusive_scan_i32_to_i64_on(hipStream_t st, const int32_t *in, int64_t *out, int64_t n, int64_t *tmp,
+                                  int64_t tmp_len);
+// Deterministic sum of block partials into out[0] (energy x6), out[1] (virial), out_h[0] (pair hits).
+void reduce_launch(hipStream_t st, int64_t nitems, const double *pe, const double *pv, const int64_t *ph,
+                   double *out, int64_t *out_h, bool globals);  // ljg_md.hip
 void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
 void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v);  // ljg_vcl.hip: p[0..n) = v on the stream

@@ -57,31 +57,38 @@ __global__ void k_scan_add(int64_t *__restrict__ out, int64_t n_out, const int64
 }
 
 template <class TIn>
-static void scan_impl(Ctx &c, const TIn *in, int64_t n_in, int64_t *out, int64_t n_out, int64_t *scratch,
+static void scan_impl(hipStream_t st, const TIn *in, int64_t n_in, int64_t *out, int64_t n_out, int64_t *scratch,
                       int64_t scratch_len) {
   const int64_t nb = (n_out + kScanTile - 1) / kScanTile;
   if (scratch_len < 2 * (nb + 1)) throw HipError{hipErrorInvalidValue, "scan scratch too small", __LINE__};
   int64_t *tot = scratch, *tot_scan = scratch + (nb + 1);
-  hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, c.stream, in, n_in, n_out, out,
-                     tot);
+  hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n_in, n_out, out, tot);
   LJG_HIP(hipGetLastError());
   if (nb > 1) {
-    scan_impl<int64_t>(c, tot, nb, tot_scan, nb, scratch + 2 * (nb + 1), scratch_len - 2 * (nb + 1));
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n_out, 256)), dim3(256), 0, c.stream, out, n_out, tot_scan);
+    scan_impl<int64_t>(st, tot, nb, tot_scan, nb, scratch + 2 * (nb + 1), scratch_len - 2 * (nb + 1));
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n_out, 256)), dim3(256), 0, st, out, n_out, tot_scan);
     LJG_HIP(hipGetLastError());
   }
 }
 
-void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t n) {
-  const int64_t n_out = n + 1;
-  int64_t need = 0, m = n_out;
+int64_t scan_tmp_len(int64_t n) {
+  int64_t need = 0, m = n + 1;
   do {
     const int64_t nb = (m + kScanTile - 1) / kScanTile;
     need += 2 * (nb + 1);
     m = nb;
   } while (m > 1);
-  c.scan_tmp.ensure((size_t)need + 8);
-  scan_impl<int32_t>(c, in, n, out, n_out, c.scan_tmp.p, (int64_t)c.scan_tmp.cap);
+  return need + 8;
+}
+
+void exclusive_scan_i32_to_i64_on(hipStream_t st, const int32_t *in, int64_t *out, int64_t n, int64_t *tmp,
+                                  int64_t tmp_len) {
+  scan_impl<int32_t>(st, in, n, out, n + 1, tmp, tmp_len);
+}
+
+void exclusive_scan_i32_to_i64(Ctx &c, const int32_t *in, int64_t *out, int64_t n) {
+  c.scan_tmp.ensure((size_t)scan_tmp_len(n));
+  scan_impl<int32_t>(c.stream, in, n, out, n + 1, c.scan_tmp.p, (int64_t)c.scan_tmp.cap);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -492,6 +499,12 @@ void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stat
   int nb = blocks_for(n, kStatThreads * 16);
   nb = nb < 256 ? nb : 256;
   hipLaunchKernelGGL(k_count_stats, dim3(nb), dim3(kStatThreads), 0, c.stream, n, cnt, stats);
+  LJG_HIP(hipGetLastError());
+}
+
+void reduce_launch(hipStream_t st, int64_t nitems, const double *pe, const double *pv, const int64_t *ph,
+                   double *out, int64_t *out_h, bool globals) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, nitems, pe, pv, ph, out, out_h, (int32_t)globals);
   LJG_HIP(hipGetLastError());
 }
 

@@ -116,6 +116,7 @@ SIGNATURES = {
     "ljg_write_checkpoint": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i64, _i32, _i64] + [_p] * 11
                              + [ctypes.c_char_p, _i32]),
     "ljg_read_checkpoint": (_i32, [ctypes.c_char_p, _p] + [_p] * 11),
+    "ljg_force_lc_noalloc": (_i32, [ctypes.POINTER(Params), _i32, _p, _p, _i64] + [_p] * 10),
 }
 
 _lib = None

@@ -227,6 +227,31 @@ ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *clust
  * (offsets[n] is the total). */
 ljg_status ljg_get_verlet_lists(ljg_ctx *ctx, int64_t *offsets, int32_t *neighbors);
 
+/* ---- allocation-free force evaluation ------------------------------------------------------------ */
+/* Limits of ljg_force_lc_noalloc: its intermediates live in one static device array of
+ * LJG_NOALLOC_SCRATCH_BYTES per GPU (held by the library until the process ends). */
+#define LJG_NOALLOC_MAX_PARTICLES 524288
+#define LJG_NOALLOC_MAX_CELLS 524288 /* CellBlock3D cells including the halo layers */
+#define LJG_NOALLOC_SCRATCH_BYTES (48u << 20)
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the plain FP64 LinkedCells force
+ * evaluation of ljg_upload + ljg_force(LJG_KERNEL_LC_FP64, calc_globals) without a context and without any
+ * allocation.  x, y, z, type, id (inputs), fx, fy, fz and globals (outputs) are DEVICE pointers on the
+ * current HIP device; type (values in [0, ntypes), clamped) and id may be NULL, as in ljg_upload, and
+ * globals may be NULL (no energy, virial or pair count).  The box, cutoff, skin and apply_shift come from
+ * params (the other fields are ignored); ntypes types with per-type epsilon and sigma are mixed as
+ * ljg_set_types does (ntypes 1 with NULL arrays: epsilon = sigma = 1).  Positions are wrapped into the box as
+ * ljg_upload does, without changing the caller's arrays.  Everything is queued on `stream` (a hipStream_t,
+ * NULL for the default stream) and the call returns without waiting; fx, fy, fz and *globals are
+ * bit-identical to the context's results.  Returns LJG_ERR_CAPACITY when n > LJG_NOALLOC_MAX_PARTICLES or
+ * the box has more than LJG_NOALLOC_MAX_CELLS cells.  The scratch array is shared by every call on a GPU
+ * and keeps the last call's contents (the call clears what must start at zero): calls that overlap on one
+ * GPU (other streams or threads) race and must be serialised by the caller. */
+ljg_status ljg_force_lc_noalloc(const ljg_params *params, int32_t ntypes, const double *epsilon, const double *sigma,
+                                int64_t n, const double *x, const double *y, const double *z, const int32_t *type,
+                                const int64_t *id, double *fx, double *fy, double *fz, ljg_globals *globals,
+                                void *stream);
+
 /* ---- checkpoints (host side; md-flexible's .pvtu/.vtu format) --------------------------------- */
 /* Write dir/<name>_Particles_<iteration>.pvtu and dir/data/<name>_Particles_0_<iteration>.vtu, the
  * iteration zero-padded to `digits` (ParallelVtkWriter.cpp:60-200,309-358,438-442), with 17 significant
