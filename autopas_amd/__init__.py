@@ -6,8 +6,10 @@ this package only binds it.
 from .engine import (  # noqa: F401
     KERNEL_AUTO,
     KERNEL_CLUSTER_FP64,
+    KERNEL_CLUSTER_FP64_N3,
     KERNEL_CLUSTER_JROWS_MIXED,
     KERNEL_CLUSTER_MIXED,
+    KERNEL_CLUSTER_MIXED_N3,
     KERNEL_LC_FP64,
     KERNEL_VERLET_MIXED,
     KERNEL_NAMES,

@@ -80,13 +80,17 @@ __global__ void k_zero_i32(int64_t n, int32_t *p) {
   if (i < n) p[i] = 0;
 }
 
+static bool is_n3_kernel(int k) { return k == LJG_KERNEL_CLUSTER_FP64_N3 || k == LJG_KERNEL_CLUSTER_MIXED_N3; }
+
 static bool is_cluster_kernel(int k) {
-  return k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_MIXED || k == LJG_KERNEL_CLUSTER_JROWS_MIXED;
+  return k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_MIXED || k == LJG_KERNEL_CLUSTER_JROWS_MIXED ||
+         is_n3_kernel(k);
 }
 
 static void rebuild_for(Ctx &c, int kernel) {
   if (is_cluster_kernel(kernel)) {  // the tower histogram wraps the positions
     c.want_jrows = kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED;
+    c.want_n3 = is_n3_kernel(kernel);
     vcl_build(c);
   } else {
     wrap_positions(c);
@@ -188,6 +192,7 @@ static void rebuild_boxes(Ctx &c, int kernel, const std::vector<int32_t> &mark, 
   }
   if (!segments_ready && !prepare_segments(c, mark)) return;
   c.want_jrows = false;
+  c.want_n3 = false;
   c.build_partial = true;
   try {
     vcl_build(c);
@@ -219,7 +224,7 @@ static bool prepare_layout(Ctx &c, int kernel) {
 static bool layout_matches(const Ctx &c, int kernel) {
   if (is_cluster_kernel(kernel))
     return c.layout == Layout::Clusters && c.built_M == c.M &&
-           c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED);
+           c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) && c.lists_n3 == is_n3_kernel(kernel);
   if (kernel == LJG_KERNEL_VERLET_MIXED) return c.layout == Layout::Verlet;
   return c.layout == Layout::Cells;
 }
@@ -228,7 +233,8 @@ static bool layout_matches(const Ctx &c, int kernel) {
 // neighbour cells (box >= 3 ilen), the j-row cluster kernel on a single image within ilen (box > 2 ilen).
 static const char *kernel_unsupported(const Ctx &c, int kernel) {
   if (c.nboxes > 0 && kernel != LJG_KERNEL_CLUSTER_MIXED)
-    return "an ensemble of boxes runs the FP32 cluster kernel only";
+    return "an ensemble of boxes runs the FP32 cluster kernel only (without Newton-3: its boxes stay bit-identical "
+           "to each box alone)";
   const double ilen = c.prm.cutoff + c.prm.skin;
   for (int d = 0; d < 3; ++d) {
     const double L = c.prm.box_max[d] - c.prm.box_min[d];
@@ -245,7 +251,9 @@ static void run_force(Ctx &c, int kernel, bool globals) {
   if (kernel == LJG_KERNEL_LC_FP64) lc_force(c, globals);
   else if (kernel == LJG_KERNEL_VERLET_MIXED) verlet_force(c, globals);
   else if (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) cluster_jrows_force(c, globals);
-  else cluster_force(c, kernel == LJG_KERNEL_CLUSTER_MIXED, globals);
+  else
+    cluster_force(c, kernel == LJG_KERNEL_CLUSTER_MIXED || kernel == LJG_KERNEL_CLUSTER_MIXED_N3, globals, nullptr,
+                  nullptr, is_n3_kernel(kernel));
 }
 
 static int64_t candidate_evals(const Ctx &c, int kernel) {
@@ -377,6 +385,19 @@ ljg_status ljg_set_types(ljg_ctx *h, int32_t ntypes, const double *eps, const do
   mix_types(c, ntypes, eps, sig, mass);
   return LJG_OK;
   LJG_API_END
+}
+
+ljg_status ljg_set_layout(ljg_ctx *h, int32_t cluster_size, int32_t tile_groups, int32_t build_groups) {
+  if (!h) return LJG_ERR_ARG;
+  if (cluster_size != 1 && cluster_size != 2 && cluster_size != 4 && cluster_size != 8) return LJG_ERR_UNSUPPORTED;
+  if (tile_groups < 0 || tile_groups > 64 || build_groups < 0 || build_groups > 64) return LJG_ERR_ARG;
+  Ctx &c = h->c;
+  if (c.nboxes > 0 && cluster_size != c.M) return fail(c, LJG_ERR_UNSUPPORTED, "an ensemble keeps its cluster size");
+  c.M = cluster_size;
+  c.prm.cluster_size = cluster_size;
+  c.prm.tile_groups = tile_groups;
+  c.prm.build_groups = build_groups;
+  return LJG_OK;
 }
 
 ljg_status ljg_set_shift(ljg_ctx *h, int32_t apply_shift) {

@@ -198,17 +198,25 @@ struct MixedBand {
   float rc, half_skin;
 };
 
-template <int M, int G, bool MIX, bool GLOB, bool INTEG>
+// N3 (Newton-3 lists, single boxes): every listed pair is evaluated once and its force goes to both
+// particles (LJFunctor.h:316-318): each lane sums its j's reaction over the M i-members of the entry and
+// adds it to the j slot atomically; the i-members' sums are added atomically at the end (the force arrays
+// were cleared before the launch).  On the cluster's own entry (shift id 13) only member pairs b > q count,
+// as SoAFunctorSingle's triangle (LJFunctor.h:204-364).  Pair hits, energy and virial count twice, so that
+// the common halving of the full-list sums (LJFunctor::endTraversal) gives the pair values.
+template <int M, int G, bool MIX, bool GLOB, bool INTEG, bool N3>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
     SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd) {
+  static_assert(!(N3 && INTEG), "the fused update needs every pair's force first");
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
+  constexpr float kW = N3 ? 2.f : 1.f;  // weight of a hit in the pair count, energy and virial sums
   __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
   if constexpr (MIX) {
     for (int k = threadIdx.x; k < 3 * ntypes * ntypes; k += kFThreads) stab[k] = (float)tab64[k];
@@ -276,6 +284,16 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
       const float yj = p.y + __int_as_float(e.z);
       const float zj = p.z + __int_as_float(e.w);
       const int tj = __float_as_int(p.w);
+      // Newton-3: this lane's j slot, and whether its entry is the cluster's own (member pairs b > q only)
+      int64_t sj_n3 = 0;
+      bool own_n3 = false;
+      const int bm = (base + l) & (M - 1);
+      if constexpr (N3) {
+        const int jc = e.x & kClusterMask;
+        sj_n3 = (int64_t)jc * M + bm;
+        own_n3 = jc == ic && ((uint32_t)e.x >> kShiftBits) == 13u;
+      }
+      float fjx = 0.f, fjy = 0.f, fjz = 0.f;  // Newton-3: the reaction on j, summed over the i-members
       // issue the next loads before the pair math
       {
         const int kn = base + LG + l;
@@ -298,10 +316,15 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         ax[q] = __builtin_fmaf(dx, fac, ax[q]);
         ay[q] = __builtin_fmaf(dy, fac, ay[q]);
         az[q] = __builtin_fmaf(dz, fac, az[q]);
-        hits += m ? 1 : 0;
+        if constexpr (N3) {
+          fjx = __builtin_fmaf(dx, fac, fjx);
+          fjy = __builtin_fmaf(dy, fac, fjy);
+          fjz = __builtin_fmaf(dz, fac, fjz);
+        }
+        hits += m ? (N3 ? 2 : 1) : 0;
         if constexpr (GLOB) {
-          ae[q] += m ? __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
-          av[q] = __builtin_fmaf(m ? r2 : 0.f, fac, av[q]);  // r2 is +inf for padding
+          ae[q] += m ? kW * __builtin_fmaf(eps24, lj6 * lj6 - lj6, shift6) : 0.f;
+          av[q] = __builtin_fmaf(m ? kW * r2 : 0.f, fac, av[q]);  // r2 is +inf for padding
         }
       };
       // pairs in the band around rc^2 are left out here; anyb collects the lanes that hold one
@@ -311,8 +334,9 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         const float dx = xi[q] - xj, dy = yi[q] - yj, dz = zi[q] - zj;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const uint32_t ub = __float_as_uint(r2) - 1u;
-        const bool m = ub < cut_lo;
-        anyb |= __builtin_amdgcn_ballot_w64(ub < cut_hi) & ~__builtin_amdgcn_ballot_w64(m);
+        const bool tri = !N3 || !own_n3 || bm > q;
+        const bool m = ub < cut_lo && tri;
+        anyb |= __builtin_amdgcn_ballot_w64(ub < cut_hi && tri) & ~__builtin_amdgcn_ballot_w64(m);
         pair_term(q, dx, dy, dz, r2, m);
       }
       if (anyb != 0ull) {
@@ -335,7 +359,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
           const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
           const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
           const uint32_t ub = __float_as_uint(r2) - 1u;
-          const bool inband = ub >= cut_lo && ub < cut_hi;
+          const bool inband = ub >= cut_lo && ub < cut_hi && (!N3 || !own_n3 || bm > q);
           if (__ballot(inband) == 0ull) continue;
           int32_t qq = q;
           asm volatile("" : "+v"(qq));
@@ -354,6 +378,14 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
             pair_term(q, dx, dy, dz, r2, in);
           }
           asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(ax[q]), "+v"(ay[q]), "+v"(az[q]));
+        }
+      }
+      if constexpr (N3) {
+        // the reaction on this lane's j (padding and lanes past the row have none)
+        if (valid && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) {
+          unsafeAtomicAdd(&fx[sj_n3], -(double)fjx);
+          unsafeAtomicAdd(&fy[sj_n3], -(double)fjy);
+          unsafeAtomicAdd(&fz[sj_n3], -(double)fjz);
         }
       }
     }
@@ -384,13 +416,13 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
       for (int r = 0; r < R; ++r) {
         const int q = q0 + r;
         const double val = v[r];
-        // padding members are never stored
-        if (q < M) {
-          if (!INTEG && ((real >> q) & 1u)) fx[s0 + q] = val;
-        } else if (q < 2 * M) {
-          if (!INTEG && ((real >> (q - M)) & 1u)) fy[s0 + q - M] = val;
-        } else if (q < 3 * M) {
-          if (!INTEG && ((real >> (q - 2 * M)) & 1u)) fz[s0 + q - 2 * M] = val;
+        // padding members are never stored; Newton-3 adds to what other workgroups' reactions left
+        if (q < 3 * M) {
+          if (!INTEG && ((real >> (q % M)) & 1u)) {
+            double *dst = (q < M ? fx : q < 2 * M ? fy : fz) + s0 + q % M;
+            if constexpr (N3) unsafeAtomicAdd(dst, val);
+            else *dst = val;
+          }
         } else if (q == 3 * M) {
           h_out = (int64_t)val;
           if (GLOB && clg) clg[3 * (int64_t)ic + 2] = val;  // per-cluster sums for per-box globals
@@ -751,8 +783,9 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
 }
 
 // FP64 cluster kernel: same tiles (one i-cluster per wave), absolute FP64 positions with the periodic
-// shift of the entry, pair term exactly as LJFunctor.h:288-320.
-template <int M, bool MIX, bool GLOB>
+// shift of the entry, pair term exactly as LJFunctor.h:288-320.  N3: Newton-3 lists, as in
+// k_force_cluster_mixed (reaction on j added atomically, own entry's member pairs b > q only).
+template <int M, bool MIX, bool GLOB, bool N3>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
     int64_t nclusters, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const int32_t *__restrict__ type, const int32_t *__restrict__ orig, int32_t row,
@@ -795,6 +828,9 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
       const double zj = z[sj] + shift_component(sid, 2) * a.L[2];
       const int tj = type[sj];
       const bool rj = valid && orig[sj] >= 0;
+      const int bm = kk & (M - 1);
+      const bool own = N3 && (e.x & kClusterMask) == ic && sid == 13;
+      double fjx = 0., fjy = 0., fjz = 0.;
 #pragma unroll
       for (int q = 0; q < M; ++q) {
         const double drx = xi[q] - xj, dry = yi[q] - yj, drz = zi[q] - zj;
@@ -804,7 +840,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
           const int t = 3 * (ti[q] * a.ntypes + tj);
           eps24 = tab[t]; sig2 = tab[t + 1]; shift6 = tab[t + 2];
         }
-        const bool m = rj && ((real >> q) & 1u) && dr2 <= a.rc2 && dr2 > 0.;
+        const bool m = rj && ((real >> q) & 1u) && dr2 <= a.rc2 && dr2 > 0. && (!own || bm > q);
         const double invdr2 = 1. / dr2;
         const double lj2 = sig2 * invdr2;
         const double lj6 = lj2 * lj2 * lj2;
@@ -814,10 +850,23 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
         ax[q] += drx * fac;
         ay[q] += dry * fac;
         az[q] += drz * fac;
-        hits += m ? 1 : 0;
+        if constexpr (N3) {
+          fjx += drx * fac;
+          fjy += dry * fac;
+          fjz += drz * fac;
+        }
+        hits += m ? (N3 ? 2 : 1) : 0;
         if constexpr (GLOB) {
-          ae[q] += m ? eps24 * lj12m6 + shift6 : 0.;
-          av[q] += drx * (drx * fac) + dry * (dry * fac) + drz * (drz * fac);
+          const double w = N3 ? 2. : 1.;
+          ae[q] += m ? w * (eps24 * lj12m6 + shift6) : 0.;
+          av[q] += w * (drx * (drx * fac) + dry * (dry * fac) + drz * (drz * fac));
+        }
+      }
+      if constexpr (N3) {
+        if (rj && (fjx != 0. || fjy != 0. || fjz != 0.)) {
+          unsafeAtomicAdd(&fx[sj], -fjx);
+          unsafeAtomicAdd(&fy[sj], -fjy);
+          unsafeAtomicAdd(&fz[sj], -fjz);
         }
       }
     }
@@ -843,12 +892,12 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
     constexpr int KV = ilog2(NV);
     if ((lane & ((1 << (6 - KV)) - 1)) == 0) {
       const int64_t s0 = (int64_t)ic * M;
-      if (q < M) {
-        if ((real >> q) & 1u) fx[s0 + q] = v[0];
-      } else if (q < 2 * M) {
-        if ((real >> (q - M)) & 1u) fy[s0 + q - M] = v[0];
-      } else if (q < 3 * M) {
-        if ((real >> (q - 2 * M)) & 1u) fz[s0 + q - 2 * M] = v[0];
+      if (q < 3 * M) {
+        if ((real >> (q % M)) & 1u) {
+          double *dst = (q < M ? fx : q < 2 * M ? fy : fz) + s0 + q % M;
+          if constexpr (N3) unsafeAtomicAdd(dst, v[0]);
+          else *dst = v[0];
+        }
       } else if (q == 3 * M) {
         h_out = (int64_t)v[0];
       } else if (GLOB && q == 3 * M + 1) {
@@ -910,9 +959,20 @@ void lc_force(Ctx &c, bool globals) {
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }
 
+__global__ void k_zero_forces(int64_t n, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+                              SkipIf sk) {
+  if (skip_launch(sk)) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    fx[i] = 0.;
+    fy[i] = 0.;
+    fz[i] = 0.;
+  }
+}
+
 template <int M, int G>
 static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, const IntegArgs *integ,
-                         const ClusterSubset *sub) {
+                         const ClusterSubset *sub, bool n3) {
   constexpr int per_block = kWavesPerBlock * G;
   const Segs sg = sub ? sub->segs : Segs{nullptr, nullptr, 0};
   const int64_t items = sub ? sub->nitems : c.nclusters;
@@ -948,38 +1008,55 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   bd.rc2 = a.rc2;
   bd.rc = (float)std::sqrt(a.rc2);
   bd.half_skin = (float)(0.5 * c.prm.skin);
-#define LJG_LAUNCH(MIXV, GLOBV, INTV)                                                                              \
-  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream,    \
+#define LJG_LAUNCH(MIXV, GLOBV, INTV, N3V)                                                                         \
+  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV, N3V>), dim3(nb), dim3(kFThreads), 0, c.stream, \
                      items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,             \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
                      c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off, c.part_hits.p + off, c.hits_acc, c.skip, \
                      clg, ia, bd)
-  if (integ) {
-    if (mix) LJG_LAUNCH(true, false, true); else LJG_LAUNCH(false, false, true);
+  if (n3) {
+    State &s = c.st[c.cur];
+    hipLaunchKernelGGL(k_zero_forces, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.fx.p,
+                       s.fy.p, s.fz.p, c.skip);
+    if (mix) {
+      if (glob) LJG_LAUNCH(true, true, false, true); else LJG_LAUNCH(true, false, false, true);
+    } else {
+      if (glob) LJG_LAUNCH(false, true, false, true); else LJG_LAUNCH(false, false, false, true);
+    }
+  } else if (integ) {
+    if (mix) LJG_LAUNCH(true, false, true, false); else LJG_LAUNCH(false, false, true, false);
   } else if (mix) {
-    if (glob) LJG_LAUNCH(true, true, false); else LJG_LAUNCH(true, false, false);
+    if (glob) LJG_LAUNCH(true, true, false, false); else LJG_LAUNCH(true, false, false, false);
   } else {
-    if (glob) LJG_LAUNCH(false, true, false); else LJG_LAUNCH(false, false, false);
+    if (glob) LJG_LAUNCH(false, true, false, false); else LJG_LAUNCH(false, false, false, false);
   }
 #undef LJG_LAUNCH
 }
 
 template <int M>
-static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab) {
+static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const double *tab, bool n3) {
   const int nb = blocks_for(c.nclusters, kWavesPerBlock);
   c.part_e.ensure(nb);
   c.part_v.ensure(nb);
   c.part_hits.ensure(nb);
   c.part_items = nb;
   State &s = c.st[c.cur];
-#define LJG_LAUNCH(MIXV, GLOBV)                                                                                   \
-  hipLaunchKernelGGL((k_force_cluster_fp64<M, MIXV, GLOBV>), dim3(nb), dim3(kFThreads), 0, c.stream, c.nclusters, \
-                     s.x.p, s.y.p, s.z.p, s.type.p, s.orig.p, c.list_row, c.list_count.p, c.list.p, a, tab, s.fx.p,\
-                     s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
-  if (mix) {
-    if (glob) LJG_LAUNCH(true, true); else LJG_LAUNCH(true, false);
+#define LJG_LAUNCH(MIXV, GLOBV, N3V)                                                                              \
+  hipLaunchKernelGGL((k_force_cluster_fp64<M, MIXV, GLOBV, N3V>), dim3(nb), dim3(kFThreads), 0, c.stream,        \
+                     c.nclusters, s.x.p, s.y.p, s.z.p, s.type.p, s.orig.p, c.list_row, c.list_count.p, c.list.p, a, \
+                     tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
+  if (n3) {
+    hipLaunchKernelGGL(k_zero_forces, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.fx.p,
+                       s.fy.p, s.fz.p, c.skip);
+    if (mix) {
+      if (glob) LJG_LAUNCH(true, true, true); else LJG_LAUNCH(true, false, true);
+    } else {
+      if (glob) LJG_LAUNCH(false, true, true); else LJG_LAUNCH(false, false, true);
+    }
+  } else if (mix) {
+    if (glob) LJG_LAUNCH(true, true, false); else LJG_LAUNCH(true, false, false);
   } else {
-    if (glob) LJG_LAUNCH(false, true); else LJG_LAUNCH(false, false);
+    if (glob) LJG_LAUNCH(false, true, false); else LJG_LAUNCH(false, false, false);
   }
 #undef LJG_LAUNCH
 }
@@ -1042,9 +1119,12 @@ int64_t cluster_force_blocks(const Ctx &c, int64_t nitems) {
   return blocks_for(nitems, kWavesPerBlock * G);
 }
 
-void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, const ClusterSubset *sub) {
+void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, const ClusterSubset *sub, bool n3) {
   if (integ && (!mixed || globals)) throw HipError{hipErrorInvalidValue, "fused update needs the FP32 kernel", -2};
   if (sub && !mixed) throw HipError{hipErrorInvalidValue, "cluster subsets need the FP32 kernel", -2};
+  if (n3 && (integ || sub || c.nboxes > 0 || !c.lists_n3))
+    throw HipError{hipErrorInvalidValue, "Newton-3 runs on the Newton-3 lists of a single box, without fused update", -2};
+  if (!n3 && c.lists_n3) throw HipError{hipErrorInvalidValue, "Newton-3 lists need a Newton-3 kernel", -2};
   const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
@@ -1052,15 +1132,15 @@ void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, con
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
   if (!mixed) {
     switch (c.M) {
-      case 1: launch_fp64<1>(c, mix, globals, a, tab); break;
-      case 2: launch_fp64<2>(c, mix, globals, a, tab); break;
-      case 4: launch_fp64<4>(c, mix, globals, a, tab); break;
-      case 8: launch_fp64<8>(c, mix, globals, a, tab); break;
+      case 1: launch_fp64<1>(c, mix, globals, a, tab, n3); break;
+      case 2: launch_fp64<2>(c, mix, globals, a, tab, n3); break;
+      case 4: launch_fp64<4>(c, mix, globals, a, tab, n3); break;
+      case 8: launch_fp64<8>(c, mix, globals, a, tab, n3); break;
       default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", __LINE__};
     }
   } else {
 #define LJG_CASE(MV, GV) \
-  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ, sub); goto launched; }
+  if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ, sub, n3); goto launched; }
     LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4)
     LJG_CASE(4, 1) LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8)
     LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)

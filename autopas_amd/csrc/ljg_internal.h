@@ -321,6 +321,8 @@ struct Ctx {
   bool want_jrows = false;       // the next cluster build emits j-particle rows instead of cluster entries
   bool jrows_valid = false;      // the current cluster rows are j-particle rows
   bool jrows_frame_ok = false;   // every row pair within ilen has the same separation in the first member's frame
+  bool want_n3 = false;          // the next cluster build emits Newton-3 (forward-only) cluster-entry lists
+  bool lists_n3 = false;         // the current cluster-entry lists are Newton-3 lists
   DevBuf<int32_t> vcount;        // neighbours per particle
   DevBuf<int32_t> vlist;         // rows of vrow entries, transposed per wave of 64 particles
   int32_t vrow = 0, vcap = 0;
@@ -453,8 +455,10 @@ struct ClusterSubset {
   int64_t nitems;
   int64_t part_offset;
 };
+// n3: the lists are Newton-3 lists and each pair's force goes to both particles (the force arrays are
+// cleared first and every contribution is added atomically).  ljg_force.hip
 void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ = nullptr,
-                   const ClusterSubset *sub = nullptr);  // ljg_force.hip
+                   const ClusterSubset *sub = nullptr, bool n3 = false);
 int64_t cluster_force_blocks(const Ctx &c, int64_t nitems);  // ljg_force.hip: workgroups of a launch
 void verlet_build(Ctx &c);                 // ljg_verlet.hip (needs the cell layout)
 void verlet_fixed(Ctx &c);                 // ljg_verlet.hip

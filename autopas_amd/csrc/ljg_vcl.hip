@@ -490,6 +490,11 @@ __global__ void k_cluster_bbox(int64_t nitems, Segs sg, int32_t M, const double 
   }
 }
 
+// Newton-3 lists keep only forward entries: a higher j-cluster, or the i-cluster itself at shift id 13 (its
+// own member pairs) or above (one of each pair of opposite images); as the oracle's
+// ljo_vcl_forward_only, VerletClusterListsRebuilder.h:339-354.
+__device__ inline bool forward_entry(int64_t ci, int64_t cj, int sid) { return cj > ci || (cj == ci && sid >= 13); }
+
 __device__ inline double box_dist2(const double *a, const double *b, double shx, double shy, double shz) {
   const double sh[3] = {shx, shy, shz};
   double s = 0.;
@@ -566,7 +571,7 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
     int64_t ntowers, Segs sg, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ box_active, const int64_t *__restrict__ cstart,
     const int32_t *__restrict__ ccount, const double *__restrict__ bbox, const double4 *__restrict__ c0,
-    int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list, int32_t *__restrict__ jcl) {
+    int32_t cap, int32_t *__restrict__ count, int4 *__restrict__ list, int32_t *__restrict__ jcl, int32_t n3) {
   // the j-cluster index pass (JCL) needs neither the reference points nor as deep a chunk, and the smaller
   // LDS footprint lets more workgroups share a CU
   constexpr int LDSN = JCL ? kListLds / 2 : kListLds;
@@ -655,8 +660,9 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
                   ++cnt;
                 }
               } else {
+                const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                if (n3 && !forward_entry(ci, jb + k, sid)) continue;
                 if (cnt < subcap) {
-                  const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
                   const double4 b = sc[k];
                   const float dx = (float)((b.x + shx) - ai.x);
                   const float dy = (float)((b.y + shy) - ai.y);
@@ -683,7 +689,7 @@ __device__ inline int32_t visit_entries(int v, int side, const TowerGeom &g, int
                                         const double *bi, const double4 &ai, const int64_t *__restrict__ cstart,
                                         const int32_t *__restrict__ ccount, const double *__restrict__ bbox,
                                         const double4 *__restrict__ c0, int32_t cap, int32_t pos,
-                                        int4 *__restrict__ row) {
+                                        int4 *__restrict__ row, int64_t ci, int32_t n3) {
   const double il2 = g.ilen * g.ilen;
   const double zwin = g.ilen * (1.0 + 1e-9) + 1e-9;
   const int oy = v / side - g.R, ox = v % side - g.R;
@@ -716,8 +722,9 @@ __device__ inline int32_t visit_entries(int v, int side, const TowerGeom &g, int
       const double *bj = bbox + 6 * k;
       if (bj[2] > zmax) break;
       if (box_dist2(bi, bj, shx, shy, shz) <= il2) {
+        const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        if (n3 && !forward_entry(ci, k, sid)) continue;
         if (WRITE && pos + n < cap) {
-          const int sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
           const double4 b = c0[k];
           const float dx = (float)((b.x + shx) - ai.x);
           const float dy = (float)((b.y + shy) - ai.y);
@@ -745,7 +752,7 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
     const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount,
     const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
-    int4 *__restrict__ list) {
+    int4 *__restrict__ list, int32_t n3) {
   const int64_t it = (int64_t)blockIdx.x * kListLaneThreads + threadIdx.x;
   if (it >= nclusters) return;
   const int64_t ci = seg_index(sg, it);
@@ -776,7 +783,7 @@ __global__ __launch_bounds__(kListLaneThreads) void k_cluster_lists_lane(
   int4 *row = list + ci * (int64_t)cap;
   int32_t cnt = 0;
   for (int v = 0; v < nvis; ++v)
-    cnt += visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, cnt, row);
+    cnt += visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, cnt, row, ci, n3);
   count[ci] = cnt;
 }
 
@@ -794,7 +801,7 @@ __global__ __launch_bounds__(kListSplitThreads) void k_cluster_lists_split(
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ cluster_box, const int32_t *__restrict__ box_active,
     const int32_t *__restrict__ cluster_tower, const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount,
     const double *__restrict__ bbox, const double4 *__restrict__ c0, int32_t cap, int32_t *__restrict__ count,
-    int4 *__restrict__ list) {
+    int4 *__restrict__ list, int32_t n3) {
   const int sl = threadIdx.x & (kSplitLanes - 1);
   const int64_t it = ((int64_t)blockIdx.x * kListSplitThreads + threadIdx.x) / kSplitLanes;
   // every lane of a group holds the same i-cluster and the same decisions; no lane leaves before the scans
@@ -840,7 +847,8 @@ __global__ __launch_bounds__(kListSplitThreads) void k_cluster_lists_split(
     const int v = r * kSplitLanes + sl;
     const bool mine = work && v < nvis;
     const int32_t n =
-        mine ? visit_entries<false>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, 0, row) : 0;
+        mine ? visit_entries<false>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, 0, row, ci, n3)
+             : 0;
     int32_t inc = n;  // inclusive scan over the group
     for (int off = 1; off < kSplitLanes; off <<= 1) {
       const int32_t y = __shfl_up(inc, off, kSplitLanes);
@@ -848,7 +856,8 @@ __global__ __launch_bounds__(kListSplitThreads) void k_cluster_lists_split(
     }
     const int32_t total = __shfl(inc, kSplitLanes - 1, kSplitLanes);
     if (mine && n > 0)
-      (void)visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, base + inc - n, row);
+      (void)visit_entries<true>(v, side, g, tb, tx, ty, bi, ai, cstart, ccount, bbox, c0, cap, base + inc - n, row,
+                                ci, n3);
     base += total;
   }
   if (act && sl == 0) count[ci] = base;
@@ -1287,41 +1296,42 @@ void vcl_build(Ctx &c) {
     const int64_t *no_i64 = nullptr;
     const int32_t *no_i32 = nullptr;
     const int64_t n_lane = partial ? c.seg_nclusters : ncl;
+    const int32_t n3 = c.want_n3 && !jrows ? 1 : 0;
     if (n_towers == 0 || n_lane == 0)
       ;  // no box marked
     else if (jrows)
       hipLaunchKernelGGL((k_cluster_lists_tower<true, kJclSplit, false>), dim3((unsigned)ntow),
                          dim3(kListThreads * kJclSplit), 0, c.stream, ntow, none, g, no_tg, no_i64, no_i32, no_i32,
                          c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p, (int4 *)nullptr,
-                         c.jcl.p);
+                         c.jcl.p, (int32_t)0);
     else if (split_lists && ens)
       hipLaunchKernelGGL((k_cluster_lists_split<true>), dim3(blocks_for(n_lane * kSplitLanes, kListSplitThreads)),
                          dim3(kListSplitThreads), 0, c.stream, n_lane, sg_clusters, g, c.box_tg.p, c.box_tbase.p,
                          c.tower_box.p, c.cluster_box.p, active, c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p,
-                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p);
+                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, n3);
     else if (split_lists)
       hipLaunchKernelGGL((k_cluster_lists_split<false>), dim3(blocks_for(ncl * kSplitLanes, kListSplitThreads)),
                          dim3(kListSplitThreads), 0, c.stream, ncl, none, g, no_tg, no_i64, no_i32, no_i32, no_i32,
                          c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
-                         c.list.p);
+                         c.list.p, n3);
     else if (lane_lists && ens)
       hipLaunchKernelGGL((k_cluster_lists_lane<true>), dim3(blocks_for(n_lane, kListLaneThreads)),
                          dim3(kListLaneThreads), 0, c.stream, n_lane, sg_clusters, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, c.cluster_box.p, active,
                          c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
-                         c.list.p);
+                         c.list.p, n3);
     else if (lane_lists)
       hipLaunchKernelGGL((k_cluster_lists_lane<false>), dim3(blocks_for(ncl, kListLaneThreads)),
                          dim3(kListLaneThreads), 0, c.stream, ncl, none, g, no_tg, no_i64, no_i32, no_i32, no_i32,
                          c.cluster_tower.p, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p, cap, c.list_count.p,
-                         c.list.p);
+                         c.list.p, n3);
     else if (ens)
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, true>), dim3((unsigned)n_towers), dim3(kListThreads), 0,
                          c.stream, n_towers, sg_towers, g, c.box_tg.p, c.box_tbase.p, c.tower_box.p, active, c.ctow_start.p, c.ctow_count.p,
-                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
+                         c.bbox.p, c.c0.p, cap, c.list_count.p, c.list.p, (int32_t *)nullptr, n3);
     else
       hipLaunchKernelGGL((k_cluster_lists_tower<false, 1, false>), dim3((unsigned)ntow), dim3(kListThreads), 0, c.stream,
                          ntow, none, g, no_tg, no_i64, no_i32, no_i32, c.ctow_start.p, c.ctow_count.p, c.bbox.p, c.c0.p,
-                         cap, c.list_count.p, c.list.p, (int32_t *)nullptr);
+                         cap, c.list_count.p, c.list.p, (int32_t *)nullptr, n3);
     LJG_HIP(hipGetLastError());
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
     if (jrows) {
@@ -1401,6 +1411,7 @@ void vcl_build(Ctx &c) {
     break;
   }
   c.jrows_valid = jrows;
+  c.lists_n3 = c.want_n3 && !jrows;
   c.built_M = M;
   c.layout = Layout::Clusters;
   if (!jrows) vcl_pos32(c, active);

@@ -24,6 +24,8 @@ KERNEL_CLUSTER_FP64 = 1
 KERNEL_CLUSTER_MIXED = 2
 KERNEL_VERLET_MIXED = 3
 KERNEL_CLUSTER_JROWS_MIXED = 4
+KERNEL_CLUSTER_FP64_N3 = 5
+KERNEL_CLUSTER_MIXED_N3 = 6
 KERNEL_AUTO = 100
 MAX_TUNE_CANDIDATES = 16
 KERNEL_NAMES = {
@@ -32,6 +34,8 @@ KERNEL_NAMES = {
     KERNEL_CLUSTER_MIXED: "cluster_fp32pair_fp64acc",
     KERNEL_VERLET_MIXED: "verlet_fp32pair_fp64acc",
     KERNEL_CLUSTER_JROWS_MIXED: "cluster_jrows_fp32pair_fp64acc",
+    KERNEL_CLUSTER_FP64_N3: "cluster_fp64_newton3",
+    KERNEL_CLUSTER_MIXED_N3: "cluster_fp32pair_fp64acc_newton3",
 }
 
 _d = ctypes.c_double
@@ -101,6 +105,7 @@ SIGNATURES = {
     "ljg_update_positions": (_i32, [_p] + [_p] * 7),
     "ljg_rebuild": (_i32, [_p, _i32]),
     "ljg_set_shift": (_i32, [_p, _i32]),
+    "ljg_set_layout": (_i32, [_p, _i32, _i32, _i32]),
     "ljg_bin": (_i32, [_p]),
     "ljg_build_pairs": (_i32, [_p]),
     "ljg_force": (_i32, [_p, _i32, _i32, ctypes.POINTER(Globals)]),
@@ -418,6 +423,10 @@ class Engine:
 
     def rebuild(self, kernel=KERNEL_CLUSTER_MIXED):
         self._check(self.lib.ljg_rebuild(self.h, kernel), "ljg_rebuild")
+
+    def set_layout(self, cluster_size, tile_groups=0, build_groups=0):
+        """Cluster size and tile shape of the next build (a tuner switching configurations)."""
+        self._check(self.lib.ljg_set_layout(self.h, cluster_size, tile_groups, build_groups), "ljg_set_layout")
 
     def set_shift(self, apply_shift):
         self._check(self.lib.ljg_set_shift(self.h, int(bool(apply_shift))), "ljg_set_shift")

@@ -19,20 +19,27 @@
  *   - the functor's per-pair and per-SoA entry points are not provided (the GPU evaluates the whole
  *     container); they throw "not implemented" like PairwiseFunctor's defaults (PairwiseFunctor.h:56-104);
  *   - step(n, dt) runs n velocity-Verlet steps on the device (the engine's own loop);
- *   - setKernel() picks the GPU layout (AutoPas picks containers and traversals through its tuner).
+ *   - setKernel() picks the GPU layout; setKernel(LJG_KERNEL_AUTO) lets the AutoTuner below pick it among the
+ *     allowed GPU configurations, as AutoPas picks containers and traversals.
  * Every error is thrown as autopas_amd::AutoPasException, as ExceptionHandler does by default
  * (ExceptionHandler.cpp:12).
  */
 #ifndef AUTOPAS_GPU_HPP
 #define AUTOPAS_GPU_HPP
 
+#include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
+#include <map>
+#include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -293,6 +300,215 @@ class LJFunctor {
   size_t _pairs = 0, _distanceCalls = 0;
 };
 
+/* ---- the GPU configurations in AutoPas's tuning space -------------------------------------------------
+ * AutoPas tunes over Configurations (container, traversal, load estimator, data layout, newton3, cell size
+ * factor; Configuration.h:26-171), the cartesian product of the allowed options filtered by compatibility
+ * (SearchSpaceGenerators.cpp:18-67).  The engine enters that space as new container and traversal option
+ * values, the Newton-3 option, and two GPU parameters: the precision of the pair term (the sums are FP64
+ * in every kernel) and the cluster size with the i-clusters per wavefront of the force kernel. */
+namespace options {
+/* ContainerOption / TraversalOption entries for the engine (ContainerOption.h, TraversalOption.h:25-199). */
+enum class ContainerOption { gpuLinkedCells, gpuVerletLists, gpuClusterLists };
+enum class TraversalOption { gpu_lc_full_shell, gpu_verlet_lists, gpu_cluster_tiles, gpu_cluster_jrows };
+/* autopas::Newton3Option (Newton3Option.h). */
+enum class Newton3Option { disabled, enabled };
+/* GPU parameter: precision of the pair term. */
+enum class PrecisionOption { fp64, fp32pair };
+
+inline const char *toString(ContainerOption o) {
+  switch (o) {
+    case ContainerOption::gpuLinkedCells: return "gpuLinkedCells";
+    case ContainerOption::gpuVerletLists: return "gpuVerletLists";
+    default: return "gpuClusterLists";
+  }
+}
+inline const char *toString(TraversalOption o) {
+  switch (o) {
+    case TraversalOption::gpu_lc_full_shell: return "gpu_lc_full_shell";
+    case TraversalOption::gpu_verlet_lists: return "gpu_verlet_lists";
+    case TraversalOption::gpu_cluster_tiles: return "gpu_cluster_tiles";
+    default: return "gpu_cluster_jrows";
+  }
+}
+}  // namespace options
+
+/* autopas::Configuration (Configuration.h:26-171) with the engine's parameters.  clusterSize and
+ * tileGroups are 0 where they do not apply (as the load estimator is "none" where it does not). */
+struct Configuration {
+  options::ContainerOption container = options::ContainerOption::gpuClusterLists;
+  options::TraversalOption traversal = options::TraversalOption::gpu_cluster_jrows;
+  options::Newton3Option newton3 = options::Newton3Option::disabled;
+  options::PrecisionOption precision = options::PrecisionOption::fp32pair;
+  int clusterSize = 0;
+  int tileGroups = 0;  // i-clusters per wavefront of the force kernel; 0: the engine's default
+
+  auto key() const { return std::make_tuple(container, traversal, newton3, precision, clusterSize, tileGroups); }
+  bool operator<(const Configuration &o) const { return key() < o.key(); }
+  bool operator==(const Configuration &o) const { return key() == o.key(); }
+  bool operator!=(const Configuration &o) const { return !(*this == o); }
+
+  /* The engine kernel of this configuration (LJG_KERNEL_*), -1 when there is none. */
+  int32_t kernel() const {
+    using namespace options;
+    const bool n3 = newton3 == Newton3Option::enabled, f64 = precision == PrecisionOption::fp64;
+    if (container == ContainerOption::gpuLinkedCells && traversal == TraversalOption::gpu_lc_full_shell)
+      return f64 && !n3 ? LJG_KERNEL_LC_FP64 : -1;
+    if (container == ContainerOption::gpuVerletLists && traversal == TraversalOption::gpu_verlet_lists)
+      return !f64 && !n3 ? LJG_KERNEL_VERLET_MIXED : -1;
+    if (container != ContainerOption::gpuClusterLists) return -1;
+    if (traversal == TraversalOption::gpu_cluster_tiles)
+      return f64 ? (n3 ? LJG_KERNEL_CLUSTER_FP64_N3 : LJG_KERNEL_CLUSTER_FP64)
+                 : (n3 ? LJG_KERNEL_CLUSTER_MIXED_N3 : LJG_KERNEL_CLUSTER_MIXED);
+    if (traversal == TraversalOption::gpu_cluster_jrows) return !f64 && !n3 ? LJG_KERNEL_CLUSTER_JROWS_MIXED : -1;
+    return -1;
+  }
+
+  /* Configuration::hasCompatibleValues (Configuration.h): a kernel exists, the cluster parameters are set
+   * exactly where they apply, and the tile shape is one the kernel is built for (0 = its default). */
+  bool hasCompatibleValues() const {
+    const int32_t k = kernel();
+    if (k < 0) return false;
+    const bool cl = container == options::ContainerOption::gpuClusterLists;
+    if (!cl) return clusterSize == 0 && tileGroups == 0;
+    if (clusterSize != 1 && clusterSize != 2 && clusterSize != 4 && clusterSize != 8) return false;
+    if (tileGroups == 0) return true;
+    if (k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_FP64_N3) return false;  // one i-cluster per wave
+    static const std::map<std::pair<int, int>, std::set<int>> tiles = {
+        {{LJG_KERNEL_CLUSTER_MIXED, 8}, {1, 2, 4}},        {{LJG_KERNEL_CLUSTER_MIXED, 4}, {1, 2, 4, 8}},
+        {{LJG_KERNEL_CLUSTER_MIXED, 2}, {1, 2, 4, 8, 16}}, {{LJG_KERNEL_CLUSTER_MIXED, 1}, {4, 8, 16, 32, 64}},
+        {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 8}, {2, 4, 8}},  {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 4}, {4, 8, 16}},
+        {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 2}, {8, 16}},    {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 1}, {16, 32}}};
+    const int kk = k == LJG_KERNEL_CLUSTER_MIXED_N3 ? LJG_KERNEL_CLUSTER_MIXED : k;
+    const auto it = tiles.find({kk, clusterSize});
+    return it != tiles.end() && it->second.count(tileGroups) > 0;
+  }
+
+  /* Configuration::toShortString. */
+  std::string toString() const {
+    return std::string("{") + options::toString(container) + " , " + options::toString(traversal) + " , " +
+           (newton3 == options::Newton3Option::enabled ? "N3" : "noN3") + " , " +
+           (precision == options::PrecisionOption::fp64 ? "fp64" : "fp32pair") + " , M" + std::to_string(clusterSize) +
+           " , G" + std::to_string(tileGroups) + "}";
+  }
+};
+
+/* SearchSpaceGenerators::cartesianProduct (SearchSpaceGenerators.cpp:18-67): every combination of the
+ * allowed values that has compatible values; the cluster parameters collapse to 0 where they do not
+ * apply.  Throws when nothing is left, as the reference does. */
+inline std::set<Configuration> cartesianProduct(const std::set<options::ContainerOption> &containers,
+                                                const std::set<options::TraversalOption> &traversals,
+                                                const std::set<options::Newton3Option> &newton3,
+                                                const std::set<options::PrecisionOption> &precisions,
+                                                const std::set<int> &clusterSizes, const std::set<int> &tileGroups) {
+  std::set<Configuration> space;
+  for (auto c : containers)
+    for (auto t : traversals)
+      for (auto n : newton3)
+        for (auto p : precisions)
+          for (int m : clusterSizes)
+            for (int g : tileGroups) {
+              Configuration cfg{c, t, n, p, m, g};
+              if (c != options::ContainerOption::gpuClusterLists) cfg.clusterSize = cfg.tileGroups = 0;
+              if (cfg.hasCompatibleValues()) space.insert(cfg);
+            }
+  if (space.empty()) throw AutoPasException("No valid configurations could be created.");
+  return space;
+}
+
+/* autopas::AutoTuner (AutoTuner.h:40-443) with the full-search strategy and the fastestAbs selector
+ * (AutoTunerInfo.h: maxSamples 3, tuningInterval 5000): a tuning phase starts every tuningInterval
+ * iterations; it takes the configurations from the back of the queue, maxSamples iterations each, and
+ * scores each by estimateRuntimeFromSamples (AutoTuner.cpp:360-369): the fastest rebuilding sample /
+ * rebuild frequency + the fastest traversal sample.  The lowest score is used until the next phase.
+ * Configurations the engine cannot run on this box are dropped (LogicHandler's reject loop,
+ * LogicHandler.h:1212-1230). */
+class AutoTuner {
+ public:
+  AutoTuner(std::set<Configuration> space, unsigned int maxSamples, unsigned int tuningInterval,
+            unsigned int rebuildFrequency)
+      : _space(std::move(space)),
+        _maxSamples(maxSamples == 0 ? 1 : maxSamples),
+        _tuningInterval(tuningInterval),
+        _rebuildFrequency(rebuildFrequency == 0 ? 1 : rebuildFrequency) {
+    if (_space.empty()) throw AutoPasException("AutoTuner: empty search space");
+    _current = *_space.rbegin();
+  }
+
+  /* AutoTuner::tuneConfiguration (AutoTuner.cpp:85-175) for the iteration about to run; returns whether a
+   * tuning phase is in progress (the iteration's measurement is then expected). */
+  bool tuneConfiguration(size_t iteration) {
+    if (!_isTuning && (iteration == 0 || (_tuningInterval > 0 && iteration - _phaseEnd >= _tuningInterval))) {
+      _isTuning = true;
+      ++_tuningPhase;
+      _queue.assign(_space.begin(), _space.end());  // the back is tested first (FILO)
+      _evidence.clear();
+      resetSamples();
+    }
+    if (!_isTuning) return false;
+    if (_traverse.size() >= _maxSamples) {
+      _evidence[_queue.back()] = estimate();
+      _queue.pop_back();
+      resetSamples();
+    }
+    if (_queue.empty()) {
+      selectBest(iteration);
+      return false;
+    }
+    _current = _queue.back();
+    return true;
+  }
+
+  /* AutoTuner::addMeasurement (AutoTuner.cpp:250-330): rebuild samples only from iterations that rebuilt. */
+  void addMeasurement(long rebuildNs, long traverseNs, bool rebuilt) {
+    if (!_isTuning) return;
+    if (rebuilt) _rebuild.push_back(rebuildNs);
+    _traverse.push_back(traverseNs);
+  }
+
+  /* The current configuration cannot run here: drop it from the space and the queue. */
+  void rejectConfiguration(const Configuration &cfg) {
+    _space.erase(cfg);
+    _queue.erase(std::remove(_queue.begin(), _queue.end(), cfg), _queue.end());
+    resetSamples();
+    if (_space.empty()) throw AutoPasException("AutoTuner: no configuration is applicable");
+    if (!_isTuning && cfg == _current) _current = *_space.rbegin();
+  }
+
+  const Configuration &getCurrentConfig() const { return _current; }
+  bool inTuningPhase() const { return _isTuning; }
+  size_t getTuningPhase() const { return _tuningPhase; }
+  const std::set<Configuration> &getSearchSpace() const { return _space; }
+  /* Effective time (ns) of every configuration scored in the last tuning phase. */
+  const std::map<Configuration, long> &getEvidence() const { return _evidence; }
+
+ private:
+  void resetSamples() {
+    _rebuild.clear();
+    _traverse.clear();
+  }
+  long estimate() const {
+    const long reb = _rebuild.empty() ? 0 : *std::min_element(_rebuild.begin(), _rebuild.end());
+    const long trav = *std::min_element(_traverse.begin(), _traverse.end());
+    return static_cast<long>(reb / _rebuildFrequency) + trav;
+  }
+  void selectBest(size_t iteration) {
+    _isTuning = false;
+    _phaseEnd = iteration;
+    auto best = std::min_element(_evidence.begin(), _evidence.end(),
+                                 [](const auto &a, const auto &b) { return a.second < b.second; });
+    if (best != _evidence.end()) _current = best->first;
+  }
+
+  std::set<Configuration> _space;
+  unsigned int _maxSamples, _tuningInterval, _rebuildFrequency;
+  std::vector<Configuration> _queue;
+  std::map<Configuration, long> _evidence;
+  std::vector<long> _rebuild, _traverse;
+  Configuration _current;
+  bool _isTuning = false;
+  size_t _tuningPhase = 0, _phaseEnd = 0;
+};
+
 /* autopas::AutoPas<Particle_T> (AutoPasDecl.h) for the Lennard-Jones path: owned particles in a periodic
  * box, kept host-side in insertion order and mirrored on the device. */
 template <class Particle_T>
@@ -326,8 +542,25 @@ class AutoPas {
   unsigned int getVerletClusterSize() const { return _clusterSize; }
 
   /* GPU layout (LJG_KERNEL_*); -1 (default): the j-row kernel when every box side exceeds 2 (rc + skin),
-   * else the FP32 cluster kernel.  LJG_KERNEL_AUTO applies to step() only. */
+   * else the FP32 cluster kernel.  LJG_KERNEL_AUTO: computeInteractions picks the configuration with the
+   * AutoTuner over the allowed options below, and step() with the engine's own tuning phases. */
   void setKernel(int32_t kernel) { _kernel = kernel; }
+
+  /* The tuning space of LJG_KERNEL_AUTO (AutoPas::setAllowedContainers / setAllowedTraversals /
+   * setAllowedNewton3Options, AutoPasDecl.h:800-1000, plus the GPU parameters), and the tuner's settings
+   * (setTuningInterval, setMaxSamples). */
+  void setAllowedContainers(const std::set<options::ContainerOption> &v) { _allowedContainers = v; }
+  void setAllowedTraversals(const std::set<options::TraversalOption> &v) { _allowedTraversals = v; }
+  void setAllowedNewton3Options(const std::set<options::Newton3Option> &v) { _allowedNewton3 = v; }
+  void setAllowedPrecisions(const std::set<options::PrecisionOption> &v) { _allowedPrecisions = v; }
+  void setAllowedClusterSizes(const std::set<int> &v) { _allowedClusterSizes = v; }
+  void setAllowedTileGroups(const std::set<int> &v) { _allowedTileGroups = v; }
+  void setTuningInterval(unsigned int v) { _tuningInterval = v; }
+  void setMaxSamples(unsigned int v) { _maxSamples = v; }
+  /* The configuration of the last computeInteractions (AutoPas::getCurrentConfigs). */
+  const Configuration &getCurrentConfig() const { return _applied; }
+  /* The tuner of LJG_KERNEL_AUTO (null before the first computeInteractions with it). */
+  const AutoTuner *getAutoTuner() const { return _tuner.get(); }
   int32_t getKernel() const { return _kernel; }
   void setDevice(int32_t device) { _device = device; }
 
@@ -398,7 +631,6 @@ class AutoPas {
     if (f->getCutoff() != _cutoff)
       throw AutoPasException("AutoPas::computeInteractions: the functor's cutoff differs from AutoPas's cutoff");
     applyFunctor(*f);
-    const int32_t kernel = forceKernel();
     const size_t n = _particles.size();
     if (n == 0) {
       f->initTraversal();
@@ -406,21 +638,61 @@ class AutoPas {
       return false;
     }
     pushState();
-    if (_rebuildDue || _stepsSinceRebuild >= _rebuildFrequency) {
-      detail::check(_ctx, ljg_rebuild(_ctx, kernel), "AutoPas::computeInteractions: rebuild");
-      _stepsSinceRebuild = 0;
-      _rebuildDue = false;
-      ++_numRebuilds;
-    }
-    f->initTraversal();
+    const bool autoTune = _kernel == LJG_KERNEL_AUTO;
+    bool tuning = false;
     ljg_globals g{};
-    detail::check(_ctx, ljg_force(_ctx, kernel, Functor::kCalculateGlobals ? 1 : 0, &g),
-                  "AutoPas::computeInteractions: ljg_force");
+    for (;;) {
+      int32_t kernel = forceKernel();
+      bool configChanged = false;
+      if (autoTune) {
+        // LogicHandler::selectConfiguration (LogicHandler.h:1169-1231): the tuner's configuration for this
+        // iteration; a new one rebuilds the lists (a new container and traversal)
+        if (!_tuner)
+          _tuner.reset(new AutoTuner(cartesianProduct(_allowedContainers, _allowedTraversals, _allowedNewton3,
+                                                      _allowedPrecisions, _allowedClusterSizes, _allowedTileGroups),
+                                     _maxSamples, _tuningInterval, _rebuildFrequency));
+        tuning = _tuner->tuneConfiguration(_iterations);
+        const Configuration cfg = _tuner->getCurrentConfig();
+        kernel = cfg.kernel();
+        configChanged = !_haveApplied || cfg != _applied;
+        if (configChanged) {
+          const int m = cfg.clusterSize > 0 ? cfg.clusterSize : (int)_clusterSize;
+          detail::check(_ctx, ljg_set_layout(_ctx, m, cfg.tileGroups, 0), "AutoPas::computeInteractions: layout");
+          _applied = cfg;
+          _haveApplied = true;
+        }
+      }
+      const auto t0 = std::chrono::steady_clock::now();
+      bool rebuilt = false;
+      if (configChanged || _rebuildDue || _stepsSinceRebuild >= _rebuildFrequency) {
+        const ljg_status st = ljg_rebuild(_ctx, kernel);
+        if (autoTune && st == LJG_ERR_UNSUPPORTED) {  // not applicable to this box: the reject loop
+          _tuner->rejectConfiguration(_applied);
+          _haveApplied = false;
+          continue;
+        }
+        detail::check(_ctx, st, "AutoPas::computeInteractions: rebuild");
+        _stepsSinceRebuild = 0;
+        _rebuildDue = false;
+        ++_numRebuilds;
+        rebuilt = true;
+      }
+      const auto t1 = std::chrono::steady_clock::now();
+      f->initTraversal();
+      detail::check(_ctx, ljg_force(_ctx, kernel, Functor::kCalculateGlobals ? 1 : 0, &g),
+                    "AutoPas::computeInteractions: ljg_force");
+      const auto t2 = std::chrono::steady_clock::now();
+      // LogicHandler.h:1309-1324: the iteration's rebuild and traversal times go to the tuner
+      if (tuning)
+        _tuner->addMeasurement(std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count(),
+                               std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count(), rebuilt);
+      break;
+    }
     f->setTraversalResults(g);
     pullState(true);
     ++_stepsSinceRebuild;
     ++_iterations;
-    return false;
+    return tuning;
   }
 
   /* AutoPas::getMeanRebuildFrequency (AutoPasDecl.h:1111, LogicHandler.h:633-646): iterations per list
@@ -585,6 +857,21 @@ class AutoPas {
   unsigned int _stepsSinceRebuild = 0;
   size_t _numRebuilds = 0;
   size_t _iterations = 0;
+  std::set<options::ContainerOption> _allowedContainers{options::ContainerOption::gpuLinkedCells,
+                                                         options::ContainerOption::gpuVerletLists,
+                                                         options::ContainerOption::gpuClusterLists};
+  std::set<options::TraversalOption> _allowedTraversals{
+      options::TraversalOption::gpu_lc_full_shell, options::TraversalOption::gpu_verlet_lists,
+      options::TraversalOption::gpu_cluster_tiles, options::TraversalOption::gpu_cluster_jrows};
+  std::set<options::Newton3Option> _allowedNewton3{options::Newton3Option::disabled, options::Newton3Option::enabled};
+  std::set<options::PrecisionOption> _allowedPrecisions{options::PrecisionOption::fp32pair};
+  std::set<int> _allowedClusterSizes{4, 8};
+  std::set<int> _allowedTileGroups{0};
+  unsigned int _tuningInterval = 5000;  // AutoTunerInfo.h
+  unsigned int _maxSamples = 3;
+  std::unique_ptr<AutoTuner> _tuner;
+  Configuration _applied{};
+  bool _haveApplied = false;
   int32_t _shift = 0;
   std::vector<double> _eps, _sig, _mass;
 };

@@ -42,7 +42,13 @@ typedef int ljg_status;
 #define LJG_KERNEL_CLUSTER_JROWS_MIXED 4 /* Verlet-cluster i-clusters against j-particle rows (the union of
                                       the members' neighbour lists), fixed-point positions, FP32 pair term,
                                       FP64 per-particle sums; boxes > 2 (rc + skin) */
-#define LJG_KERNEL_LAST 4
+#define LJG_KERNEL_CLUSTER_FP64_N3 5 /* LJG_KERNEL_CLUSTER_FP64 on Newton-3 lists: forward neighbours only, each
+                                      interacting cluster pair listed once (VerletClusterListsRebuilder.h:339-354),
+                                      each pair's force added to both particles (LJFunctor.h:316-318,
+                                      SoAFunctorPairImpl<newton3 = true> :386-559) through device atomics; single
+                                      boxes; not bit-reproducible run to run (the order of the additions) */
+#define LJG_KERNEL_CLUSTER_MIXED_N3 6 /* LJG_KERNEL_CLUSTER_MIXED on Newton-3 lists, as LJG_KERNEL_CLUSTER_FP64_N3 */
+#define LJG_KERNEL_LAST 6
 /* ljg_step only: tune the layout while stepping (AutoPas's tuning phases, AutoTuner.cpp:85-175) and use
  * the fastest; see ljg_params.tuning_samples / tuning_interval and ljg_get_tuning. */
 #define LJG_KERNEL_AUTO 100
@@ -173,6 +179,11 @@ ljg_status ljg_update_positions(ljg_ctx *ctx, const double *x, const double *y, 
 /* Rebuild the neighbour structure the given kernel runs on, now (LogicHandler::rebuildNeighborLists,
  * LogicHandler.h:1082-1090): wrap into the box, bin or cluster, build lists. */
 ljg_status ljg_rebuild(ljg_ctx *ctx, int32_t kernel);
+/* Cluster size and tile shape of the next build (VerletClusterLists clusterSize, ljg_params.cluster_size,
+ * tile_groups, build_groups): a tuner switching between GPU configurations (AutoTuner::tuneConfiguration,
+ * AutoTuner.cpp:85-175) changes them between force evaluations; a different cluster size rebuilds the
+ * lists at the next evaluation. */
+ljg_status ljg_set_layout(ljg_ctx *ctx, int32_t cluster_size, int32_t tile_groups, int32_t build_groups);
 /* Truncated-shifted energy on or off (LJFunctor's applyShift template argument, LJFunctor.h:39-44); the
  * mixing table is recomputed from the last ljg_set_types parameters. */
 ljg_status ljg_set_shift(ljg_ctx *ctx, int32_t apply_shift);

@@ -781,6 +781,35 @@ void ljo_vcl_get(const ljo_vcl *v, int64_t *slot_particle, int32_t *cluster_towe
   if (bbox) memcpy(bbox, v->bbox, v->num_clusters * 6 * sizeof(double));
 }
 
+/* Newton-3 lists: only forward neighbours (VerletClusterListsRebuilder.h:278-306,339-354,366-409), so that
+ * an interacting cluster pair is listed once instead of in both clusters' lists.  The reference orders
+ * towers by interaction cell and then tower index, and within a tower takes the clusters after A; in the
+ * periodic variant (no halo towers, neighbours reached through image shifts) the order must be antisymmetric
+ * under (A, B, s) <-> (B, A, -s), so an entry (J, s) of cluster I is forward when J > I (clusters are
+ * numbered tower by tower, z-sorted inside a tower: the reference's same-tower rule), or J == I and
+ * s >= 13 (shift ids s and 26 - s are opposite images; 13 is no shift, the cluster itself, whose member
+ * pairs a Newton-3 kernel takes once, as SoAFunctorSingle does, VCLClusterFunctor.h:56-72).  The order of
+ * the kept entries is that of the full list. */
+void ljo_vcl_forward_only(ljo_vcl *v) {
+  int64_t w = 0, prev = 0;
+  for (int64_t ci = 0; ci < v->num_clusters; ++ci) {
+    const int64_t b = prev, e = v->list_offsets[ci + 1];
+    prev = e;
+    v->list_offsets[ci] = w;
+    for (int64_t k = b; k < e; ++k) {
+      const int64_t cj = v->list_j[k];
+      const int sid = v->list_shift[k];
+      if (cj > ci || (cj == ci && sid >= 13)) {
+        v->list_j[w] = v->list_j[k];
+        v->list_shift[w] = v->list_shift[k];
+        ++w;
+      }
+    }
+  }
+  v->list_offsets[v->num_clusters] = w;
+  v->num_entries = w;
+}
+
 void ljo_vcl_free(ljo_vcl *v) {
   if (!v) return;
   free(v->slot_particle); free(v->cluster_tower); free(v->list_offsets);

@@ -111,6 +111,8 @@ void ljo_vcl_sizes(const ljo_vcl *v, int64_t *num_slots, int64_t *num_clusters, 
  * bbox[6*num_clusters] (xlo,ylo,zlo,xhi,yhi,zhi). */
 void ljo_vcl_get(const ljo_vcl *v, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
                  int32_t *list_j, int32_t *list_shift, double *bbox);
+/* Keep only the forward (Newton-3) entries of every list, in list order (VerletClusterListsRebuilder.h:339-354). */
+void ljo_vcl_forward_only(ljo_vcl *v);
 void ljo_vcl_free(ljo_vcl *v);
 
 /* ---- Verlet lists (VerletLists / SoAFunctorVerlet neighbour sets, VerletLists.h) ----------------

@@ -5,6 +5,7 @@
 //   autopas_api_test forces <file>     computeInteractions on a seeded lattice; writes x y z fx fy fz per
 //                                      particle and the globals, for comparison with the oracle
 //   autopas_api_test rebuilds          rebuild count of computeInteractions calls (LogicHandler's rule)
+//   autopas_api_test tune <file>       computeInteractions with the AutoTuner over the GPU configurations
 //   autopas_api_test loop <file>       md-flexible's step sequence through the API (host integrator,
 //                                      TimeDiscretization.cpp:17-165) next to AutoPas::step on the device;
 //                                      writes both end states
@@ -13,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -249,6 +251,57 @@ int mode_rebuilds() {
   return 0;
 }
 
+// computeInteractions with LJG_KERNEL_AUTO: the AutoTuner runs one tuning phase over the GPU
+// configurations (cartesian product of the allowed options), maxSamples iterations each, then keeps the
+// configuration with the lowest effective time.  Writes the search space, the evidence, the chosen
+// configuration and the final state (positions and forces of the last call) for comparison with the oracle.
+int mode_tune(const char *path) {
+  FILE *out = std::fopen(path, "w");
+  if (!out) return fail("cannot open output");
+  namespace o = autopas_amd::options;
+  const Lattice s = lattice(12, 0.75, 9, 1);  // L = 13.2: every kernel applies (> 3 (rc + skin))
+  AutoPas<MoleculeLJ> ap;
+  setup(ap, s.L, LJG_KERNEL_AUTO);
+  ap.setAllowedContainers({o::ContainerOption::gpuLinkedCells, o::ContainerOption::gpuVerletLists,
+                           o::ContainerOption::gpuClusterLists});
+  ap.setAllowedPrecisions({o::PrecisionOption::fp64, o::PrecisionOption::fp32pair});
+  ap.setAllowedNewton3Options({o::Newton3Option::disabled, o::Newton3Option::enabled});
+  ap.setAllowedClusterSizes({4, 8});
+  ap.setMaxSamples(3);
+  ap.setTuningInterval(1000);
+  add_all(ap, s, false);
+  autopas_amd::LJFunctor<MoleculeLJ, true, false, FunctorN3Modes::Both, true> f(kCutoff);
+  f.setParticleProperties(24., 1.);
+  int tuningCalls = 0, calls = 0;
+  std::set<std::string> seen;
+  for (;; ++calls) {
+    for (auto &p : ap) p.setF({0., 0., 0.});
+    const bool tuning = ap.computeInteractions(&f);
+    seen.insert(ap.getCurrentConfig().toString());
+    if (!tuning) break;
+    ++tuningCalls;
+    if (calls > 200) return fail("the tuning phase did not end");
+  }
+  const auto *tuner = ap.getAutoTuner();
+  if (!tuner || tuner->inTuningPhase()) return fail("no tuner or still tuning");
+  const size_t space = tuner->getSearchSpace().size();
+  std::fprintf(out, "# space %zu calls %d tuning_calls %d seen %zu\n", space, calls + 1, tuningCalls, seen.size());
+  long best = -1;
+  for (const auto &kv : tuner->getEvidence()) {
+    std::fprintf(out, "# evidence %s %ld\n", kv.first.toString().c_str(), kv.second);
+    if (best < 0 || kv.second < best) best = kv.second;
+  }
+  const auto &chosen = ap.getCurrentConfig();
+  const auto it = tuner->getEvidence().find(chosen);
+  if (it == tuner->getEvidence().end() || it->second != best) return fail("the chosen configuration is not the fastest");
+  std::fprintf(out, "# chosen %s kernel %d\n", chosen.toString().c_str(), (int)chosen.kernel());
+  std::fprintf(out, "globals %.17g %.17g 0 0\n", f.getPotentialEnergy(), f.getVirial());
+  write_state(out, "final", ap);
+  std::fclose(out);
+  std::printf("ok tune\n");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -258,6 +311,7 @@ int main(int argc, char **argv) {
     if (mode == "errors") return mode_errors();
     if (mode == "nodevice") return mode_nodevice();
     if (mode == "rebuilds") return mode_rebuilds();
+    if (mode == "tune" && argc >= 3) return mode_tune(argv[2]);
     if (mode == "forces" && argc >= 3) return mode_forces(argv[2]);
     if (mode == "loop" && argc >= 3) return mode_loop(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
   } catch (const AutoPasException &e) {

@@ -74,6 +74,8 @@ def lib():
     L.ljo_vcl_get.argtypes = [_p] * 7
     L.ljo_vcl_free.restype = None
     L.ljo_vcl_free.argtypes = [_p]
+    L.ljo_vcl_forward_only.restype = None
+    L.ljo_vcl_forward_only.argtypes = [_p]
     L.ljo_verlet_pairs_fixed.restype = _i64
     L.ljo_verlet_pairs_fixed.argtypes = [_i64, _p, _p, _p, _p, _p, _d, _p, _p, _i64]
     L.ljo_verlet_positions.restype = None
@@ -180,12 +182,14 @@ def c08_forces_halo(x, y, z, cutoff, skin, box_min, box_max, types=None, eps=(1.
     return dict(fx=fx, fy=fy, fz=fz, epot=ep[0], virial=vi[0], pairs=int(pairs), traverse_seconds=ts[0])
 
 
-def vcl_periodic(x, y, z, ids, box_min, box_max, ilen, cluster_size):
+def vcl_periodic(x, y, z, ids, box_min, box_max, ilen, cluster_size, newton3=False):
     L = lib()
     idarr = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64)
     h = L.ljo_vcl_build_periodic(len(x), P(f64(x)), P(f64(y)), P(f64(z)), P(idarr), P(f64(box_min)),
                                  P(f64(box_max)), ilen, cluster_size)
     try:
+        if newton3:
+            L.ljo_vcl_forward_only(h)
         ns, nc, ne = (np.zeros(1, dtype=np.int64) for _ in range(3))
         tpd = np.zeros(2, dtype=np.int64)
         side = np.zeros(2)

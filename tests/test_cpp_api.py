@@ -40,6 +40,8 @@ def read_states(path):
     out, cur = {}, None
     with open(path) as f:
         for line in f:
+            if line.startswith("#"):
+                continue
             w = line.split()
             if w[0] in ("globals", "stats"):
                 out.setdefault(w[0], []).append([float(v) for v in w[1:]])
@@ -100,6 +102,35 @@ def test_compute_interactions_matches_direct_sum(driver, oracle, tmp_path):
 def test_compute_interactions_rebuild_policy(driver):
     announce("C++ API computeInteractions: rebuild every 10 calls and on a skin/2 move")
     assert "ok rebuilds" in run(driver, "rebuilds")
+
+
+@pytest.mark.gpu
+def test_compute_interactions_autotuner_picks_gpu_configuration(driver, oracle, tmp_path):
+    # LJG_KERNEL_AUTO: one tuning phase over the cartesian product of the allowed GPU options (3 samples per
+    # configuration), then the fastest configuration by the tuner's effective time; the forces of the call
+    # after the phase match the FP64 direct sum (FP32-pair tolerance: the choice may be an FP32-pair kernel)
+    announce("C++ API AutoTuner over the GPU configurations: 12^3 lattice")
+    path = str(tmp_path / "tune.txt")
+    run(driver, "tune", path, timeout=300)
+    lines = open(path).read().splitlines()
+    head = lines[0].split()
+    space, calls, tuning_calls, seen = int(head[2]), int(head[4]), int(head[6]), int(head[8])
+    ev = [ln for ln in lines if ln.startswith("# evidence")]
+    print("\n".join(ev + [ln for ln in lines if ln.startswith("# chosen")]), flush=True)
+    # LC fp64; Verlet fp32pair; cluster tiles fp64/fp32pair x N3 off/on x M 4/8; j-rows fp32pair x M 4/8
+    assert space == 1 + 1 + 8 + 2
+    assert len(ev) == space and seen == space
+    assert tuning_calls == 3 * space
+    st = read_states(path)
+    s = st["final"][np.argsort(st["final"][:, 0])]
+    x, y, z, f = s[:, 1], s[:, 2], s[:, 3], s[:, 7:10]
+    L = 12 * 0.75 ** (-1.0 / 3.0)
+    ref = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L))
+    fr = np.stack([ref["fx"], ref["fy"], ref["fz"]], axis=1)
+    assert rms_rel(f, fr) <= 1e-5
+    e, v = st["globals"][0][0], st["globals"][0][1]
+    assert abs(e - ref["epot"]) <= 1e-5 * abs(ref["epot"])
+    assert abs(v - ref["virial"]) <= 1e-5 * abs(ref["virial"])
 
 
 @pytest.mark.gpu

@@ -85,9 +85,28 @@ def test_cluster_lists_bit_exact(oracle, M, list_mode):
         assert np.array_equal(g[k], r[k]), k
 
 
-MIXED = (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED)
+@pytest.mark.parametrize("list_mode", [1, 2, 3])
+@pytest.mark.parametrize("M,n_side", [(1, 10), (2, 10), (4, 10), (8, 10), (4, 4), (8, 4)])
+def test_cluster_lists_newton3_bit_exact(oracle, M, n_side, list_mode):
+    # Newton-3 (forward-only) cluster-pair lists of the N3 kernels against the oracle's, entry by entry, in
+    # every list kernel; n_side 4 (L = 4.4 < 2 rc) has clusters meeting their own images
+    announce(f"Newton-3 cluster lists bit-exact M={M} list_mode={list_mode}: {n_side}^3 lattice rho 0.75")
+    s = lattice_box(n_side, 0.75, seed=7)
+    L = s["L"]
+    ids = np.arange(s["n"], dtype=np.int64)[::-1].copy()
+    with make_engine(L, M=M, list_mode=list_mode) as e:
+        e.upload(s["x"], s["y"], s["z"], id=ids)
+        e.rebuild(ap.KERNEL_CLUSTER_MIXED_N3)
+        g = e.clusters()
+    wx, wy, wz = oracle.wrap(s["x"], s["y"], s["z"], (0, 0, 0), (L, L, L))
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), (L, L, L), RC + SKIN, M, newton3=True)
+    for k in ("slot_particle", "cluster_tower", "list_offsets", "list_j", "list_shift", "bbox"):
+        assert np.array_equal(g[k], r[k]), k
+
+
+MIXED = (ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED, ap.KERNEL_CLUSTER_JROWS_MIXED, ap.KERNEL_CLUSTER_MIXED_N3)
 ALL_KERNELS = [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_VERLET_MIXED,
-               ap.KERNEL_CLUSTER_JROWS_MIXED]
+               ap.KERNEL_CLUSTER_JROWS_MIXED, ap.KERNEL_CLUSTER_FP64_N3, ap.KERNEL_CLUSTER_MIXED_N3]
 
 
 def test_verlet_lists_exact(oracle):
@@ -200,7 +219,8 @@ def test_fixed_point_kernels_reject_small_box(kernel):
         assert ei.value.status == 5
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED])
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_MIXED,
+                                    ap.KERNEL_CLUSTER_FP64_N3, ap.KERNEL_CLUSTER_MIXED_N3])
 def test_small_box_multiple_images(oracle, kernel):
     # L = 4.31 < 2 * (rc + skin): a particle sees several images of the same neighbour.
     announce(f"small box images kernel={ap.KERNEL_NAMES[kernel]}: 4^3 lattice rho 0.75, L=5.29")
@@ -212,7 +232,7 @@ def test_small_box_multiple_images(oracle, kernel):
         g = e.force(kernel, globals=True)
         f = e.download()
     ref = oracle.direct_sum(f["x"], f["y"], f["z"], RC, box_len=(L, L, L))
-    tol = 1e-5 if kernel == ap.KERNEL_CLUSTER_MIXED else 1e-10
+    tol = 1e-5 if kernel in MIXED else 1e-10
     assert rms_rel(f, ref) <= tol
     assert g.pair_count == ref["pairs"]
 
@@ -307,7 +327,8 @@ def test_md_trajectory_lc_fp64(oracle):
     assert np.max(np.abs(g["vx"] - vx)) < 1e-8
 
 
-@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_JROWS_MIXED])
+@pytest.mark.parametrize("kernel", [ap.KERNEL_LC_FP64, ap.KERNEL_CLUSTER_FP64, ap.KERNEL_CLUSTER_JROWS_MIXED,
+                                    ap.KERNEL_CLUSTER_FP64_N3, ap.KERNEL_CLUSTER_MIXED_N3])
 def test_dynamic_rebuild_trajectory(oracle, kernel):
     # Hot system, no periodic rebuilds: every rebuild comes from the skin/2 displacement trigger
     # (LogicHandler.h:1000-1015), whose step launches its force before the host has seen the trigger and
@@ -336,12 +357,13 @@ def test_dynamic_rebuild_trajectory(oracle, kernel):
     assert rs.rebuilds >= 3  # the initial build plus displacement-triggered ones
     d = np.stack([g["x"] - x, g["y"] - y, g["z"] - z])
     d -= L * np.round(d / L)
-    if kernel == ap.KERNEL_CLUSTER_JROWS_MIXED:
-        # FP32 pair term; the velocity-Verlet update runs in the force kernel's epilogue
+    if kernel in (ap.KERNEL_CLUSTER_JROWS_MIXED, ap.KERNEL_CLUSTER_MIXED_N3):
+        # FP32 pair term (the j-row kernel also runs the velocity-Verlet update in its epilogue)
         assert abs(rs.pair_interactions - pairs) <= 1e-4 * pairs
         assert np.max(np.abs(d)) < 2e-5
         assert np.max(np.abs(g["vx"] - vx)) < 1e-3
     else:
+        # FP64 (the Newton-3 kernel adds its pair forces atomically, in no fixed order: a few ulps)
         assert rs.pair_interactions == pairs
         assert np.max(np.abs(d)) < 1e-8
         assert np.max(np.abs(g["vx"] - vx)) < 1e-8

@@ -271,6 +271,55 @@ def test_vcl_lists_cover_every_pair_once(oracle, M):
                     assert (cluster_of[i], cluster_of[j], sid) in entries
 
 
+@pytest.mark.parametrize("M", [1, 2, 4, 8])
+@pytest.mark.parametrize("n_side,rho", [(6, 0.6), (3, 0.5)])
+def test_vcl_newton3_lists_cover_every_pair_once(oracle, M, n_side, rho):
+    # Newton-3 lists (forward neighbours only): every unordered interacting pair {(i, 0), (j, image)} within
+    # rc is covered by exactly one entry, either (cluster i -> cluster j, image) or (cluster j -> cluster i,
+    # -image), as VerletClusterListsTest.cpp:215-256 checks its N3 lists against the non-N3 ones; the entries
+    # are the full lists' forward entries in list order.  The 3^3 box (L = 4.05 < 2 rc) has clusters that
+    # meet their own images.
+    a = rho ** (-1 / 3)
+    L = n_side * a
+    x, y, z = _lattice(n_side, a, jitter=0.2, seed=M)
+    x, y, z = oracle.wrap(x, y, z, (0, 0, 0), (L, L, L))
+    rc, ilen = 2.5, 2.8
+    full = oracle.vcl_periodic(x, y, z, None, (0, 0, 0), (L, L, L), ilen, M)
+    r = oracle.vcl_periodic(x, y, z, None, (0, 0, 0), (L, L, L), ilen, M, newton3=True)
+    assert np.array_equal(r["slot_particle"], full["slot_particle"])
+    nc = len(r["cluster_tower"])
+    keep = []
+    for ci in range(nc):
+        for e in range(full["list_offsets"][ci], full["list_offsets"][ci + 1]):
+            cj, sid = int(full["list_j"][e]), int(full["list_shift"][e])
+            if cj > ci or (cj == ci and sid >= 13):
+                keep.append((ci, cj, sid))
+    got = [(ci, int(r["list_j"][e]), int(r["list_shift"][e]))
+           for ci in range(nc) for e in range(r["list_offsets"][ci], r["list_offsets"][ci + 1])]
+    assert got == keep
+    entries = set(got)
+    assert len(entries) == len(got)
+    slot = r["slot_particle"]
+    cluster_of = {int(p): s // M for s, p in enumerate(slot) if p >= 0}
+    n = len(x)
+    for i in range(n):
+        for j in range(i, n):
+            for sx, sy, sz in itertools.product((-1, 0, 1), repeat=3):
+                if i == j and sx == sy == sz == 0:
+                    continue
+                d = (x[i] - x[j] - sx * L, y[i] - y[j] - sy * L, z[i] - z[j] - sz * L)
+                if d[0] ** 2 + d[1] ** 2 + d[2] ** 2 > rc * rc:
+                    continue
+                sid = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)
+                ci, cj = cluster_of[i], cluster_of[j]
+                fwd = (ci, cj, sid) in entries
+                back = (cj, ci, 26 - sid) in entries
+                if ci == cj and sid == 13:
+                    assert fwd  # the cluster's own entry: its member pairs
+                else:
+                    assert fwd != back, (i, j, sid)
+
+
 def test_verlet_pairs_fixed_match_fp64_within_ilen(oracle):
     n_side, rho = 7, 0.7
     a = rho ** (-1 / 3)
