@@ -286,6 +286,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->build_groups < 0 || p->build_groups > 64) return LJG_ERR_ARG;
   if (p->list_mode < 0 || p->list_mode > 3) return LJG_ERR_ARG;
   if (p->shared_rebuilds < 0 || p->shared_rebuilds > 1) return LJG_ERR_ARG;
+  if (p->graphs < 0 || p->graphs > 1) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;
@@ -341,6 +342,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.pos32_alt.release(); c.pos32n_alt.release(); c.posd_alt.release(); c.posdn_alt.release(); c.bbox_alt.release(); c.c0_alt.release(); c.lcount_alt.release();
   c.ctower_alt.release();
   c.cl_glob.release(); c.box_glob.release();
+  if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.host_flag) (void)hipHostFree(c.host_flag);
   if (c.pin_mark) (void)hipHostFree(c.pin_mark);
   if (c.pin_seg) (void)hipHostFree(c.pin_seg);
@@ -853,6 +855,8 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   LJG_HIP(hipGetLastError());
   const bool saved_timing = c.timing;
   c.timing = false;  // per-launch events are recorded here instead
+  const bool graphs = c.prm.graphs != 0;
+  if (graphs) timing = false;  // no event records inside the captured chunks
   // clears both copies of the flag; every queued launch (a drift that may still raise it) has run first
   auto clear_flag = [&]() {
     stream_wait(c);
@@ -947,9 +951,52 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     if (k64 > nsteps - s) k64 = nsteps - s;
     if (k64 > kChunkSteps) k64 = kChunkSteps;
     const int32_t k = (int32_t)k64;
-    for (int32_t j = 0; j < k; ++j) {
-      drift_step(s + j, true);
-      force_step(s + j, true);
+    if (graphs && k > 0) {
+      // the chunk's launches as one graph: captured from the stream, then the executable graph of the
+      // previous chunk is updated with it when the launch sequence is the same, else instantiated anew
+      LJG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+      try {
+        for (int32_t j = 0; j < k; ++j) {
+          drift_step(s + j, true);
+          force_step(s + j, true);
+        }
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipGraph_t g = nullptr;
+      LJG_HIP(hipStreamEndCapture(c.stream, &g));
+      bool updated = false;
+      if (c.graph_exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(c.graph_exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+          (void)hipGetLastError();
+          LJG_HIP(hipGraphExecDestroy(c.graph_exec));
+          c.graph_exec = nullptr;
+        }
+      }
+      if (!updated) {
+        const hipError_t e = hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          c.graph_exec = nullptr;
+          throw HipError{e, "hipGraphInstantiate", __LINE__};
+        }
+        ++c.graph_instantiations;
+      }
+      const hipError_t e = hipGraphLaunch(c.graph_exec, c.stream);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) throw HipError{e, "hipGraphLaunch", __LINE__};
+      ++c.graph_launches;
+    } else {
+      for (int32_t j = 0; j < k; ++j) {
+        drift_step(s + j, true);
+        force_step(s + j, true);
+      }
     }
     const bool due = s + k < nsteps && steps_to_due() - k <= 0;
     // A scheduled rebuild before the force of step s + k is queued at once, after that step's drift: it
@@ -1138,8 +1185,9 @@ struct TuneCandidate {
 };
 static const TuneCandidate kTuneCandidates[] = {
     {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 8, 4}, {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 4, 4},
-    {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 8, 4}, {LJG_KERNEL_CLUSTER_MIXED, 4, 8, 0},
-    {LJG_KERNEL_CLUSTER_MIXED, 8, 2, 0},       {LJG_KERNEL_VERLET_MIXED, 4, 0, 0},
+    {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 8, 4}, {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 16, 4},
+    {LJG_KERNEL_CLUSTER_MIXED, 4, 8, 0},       {LJG_KERNEL_CLUSTER_MIXED, 8, 2, 0},
+    {LJG_KERNEL_CLUSTER_MIXED_N3, 8, 2, 0},    {LJG_KERNEL_VERLET_MIXED, 4, 0, 0},
 };
 constexpr int kNumTuneCandidates = (int)(sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]));
 static_assert(kNumTuneCandidates <= LJG_MAX_TUNE_CANDIDATES, "ljg_tuning holds every candidate");

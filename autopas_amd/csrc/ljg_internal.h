@@ -413,6 +413,9 @@ struct Ctx {
   // second stream: a partial rebuild and the force of its boxes run beside the force of the others
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // ljg_params.graphs: the executable graph of the last captured chunk of steps (updated or replaced per chunk)
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_launches = 0, graph_instantiations = 0;
   DevBuf<double> cl_glob;              // per i-cluster (energy x6, virial, pairs) of the last globals launch
   DevBuf<double> box_glob;             // per box (energy, virial, pairs) after endTraversal scaling
   bool box_glob_valid = false;

@@ -61,7 +61,7 @@ class Params(ctypes.Structure):
         ("tuning_interval", _i32),
         ("list_mode", _i32),
         ("shared_rebuilds", _i32),
-        ("reserved", _i32),
+        ("graphs", _i32),
     ]
 
 
@@ -236,7 +236,7 @@ class Engine:
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
-                 tuning_interval=0, list_mode=0, shared_rebuilds=False):
+                 tuning_interval=0, list_mode=0, shared_rebuilds=False, graphs=False):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -254,6 +254,7 @@ class Engine:
         p.tuning_interval = tuning_interval
         p.list_mode = list_mode
         p.shared_rebuilds = int(bool(shared_rebuilds))
+        p.graphs = int(bool(graphs))
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))

@@ -186,45 +186,99 @@ def run(sysd, kernel, cluster_size, steps, warmup, timing=True, tile_groups=0):
                 pairs_per_s=rs.pair_interactions / wall, device=name)
 
 
-# Candidate FP32-pair / FP64-accumulate layouts tried by the tuning pass: (kernel, cluster size,
-# i-clusters per wavefront).
-CANDIDATES = (
-    ("cluster", 4, 8), ("cluster", 8, 2), ("jrows", 4, 8), ("jrows", 4, 16), ("jrows", 8, 4), ("jrows", 8, 8),
-    ("verlet", 1, 0),
-)
-KERNEL_OF = {"cluster": "KERNEL_CLUSTER_MIXED", "jrows": "KERNEL_CLUSTER_JROWS_MIXED", "verlet": "KERNEL_VERLET_MIXED"}
+KERNEL_OF = {"cluster": "KERNEL_CLUSTER_MIXED", "jrows": "KERNEL_CLUSTER_JROWS_MIXED", "verlet": "KERNEL_VERLET_MIXED",
+             "clustern3": "KERNEL_CLUSTER_MIXED_N3"}
 
 
-def tune(sysd, steps=30):
+def layout_name(kernel, M, G):
     import autopas_amd as ap
 
-    res = {}
-    for kind, m, g in CANDIDATES:
-        kernel = getattr(ap, KERNEL_OF[kind])
-        r = run(sysd, kernel, max(m, 1), steps, 5, timing=False, tile_groups=g)
-        res[f"{kind}_M{m}_G{g}"] = (r["pairs_per_s"], kernel, max(m, 1), g)
-    best = max(res, key=lambda k: res[k][0])
-    return best, res
+    kind = {ap.KERNEL_CLUSTER_MIXED: "cluster", ap.KERNEL_CLUSTER_JROWS_MIXED: "jrows",
+            ap.KERNEL_VERLET_MIXED: "verlet", ap.KERNEL_CLUSTER_MIXED_N3: "clustern3"}[kernel]
+    return f"{kind}_M{M}_G{G}"
 
 
-def cpu_baseline(seconds):
+def run_tuned(sysd, steps, warmup, samples=10):
+    """The engine's own layout tuning (LJG_KERNEL_AUTO, AutoTuner.cpp:85-175 in the engine): one tuning phase
+    over its candidate layouts, `samples` steps of the real run each, scored as rebuild / rebuild frequency +
+    the rest of the step; then `warmup` steps and the timed `steps` with the chosen layout."""
+    import autopas_amd as ap
+
+    L = sysd["L"]
+    e = ap.Engine((0.0, 0.0, 0.0), (L, L, L), 2.5, 0.3, cluster_size=8, rebuild_frequency=10,
+                  dynamic_rebuild=True, apply_shift=True, tuning_samples=samples, tuning_interval=0)
+    try:
+        e.upload(sysd["x"], sysd["y"], sysd["z"], sysd["vx"], sysd["vy"], sysd["vz"])
+        ncand = len(e.tuning()["candidates"])
+        e.step(ncand * samples + warmup, 0.002, ap.KERNEL_AUTO)
+        t = e.tuning()
+        if t["tuning"] or t["best"] < 0:
+            raise RuntimeError("the tuning phase did not finish")
+        e.set_timing(True)
+        t0 = time.perf_counter()
+        rs = e.step(steps, 0.002, ap.KERNEL_AUTO)
+        t1 = time.perf_counter()
+        name = e.device_name()
+    finally:
+        e.close()
+    best = t["candidates"][t["best"]]
+    table = {layout_name(c["kernel"], c["cluster_size"], c["tile_groups"]): c["effective_ms"] for c in t["candidates"]
+             if c["effective_ms"] > 0}
+    wall = t1 - t0
+    res = dict(wall_s=wall, pairs=rs.pair_interactions, steps=rs.steps, rebuilds=rs.rebuilds, force_ms=rs.force_ms,
+               force_launches=rs.force_launches, rebuild_ms=rs.rebuild_ms, pairs_per_s=rs.pair_interactions / wall,
+               device=name)
+    return res, best["kernel"], best["cluster_size"], best["tile_groups"], table
+
+
+def host_cpu():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_baseline(seconds, layout_kernel=None, M=8, G=0):
     """FP64 LinkedCells + C08 + Newton-3 traversal (md-flexible's CPU path, restated in oracle/) on the
-    host cores, on a 32^3-particle sample of the same liquid (same density, seed and jitter)."""
+    host cores, on a 32^3-particle sample of the same liquid (same density, seed and jitter).  The box's
+    CPU share is 16 cores per GPU: the threads are OMP_NUM_THREADS when set, else min(16, nproc).  The same
+    oracle forces are the FP64 reference of the subsample on which the chosen GPU layout's force error is
+    quoted (the bulk-liquid configuration's force RMS error vs the FP64 oracle on a 32k-particle box)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
 
     s = make_system(32)
     L = s["L"]
-    threads = min(16, os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     probe = oracle.c08_forces_halo(s["x"], s["y"], s["z"], 2.5, 0.3, (0, 0, 0), (L, L, L), threads=threads,
                                    repeats=1)
     reps = max(1, int(seconds / max(probe["traverse_seconds"], 1e-6)))
     r = oracle.c08_forces_halo(s["x"], s["y"], s["z"], 2.5, 0.3, (0, 0, 0), (L, L, L), threads=threads,
                                repeats=reps)
-    return dict(value=r["pairs"] / r["traverse_seconds"], unit="pair-interactions/s", cores=threads,
-                kind="port",
-                sample=f"32768-particle box (32^3 lattice, rho 0.75, seed 2), {reps} C08 traversals, "
-                       f"{r['pairs']} pairs each, {r['traverse_seconds'] * 1e3:.2f} ms per traversal")
+    out = dict(value=r["pairs"] / r["traverse_seconds"], unit="pair-interactions/s", cores=threads,
+               kind="port", nproc=os.cpu_count(), host_cpu=host_cpu(),
+               sample=f"32768-particle box (32^3 lattice, rho 0.75, seed 2), {reps} C08 traversals, "
+                      f"{r['pairs']} pairs each, {r['traverse_seconds'] * 1e3:.2f} ms per traversal")
+    if layout_kernel is not None:
+        import autopas_amd as ap
+
+        with ap.Engine((0.0, 0.0, 0.0), (L, L, L), 2.5, 0.3, cluster_size=M, tile_groups=G) as e:
+            e.upload(s["x"], s["y"], s["z"])
+            g = e.force(layout_kernel, globals=True)
+            f = e.download()
+        num = sum(float(np.sum((f[k] - r[k]) ** 2)) for k in ("fx", "fy", "fz"))
+        den = sum(float(np.sum(r[k] ** 2)) for k in ("fx", "fy", "fz"))
+        out["gpu_vs_oracle_32k"] = dict(
+            kernel=ap.KERNEL_NAMES[layout_kernel], force_rms_rel_err=float(np.sqrt(num / den)),
+            pair_count=g.pair_count, pair_count_oracle=int(r["pairs"]),
+            energy_rel_err=abs(g.potential_energy - r["epot"]) / abs(r["epot"]))
+    return out
 
 
 def serving(jobs, workers):
@@ -396,10 +450,10 @@ def main():
         best, table = args.layout, {}
         kernel = getattr(ap, KERNEL_OF[kind])
         M, G = int(m[1:]), int(g[1:])
+        mixed = run(s, kernel, M, args.steps, args.warmup, tile_groups=G)
     else:
-        best, table = tune(s)
-        _, kernel, M, G = table[best]
-    mixed = run(s, kernel, M, args.steps, args.warmup, tile_groups=G)
+        mixed, kernel, M, G, table = run_tuned(s, args.steps, args.warmup)
+        best = layout_name(kernel, M, G)
     if args.profile_only:
         if args.copy_calibration:
             copy_calibration()
@@ -432,7 +486,8 @@ def main():
             "kernel": ap.KERNEL_NAMES[kernel],
             "cluster_size": M,
             "i_clusters_per_wave": G,
-            "tuning_pairs_per_s": {k: v[0] for k, v in table.items()},
+            "tuning_ms_per_step": table,
+            "tuner": "the engine's tuning phase (LJG_KERNEL_AUTO) before the timed steps",
         },
         "plain_fp64": {
             "kernel": "linked cells, 27-cell full shell, FP64",
@@ -459,6 +514,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": TRAFFIC_BYTES.get(best),
+            "traffic_note": None if best in TRAFFIC_BYTES else f"no counter profile recorded for layout {best}",
             "hbm_gbs": (TRAFFIC_BYTES[best] / force_avg_s / 1e9) if best in TRAFFIC_BYTES else None,
             "hbm_frac": (TRAFFIC_BYTES[best] / force_avg_s / 1e9 / PEAK_HBM_GBS) if best in TRAFFIC_BYTES else None,
             "kernel": best,
@@ -484,7 +540,7 @@ def main():
     if args.slab_steps > 0:
         out["two_phase_slab"] = slab_bench(args.slab_steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, kernel, M, G)
     print(json.dumps(out))
 
 

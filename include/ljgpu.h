@@ -77,7 +77,11 @@ typedef struct {
                             * displacement trigger per box, results bit-identical to the box alone); 1 = one
                             * schedule for all boxes (a trigger in any box rebuilds every box; results equal to
                             * the box alone up to the FP32 rounding of the pair sums), fewer and larger rebuilds */
-  int32_t reserved;
+  int32_t graphs;          /* ljg_step: 1 = the launches of each chunk of steps between two list rebuilds are
+                            * captured into a hipGraph (stream capture) and launched as one graph; the graph
+                            * of the previous chunk is updated in place when the launch sequence matches
+                            * (hipGraphExecUpdate), else instantiated anew.  Per-launch timing events are
+                            * then off (ljg_run_stats.force_ms stays 0). */
 } ljg_params;
 
 typedef struct {

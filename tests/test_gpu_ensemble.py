@@ -64,6 +64,25 @@ def test_ensemble_matches_boxes_alone(M, G):
     assert rs.pair_interactions == pairs
 
 
+def test_ensemble_graph_captured_steps_match_stream_launches():
+    # the ensemble's step chunks as captured hipGraphs: every box ends as with plain stream launches
+    boxes = small_boxes()
+    lo, hi, counts, st = en.ensemble_arrays(boxes)
+    announce(f"ensemble, graph-captured step chunks: {len(boxes)} boxes, 40 steps")
+    outs = []
+    for graphs in (False, True):
+        e = ap.Engine(lo[0], hi[0], en.RC, en.SKIN, cluster_size=8, tile_groups=4, rebuild_frequency=10,
+                      dynamic_rebuild=True, graphs=graphs)
+        with e:
+            e.set_boxes(lo, hi, counts)
+            e.upload(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"])
+            rs = e.step(40, en.DT, ap.KERNEL_CLUSTER_MIXED)
+            outs.append((rs.pair_interactions, e.download()))
+    assert outs[0][0] == outs[1][0]
+    for k in outs[0][1]:
+        assert np.array_equal(outs[0][1][k], outs[1][1][k]), k
+
+
 def test_ensemble_dynamic_rebuild_matches_boxes_alone():
     # every box keeps the rebuild schedule it has alone (frequency 20 and its own displacement trigger):
     # boxes at four temperatures move skin/2 at different steps, each is rebuilt in place at its own steps

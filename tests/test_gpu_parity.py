@@ -506,6 +506,24 @@ def test_cutoff_decision_matches_fp64(kernel, M):
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("kernel", [ap.KERNEL_CLUSTER_JROWS_MIXED, ap.KERNEL_CLUSTER_MIXED, ap.KERNEL_LC_FP64])
+def test_graph_captured_steps_match_stream_launches(kernel):
+    # ljg_params.graphs: each chunk of steps between rebuilds runs as one captured hipGraph (updated in place
+    # from chunk to chunk); the trajectory, pair count and rebuild steps are those of the plain stream
+    # launches, bit for bit, with the displacement trigger active (hot system)
+    announce(f"graph-captured step chunks vs stream launches, kernel={ap.KERNEL_NAMES[kernel]}: 12^3 lattice T 3")
+    s = lattice_box(12, 0.75, seed=23, temperature=3.0)
+    outs = []
+    for graphs in (False, True):
+        with make_engine(s["L"], M=4, rebuild_frequency=10, dynamic_rebuild=True, graphs=graphs) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            rs = e.step(75, 0.002, kernel)
+            outs.append((rs.pair_interactions, rs.rebuilds, e.download()))
+    assert outs[0][0] == outs[1][0] and outs[0][1] == outs[1][1]
+    for k in outs[0][2]:
+        assert np.array_equal(outs[0][2][k], outs[1][2][k]), k
+
+
 def test_checkpoint_restart_continues_the_trajectory(tmp_path):
     # 30 steps in one go vs 15 steps, checkpoint, a fresh context restored from it, 15 more steps (FP64)
     announce("checkpoint restart: 6^3 lattice, 15 + 15 steps vs 30 steps, LC FP64")
@@ -537,15 +555,16 @@ def test_checkpoint_restart_continues_the_trajectory(tmp_path):
 def test_auto_layout_tuning():
     # KERNEL_AUTO: a tuning phase samples every candidate that fits the box for tuning_samples steps of
     # the real run, picks the lowest score and keeps it; a second phase starts after tuning_interval.
-    announce("layout tuning: 20^3 lattice, 6 candidates x 4 steps, interval 10")
+    announce("layout tuning: 20^3 lattice, 8 candidates x 4 steps, interval 10")
     s = lattice_box(20, 0.75, seed=6, temperature=0.9)
     L = s["L"]
     with make_engine(L, M=4, rebuild_frequency=4, tuning_samples=4, tuning_interval=10) as e:
         e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
         t0 = e.tuning()
         assert t0["best"] == -1 and t0["phases"] == 0
-        rs = e.step(24, 0.002, ap.KERNEL_AUTO)  # exactly one phase: 6 candidates x 4 steps
-        assert rs.steps == 24
+        ncand = len(t0["candidates"])
+        rs = e.step(4 * ncand, 0.002, ap.KERNEL_AUTO)  # exactly one phase: every candidate x 4 steps
+        assert rs.steps == 4 * ncand
         t1 = e.tuning()
         n = len(t1["candidates"])
         assert not t1["tuning"] and t1["phases"] == 1 and 0 <= t1["best"] < n
@@ -556,10 +575,10 @@ def test_auto_layout_tuning():
         assert rs.steps == 16
         assert e.tuning()["phases"] == 2 and e.tuning()["tuning"]
         st = e.download()
-    # the run stays on the physical trajectory: compare with the FP64 linked-cells kernel over 40 steps
+    # the run stays on the physical trajectory: compare with the FP64 linked-cells kernel over as many steps
     with make_engine(L, M=4, rebuild_frequency=4) as e:
         e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
-        e.step(40, 0.002, ap.KERNEL_LC_FP64)
+        e.step(4 * ncand + 16, 0.002, ap.KERNEL_LC_FP64)
         ref = e.download()
     d = np.stack([st["x"] - ref["x"], st["y"] - ref["y"], st["z"] - ref["z"]])
     d -= L * np.round(d / L)
