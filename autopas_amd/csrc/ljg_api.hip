@@ -1177,9 +1177,11 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
 // ---- layout tuning (AutoTuner.cpp:85-175,250-369 in spirit) ----------------------------------------
 // Candidates: FP32-pair/FP64-accumulate layouts (kernel, cluster size, i-clusters per wavefront of the
 // force kernel, of the j-row expansion pass).  A tuning phase runs each candidate the box supports for
-// tuning_samples steps of the real simulation and scores it as AutoTuner's estimateRuntimeFromSamples
-// does with the mean selector: rebuild time / rebuild_frequency + the rest of the step.  The lowest
-// score is used until tuning_interval steps later (0: never again).
+// one unscored step (it absorbs the candidate's one-time costs: the first launches of its kernels and the
+// growth of its buffers, which AutoPas's tuner spreads over many more iterations) and then tuning_samples
+// scored steps of the real simulation, the first of them after a rebuild, and scores it as AutoTuner's
+// estimateRuntimeFromSamples does with the mean selector: rebuild time / rebuild_frequency + the rest of
+// the step.  The lowest score is used until tuning_interval steps later (0: never again).
 struct TuneCandidate {
   int32_t kernel, M, G, B;
 };
@@ -1214,6 +1216,7 @@ static void run_auto(Ctx &c, int32_t nsteps, double dt, ljg_run_stats &rs) {
       t.active = true;
       t.ci = next_candidate(c, 0);
       t.done = 0;
+      t.warmed = false;
       t.wall_ms = t.rebuild_ms = 0.;
       t.rebuilds = 0;
       for (double &e : t.effective_ms) e = -1.;
@@ -1222,6 +1225,16 @@ static void run_auto(Ctx &c, int32_t nsteps, double dt, ljg_run_stats &rs) {
     if (t.active) {
       const TuneCandidate &k = kTuneCandidates[t.ci];
       apply_candidate(c, k);
+      if (!t.warmed) {
+        ljg_run_stats sub{};
+        run_steps(c, k.kernel, 1, dt, false, sub);
+        t.warmed = true;
+        --left;
+        rs.steps += sub.steps;
+        rs.rebuilds += sub.rebuilds;
+        rs.pair_interactions += sub.pair_interactions;
+        continue;
+      }
       // every candidate's samples start with a rebuild, so each score carries its rebuild cost
       if (t.done == 0) c.steps_since_rebuild = c.prm.rebuild_frequency;
       const int32_t run = left < samples - t.done ? left : samples - t.done;
@@ -1248,6 +1261,7 @@ static void run_auto(Ctx &c, int32_t nsteps, double dt, ljg_run_stats &rs) {
         if (nxt >= 0) {
           t.ci = nxt;
           t.done = 0;
+          t.warmed = false;
           t.wall_ms = t.rebuild_ms = 0.;
           t.rebuilds = 0;
         } else {

@@ -240,7 +240,8 @@ enum class Layout { None, Upload, Cells, Clusters, Verlet };
 struct TuneState {
   bool active = false;
   int ci = 0;                  // candidate being sampled
-  int32_t done = 0;            // its steps so far
+  int32_t done = 0;            // its scored steps so far
+  bool warmed = false;         // its unscored first step has run
   double wall_ms = 0., rebuild_ms = 0.;
   int32_t rebuilds = 0;
   int best = -1;               // chosen candidate

@@ -198,7 +198,7 @@ def layout_name(kernel, M, G):
     return f"{kind}_M{M}_G{G}"
 
 
-def run_tuned(sysd, steps, warmup, samples=10):
+def run_tuned(sysd, steps, warmup, samples=20):
     """The engine's own layout tuning (LJG_KERNEL_AUTO, AutoTuner.cpp:85-175 in the engine): one tuning phase
     over its candidate layouts, `samples` steps of the real run each, scored as rebuild / rebuild frequency +
     the rest of the step; then `warmup` steps and the timed `steps` with the chosen layout."""
@@ -210,7 +210,7 @@ def run_tuned(sysd, steps, warmup, samples=10):
     try:
         e.upload(sysd["x"], sysd["y"], sysd["z"], sysd["vx"], sysd["vy"], sysd["vz"])
         ncand = len(e.tuning()["candidates"])
-        e.step(ncand * samples + warmup, 0.002, ap.KERNEL_AUTO)
+        e.step(ncand * (samples + 1) + warmup, 0.002, ap.KERNEL_AUTO)
         t = e.tuning()
         if t["tuning"] or t["best"] < 0:
             raise RuntimeError("the tuning phase did not finish")

@@ -555,7 +555,7 @@ def test_checkpoint_restart_continues_the_trajectory(tmp_path):
 def test_auto_layout_tuning():
     # KERNEL_AUTO: a tuning phase samples every candidate that fits the box for tuning_samples steps of
     # the real run, picks the lowest score and keeps it; a second phase starts after tuning_interval.
-    announce("layout tuning: 20^3 lattice, 8 candidates x 4 steps, interval 10")
+    announce("layout tuning: 20^3 lattice, 8 candidates x (1 + 4) steps, interval 10")
     s = lattice_box(20, 0.75, seed=6, temperature=0.9)
     L = s["L"]
     with make_engine(L, M=4, rebuild_frequency=4, tuning_samples=4, tuning_interval=10) as e:
@@ -563,8 +563,8 @@ def test_auto_layout_tuning():
         t0 = e.tuning()
         assert t0["best"] == -1 and t0["phases"] == 0
         ncand = len(t0["candidates"])
-        rs = e.step(4 * ncand, 0.002, ap.KERNEL_AUTO)  # exactly one phase: every candidate x 4 steps
-        assert rs.steps == 4 * ncand
+        rs = e.step(5 * ncand, 0.002, ap.KERNEL_AUTO)  # exactly one phase: every candidate 1 unscored + 4 steps
+        assert rs.steps == 5 * ncand
         t1 = e.tuning()
         n = len(t1["candidates"])
         assert not t1["tuning"] and t1["phases"] == 1 and 0 <= t1["best"] < n
@@ -578,7 +578,7 @@ def test_auto_layout_tuning():
     # the run stays on the physical trajectory: compare with the FP64 linked-cells kernel over as many steps
     with make_engine(L, M=4, rebuild_frequency=4) as e:
         e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
-        e.step(4 * ncand + 16, 0.002, ap.KERNEL_LC_FP64)
+        e.step(5 * ncand + 16, 0.002, ap.KERNEL_LC_FP64)
         ref = e.download()
     d = np.stack([st["x"] - ref["x"], st["y"] - ref["y"], st["z"] - ref["z"]])
     d -= L * np.round(d / L)
