@@ -1154,18 +1154,23 @@ void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, con
 }
 
 // i-clusters per wave for the j-row kernel (rows are ~1.5x longer than a member's own list).
-static int default_jrow_groups(int M) { return M >= 8 ? 4 : M == 4 ? 8 : M == 2 ? 16 : 32; }
+// Small builds (few i-clusters) are bound by the latency of one wave's row, not by throughput: fewer
+// i-clusters per wave spread the row over more lanes (measured for M = 8 from 512 to 64,000 particles).
+static int default_jrow_groups(int M, int64_t nclusters) {
+  if (M >= 8) return nclusters <= 2048 ? 1 : nclusters <= 8192 ? 2 : 4;
+  return M == 4 ? 8 : M == 2 ? 16 : 32;
+}
 
 void cluster_jrows_force(Ctx &c, bool globals, const IntegArgs *integ) {
   const ForceArgs a = make_args(c);
   const bool mix = c.lj.ntypes > 1;
   const double *tab = mix ? c.tab.p : nullptr;
-  const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_jrow_groups(c.M);
+  const int G = c.prm.tile_groups > 0 ? c.prm.tile_groups : default_jrow_groups(c.M, c.nclusters);
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
 #define LJG_CASE(MV, GV) \
   if (c.M == MV && G == GV) { launch_jrows<MV, GV>(c, mix, globals && !integ, a, tab, integ); goto launched; }
-  LJG_CASE(8, 2) LJG_CASE(8, 4) LJG_CASE(8, 8)
-  LJG_CASE(4, 4) LJG_CASE(4, 8) LJG_CASE(4, 16)
+  LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4) LJG_CASE(8, 8)
+  LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8) LJG_CASE(4, 16)
   LJG_CASE(2, 8) LJG_CASE(2, 16)
   LJG_CASE(1, 16) LJG_CASE(1, 32)
 #undef LJG_CASE

@@ -1336,7 +1336,9 @@ void vcl_build(Ctx &c) {
     count_stats(c, c.list_count.p, jrows ? ncl * kJclSplit : ncl, c.list_stats.p);
     if (jrows) {
       // i-clusters per wavefront of the expansion pass (tuned on the 10^6-particle liquid: 4 for M >= 4)
-      const int eg = c.prm.build_groups > 0 ? c.prm.build_groups : (M >= 4 ? 4 : M == 2 ? 4 : 8);
+      // (one i-cluster per wave below 2048 clusters: latency-bound small builds)
+      const int eg = c.prm.build_groups > 0 ? c.prm.build_groups
+                                             : (M >= 4 ? (ncl <= 2048 ? 1 : 4) : M == 2 ? 4 : 8);
       // FP32 member test of the cubic variant: ilen^2 in fixed-point units, band 2^-18 relative
       uint32_t cut_lo = 0, cut_hi = 0;
       cutoff_band((float)(ilen * ilen * c.fg.inv_scale[0] * c.fg.inv_scale[0]), std::ldexp(1.0, -18), cut_lo, cut_hi);

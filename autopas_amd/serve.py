@@ -87,10 +87,26 @@ def job_system(job):
     return L, (x, y, z, vx, vy, vz)
 
 
+KERNEL_BY_SIZE = None  # run_job's kernel argument: pick the layout from the job's size (job_layout)
+
+
+def job_layout(n):
+    """Layout for a job of n particles: (kernel, cluster size, i-clusters per wavefront).  Serving-sized boxes
+    (up to 65,536 particles) are bound by launch and rebuild latency, where the cluster tiles (whose build
+    has no expansion pass) with few i-clusters per wavefront were fastest on MI355X at every size from 512
+    to 64,000 particles (15-30 us per step)."""
+    clusters = (n + 7) // 8
+    return engine.KERNEL_CLUSTER_MIXED, 8, (1 if clusters <= 2048 else 2 if clusters <= 4096 else 4)
+
+
 def run_job(eng, job, kernel):
-    """Run one job on an engine context (box, upload, steps, download); returns (pairs, rebuilds, state)."""
+    """Run one job on an engine context (box, upload, steps, download); returns (pairs, rebuilds, state).
+    kernel KERNEL_BY_SIZE picks the layout from the job's size (job_layout)."""
     L, (x, y, z, vx, vy, vz) = job_system(job)
     eng.set_box((0.0, 0.0, 0.0), (L, L, L))
+    if kernel is KERNEL_BY_SIZE:
+        kernel, m, g = job_layout(job.n)
+        eng.set_layout(m, g)
     eng.upload(x, y, z, vx, vy, vz)
     rs = eng.step(job.steps, DT, kernel)
     return rs.pair_interactions, rs.rebuilds, eng.download()
@@ -309,5 +325,7 @@ def serve_open_loop(count, workers=4, kernel=engine.KERNEL_CLUSTER_MIXED, load=0
         jobs = make_jobs(count, load * capacity, seed=seed, **job_kw)
         wall = srv.run(jobs)
     out = latency_summary(jobs, wall)
-    out.update(workers=workers, load=load, capacity_jobs_per_s=capacity, kernel=engine.KERNEL_NAMES[kernel])
+    out.update(workers=workers, load=load, capacity_jobs_per_s=capacity,
+               kernel="cluster tiles M8, tile shape by job size" if kernel is KERNEL_BY_SIZE
+               else engine.KERNEL_NAMES[kernel])
     return out, jobs

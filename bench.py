@@ -284,13 +284,14 @@ def cpu_baseline(seconds, layout_kernel=None, M=8, G=0):
 def serving(jobs, workers):
     """p50/p99 job latency: independent small boxes (N log-uniform in [256, 65536], rho in [0.4, 0.85],
     10-1000 steps) arriving as a Poisson process at 80% of the worker pool's measured throughput, served
-    (a) by `workers` reused contexts with the j-row kernel and (b) with continuous batching in one ensemble
+    (a) by `workers` reused contexts, each job with the cluster-tile kernel at a tile shape chosen from its
+    size, and (b) with continuous batching in one ensemble
     context, both on the same arrival stream; beside them the same jobs run one at a time with the plain
     FP64 kernel (closed loop, so its latency is the bare service time)."""
     import autopas_amd as ap
     from autopas_amd import serve
 
-    out, _ = serve.serve_open_loop(jobs, workers=workers, kernel=ap.KERNEL_CLUSTER_JROWS_MIXED, load=0.8, seed=5,
+    out, _ = serve.serve_open_loop(jobs, workers=workers, kernel=serve.KERNEL_BY_SIZE, load=0.8, seed=5,
                                    cluster_size=8)
     base_jobs = serve.make_jobs(max(jobs // 4, 8), None, seed=5)
     with serve.Server(1, ap.KERNEL_LC_FP64, 4) as srv:
