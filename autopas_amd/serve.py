@@ -15,6 +15,7 @@ numpy's Philox4x32 generator.
 Worker threads spend their time inside the C ABI, which releases the GIL, so the contexts' streams run
 concurrently on the device.
 """
+import collections
 import math
 import queue
 import threading
@@ -185,41 +186,90 @@ class Server:
         return max(j.finish for j in jobs) if jobs else 0.0
 
 
+class _Lane:
+    """One batching context of a BatchedServer: its engine (an ensemble context with its own stream), the
+    jobs it runs as boxes, the jobs dispatched to it and not yet admitted, and its phase."""
+
+    def __init__(self, eng):
+        self.engine = eng
+        self.active = []  # [job, steps left], in box order
+        self.inbox = collections.deque()  # dispatched jobs, in arrival order
+        self.phase = 0  # batch steps since the last phase boundary
+
+    def work_left(self):
+        """Particle-steps still to run for the jobs running and dispatched here (the dispatch criterion)."""
+        return sum(j.n * left for j, left in list(self.active)) + sum(j.n * j.steps for j in list(self.inbox))
+
+
 class BatchedServer:
-    """Continuous batching on one engine context: every running job is a box of one ensemble, so each
-    step advances all running jobs in the same launches.  Between epochs of at most ``epoch`` steps the
-    finished jobs are downloaded and retired and the waiting jobs that have arrived are admitted, first come
-    first served, while the particles in flight stay within ``particle_budget`` (the HBM budget) and the
-    batch holds at most ``max_jobs`` jobs (fewer jobs in a batch finish sooner each); the running
-    jobs' state stays resident on the device (``Engine.edit_boxes``).  An epoch ends early when a running job
-    reaches its step count, so every job runs exactly its own number of steps.
+    """Continuous batching: every running job is a box of an ensemble context, so each step advances all
+    running jobs of the context in the same launches.  Between epochs of at most ``epoch`` steps the
+    finished jobs are downloaded and retired and the waiting jobs are admitted, first come first served,
+    while the particles in flight stay within ``particle_budget`` (the HBM budget) and the batch holds at
+    most ``max_jobs`` jobs (fewer jobs in a batch finish sooner each); the running jobs' state stays resident
+    on the device (``Engine.edit_boxes``).  An epoch ends early when a running job reaches its step count, so
+    every job runs exactly its own number of steps.
+
+    ``lanes`` ensemble contexts batch at once, each on its own thread and HIP stream, so the batches' launches
+    overlap on the device; an arriving job is dispatched to the lane with the least work left (particle-steps
+    of its running and waiting jobs), and each lane holds a share of the particle budget.  One lane is a
+    single batch.
 
     Each job keeps the rebuild schedule it has alone (its own frequency and displacement trigger, its
     layout and lists kept through every change of the batch), so its trajectory is bit-identical to the job
-    run alone.  With ``shared_rebuilds`` the batch has one schedule (a trigger in any job rebuilds all), which
+    run alone.  With ``shared_rebuilds`` a batch has one schedule (a trigger in any job rebuilds all), which
     takes fewer, larger rebuilds; a job's trajectory then equals its run alone up to the rounding of the
-    FP32 pair sums, which its rebuilds at other steps take in another order."""
+    FP32 pair sums, which its rebuilds at other steps take in another order.
+
+    With ``aligned`` (the default) waiting jobs are admitted only at the batch's phase boundaries, every
+    ``rebuild_frequency`` steps of the batch: a job's scheduled rebuilds fall every ``rebuild_frequency`` of
+    its own steps, so jobs admitted at phase boundaries are due together and the batch takes one
+    partial rebuild per period instead of one every step or two (only a job whose displacement trigger fires
+    leaves the common phase).  A job that arrives mid-period waits at most ``rebuild_frequency`` batch
+    steps; finished jobs are retired at any step."""
 
     def __init__(self, particle_budget=None, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=8, tile_groups=4,
-                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None, shared_rebuilds=False):
+                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None, shared_rebuilds=False,
+                 aligned=True, rebuild_frequency=10, lanes=1):
+        if lanes < 1:
+            raise ValueError("a batched server needs at least one lane")
         self.kernel = kernel
         self.epoch = epoch
+        self.aligned = aligned
+        self.frequency = rebuild_frequency
         self.max_jobs = max_jobs  # jobs in one batch at most (None: only the particle budget limits)
         self.keep_results = keep_results
         self.shared_rebuilds = shared_rebuilds
-        self.engine = engine.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), RC, SKIN, cluster_size=cluster_size,
-                                    rebuild_frequency=10, dynamic_rebuild=True, tile_groups=tile_groups,
-                                    shared_rebuilds=shared_rebuilds)
+        self.lanes = []
+        try:
+            for _ in range(lanes):
+                self.lanes.append(_Lane(engine.Engine(
+                    (0.0, 0.0, 0.0), (8.0, 8.0, 8.0), RC, SKIN, cluster_size=cluster_size,
+                    rebuild_frequency=rebuild_frequency, dynamic_rebuild=True, tile_groups=tile_groups,
+                    shared_rebuilds=shared_rebuilds)))
+        except Exception:
+            self.close()
+            raise
         if particle_budget is None:
             free, _ = self.engine.memory_info()
             particle_budget = int(0.8 * free / bytes_per_particle)
         self.particle_budget = particle_budget
+        self.lane_budget = max(1, particle_budget // lanes)
         self.pairs = 0
         self.epochs = 0
         self.edits = 0
+        self.rebuild_passes = 0  # build passes of the batches (each over the boxes due at that step)
+        self._lock = threading.Lock()  # the counters above, summed over the lanes' threads
+
+    @property
+    def engine(self):
+        """The first lane's context (the only one with one lane)."""
+        return self.lanes[0].engine
 
     def close(self):
-        self.engine.close()
+        for lane in self.lanes:
+            lane.engine.close()
+        self.lanes = []
 
     def __enter__(self):
         return self
@@ -227,7 +277,7 @@ class BatchedServer:
     def __exit__(self, *exc):
         self.close()
 
-    def _edit(self, keep, admit):
+    def _edit(self, lane, keep, admit):
         lo = hi = cnt = None
         parts = {}
         if admit:
@@ -237,51 +287,109 @@ class BatchedServer:
             cnt = np.array([j.n for j in admit], dtype=np.int64)
             for k, key in enumerate(("x", "y", "z", "vx", "vy", "vz")):
                 parts[key] = np.concatenate([st[k] for _, st in systems])
-        self.engine.edit_boxes(keep, lo, hi, cnt, **parts)
-        self.edits += 1
+        lane.engine.edit_boxes(keep, lo, hi, cnt, **parts)
+        with self._lock:
+            self.edits += 1
 
-    def run(self, jobs):
-        """Serve ``jobs`` (arrival times relative to now); returns the wall time to the last completion."""
-        t0 = time.perf_counter()
-        pending = sorted(jobs, key=lambda j: j.arrival)
-        head = 0
-        active = []  # [job, steps left], in box order
-        while head < len(pending) or active:
+    def _serve_lane(self, w, t0, cond, state):
+        """The loop of lane w: retire, admit from the lane's inbox, run an epoch; returns when the feeder is
+        done and the lane has nothing left."""
+        lane = self.lanes[w]
+        while True:
+            with cond:
+                while not lane.active and not lane.inbox:
+                    if state["fed"] or state["error"]:
+                        return
+                    cond.wait()
+                if state["error"]:
+                    return
             now = time.perf_counter() - t0
+            active = lane.active
             done = [k for k, (_, left) in enumerate(active) if left == 0]
             if done:
                 offsets = np.concatenate([[0], np.cumsum([j.n for j, _ in active])])
                 for k in done:
                     job = active[k][0]
                     if self.keep_results:
-                        job.result = self.engine.download_range(int(offsets[k]), job.n)
+                        job.result = lane.engine.download_range(int(offsets[k]), job.n)
                     job.finish = time.perf_counter() - t0
             keep = [k for k, (_, left) in enumerate(active) if left > 0]
             inflight = sum(active[k][0].n for k in keep)
+            if not keep:
+                lane.phase = 0  # an empty batch starts a new phase
             admit = []
-            while head < len(pending) and pending[head].arrival <= now and (
-                    self.max_jobs is None or len(keep) + len(admit) < self.max_jobs) and (
-                    inflight + pending[head].n <= self.particle_budget or not (keep or admit)):
-                job = pending[head]
-                head += 1
-                job.start = now
-                job.worker = 0
-                admit.append(job)
-                inflight += job.n
+            with cond:
+                while (lane.phase == 0 or not self.aligned) and lane.inbox and (
+                        self.max_jobs is None or len(keep) + len(admit) < self.max_jobs) and (
+                        inflight + lane.inbox[0].n <= self.lane_budget or not (keep or admit)):
+                    job = lane.inbox.popleft()
+                    job.start = now
+                    job.worker = w
+                    admit.append(job)
+                    inflight += job.n
             if admit or done:
                 if keep or admit:
-                    self._edit(keep, admit)
-                active = [active[k] for k in keep] + [[j, j.steps] for j in admit]
+                    self._edit(lane, keep, admit)
+                lane.active = active = [active[k] for k in keep] + [[j, j.steps] for j in admit]
             if not active:
-                if head < len(pending):
-                    time.sleep(max(0.0, pending[head].arrival - (time.perf_counter() - t0)))
+                with cond:  # jobs wait for the phase boundary only in a running batch
+                    if lane.inbox:
+                        continue
+                    if not state["fed"]:
+                        cond.wait(0.001)
                 continue
             steps = min(self.epoch, min(left for _, left in active))
-            rs = self.engine.step(steps, DT, self.kernel)
-            self.pairs += rs.pair_interactions
-            self.epochs += 1
+            if self.aligned:
+                steps = min(steps, self.frequency - lane.phase)
+            rs = lane.engine.step(steps, DT, self.kernel)
+            lane.phase = (lane.phase + steps) % self.frequency
+            with self._lock:
+                self.pairs += rs.pair_interactions
+                self.rebuild_passes += rs.rebuilds
+                self.epochs += 1
             for a in active:
                 a[1] -= steps
+
+    def run(self, jobs):
+        """Serve ``jobs`` (arrival times relative to now); returns the wall time to the last completion."""
+        t0 = time.perf_counter()
+        cond = threading.Condition()
+        state = dict(fed=False, error=None)
+        for lane in self.lanes:
+            lane.active = []
+            lane.inbox.clear()
+            lane.phase = 0
+
+        def lane_main(w):
+            try:
+                self._serve_lane(w, t0, cond, state)
+            except BaseException as exc:  # re-raised by run()
+                with cond:
+                    state["error"] = state["error"] or exc
+                    cond.notify_all()
+
+        threads = [threading.Thread(target=lane_main, args=(w,), daemon=True) for w in range(len(self.lanes))]
+        for th in threads:
+            th.start()
+        try:
+            for job in sorted(jobs, key=lambda j: j.arrival):
+                delay = job.arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                with cond:
+                    if state["error"]:
+                        break
+                    lane = min(self.lanes, key=_Lane.work_left)
+                    lane.inbox.append(job)
+                    cond.notify_all()
+        finally:
+            with cond:
+                state["fed"] = True
+                cond.notify_all()
+            for th in threads:
+                th.join()
+        if state["error"] is not None:
+            raise state["error"]
         return max(j.finish for j in jobs) if jobs else 0.0
 
 
@@ -295,12 +403,13 @@ def serve_open_loop_batched(count, load=0.8, seed=5, calibration_jobs=None, rate
         capacity = len(calib) / wall
         arrival_rate = rate if rate is not None else load * capacity
         jobs = make_jobs(count, arrival_rate, seed=seed)
-        srv.pairs = srv.epochs = srv.edits = 0
+        srv.pairs = srv.epochs = srv.edits = srv.rebuild_passes = 0
         wall = srv.run(jobs)
         out = latency_summary(jobs, wall)
         out.update(pairs=int(srv.pairs), pairs_per_s=srv.pairs / wall if wall else 0.0,
                    load=arrival_rate / capacity, arrival_rate_jobs_per_s=arrival_rate, capacity_jobs_per_s=capacity,
                    kernel=engine.KERNEL_NAMES[srv.kernel], epochs=srv.epochs, batch_edits=srv.edits,
+                   rebuild_passes=srv.rebuild_passes, aligned=srv.aligned, lanes=len(srv.lanes),
                    particle_budget=srv.particle_budget, epoch_steps=srv.epoch,
                    schedule="shared" if srv.shared_rebuilds else "per job")
     return out, jobs

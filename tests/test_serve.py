@@ -100,6 +100,8 @@ class _FakeEnsemble:
         self.peak = 0
         self.most = 0
         self.kwargs = kwargs
+        self.clock = 0  # steps run since the box list was last empty
+        self.admit_clocks = []  # the clock at each edit that added boxes to a running batch
 
     def memory_info(self):
         return 1 << 40, 1 << 41
@@ -107,6 +109,10 @@ class _FakeEnsemble:
     def edit_boxes(self, keep, box_min=None, box_max=None, counts=None, x=None, **parts):
         assert len(set(keep)) == len(keep) and all(0 <= k < len(self.boxes) for k in keep)
         kept = [self.boxes[k] for k in keep]
+        if not kept:
+            self.clock = 0
+        elif counts is not None:
+            self.admit_clocks.append(self.clock)
         added = []
         if counts is not None:
             ends = np.cumsum(counts)
@@ -118,11 +124,14 @@ class _FakeEnsemble:
 
     def step(self, steps, dt, kernel):
         assert steps >= 1
+        time.sleep(1e-4 * steps)  # a step takes time, so batches overlap the arrivals
+        self.clock += steps
         for b in self.boxes:
             b[2] += steps
 
         class R:
             pair_interactions = 0
+            rebuilds = 0
         return R()
 
     def download_range(self, first, count):
@@ -135,22 +144,33 @@ class _FakeEnsemble:
         pass
 
 
-@pytest.mark.parametrize("max_jobs", [None, 3])
-def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs):
+@pytest.mark.parametrize("max_jobs,aligned,lanes", [(None, True, 1), (3, True, 1), (None, False, 1), (None, True, 3)])
+def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs, aligned, lanes):
     # jobs arrive over ~20 ms; the budget holds a few at once; every job must run exactly its own steps,
-    # start no earlier than it arrived, and be read back from its own place in the batch
+    # start no earlier than it arrived, and be read back from its own place in its batch; aligned, jobs join
+    # a running batch only at multiples of the rebuild frequency of its steps; with several lanes the jobs
+    # spread over the lanes' contexts, each within its share of the budget
     monkeypatch.setattr(serve.engine, "Engine", _FakeEnsemble)
     jobs = serve.make_jobs(16, rate=800.0, seed=9, n_range=(256, 4000), steps_range=(3, 40))
     budget = 6000
     with serve.BatchedServer(particle_budget=budget, keep_results=True, epoch=10, max_jobs=max_jobs,
-                             shared_rebuilds=True) as srv:
-        assert srv.engine.kwargs["shared_rebuilds"] is True
+                             shared_rebuilds=True, aligned=aligned, rebuild_frequency=10, lanes=lanes) as srv:
+        engines = [lane.engine for lane in srv.lanes]
+        assert len(engines) == lanes and srv.engine is engines[0]
+        assert all(e.kwargs["shared_rebuilds"] is True and e.kwargs["rebuild_frequency"] == 10 for e in engines)
         srv.run(jobs)
-        peak, most = srv.engine.peak, srv.engine.most
         assert srv.edits >= 3 and srv.epochs >= 4
-    assert peak <= max(budget, max(j.n for j in jobs))
-    if max_jobs is not None:
-        assert most <= max_jobs
+    share = budget // lanes
+    for e in engines:
+        assert e.peak <= max(share, max(j.n for j in jobs))
+        if max_jobs is not None:
+            assert e.most <= max_jobs
+    clocks = [c for e in engines for c in e.admit_clocks]
+    if aligned:
+        assert all(c % 10 == 0 for c in clocks), clocks
+    else:
+        assert any(c % 10 != 0 for c in clocks), clocks
+    assert {j.worker for j in jobs} == set(range(lanes))
     for j in jobs:
         tag = float(serve.job_system(j)[1][0][0])
         assert j.result["tag"] == tag, j.index
