@@ -302,9 +302,9 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
     LJG_HIP(hipHostMalloc((void **)&c.host_flag, 4 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(c.host_flag, 0, 4 * sizeof(int32_t));
     LJG_HIP(hipHostGetDevicePointer((void **)&c.host_flag_dev, c.host_flag, 0));
-    LJG_HIP(hipHostMalloc((void **)&c.pin_mark, 16 * sizeof(unsigned long long),
+    LJG_HIP(hipHostMalloc((void **)&c.pin_mark, (1 + kMarkWords) * sizeof(unsigned long long),
                           hipHostMallocMapped | hipHostMallocCoherent));
-    std::memset(c.pin_mark, 0, 16 * sizeof(unsigned long long));
+    std::memset(c.pin_mark, 0, (1 + kMarkWords) * sizeof(unsigned long long));
     LJG_HIP(hipHostGetDevicePointer((void **)&c.pin_mark_dev, c.pin_mark, 0));
     c.flag.ensure(4);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, (int64_t)4, c.flag.p);
@@ -507,8 +507,8 @@ static void set_box_tables(Ctx &c, int32_t nboxes, const double *box_min, const 
   c.build_partial = false;
   c.box_since.assign((size_t)nboxes, 0);
   c.box_pending.clear();
-  c.box_flag.ensure((size_t)nboxes);
-  fill_i32(c, c.box_flag.p, nboxes, 0);
+  c.box_flag.ensure((size_t)nboxes + 1);  // (+1: read back as whole 8-byte words)
+  fill_i32(c, c.box_flag.p, nboxes + 1, 0);
   LJG_HIP(hipStreamSynchronize(c.stream));  // the host tables go out of scope
 }
 
@@ -825,7 +825,12 @@ ljg_status ljg_set_timing(ljg_ctx *h, int32_t enabled) {
 constexpr int32_t kTimingStride = 8;
 constexpr int32_t kChunkSteps = 32;
 
-__global__ void k_clear_flag(int32_t *flag) { flag[0] = 0; }
+// Clears the displacement flag and, when given, its host-mapped copy: in stream order, after every queued
+// drift that may still raise it, so the host needs no synchronisation first.
+__global__ void k_clear_flag(int32_t *flag, int32_t *host_copy) {
+  flag[0] = 0;
+  if (host_copy) host_copy[0] = 0;
+}
 
 // The pair counts of the step loop's kAccSlots lines, summed into *sum.
 __global__ void k_sum_slots(const unsigned long long *__restrict__ slots, unsigned long long *__restrict__ sum) {
@@ -850,6 +855,9 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   std::vector<hipEvent_t> rev;                           // pairs (start, stop) per rebuild
   std::vector<std::pair<int32_t, hipEvent_t>> fev;       // (step, event): start, stop per timed force launch
   c.hits_slots.ensure(kAccSlots * kAccStride + 1);  // (+1: their sum, read back at the end)
+  unsigned long long *const sum_slots_out = c.hits_slots.p + kAccSlots * kAccStride;
+  unsigned long long total_hits = 0;
+  bool total_read = false;  // total_hits holds the run's sum, read with the last chunk's synchronisation
   hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kAccSlots * kAccStride), 0, c.stream, (int64_t)(kAccSlots * kAccStride),
                      reinterpret_cast<int64_t *>(c.hits_slots.p));
   LJG_HIP(hipGetLastError());
@@ -857,11 +865,10 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
   c.timing = false;  // per-launch events are recorded here instead
   const bool graphs = c.prm.graphs != 0;
   if (graphs) timing = false;  // no event records inside the captured chunks
-  // clears both copies of the flag; every queued launch (a drift that may still raise it) has run first
+  // clears both copies of the flag on the stream, after every queued launch (a drift that may still raise
+  // it); the host reads its copy only after a later synchronisation
   auto clear_flag = [&]() {
-    stream_wait(c);
-    static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p, c.host_flag_dev);
     LJG_HIP(hipGetLastError());
     if (c.nboxes > 0) fill_i32(c, c.box_flag.p, c.nboxes, 0);
   };
@@ -1008,32 +1015,54 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     // the flag was raised at an earlier step, when the host rebuilds at that step instead
     const bool force_next = rebuild_next && fused;
     if (due) drift_step(s + k, true);
-    if (per_box && dyn) {  // the boxes' flags come back with the same synchronisation
-      if (c.pin_flags_cap < (size_t)c.nboxes) {
+    // The boxes' flags come back with the same synchronisation: in the marker's words for up to
+    // kMarkFlagBoxes boxes, else by a copy queued before it.
+    const int flag_words = per_box && dyn && c.nboxes <= kMarkFlagBoxes ? (c.nboxes + 1) / 2 : 0;
+    if (per_box && dyn) {
+      if (c.pin_flags_cap < (size_t)c.nboxes + 1) {
         if (c.pin_flags) LJG_HIP(hipHostFree(c.pin_flags));
         c.pin_flags = nullptr;
         c.pin_flags_cap = 0;
-        LJG_HIP(hipHostMalloc((void **)&c.pin_flags, (size_t)c.nboxes * sizeof(int32_t), hipHostMallocDefault));
-        c.pin_flags_cap = (size_t)c.nboxes;
+        LJG_HIP(hipHostMalloc((void **)&c.pin_flags, ((size_t)c.nboxes + 1) * sizeof(int32_t), hipHostMallocDefault));
+        c.pin_flags_cap = (size_t)c.nboxes + 1;
       }
-      LJG_HIP(hipMemcpyAsync(c.pin_flags, c.box_flag.p, (size_t)c.nboxes * sizeof(int32_t), hipMemcpyDeviceToHost,
-                             c.stream));
+      if (!flag_words)
+        LJG_HIP(hipMemcpyAsync(c.pin_flags, c.box_flag.p, (size_t)c.nboxes * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               c.stream));
     }
+    // The run ends with this chunk unless a flag was raised: its pair count comes back with the same
+    // synchronisation.
+    const bool last_chunk = !due && s + k == nsteps;
+    bool sum_read = false;
     if (rebuild_next) {
       rebuild_now();
       if (force_next) force_step(s + k, true, stamp_of(s + k) - 1);
       stream_wait(c);
     } else if (dyn || due) {
-      stream_wait(c);
+      unsigned long long words[kMarkWords] = {};
+      const int nsum = last_chunk ? 1 : 0;
+      if (nsum) {
+        hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, c.stream, c.hits_slots.p, sum_slots_out);
+        LJG_HIP(hipGetLastError());
+      }
+      stream_wait(c, reinterpret_cast<const unsigned long long *>(c.box_flag.p), flag_words, words,
+                  nsum ? sum_slots_out : nullptr, nsum);
+      if (flag_words) std::memcpy(c.pin_flags, words, (size_t)c.nboxes * sizeof(int32_t));
+      if (nsum) {
+        total_hits = words[flag_words];
+        sum_read = true;
+      }
     }
     const int32_t raised = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
+    // (the sum read with this chunk's synchronisation holds unless more steps follow)
+    total_read = sum_read && raised == 0;
     if (force_next && (raised == 0 || raised >= stamp_of(s + k))) {
       // The queued force of step s + k ran.  A flag raised by the drift of step s + k asked for the
       // rebuild that preceded it: cleared (nothing is queued, so no wait).  One raised by that force's own
       // drift of step s + k + 1 stays for the next chunk, whose launches then exit until it is handled.
       if (raised == stamp_of(s + k)) {
         static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
-        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p, (int32_t *)nullptr);
         LJG_HIP(hipGetLastError());
       }
       c.steps_since_rebuild = 1;
@@ -1088,7 +1117,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
         // boxes; then the force of the marked boxes.  The two sets share no cluster, slot or list row.
         // Everything queued has run (the chunk's synchronisation), so the flags clear without waiting.
         static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
-        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p);
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p, (int32_t *)nullptr);
         LJG_HIP(hipGetLastError());
         fill_i32(c, c.box_flag.p, c.nboxes, 0);
         const bool last = t + 1 == nsteps;
@@ -1147,14 +1176,17 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     s = t + 1;
   }
   c.hits_acc = nullptr;
-  if (nsteps > 0 && !fused) md_kick_drift(c, true, false, dt, 0);  // final velocity update
+  if (nsteps > 0 && !fused) {
+    md_kick_drift(c, true, false, dt, 0);  // final velocity update
+    total_read = false;
+  }
   c.timing = saved_timing;
-  unsigned long long *sum = c.hits_slots.p + kAccSlots * kAccStride;
-  hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, c.stream, c.hits_slots.p, sum);
-  LJG_HIP(hipGetLastError());
-  unsigned long long total = 0;
-  stream_wait(c, sum, 1, &total);
-  const int64_t hits2 = (int64_t)total;
+  if (!total_read) {
+    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, c.stream, c.hits_slots.p, sum_slots_out);
+    LJG_HIP(hipGetLastError());
+    stream_wait(c, sum_slots_out, 1, &total_hits);
+  }
+  const int64_t hits2 = (int64_t)total_hits;
   rs.steps += nsteps;
   rs.pair_interactions += hits2 / 2;
   for (size_t k = 0; k + 1 < fev.size(); k += 2) {

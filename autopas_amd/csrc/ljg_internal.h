@@ -484,7 +484,12 @@ void md_set_positions(Ctx &c, const double *pos, const double *vel);
 // a one-thread kernel copies them into pinned mapped memory and then writes a fresh marker there, on
 // which the host spins (a stream synchronisation wakes the host tens of microseconds after the work
 // ends).  After 2 s of spinning it synchronises the stream instead.  ljg_md.hip
-void stream_wait(Ctx &c, const unsigned long long *src = nullptr, int n = 0, unsigned long long *out = nullptr);
+constexpr int kMarkWords = 63;                          // words one marker read returns at most
+constexpr int kMarkFlagBoxes = 2 * (kMarkWords - 1);    // ensembles up to this size read box flags with it
+// Waits for the work queued on the stream and reads n words of src, then n2 words of src2 (device
+// memory), into out, with one marker kernel whose pinned word the host polls.
+void stream_wait(Ctx &c, const unsigned long long *src = nullptr, int n = 0, unsigned long long *out = nullptr,
+                 const unsigned long long *src2 = nullptr, int n2 = 0);
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
 // Ensemble (ljg_ensemble.hip): per-box tower grids and fixed regions for cluster size c.M, the tower

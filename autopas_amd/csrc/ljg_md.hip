@@ -466,9 +466,12 @@ __global__ __launch_bounds__(kStatThreads) void k_count_stats(int64_t n, const i
   }
 }
 
+// Copies n words of src, then n2 words of src2, behind the marker word, and writes the marker last.
 __global__ void k_mark(unsigned long long *__restrict__ dst, const unsigned long long *__restrict__ src, int32_t n,
-                       unsigned long long seq) {
-  if ((int32_t)threadIdx.x < n) dst[1 + threadIdx.x] = src[threadIdx.x];
+                       const unsigned long long *__restrict__ src2, int32_t n2, unsigned long long seq) {
+  const int32_t i = (int32_t)threadIdx.x;
+  if (i < n) dst[1 + i] = src[i];
+  else if (i < n + n2) dst[1 + i] = src2[i - n];
   __threadfence_system();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -477,10 +480,13 @@ __global__ void k_mark(unsigned long long *__restrict__ dst, const unsigned long
   }
 }
 
-void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long long *out) {
-  if (n < 0 || n > 15) throw HipError{hipErrorInvalidValue, "stream_wait reads at most 15 words", __LINE__};
+void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long long *out,
+                 const unsigned long long *src2, int n2) {
+  if (n < 0 || n2 < 0 || n + n2 > kMarkWords)
+    throw HipError{hipErrorInvalidValue, "stream_wait reads at most kMarkWords words", __LINE__};
+  static_assert(kMarkWords < 64, "k_mark copies a word per lane of one wavefront");
   const unsigned long long seq = ++c.mark_seq;
-  hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, c.stream, c.pin_mark_dev, src, (int32_t)n, seq);
+  hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, c.stream, c.pin_mark_dev, src, (int32_t)n, src2, (int32_t)n2, seq);
   LJG_HIP(hipGetLastError());
   volatile unsigned long long *m = c.pin_mark;
   const auto t0 = std::chrono::steady_clock::now();
@@ -491,7 +497,7 @@ void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long lon
       break;
     }
   }
-  for (int i = 0; i < n; ++i) out[i] = m[1 + i];
+  for (int i = 0; i < n + n2; ++i) out[i] = m[1 + i];
 }
 
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats) {

@@ -197,8 +197,12 @@ class _Lane:
         self.phase = 0  # batch steps since the last phase boundary
 
     def work_left(self):
-        """Particle-steps still to run for the jobs running and dispatched here (the dispatch criterion)."""
+        """Particle-steps still to run for the jobs running and dispatched here."""
         return sum(j.n * left for j, left in list(self.active)) + sum(j.n * j.steps for j in list(self.inbox))
+
+    def particles(self):
+        """Particles of the jobs running and dispatched here (a batch's step time grows with them)."""
+        return sum(j.n for j, left in list(self.active) if left > 0) + sum(j.n for j in list(self.inbox))
 
 
 class BatchedServer:
@@ -211,9 +215,13 @@ class BatchedServer:
     every job runs exactly its own number of steps.
 
     ``lanes`` ensemble contexts batch at once, each on its own thread and HIP stream, so the batches' launches
-    overlap on the device; an arriving job is dispatched to the lane with the least work left (particle-steps
-    of its running and waiting jobs), and each lane holds a share of the particle budget.  One lane is a
-    single batch.
+    overlap on the device; an arriving job is dispatched to the lane with the least work left (``dispatch``
+    "work": particle-steps of its running and waiting jobs) or the fewest particles ("particles"), and each
+    lane holds a share of the particle budget.  One lane is a
+    single batch.  A batch of serving jobs holds a few thousand to a few tens of thousands of clusters, where
+    the force launch is bound by the latency of one wavefront's list: one i-cluster per wavefront
+    (``tile_groups`` 1) spreads it over the most lanes (measured on MI355X: a 1-box batch runs at the
+    speed of the box's own context with it, 10-60% slower with 4).
 
     Each job keeps the rebuild schedule it has alone (its own frequency and displacement trigger, its
     layout and lists kept through every change of the batch), so its trajectory is bit-identical to the job
@@ -226,13 +234,17 @@ class BatchedServer:
     its own steps, so jobs admitted at phase boundaries are due together and the batch takes one
     partial rebuild per period instead of one every step or two (only a job whose displacement trigger fires
     leaves the common phase).  A job that arrives mid-period waits at most ``rebuild_frequency`` batch
-    steps; finished jobs are retired at any step."""
+    steps (``epoch`` steps at most when the lane runs several periods at once, as it does while no job
+    waits for it); finished jobs are retired at any step."""
 
-    def __init__(self, particle_budget=None, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=8, tile_groups=4,
-                 epoch=10, keep_results=False, bytes_per_particle=1024, max_jobs=None, shared_rebuilds=False,
-                 aligned=True, rebuild_frequency=10, lanes=1):
+    def __init__(self, particle_budget=None, kernel=engine.KERNEL_CLUSTER_MIXED, cluster_size=8, tile_groups=1,
+                 epoch=50, keep_results=False, bytes_per_particle=1024, max_jobs=None, shared_rebuilds=False,
+                 aligned=True, rebuild_frequency=10, lanes=1, dispatch="particles"):
         if lanes < 1:
             raise ValueError("a batched server needs at least one lane")
+        if dispatch not in ("work", "particles"):
+            raise ValueError("dispatch is 'work' or 'particles'")
+        self.dispatch = _Lane.work_left if dispatch == "work" else _Lane.particles
         self.kernel = kernel
         self.epoch = epoch
         self.aligned = aligned
@@ -340,7 +352,9 @@ class BatchedServer:
                 continue
             steps = min(self.epoch, min(left for _, left in active))
             if self.aligned:
-                steps = min(steps, self.frequency - lane.phase)
+                # to the next phase boundary, or with no job waiting to a later one within the epoch
+                periods = 1 if lane.inbox else max(1, self.epoch // self.frequency)
+                steps = min(steps, self.frequency - lane.phase + (periods - 1) * self.frequency)
             rs = lane.engine.step(steps, DT, self.kernel)
             lane.phase = (lane.phase + steps) % self.frequency
             with self._lock:
@@ -379,7 +393,7 @@ class BatchedServer:
                 with cond:
                     if state["error"]:
                         break
-                    lane = min(self.lanes, key=_Lane.work_left)
+                    lane = min(self.lanes, key=self.dispatch)
                     lane.inbox.append(job)
                     cond.notify_all()
         finally:

@@ -285,9 +285,9 @@ def serving(jobs, workers):
     """p50/p99 job latency: independent small boxes (N log-uniform in [256, 65536], rho in [0.4, 0.85],
     10-1000 steps) arriving as a Poisson process at 80% of the worker pool's measured throughput, served
     (a) by `workers` reused contexts, each job with the cluster-tile kernel at a tile shape chosen from its
-    size, and (b) with continuous batching in one ensemble
-    context, both on the same arrival stream; beside them the same jobs run one at a time with the plain
-    FP64 kernel (closed loop, so its latency is the bare service time)."""
+    size, and (b) with continuous batching in `workers` ensemble contexts that batch concurrently (and in
+    one), all on the same arrival stream; beside them the same jobs run one at a time with the plain FP64
+    kernel (closed loop, so its latency is the bare service time)."""
     import autopas_amd as ap
     from autopas_amd import serve
 
@@ -298,21 +298,27 @@ def serving(jobs, workers):
         wall = srv.run(base_jobs)
     base = serve.latency_summary(base_jobs, wall)
     service = np.array([j.finish - j.start for j in base_jobs]) * 1e3
-    for key, shared in (("batched", False), ("batched_shared_schedule", True)):
+    for key, shared, lanes in (("batched", False, workers), ("batched_one_context", False, 1),
+                               ("batched_shared_schedule", True, workers)):
         batched, _ = serve.serve_open_loop_batched(jobs, seed=5, rate=out["load"] * out["capacity_jobs_per_s"],
-                                                   shared_rebuilds=shared)
+                                                   shared_rebuilds=shared, lanes=lanes)
         out[key] = dict(
             p50_ms=batched["p50_ms"], p99_ms=batched["p99_ms"], mean_ms=batched["mean_ms"], jobs=batched["jobs"],
             pairs_per_s=batched["pairs_per_s"], jobs_per_s=batched["jobs_per_s"],
             arrival_rate_jobs_per_s=batched["arrival_rate_jobs_per_s"],
             capacity_jobs_per_s=batched["capacity_jobs_per_s"], load=batched["load"], epochs=batched["epochs"],
-            batch_edits=batched["batch_edits"], particle_budget=batched["particle_budget"],
-            epoch_steps=batched["epoch_steps"], kernel=batched["kernel"], schedule=batched["schedule"],
-            note="continuous batching: every running job is a box of one ensemble context, advanced in the same "
-                 "launches; jobs admitted between epochs under the HBM particle budget, finished ones retired; "
+            batch_edits=batched["batch_edits"], rebuild_passes=batched["rebuild_passes"], lanes=batched["lanes"],
+            particle_budget=batched["particle_budget"], epoch_steps=batched["epoch_steps"], kernel=batched["kernel"],
+            schedule=batched["schedule"],
+            note="continuous batching: every running job is a box of an ensemble context and advances in the "
+                 "same launches as the other jobs of its context; "
+                 + (f"{lanes} such contexts batch concurrently on their own streams, each arriving job sent to the "
+                    "one with the least work left; " if lanes > 1 else "one context; ")
+                 + "jobs admitted at the batch's rebuild-period boundaries under the HBM particle budget, finished "
+                   "ones retired at once; "
                  + ("one rebuild schedule for the batch (results equal to each job alone up to FP32 rounding)"
                     if shared else "each job keeps its own rebuild schedule (results bit-identical to the job "
-                                   "alone)"))
+                                   "alone with the same tile shape)"))
     out["fp64_one_at_a_time"] = dict(
         jobs=base["jobs"], service_p50_ms=float(np.percentile(service, 50)),
         service_p99_ms=float(np.percentile(service, 99)), pairs_per_s=base["pairs_per_s"],

@@ -314,20 +314,25 @@ def test_edit_adds_a_box_with_long_towers():
         assert np.array_equal(part[k], alone[k]), k
 
 
-def test_batched_server_results_match_jobs_alone():
+@pytest.mark.parametrize("lanes,G", [(1, 4), (3, 1)])
+def test_batched_server_results_match_jobs_alone(lanes, G):
     # continuous batching: jobs arrive over time, are admitted under a particle budget that holds only a
-    # few of them at once, and each job's end state is bit-identical to the job run alone in a context of
-    # its own (dynamic rebuild on: each job keeps its own rebuild steps through every change of the batch)
+    # few of them at once (split over `lanes` batching contexts that run concurrently), and each job's end
+    # state is bit-identical to the job run alone in a context of its own with the same tile shape (dynamic
+    # rebuild on: each job keeps its own rebuild steps through every change of its batch)
     from autopas_amd import serve
 
     jobs = serve.make_jobs(12, rate=400.0, seed=7, n_range=(256, 3000), steps_range=(10, 60))
-    announce(f"batched serving: {len(jobs)} jobs, budget 6000 particles")
-    with serve.BatchedServer(particle_budget=6000, keep_results=True, epoch=10) as srv:
+    announce(f"batched serving: {len(jobs)} jobs, budget 6000 particles, {lanes} lanes, G={G}")
+    with serve.BatchedServer(particle_budget=6000, keep_results=True, epoch=10, lanes=lanes,
+                             tile_groups=G) as srv:
         srv.run(jobs)
         assert srv.edits >= 3
+    if lanes > 1:
+        assert len({j.worker for j in jobs}) > 1
     for job in jobs:
         assert job.finish >= job.arrival and job.start >= job.arrival - 1e-9
-        with ap.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), serve.RC, serve.SKIN, cluster_size=8, tile_groups=4,
+        with ap.Engine((0.0, 0.0, 0.0), (8.0, 8.0, 8.0), serve.RC, serve.SKIN, cluster_size=8, tile_groups=G,
                        rebuild_frequency=10, dynamic_rebuild=True) as a:
             _, _, alone = serve.run_job(a, job, ap.KERNEL_CLUSTER_MIXED)
         for k in alone:
@@ -345,7 +350,7 @@ def test_batched_serving_workload_vs_oracle(oracle):
     jobs[2].n = 65536
     announce(f"batched serving workload vs oracle: {len(jobs)} jobs, N {[j.n for j in jobs]}, "
              f"steps {[j.steps for j in jobs]}")
-    with serve.BatchedServer(particle_budget=100000, keep_results=True, epoch=10) as srv:
+    with serve.BatchedServer(particle_budget=100000, keep_results=True, epoch=10, tile_groups=4) as srv:
         srv.run(jobs)
     errs = []
     for job in jobs:
@@ -385,7 +390,8 @@ def test_batched_server_shared_schedule_within_tolerance():
 
     jobs = serve.make_jobs(10, rate=400.0, seed=8, n_range=(256, 3000), steps_range=(10, 60))
     announce(f"batched serving, shared schedule: {len(jobs)} jobs, budget 6000 particles")
-    with serve.BatchedServer(particle_budget=6000, keep_results=True, epoch=10, shared_rebuilds=True) as srv:
+    with serve.BatchedServer(particle_budget=6000, keep_results=True, epoch=10, shared_rebuilds=True,
+                             tile_groups=4) as srv:
         srv.run(jobs)
         assert srv.edits >= 3
     for job in jobs:
