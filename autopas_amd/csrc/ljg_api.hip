@@ -437,12 +437,8 @@ static void mix_types(Ctx &c, int32_t ntypes, const double *eps, const double *s
   c.types_set = true;
 }
 
-ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max[3]) {
-  if (!h || !box_min || !box_max) return LJG_ERR_ARG;
-  LJG_API_BEGIN(h)
-  const double ilen = c.prm.cutoff + c.prm.skin;
-  for (int d = 0; d < 3; ++d)
-    if (!(box_max[d] - box_min[d] >= ilen)) return fail(c, LJG_ERR_ARG, "box shorter than rc + skin");
+// The periodic box the kernels work in; the particles are dropped.
+static void reset_box(Ctx &c, const double box_min[3], const double box_max[3]) {
   LJG_HIP(hipStreamSynchronize(c.stream));
   for (int d = 0; d < 3; ++d) {
     c.prm.box_min[d] = box_min[d];
@@ -452,9 +448,68 @@ ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max
   c.box_glob_valid = false;
   c.n = 0;
   c.nslots = 0;
+  c.n_owned = -1;
   c.layout = Layout::None;
   c.have_forces = false;
   c.steps_since_rebuild = 0;
+}
+
+ljg_status ljg_set_open_box(ljg_ctx *h, const double box_min[3], const double box_max[3]) {
+  if (!h || !box_min || !box_max) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  for (int d = 0; d < 3; ++d)
+    if (!(box_max[d] > box_min[d])) return fail(c, LJG_ERR_ARG, "empty box");
+  // the box and its halo region [min - ilen, max + ilen], then ilen of empty space before the periodic
+  // image of the low halo edge: two particles of the region are at least ilen apart across the seam
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = box_min[d] - ilen;
+    hi[d] = box_max[d] + 2.0 * ilen;
+  }
+  reset_box(c, lo, hi);
+  c.open = true;
+  for (int d = 0; d < 3; ++d) {
+    c.open_min[d] = box_min[d];
+    c.open_max[d] = box_max[d];
+  }
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_set_owned(ljg_ctx *h, int64_t n_owned) {
+  if (!h) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  if (c.nboxes > 0) return fail(c, LJG_ERR_UNSUPPORTED, "ownership on an ensemble of boxes");
+  if (n_owned < 0 || n_owned > c.n) return fail(c, LJG_ERR_ARG, "owned count outside [0, uploaded particles]");
+  c.n_owned = n_owned;
+  c.box_glob_valid = false;
+  return LJG_OK;
+  LJG_API_END
+}
+
+// An open box takes particles in its halo region only (host arrays, checked before any copy).
+static const char *outside_open_region(const Ctx &c, int64_t n, const double *x, const double *y,
+                                       const double *z) {
+  if (!c.open) return nullptr;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  const double *p[3] = {x, y, z};
+  for (int d = 0; d < 3; ++d) {
+    const double lo = c.open_min[d] - ilen, hi = c.open_max[d] + ilen;
+    for (int64_t i = 0; i < n; ++i)
+      if (!(p[d][i] >= lo && p[d][i] <= hi)) return "particle outside the open box and its halo region";
+  }
+  return nullptr;
+}
+
+ljg_status ljg_set_box(ljg_ctx *h, const double box_min[3], const double box_max[3]) {
+  if (!h || !box_min || !box_max) return LJG_ERR_ARG;
+  LJG_API_BEGIN(h)
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  for (int d = 0; d < 3; ++d)
+    if (!(box_max[d] - box_min[d] >= ilen)) return fail(c, LJG_ERR_ARG, "box shorter than rc + skin");
+  c.open = false;
+  reset_box(c, box_min, box_max);
   return LJG_OK;
   LJG_API_END
 }
@@ -520,6 +575,8 @@ ljg_status ljg_set_boxes(ljg_ctx *h, int32_t nboxes, const double *box_min, cons
   if (const char *why = check_boxes(c, nboxes, box_min, box_max, count, &total)) return fail(c, LJG_ERR_ARG, why);
   LJG_HIP(hipStreamSynchronize(c.stream));
   set_box_tables(c, nboxes, box_min, box_max, count, total);
+  c.open = false;  // the boxes are periodic
+  c.n_owned = -1;
   c.n = 0;
   c.nslots = 0;
   c.layout = Layout::None;
@@ -537,6 +594,7 @@ ljg_status ljg_edit_boxes(ljg_ctx *h, int32_t nkeep, const int32_t *keep, int32_
   if (nadd > 0 && (!box_min || !box_max || !count || !x || !y || !z)) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (nkeep > 0 && (c.nboxes == 0 || c.n == 0)) return fail(c, LJG_ERR_STATE, "no ensemble to keep boxes from");
+  if (c.open) return fail(c, LJG_ERR_UNSUPPORTED, "an open box is one domain, not an ensemble");
   const int32_t old_nb = c.nboxes;
   std::vector<int64_t> old_pstart((size_t)old_nb + 1, 0), new_base((size_t)old_nb, -1);
   for (int32_t b = 0; b < old_nb; ++b) old_pstart[b + 1] = old_pstart[b] + c.box_count[b];
@@ -668,7 +726,9 @@ ljg_status ljg_upload(ljg_ctx *h, int64_t n, const double *x, const double *y, c
     for (int64_t k : c.box_count) total += k;
     if (n != total) return fail(c, LJG_ERR_ARG, "particle count differs from the ensemble's boxes");
   }
+  if (const char *why = outside_open_region(c, n, x, y, z)) return fail(c, LJG_ERR_ARG, why);
   c.box_glob_valid = false;
+  c.n_owned = -1;
   c.cur = 0;
   c.st[0].ensure((size_t)n);
   c.n = n;
@@ -733,6 +793,7 @@ ljg_status ljg_update_positions(ljg_ctx *h, const double *x, const double *y, co
   if (!h || !x || !y || !z) return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (const char *why = outside_open_region(c, c.n, x, y, z)) return fail(c, LJG_ERR_ARG, why);
   const bool vel = vx && vy && vz;
   const size_t n = (size_t)c.n;
   c.part_v.ensure((vel ? 6 : 3) * n);
@@ -788,6 +849,9 @@ ljg_status ljg_force(ljg_ctx *h, int32_t kernel, int32_t calc_globals, ljg_globa
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
   if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
+  if (calc_globals && is_n3_kernel(kernel) && owned_of(c).orig)
+    return fail(c, LJG_ERR_UNSUPPORTED, "Newton-3 kernels count the globals of every particle: with halo particles "
+                                        "use a full-list kernel");
   prepare_layout(c, kernel);
   run_force(c, kernel, calc_globals != 0);
   if (c.nboxes > 0 && calc_globals) ensemble_box_globals(c);
@@ -1323,6 +1387,9 @@ ljg_status ljg_step(ljg_ctx *h, int32_t nsteps, double dt, int32_t kernel, ljg_r
     return LJG_ERR_ARG;
   LJG_API_BEGIN(h)
   if (c.n == 0) return fail(c, LJG_ERR_STATE, "no particles uploaded");
+  if (c.open)
+    return fail(c, LJG_ERR_UNSUPPORTED, "an open box takes one force evaluation at a time: the caller moves the "
+                                        "particles and exchanges halos between them");
   if (kernel != LJG_KERNEL_AUTO)
     if (const char *why = kernel_unsupported(c, kernel)) return fail(c, LJG_ERR_UNSUPPORTED, why);
   const auto t0 = std::chrono::steady_clock::now();

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_lc_fp64(
     const int32_t *__restrict__ type, const int32_t *__restrict__ cell_of, const int64_t *__restrict__ cell_start,
     CellGeom g, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
-    int32_t globals, unsigned long long *__restrict__ acc, SkipIf sk) {
+    int32_t globals, unsigned long long *__restrict__ acc, SkipIf sk, Owned own) {
   if (skip_launch(sk)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ax = 0., ay = 0., az = 0., ae = 0., av = 0.;
@@ -128,6 +128,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_lc_fp64(
     fx[i] = ax;
     fy[i] = ay;
     fz[i] = az;
+    if (globals && !is_owned(own, i)) ae = av = 0.;
   }
   block_partials(ae, av, hits, pe, pv, ph, globals != 0, acc);
 }
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
-    SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd) {
+    SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd, Owned own) {
   static_assert(!(N3 && INTEG), "the fused update needs every pair's force first");
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
@@ -402,7 +403,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     if constexpr (GLOB) {
       double se = 0., sv = 0.;
 #pragma unroll
-      for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
+      for (int q = 0; q < M; ++q)
+        if (is_owned(own, (int64_t)ic * M + q)) { se += (double)ae[q]; sv += (double)av[q]; }
       v[3 * M + 1] = se;
       v[3 * M + 2] = sv;
     }
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     uint32_t cut_hi, double rc2d, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
-    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, IntegArgs ia) {
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, IntegArgs ia, Owned own) {
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + (GLOB ? 2 : 0);
@@ -696,7 +698,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
   if constexpr (GLOB) {
     double se = 0., sv = 0.;
 #pragma unroll
-    for (int q = 0; q < M; ++q) { se += (double)ae[q]; sv += (double)av[q]; }
+    for (int q = 0; q < M; ++q)
+      if (is_owned(own, icl * M + q)) { se += (double)ae[q]; sv += (double)av[q]; }
     v[3 * M] = se;
     v[3 * M + 1] = sv;
   }
@@ -791,7 +794,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
     const int32_t *__restrict__ type, const int32_t *__restrict__ orig, int32_t row,
     const int32_t *__restrict__ lcount, const int4 *__restrict__ list, ForceArgs a, const double *__restrict__ tab, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv,
-    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk) {
+    int64_t *__restrict__ ph, unsigned long long *__restrict__ acc, SkipIf sk, Owned own) {
   if (skip_launch(sk)) return;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
   constexpr int NV = pow2_at_least(NVAL);
@@ -883,7 +886,8 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_fp64(
     if constexpr (GLOB) {
       double se = 0., sv = 0.;
 #pragma unroll
-      for (int q = 0; q < M; ++q) { se += ae[q]; sv += av[q]; }
+      for (int q = 0; q < M; ++q)
+        if (is_owned(own, (int64_t)ic * M + q)) { se += ae[q]; sv += av[q]; }
       v[3 * M + 1] = se;
       v[3 * M + 2] = sv;
     }
@@ -925,7 +929,7 @@ void lc_force_launch(hipStream_t st, int64_t n, const double *x, const double *y
                      const int32_t *type, const int32_t *cell_of, const int64_t *cell_start, const CellGeom &g,
                      double rc2, int32_t ntypes, const double *tab, double eps24, double sig2, double shift6,
                      double *fx, double *fy, double *fz, double *pe, double *pv, int64_t *ph, bool globals,
-                     unsigned long long *acc, SkipIf skip) {
+                     unsigned long long *acc, SkipIf skip, Owned own) {
   ForceArgs a{};
   for (int d = 0; d < 3; ++d) a.L[d] = g.L[d];
   a.rc2 = rc2;
@@ -936,10 +940,10 @@ void lc_force_launch(hipStream_t st, int64_t n, const double *x, const double *y
   const int nb = blocks_for(n, kFThreads);
   if (ntypes > 1)
     hipLaunchKernelGGL(k_force_lc_fp64<true>, dim3(nb), dim3(kFThreads), 0, st, n, x, y, z, type, cell_of,
-                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip);
+                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip, own);
   else
     hipLaunchKernelGGL(k_force_lc_fp64<false>, dim3(nb), dim3(kFThreads), 0, st, n, x, y, z, type, cell_of,
-                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip);
+                       cell_start, g, a, tab, fx, fy, fz, pe, pv, ph, (int32_t)globals, acc, skip, own);
   LJG_HIP(hipGetLastError());
 }
 
@@ -955,7 +959,7 @@ void lc_force(Ctx &c, bool globals) {
   if (c.timing) LJG_HIP(hipEventRecord(c.ev0, c.stream));
   lc_force_launch(c.stream, c.n, s.x.p, s.y.p, s.z.p, s.type.p, c.cell_of.p, c.cell_start.p, c.cg, c.lj.rc2,
                   c.lj.ntypes, tab, c.lj.eps24[0], c.lj.sig2[0], c.lj.shift6[0], s.fx.p, s.fy.p, s.fz.p, c.part_e.p,
-                  c.part_v.p, c.part_hits.p, globals, c.hits_acc, c.skip);
+                  c.part_v.p, c.part_hits.p, globals, c.hits_acc, c.skip, owned_of(c));
   if (c.timing) LJG_HIP(hipEventRecord(c.ev1, c.stream));
 }
 
@@ -1013,7 +1017,7 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
                      items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,             \
                      (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
                      c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off, c.part_hits.p + off, c.hits_acc, c.skip, \
-                     clg, ia, bd)
+                     clg, ia, bd, owned_of(c))
   if (n3) {
     State &s = c.st[c.cur];
     hipLaunchKernelGGL(k_zero_forces, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.fx.p,
@@ -1044,7 +1048,8 @@ static void launch_fp64(Ctx &c, bool mix, bool glob, const ForceArgs &a, const d
 #define LJG_LAUNCH(MIXV, GLOBV, N3V)                                                                              \
   hipLaunchKernelGGL((k_force_cluster_fp64<M, MIXV, GLOBV, N3V>), dim3(nb), dim3(kFThreads), 0, c.stream,        \
                      c.nclusters, s.x.p, s.y.p, s.z.p, s.type.p, s.orig.p, c.list_row, c.list_count.p, c.list.p, a, \
-                     tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
+                     tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip,    \
+                     owned_of(c))
   if (n3) {
     hipLaunchKernelGGL(k_zero_forces, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.fx.p,
                        s.fy.p, s.fz.p, c.skip);
@@ -1093,7 +1098,7 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream, \
                      c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, cut_lo, cut_hi, a.rc2, \
                      (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,    \
-                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia)
+                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia, owned_of(c))
 #define LJG_LAUNCH2(CUB)                                                                                 \
   if (integ) {                                                                                           \
     if (mix) LJG_LAUNCH(CUB, true, false, true); else LJG_LAUNCH(CUB, false, false, true);              \

@@ -144,6 +144,21 @@ __device__ inline bool skip_launch(const SkipIf &s) {
   return f != 0 && f <= s.stamp;
 }
 
+// Whose potential energy and virial count in the globals.  LJFunctor adds a pair's potential and virial
+// to the sums only for an owned i (and an owned j with Newton-3; LJFunctor.h:174-189): with halo
+// particles present, a pair of two owned particles counts whole, an owned-halo pair half and a halo-halo
+// pair not at all.  The owned particles are those whose upload index is below n; every particle when
+// orig is null.
+struct Owned {
+  const int32_t *orig;  // upload index of each slot (-1: padding)
+  int64_t n;
+};
+__device__ inline bool is_owned(const Owned &o, int64_t s) {
+  if (o.orig == nullptr) return true;
+  const int32_t k = o.orig[s];
+  return k >= 0 && (int64_t)k < o.n;
+}
+
 // The index ranges a partial ensemble build works on (the marked boxes' slots, towers or clusters):
 // segment j covers [base[j], base[j] + prefix[j+1] - prefix[j]), and a launch over prefix[n] items maps
 // item i to its index.  n == 0: the identity over every index (a whole build).
@@ -261,6 +276,12 @@ struct Ctx {
 
   int64_t n = 0;       // real particles
   int64_t nslots = 0;  // storage slots in the current layout
+  // Open box (ljg_set_open_box): the domain [open_min, open_max) and its halo region, embedded in the
+  // periodic box prm.box_min/box_max with ilen of empty space beyond the halo, so that no pair meets
+  // across the periodic seam.  n_owned: the first n_owned particles of the upload are owned (-1: all).
+  bool open = false;
+  double open_min[3] = {0., 0., 0.}, open_max[3] = {0., 0., 0.};
+  int64_t n_owned = -1;
   Layout layout = Layout::None;
   State st[2];
   int cur = 0;
@@ -440,7 +461,7 @@ void lc_force_launch(hipStream_t st, int64_t n, const double *x, const double *y
                      const int32_t *type, const int32_t *cell_of, const int64_t *cell_start, const CellGeom &g,
                      double rc2, int32_t ntypes, const double *tab, double eps24, double sig2, double shift6,
                      double *fx, double *fy, double *fz, double *pe, double *pv, int64_t *ph, bool globals,
-                     unsigned long long *acc, SkipIf skip);
+                     unsigned long long *acc, SkipIf skip, Owned owned = Owned{nullptr, 0});
 // Exclusive scan on a given stream with caller-provided temporary storage of scan_tmp_len(n) entries.
 int64_t scan_tmp_len(int64_t n);  // ljg_md.hip
 void exclusive_scan_i32_to_i64_on(hipStream_t st, const int32_t *in, int64_t *out, int64_t n, int64_t *tmp,
@@ -542,6 +563,13 @@ inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) /
 
 // The FP32 cluster kernel's two position buffers change roles (its fused integrator wrote the next step's
 // positions into the second): the FP32 and the FP64 copies always together.
+// The ownership the force kernels mask the globals with: none (every particle counts) unless the caller
+// marked fewer owned particles than there are.
+inline Owned owned_of(Ctx &c) {
+  if (c.n_owned < 0 || c.n_owned >= c.n) return Owned{nullptr, 0};
+  return Owned{c.st[c.cur].orig.p, c.n_owned};
+}
+
 inline void swap_cluster_positions(Ctx &c) {
   std::swap(c.pos32, c.pos32_next);
   std::swap(c.posd, c.posd_next);

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
     float sig2_1, float shift6_1,
     int32_t ntypes, const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz, double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph,
-    unsigned long long *__restrict__ acc, SkipIf sk) {
+    unsigned long long *__restrict__ acc, SkipIf sk, Owned own) {
   if (skip_launch(sk)) return;
   __shared__ float stab[MIX ? 3 * kMaxTypes * kMaxTypes : 1];
   if constexpr (MIX) {
@@ -267,8 +267,9 @@ __global__ __launch_bounds__(kVThreads) void k_force_verlet(
   // per-block partials (fixed tree) and the pair count
   __shared__ double se[kVThreads], sv[kVThreads];
   __shared__ int64_t sh[kVThreads];
-  se[threadIdx.x] = GLOB ? dae : 0.;
-  sv[threadIdx.x] = GLOB ? dav : 0.;
+  const bool counted = GLOB && active && is_owned(own, i);
+  se[threadIdx.x] = counted ? dae : 0.;
+  sv[threadIdx.x] = counted ? dav : 0.;
   sh[threadIdx.x] = active ? hits : 0;
   __syncthreads();
   for (int s = kVThreads / 2; s > 0; s >>= 1) {
@@ -308,7 +309,8 @@ void verlet_force(Ctx &c, bool globals) {
   hipLaunchKernelGGL((k_force_verlet<CUB, MIXV, GLOBV>), dim3(nb), dim3(kVThreads), 0, c.stream, c.n, c.posfx.p, \
                      c.vrow, c.vcount.p, c.vlist.p, f, cut_lo, cut_hi, c.lj.rc2, (float)c.lj.eps24[0], sig2,      \
                      (float)c.lj.shift6[0],                                                                       \
-                     c.lj.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip)
+                     c.lj.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p, c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, \
+                     owned_of(c))
 #define LJG_LAUNCH2(CUB)                                                 \
   if (mix) {                                                             \
     if (globals) LJG_LAUNCH(CUB, true, true); else LJG_LAUNCH(CUB, true, false); \

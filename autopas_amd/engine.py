@@ -94,6 +94,8 @@ SIGNATURES = {
     "ljg_device_name": (_i32, [_p, ctypes.c_char_p, _i32]),
     "ljg_memory_info": (_i32, [_p, _p, _p]),
     "ljg_set_box": (_i32, [_p, _p, _p]),
+    "ljg_set_open_box": (_i32, [_p, _p, _p]),
+    "ljg_set_owned": (_i32, [_p, _i64]),
     "ljg_set_boxes": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_get_box_globals": (_i32, [_p, _i32, _p, _p, _p]),
     "ljg_edit_boxes": (_i32, [_p, _i32, _p, _i32, _p, _p, _p] + [_p] * 8),
@@ -313,6 +315,22 @@ class Engine:
         self.nboxes = 0
         self.n = 0
         self._types = self._ids = None
+
+    def set_open_box(self, box_min, box_max):
+        """Open (non-periodic) box with halo particles: owned particles in [box_min, box_max), halo
+        particles (made by the caller) up to rc + skin outside it; no periodic images.  The particles are
+        dropped; step() is refused on an open box."""
+        lo = np.ascontiguousarray(box_min, dtype=np.float64)
+        hi = np.ascontiguousarray(box_max, dtype=np.float64)
+        self._check(self.lib.ljg_set_open_box(self.h, _ptr(lo), _ptr(hi)), "ljg_set_open_box")
+        self.nboxes = 0
+        self.n = 0
+        self._types = self._ids = None
+
+    def set_owned(self, n_owned):
+        """The first n_owned particles of the last upload are owned, the rest halo: the globals count a pair
+        as LJFunctor does for owned particles (owned-halo pairs half, halo-halo pairs not at all)."""
+        self._check(self.lib.ljg_set_owned(self.h, int(n_owned)), "ljg_set_owned")
 
     def set_boxes(self, box_min, box_max, counts):
         """Turn the context into an ensemble of independent periodic boxes: box b spans

@@ -13,9 +13,14 @@
  *   for (auto &p : autoPas) { ... }                         AutoPasDecl.h:280
  *
  * compiles against the namespace autopas_amd with the same calls.  What differs:
- *   - the box is periodic inside the engine: updateContainer() returns no leaving particles, and the
- *     positions come back wrapped into the box after each list rebuild (md-flexible's periodic wrap,
- *     RegularGridDecomposition.cpp:563-587, done on the device);
+ *   - by default the box is periodic inside the engine: updateContainer() returns no leaving particles,
+ *     and the positions come back wrapped into the box after each list rebuild (md-flexible's periodic
+ *     wrap, RegularGridDecomposition.cpp:563-587, done on the device).  setPeriodicBoundaries(false)
+ *     gives AutoPas's own model instead: an open box, halo particles added by the caller
+ *     (addHaloParticle, e.g. md-flexible's halo exchange), and updateContainer() returning the owned
+ *     particles that left the box and deleting the halo particles (LogicHandler.h:165-218);
+ *   - begin()/end() iterate the owned particles (forEach and getNumberOfParticles take an
+ *     IteratorBehavior);
  *   - the functor's per-pair and per-SoA entry points are not provided (the GPU evaluates the whole
  *     container); they throw "not implemented" like PairwiseFunctor's defaults (PairwiseFunctor.h:56-104);
  *   - step(n, dt) runs n velocity-Verlet steps on the device (the engine's own loop);
@@ -68,6 +73,9 @@ inline void check(const ljg_ctx *ctx, ljg_status st, const char *what) {
 
 /* autopas::OwnershipState (OwnershipState.h:20-29). */
 enum class OwnershipState : int64_t { dummy = 0, owned = 1, halo = 2 };
+
+/* autopas::IteratorBehavior (IteratorBehavior.h:34-42): which particles an iteration or count covers. */
+enum class IteratorBehavior : int { owned = 0b01, halo = 0b10, ownedOrHalo = 0b11 };
 
 /* autopas::FunctorN3Modes (Functor.h). */
 enum class FunctorN3Modes { Newton3Only, Newton3Off, Both };
@@ -509,8 +517,8 @@ class AutoTuner {
   size_t _tuningPhase = 0, _phaseEnd = 0;
 };
 
-/* autopas::AutoPas<Particle_T> (AutoPasDecl.h) for the Lennard-Jones path: owned particles in a periodic
- * box, kept host-side in insertion order and mirrored on the device. */
+/* autopas::AutoPas<Particle_T> (AutoPasDecl.h) for the Lennard-Jones path: owned particles (and, in an open
+ * box, halo particles) kept host-side in insertion order and mirrored on the device. */
 template <class Particle_T>
 class AutoPas {
  public:
@@ -540,6 +548,10 @@ class AutoPas {
   unsigned int getVerletRebuildFrequency() const { return _rebuildFrequency; }
   void setVerletClusterSize(unsigned int m) { _clusterSize = m; }
   unsigned int getVerletClusterSize() const { return _clusterSize; }
+  /* true (the default): the box is periodic inside the engine.  false: an open box with caller-made halo
+   * particles, as AutoPas's containers hold them (takes effect at init() or resizeBox()). */
+  void setPeriodicBoundaries(bool periodic) { _periodic = periodic; }
+  bool getPeriodicBoundaries() const { return _periodic; }
 
   /* GPU layout (LJG_KERNEL_*); -1 (default): the j-row kernel when every box side exceeds 2 (rc + skin),
    * else the FP32 cluster kernel.  LJG_KERNEL_AUTO: computeInteractions picks the configuration with the
@@ -585,6 +597,8 @@ class AutoPas {
     ljg_ctx *ctx = nullptr;
     detail::check(nullptr, ljg_create(&p, &ctx), "AutoPas::init: ljg_create");
     _ctx = ctx;
+    if (!_periodic)
+      detail::check(_ctx, ljg_set_open_box(_ctx, _boxMin.data(), _boxMax.data()), "AutoPas::init: open box");
     _shift = 0;
     _structureChanged = true;
   }
@@ -592,7 +606,10 @@ class AutoPas {
   /* AutoPas::resizeBox (AutoPasDecl.h:116): particles outside the new box are removed and returned. */
   std::vector<Particle_T> resizeBox(const std::array<double, 3> &boxMin, const std::array<double, 3> &boxMax) {
     requireInit("resizeBox");
-    detail::check(_ctx, ljg_set_box(_ctx, boxMin.data(), boxMax.data()), "AutoPas::resizeBox");
+    detail::check(_ctx,
+                  _periodic ? ljg_set_box(_ctx, boxMin.data(), boxMax.data())
+                            : ljg_set_open_box(_ctx, boxMin.data(), boxMax.data()),
+                  "AutoPas::resizeBox");
     _boxMin = boxMin;
     _boxMax = boxMax;
     std::vector<Particle_T> kept, left;
@@ -607,17 +624,44 @@ class AutoPas {
     if (!inBox(p.getR()))
       throw AutoPasException("LogicHandler: Trying to add a particle that is not in the bounding box.");
     _particles.push_back(p);
+    _particles.back().setOwnershipState(OwnershipState::owned);
     _structureChanged = true;
+  }
+  /* AutoPas::addHaloParticle (AutoPasDecl.h:203, LogicHandler.h:366-391): a copy of another domain's
+   * particle or a periodic image, outside the box and within rc + skin of it.  Open boxes only: a periodic
+   * box makes its images on the device. */
+  void addHaloParticle(const Particle_T &p) {
+    if (_periodic)
+      throw AutoPasException("AutoPas::addHaloParticle: the box is periodic (setPeriodicBoundaries(false) for halo "
+                             "particles)");
+    if (inBox(p.getR()))
+      throw AutoPasException("LogicHandler: Trying to add a halo particle that is not outside the box of the container.");
+    _halo.push_back(p);
+    _halo.back().setOwnershipState(OwnershipState::halo);
   }
   void deleteAllParticles() {
     _particles.clear();
+    _halo.clear();
     _structureChanged = true;
   }
-  size_t getNumberOfParticles() const { return _particles.size(); }
+  /* AutoPas::getNumberOfParticles (AutoPasDecl.h:559). */
+  size_t getNumberOfParticles(IteratorBehavior behavior = IteratorBehavior::owned) const {
+    return (((int)behavior & (int)IteratorBehavior::owned) ? _particles.size() : 0) +
+           (((int)behavior & (int)IteratorBehavior::halo) ? _halo.size() : 0);
+  }
 
-  /* AutoPas::updateContainer (AutoPasDecl.h:140).  The engine's box is periodic: no particle leaves, and
-   * positions are wrapped into the box on the device at the next list rebuild. */
-  std::vector<Particle_T> updateContainer() { return {}; }
+  /* AutoPas::updateContainer (AutoPasDecl.h:140, LogicHandler.h:165-218).  Periodic box: no particle
+   * leaves, and positions are wrapped into the box on the device at the next list rebuild.  Open box: the
+   * owned particles outside the box are removed and returned, and every halo particle is deleted. */
+  std::vector<Particle_T> updateContainer() {
+    if (_periodic) return {};
+    std::vector<Particle_T> kept, left;
+    for (auto &p : _particles) (inBox(p.getR()) ? kept : left).push_back(p);
+    if (!left.empty()) _structureChanged = true;
+    _particles = std::move(kept);
+    _halo.clear();
+    return left;
+  }
 
   /* AutoPas::computeInteractions (AutoPasImpl.h:116-135, LogicHandler.h:1066-1143): the functor's pair
    * forces are added to the particles' forces, and the functor's globals are set.  Lists are rebuilt as
@@ -726,9 +770,13 @@ class AutoPas {
   iterator end() { return _particles.end(); }
   const_iterator begin() const { return _particles.begin(); }
   const_iterator end() const { return _particles.end(); }
+  /* AutoPas::forEach (AutoPasDecl.h:318): owned particles, then halo particles. */
   template <class Lambda>
-  void forEach(Lambda &&fn) {
-    for (auto &p : _particles) fn(p);
+  void forEach(Lambda &&fn, IteratorBehavior behavior = IteratorBehavior::ownedOrHalo) {
+    if ((int)behavior & (int)IteratorBehavior::owned)
+      for (auto &p : _particles) fn(p);
+    if ((int)behavior & (int)IteratorBehavior::halo)
+      for (auto &p : _halo) fn(p);
   }
 
   ljg_ctx *context() { return _ctx; }
@@ -760,17 +808,19 @@ class AutoPas {
       eps = ppl->epsilons();
       sig = ppl->sigmas();
       mass = ppl->masses();
-      for (const auto &p : _particles)
-        if (p.getTypeId() >= eps.size())
-          throw AutoPasException("AutoPas::computeInteractions: particle type " + std::to_string(p.getTypeId()) +
-                                 " is not registered in the ParticlePropertiesLibrary");
+      for (const auto *v : {&_particles, &_halo})
+        for (const auto &p : *v)
+          if (p.getTypeId() >= eps.size())
+            throw AutoPasException("AutoPas::computeInteractions: particle type " + std::to_string(p.getTypeId()) +
+                                   " is not registered in the ParticlePropertiesLibrary");
     } else {
       eps = {f.getEpsilon24() / 24.};
       sig = {std::sqrt(f.getSigmaSquared())};
       mass = {1.};
-      for (const auto &p : _particles)
-        if (p.getTypeId() != 0)
-          throw AutoPasException("AutoPas::computeInteractions: a functor without mixing needs type 0 particles");
+      for (const auto *v : {&_particles, &_halo})
+        for (const auto &p : *v)
+          if (p.getTypeId() != 0)
+            throw AutoPasException("AutoPas::computeInteractions: a functor without mixing needs type 0 particles");
     }
     if (eps != _eps || sig != _sig || mass != _mass) {
       detail::check(_ctx, ljg_set_types(_ctx, (int32_t)eps.size(), eps.data(), sig.data(), mass.data()),
@@ -787,28 +837,34 @@ class AutoPas {
   }
 
   /* Host particles -> device: a full upload after a change of the particle set, else positions and
-   * velocities into the current layout (which also tells whether a particle moved skin/2). */
+   * velocities into the current layout (which also tells whether a particle moved skin/2).  An open box
+   * uploads the owned particles followed by the halo particles; the halo set counts as unchanged while the
+   * ids come in the same order (md-flexible re-adds the same halo copies every iteration between
+   * rebuilds, as AutoPas's updateHaloParticle expects, LogicHandler.h:380-388). */
   void pushState() {
-    const size_t n = _particles.size();
+    const size_t no = _particles.size(), n = no + _halo.size();
+    auto at = [&](size_t i) -> const Particle_T & { return i < no ? _particles[i] : _halo[i - no]; };
     std::vector<double> a(6 * n);
+    std::vector<int64_t> id(n);
     for (size_t i = 0; i < n; ++i) {
-      const auto &p = _particles[i];
+      const auto &p = at(i);
       for (int d = 0; d < 3; ++d) {
         a[d * n + i] = p.getR()[d];
         a[(3 + d) * n + i] = p.getV()[d];
       }
+      id[i] = (int64_t)p.getID();
     }
+    if (!_periodic && (no != _uploadedOwned || id != _uploadedIds)) _structureChanged = true;
     if (_structureChanged) {
       std::vector<int32_t> type(n);
-      std::vector<int64_t> id(n);
-      for (size_t i = 0; i < n; ++i) {
-        type[i] = (int32_t)_particles[i].getTypeId();
-        id[i] = (int64_t)_particles[i].getID();
-      }
+      for (size_t i = 0; i < n; ++i) type[i] = (int32_t)at(i).getTypeId();
       detail::check(_ctx,
                     ljg_upload(_ctx, (int64_t)n, &a[0], &a[n], &a[2 * n], &a[3 * n], &a[4 * n], &a[5 * n],
                                type.data(), id.data()),
                     "AutoPas: ljg_upload");
+      if (!_periodic) detail::check(_ctx, ljg_set_owned(_ctx, (int64_t)no), "AutoPas: ljg_set_owned");
+      _uploadedOwned = no;
+      _uploadedIds = std::move(id);
       _structureChanged = false;
       _rebuildDue = true;
     } else {
@@ -820,17 +876,17 @@ class AutoPas {
     }
   }
 
-  /* Device -> host particles: positions (wrapped at rebuilds), and either the pair forces added to the
-   * particles' forces (computeInteractions) or the whole state (step). */
+  /* Device -> host particles: positions (wrapped at rebuilds in a periodic box), and either the pair
+   * forces added to the owned particles' forces (computeInteractions) or the whole state (step). */
   void pullState(bool addForces) {
-    const size_t n = _particles.size();
+    const size_t no = _particles.size(), n = no + _halo.size();
     std::vector<double> a(9 * n);
     detail::check(_ctx,
                   ljg_download(_ctx, &a[0], &a[n], &a[2 * n], addForces ? nullptr : &a[3 * n],
                                addForces ? nullptr : &a[4 * n], addForces ? nullptr : &a[5 * n], &a[6 * n],
                                &a[7 * n], &a[8 * n]),
                   "AutoPas: ljg_download");
-    for (size_t i = 0; i < n; ++i) {
+    for (size_t i = 0; i < no; ++i) {
       auto &p = _particles[i];
       p.setR({a[i], a[n + i], a[2 * n + i]});
       const std::array<double, 3> f{a[6 * n + i], a[7 * n + i], a[8 * n + i]};
@@ -851,7 +907,11 @@ class AutoPas {
   int32_t _kernel = -1;
   int32_t _device = 0;
   ljg_ctx *_ctx = nullptr;
+  bool _periodic = true;
   std::vector<Particle_T> _particles;
+  std::vector<Particle_T> _halo;       // open box: halo particles since the last updateContainer()
+  size_t _uploadedOwned = 0;           // open box: owned count and ids (owned, then halo) of the last upload
+  std::vector<int64_t> _uploadedIds;
   bool _structureChanged = true;
   bool _rebuildDue = true;
   unsigned int _stepsSinceRebuild = 0;

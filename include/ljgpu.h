@@ -133,6 +133,27 @@ ljg_status ljg_memory_info(const ljg_ctx *ctx, uint64_t *free_bytes, uint64_t *t
  * kept, so one context can serve many jobs of different sizes without reallocating. */
 ljg_status ljg_set_box(ljg_ctx *ctx, const double box_min[3], const double box_max[3]);
 
+/* Open (non-periodic) box with halo particles, the model of AutoPas's own containers: owned particles
+ * lie in [box_min, box_max) and halo particles -- copies of other domains' particles or periodic images,
+ * made by the caller as md-flexible's halo exchange does -- in the halo region up to rc + skin outside it
+ * (AutoPas::addHaloParticle, AutoPasDecl.h:196; LinkedCells.h:87, VerletClusterLists.h:194).  Every pair
+ * within rc interacts directly, without periodic images.  Particles of ljg_upload and
+ * ljg_update_positions must lie in [box_min - (rc + skin), box_max + rc + skin]; others are refused
+ * (LJG_ERR_ARG).  Forces come back for every particle.  ljg_step is refused: the caller advances the
+ * particles and exchanges halos between force evaluations.  Every kernel runs on an open box (inside the
+ * engine the box and its halo region sit in a periodic box with rc + skin of empty space beyond the halo,
+ * so that no pair meets across the periodic boundary).  The particles are dropped, as by ljg_set_box,
+ * which returns the context to a periodic box. */
+ljg_status ljg_set_open_box(ljg_ctx *ctx, const double box_min[3], const double box_max[3]);
+
+/* Ownership for the globals (AutoPas's OwnershipState, OwnershipState.h:20-30): the first n_owned particles
+ * of the last upload are owned, the others halo.  Energy and virial then count as LJFunctor counts them
+ * for owned i (LJFunctor.h:174-189): a pair of two owned particles whole, an owned-halo pair half, a
+ * halo-halo pair not at all; the pair count still counts every pair.  ljg_upload resets it to every
+ * particle owned.  With fewer owned than uploaded particles the Newton-3 kernels refuse globals
+ * (LJG_ERR_UNSUPPORTED). */
+ljg_status ljg_set_owned(ljg_ctx *ctx, int64_t n_owned);
+
 /* Ensemble of independent periodic boxes in one context.  md-flexible runs one AutoPas instance per
  * simulation (AutoPasImpl.h:55-113); here the boxes share every launch.  Box b spans
  * [box_min[3b..3b+2], box_max[3b..3b+2]) and holds count[b] particles.  The next ljg_upload takes the

@@ -9,6 +9,10 @@
 //   autopas_api_test loop <file>       md-flexible's step sequence through the API (host integrator,
 //                                      TimeDiscretization.cpp:17-165) next to AutoPas::step on the device;
 //                                      writes both end states
+//   autopas_api_test halo <file> <k>   the same host steps on an open box with md-flexible's periodic
+//                                      boundary made by the caller (leaving particles wrapped and re-added,
+//                                      periodic images added as halo particles) next to the periodic box;
+//                                      writes both end states and globals
 //
 // Every mode prints "ok <mode>" on success and exits 0; a failed check prints the reason and exits 1.
 #include <cmath>
@@ -223,6 +227,91 @@ int mode_loop(const char *path, int32_t kernel) {
   return 0;
 }
 
+// md-flexible's periodic boundary on an open box (RegularGridDecomposition.cpp:515-587): after the drift,
+// updateContainer returns the owned particles that left the box, which are wrapped back and re-added, and
+// every periodic image within rc + skin of the box is added as a halo particle; then computeInteractions
+// and the kick on the owned particles.
+template <class Functor>
+void host_step_open(AutoPas<MoleculeLJ> &ap, Functor &f, double dt, double L) {
+  for (auto &p : ap) {
+    const double ff = dt * dt / (2 * 1.0);
+    const auto &v = p.getV();
+    const auto &F = p.getF();
+    p.addR({v[0] * dt + F[0] * ff, v[1] * dt + F[1] * ff, v[2] * dt + F[2] * ff});
+    p.setOldF(F);
+    p.setF({0., 0., 0.});
+  }
+  for (auto p : ap.updateContainer()) {
+    std::array<double, 3> r = p.getR();
+    for (int d = 0; d < 3; ++d) {
+      if (r[d] < 0.) r[d] += L;
+      else if (r[d] >= L) r[d] -= L;
+    }
+    p.setR(r);
+    ap.addParticle(p);
+  }
+  const double w = kCutoff + kSkin;
+  std::vector<MoleculeLJ> halo;
+  for (auto &p : ap)
+    for (int sx = -1; sx <= 1; ++sx)
+      for (int sy = -1; sy <= 1; ++sy)
+        for (int sz = -1; sz <= 1; ++sz) {
+          if (sx == 0 && sy == 0 && sz == 0) continue;
+          const std::array<double, 3> r{p.getR()[0] + sx * L, p.getR()[1] + sy * L, p.getR()[2] + sz * L};
+          bool near = true;
+          for (int d = 0; d < 3; ++d) near = near && r[d] >= -w && r[d] <= L + w;
+          if (!near) continue;
+          MoleculeLJ h = p;
+          h.setR(r);
+          halo.push_back(h);
+        }
+  for (auto &h : halo) ap.addHaloParticle(h);
+  ap.computeInteractions(&f);
+  for (auto &p : ap) {
+    const double kf = dt / (2 * 1.0);
+    const auto &F = p.getF();
+    const auto &oF = p.getOldF();
+    p.addV({(F[0] + oF[0]) * kf, (F[1] + oF[1]) * kf, (F[2] + oF[2]) * kf});
+  }
+}
+
+int mode_halo(const char *path, int32_t kernel) {
+  FILE *out = std::fopen(path, "w");
+  if (!out) return fail("cannot open output");
+  const Lattice s = lattice(8, 0.75, 9, 1);
+  const double dt = 0.002;
+  const int steps = 30;
+  autopas_amd::LJFunctor<MoleculeLJ, true, false, FunctorN3Modes::Both, true> f(kCutoff);
+  f.setParticleProperties(24., 1.);
+  AutoPas<MoleculeLJ> open;
+  open.setPeriodicBoundaries(false);
+  setup(open, s.L, kernel);
+  add_all(open, s, false);
+  // halo particles belong to open boxes only, and must lie outside the box
+  if (!throws([&] { open.addHaloParticle(MoleculeLJ({1., 1., 1.}, {0., 0., 0.}, 0, 0)); }, "not outside the box"))
+    return fail("a halo particle inside the box was accepted");
+  size_t halo = 0;
+  for (int k = 0; k < steps; ++k) {
+    host_step_open(open, f, dt, s.L);
+    halo += open.getNumberOfParticles(autopas_amd::IteratorBehavior::halo);
+  }
+  const double eo = f.getPotentialEnergy(), vo = f.getVirial();
+  if (open.getNumberOfParticles() != s.x.size()) return fail("owned particles lost on the open box");
+  std::fprintf(out, "globals %.17g %.17g %zu %zu\n", eo, vo, halo / steps, open.getNumRebuilds());
+  write_state(out, "open", open);
+  AutoPas<MoleculeLJ> per;
+  setup(per, s.L, kernel);
+  add_all(per, s, false);
+  if (!throws([&] { per.addHaloParticle(MoleculeLJ({-1., 1., 1.}, {0., 0., 0.}, 0, 0)); }, "periodic"))
+    return fail("a halo particle was accepted by a periodic box");
+  for (int k = 0; k < steps; ++k) host_step(per, f, dt);
+  std::fprintf(out, "globals %.17g %.17g 0 %zu\n", f.getPotentialEnergy(), f.getVirial(), per.getNumRebuilds());
+  write_state(out, "periodic", per);
+  std::fclose(out);
+  std::printf("ok halo\n");
+  return 0;
+}
+
 // LogicHandler's rebuild rule through computeInteractions (LogicHandler.h:987-1015): K calls that move
 // every particle by much less than skin/2 rebuild on the first call and then every rebuild frequency
 // calls; one particle moved by skin/2 since the last rebuild forces a rebuild at the next call.
@@ -314,6 +403,7 @@ int main(int argc, char **argv) {
     if (mode == "tune" && argc >= 3) return mode_tune(argv[2]);
     if (mode == "forces" && argc >= 3) return mode_forces(argv[2]);
     if (mode == "loop" && argc >= 3) return mode_loop(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
+    if (mode == "halo" && argc >= 3) return mode_halo(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
   } catch (const AutoPasException &e) {
     return fail(std::string("AutoPasException: ") + e.what());
   }

@@ -3,7 +3,8 @@
 CPU: misuse raises AutoPasException with the reference's messages; init() without a device throws.
 GPU: computeInteractions forces and globals against the FP64 direct sum (RMS relative force error <= 1e-5,
 energy and virial 1e-5 relative, for the FP32-pair kernels); md-flexible's step sequence through the API
-(host integrator, device forces) against AutoPas::step on the device.  The two loops build their lists
+(host integrator, device forces) against AutoPas::step on the device; the same host steps on an open box
+with md-flexible's periodic boundary made by the caller (halo particles) against the periodic box.  The two loops build their lists
 at different points (the host loop after the first drift, the device loop before it), so they agree to
 the rounding of the FP32 pair term: positions within 1e-6, velocities within 1e-4 after 25 steps; with
 the FP64 cluster kernel within 1e-10 and 1e-8.
@@ -151,3 +152,28 @@ def test_md_loop_through_api_matches_device_steps(driver, tmp_path, kernel, tol_
     assert np.max(np.abs(h[:, 4:7] - d[:, 4:7])) <= tol_v
     # the loop moved the particles: not a trivially equal pair of initial states
     assert np.max(np.abs(h[:, 4:7])) > 0.1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,tol_x,tol_e", [(0, 1e-9, 1e-10), (4, 1e-6, 1e-5)])
+def test_open_box_with_halo_exchange_matches_periodic_box(driver, tmp_path, kernel, tol_x, tol_e):
+    # md-flexible's periodic boundary through AutoPas's own model: an open box, leaving particles wrapped
+    # and re-added, periodic images added as halo particles each step.  Same trajectory and globals as the
+    # periodic box (the two build different cell grids, so the sums round differently: kernel 0 is the FP64
+    # linked-cells kernel, 4 the FP32-pair j-row kernel)
+    announce(f"C++ API open box with halo exchange vs periodic box: kernel {kernel}, 8^3 lattice, 30 steps")
+    path = str(tmp_path / "halo.txt")
+    run(driver, "halo", path, str(kernel))
+    s = read_states(path)
+    (eo, vo, halo, _), (ep, vp, _, _) = s["globals"]
+    assert halo > 1000  # the halo shell of an 8.8-sigma box
+    assert abs(eo - ep) <= tol_e * abs(ep) and abs(vo - vp) <= tol_e * abs(vp)
+    o = s["open"][np.argsort(s["open"][:, 0])]
+    p = s["periodic"][np.argsort(s["periodic"][:, 0])]
+    assert np.array_equal(o[:, 0], p[:, 0])
+    L = 8 * 0.75 ** (-1.0 / 3.0)
+    dx = o[:, 1:4] - p[:, 1:4]
+    dx -= L * np.round(dx / L)
+    assert np.max(np.abs(dx)) <= tol_x
+    assert np.max(np.abs(o[:, 4:7] - p[:, 4:7])) <= 100 * tol_x
+    assert np.max(np.abs(p[:, 4:7])) > 0.1  # the particles moved
