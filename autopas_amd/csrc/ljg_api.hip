@@ -1099,7 +1099,27 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     const bool last_chunk = !due && s + k == nsteps;
     bool sum_read = false;
     if (rebuild_next) {
+      const unsigned long long marks = c.mark_seq;
       rebuild_now();
+      if (force_next && (!dyn || c.mark_seq != marks)) {
+        // The build waited for the stream, so every drift queued before it has run and the host copy of
+        // the flag is current.  Raised by no drift, or only by the one of step s + k (the rebuild just made
+        // answers it): the force of step s + k runs, and the next chunk queues behind it without a wait.
+        const int32_t early = dyn ? static_cast<volatile int32_t *>(c.host_flag)[0] : 0;
+        if (early == 0 || early == stamp_of(s + k)) {
+          if (early != 0) {
+            static_cast<volatile int32_t *>(c.host_flag)[0] = 0;
+            hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p, (int32_t *)nullptr);
+            LJG_HIP(hipGetLastError());
+          }
+          force_step(s + k, true, stamp_of(s + k) - 1);
+          c.steps_since_rebuild = 1;
+          for (auto &v : c.box_since) v = 1;
+          total_read = false;
+          s += k + 1;
+          continue;
+        }
+      }
       if (force_next) force_step(s + k, true, stamp_of(s + k) - 1);
       stream_wait(c);
     } else if (dyn || due) {
