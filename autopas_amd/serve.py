@@ -331,6 +331,9 @@ class BatchedServer:
                 lane.phase = 0  # an empty batch starts a new phase
             admit = []
             with cond:
+                # the admission time is read under the lock the feeder appends under: a job that arrived
+                # after `now` above is admitted at a later time than its arrival
+                now = time.perf_counter() - t0
                 while (lane.phase == 0 or not self.aligned) and lane.inbox and (
                         self.max_jobs is None or len(keep) + len(admit) < self.max_jobs) and (
                         inflight + lane.inbox[0].n <= self.lane_budget or not (keep or admit)):
