@@ -222,9 +222,12 @@ static bool prepare_layout(Ctx &c, int kernel) {
 }
 
 static bool layout_matches(const Ctx &c, int kernel) {
-  if (is_cluster_kernel(kernel))
+  if (is_cluster_kernel(kernel)) {
+    const bool entries = kernel != LJG_KERNEL_CLUSTER_JROWS_MIXED && !is_n3_kernel(kernel);
     return c.layout == Layout::Clusters && c.built_M == c.M &&
-           c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) && c.lists_n3 == is_n3_kernel(kernel);
+           c.jrows_valid == (kernel == LJG_KERNEL_CLUSTER_JROWS_MIXED) && c.lists_n3 == is_n3_kernel(kernel) &&
+           (!entries || c.xrows_valid == !c.prm.whole_cluster_rows);
+  }
   if (kernel == LJG_KERNEL_VERLET_MIXED) return c.layout == Layout::Verlet;
   return c.layout == Layout::Cells;
 }
@@ -334,7 +337,7 @@ ljg_status ljg_destroy(ljg_ctx *h) {
   c.cell_of_particle.release(); c.tower_of.release(); c.tower_count.release(); c.tower_start.release();
   c.tower_items.release(); c.tower_z.release(); c.tower_id.release(); c.ctow_count.release(); c.ctow_start.release(); c.cluster_tower.release();
   c.bbox.release(); c.c0.release(); c.list_count.release(); c.list_stats.release(); c.list.release();
-  c.pos32.release(); c.pos32_next.release(); c.posd.release(); c.posd_next.release(); c.pos32_ext.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
+  c.pos32.release(); c.pos32_next.release(); c.posd.release(); c.posd_next.release(); c.pos32_ext.release(); c.posfx.release(); c.jcl.release(); c.jlist.release(); c.jcount.release(); c.xrow.release(); c.xcount.release(); c.xrow_tmp.release(); c.xcount_tmp.release(); c.tstats.release(); c.posfx_next.release(); c.hits_slots.release(); c.vcount.release(); c.vlist.release(); c.tab.release(); c.part_e.release(); c.part_v.release(); c.part_hits.release();
   c.red_out.release(); c.red_hits.release(); c.flag.release(); c.scan_tmp.release();
   c.box_tg.release(); c.box_wrap.release(); c.box_of.release(); c.box_tbase.release(); c.tower_box.release();
   c.box_cbase.release(); c.box_pbase.release(); c.cluster_box.release();
@@ -1525,6 +1528,26 @@ ljg_status ljg_get_clusters(ljg_ctx *h, int64_t *slot_particle, int32_t *cluster
     }
   }
   if (bbox) LJG_HIP(hipMemcpy(bbox, c.bbox.p, (size_t)c.nclusters * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  return LJG_OK;
+  LJG_API_END
+}
+
+ljg_status ljg_get_member_rows(ljg_ctx *h, int64_t *offsets, int32_t *rows) {
+  LJG_API_BEGIN(h)
+  if (c.layout != Layout::Clusters || !c.xrows_valid) return fail(c, LJG_ERR_STATE, "no member rows built");
+  if (!offsets) return fail(c, LJG_ERR_ARG, "offsets is required");
+  LJG_HIP(hipStreamSynchronize(c.stream));
+  const size_t ncl = (size_t)c.nclusters;
+  std::vector<int32_t> cnt(ncl);
+  if (ncl > 0) LJG_HIP(hipMemcpy(cnt.data(), c.xcount.p, ncl * sizeof(int32_t), hipMemcpyDeviceToHost));
+  offsets[0] = 0;
+  for (size_t k = 0; k < ncl; ++k) offsets[k + 1] = offsets[k] + cnt[k];
+  if (rows && ncl > 0) {
+    std::vector<int32_t> r(ncl * (size_t)c.xstride);
+    LJG_HIP(hipMemcpy(r.data(), c.xrow.p, r.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ncl; ++k)
+      std::memcpy(rows + offsets[k], r.data() + k * (size_t)c.xstride, (size_t)cnt[k] * sizeof(int32_t));
+  }
   return LJG_OK;
   LJG_API_END
 }

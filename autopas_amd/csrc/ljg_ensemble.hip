@@ -341,6 +341,29 @@ __global__ __launch_bounds__(kMoveThreads) void k_move_clusters(
   }
 }
 
+// The same move for the member rows of the FP32 cluster kernel (entry << 3 | member: relative to the
+// cluster's own row, so copied as they are); an added box's clusters get empty rows until its build.
+__global__ __launch_bounds__(kMoveThreads) void k_move_xrows(int64_t nclusters, const int32_t *__restrict__ cluster_box,
+                                                             const int64_t *__restrict__ cbase,
+                                                             const int64_t *__restrict__ src_cbase,
+                                                             const int32_t *__restrict__ count_old,
+                                                             const int32_t *__restrict__ xrow_old, int32_t stride,
+                                                             int32_t *__restrict__ count, int32_t *__restrict__ xrow) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = (int64_t)blockIdx.x * (kMoveThreads / kWave) + threadIdx.x / kWave;
+  if (c >= nclusters) return;
+  const int32_t k = cluster_box[c];
+  const int64_t sc = src_cbase[k];
+  if (sc < 0) {
+    if (lane == 0) count[c] = 0;
+    return;
+  }
+  const int64_t f = c + (sc - cbase[k]);
+  const int32_t n = count_old[f] < stride ? count_old[f] : stride;
+  if (lane == 0) count[c] = n;
+  for (int32_t e = lane; e < n; e += kWave) xrow[c * (int64_t)stride + e] = xrow_old[f * (int64_t)stride + e];
+}
+
 // Added particle k (upload index first + k) into its box's region: box b by its upload index
 // (pbase[b0 ..] searched), slot cbase[b] M + (index - pbase[b]).
 __global__ void k_add_to_regions(int64_t first, int64_t count, int32_t b0, int32_t nboxes, int32_t M,
@@ -426,6 +449,15 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
                      c.cluster_box.p, c.box_cbase.p, d_tab, d_tab + nb, c.bbox.p, c.c0.p, c.list_count.p,
                      c.cluster_tower.p, c.list.p, cap, bb.p, cc.p, cnt.p, ctw.p, rows.p);
   LJG_HIP(hipGetLastError());
+  const bool xrows = c.xrows_valid && c.xstride == cap * M;
+  if (xrows) {
+    c.xrow_tmp.ensure((size_t)ncl * c.xstride + 1);
+    c.xcount_tmp.ensure((size_t)ncl + 1);
+    hipLaunchKernelGGL(k_move_xrows, dim3(blocks_for(ncl, per_block)), dim3(kMoveThreads), 0, c.stream, ncl,
+                       c.cluster_box.p, c.box_cbase.p, d_tab, c.xcount.p, c.xrow.p, c.xstride, c.xcount_tmp.p,
+                       c.xrow_tmp.p);
+    LJG_HIP(hipGetLastError());
+  }
   if (added > 0) {
     const bool have_v = vx && vy && vz;
     const size_t n = (size_t)added;
@@ -459,6 +491,11 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
   std::swap(c.list_count, cnt);
   std::swap(c.cluster_tower, ctw);
   std::swap(c.list, rows);
+  if (xrows) {
+    std::swap(c.xrow, c.xrow_tmp);
+    std::swap(c.xcount, c.xcount_tmp);
+  }
+  c.xrows_valid = xrows;  // without them (another stride) the next build of the boxes makes them all again
   c.cur = other;
   c.nslots = ns;
   c.nclusters = ncl;

@@ -205,14 +205,19 @@ struct MixedBand {
 // were cleared before the launch).  On the cluster's own entry (shift id 13) only member pairs b > q count,
 // as SoAFunctorSingle's triangle (LJFunctor.h:204-364).  Pair hits, energy and virial count twice, so that
 // the common halving of the full-list sums (LJFunctor::endTraversal) gives the pair values.
-template <int M, int G, bool MIX, bool GLOB, bool INTEG, bool N3>
+// XR (member rows): the lanes stream the i-cluster's member row (xcount[ic] elements entry << 3 | member at
+// xrow[ic * xstride]: the listed j-cluster members within rc + skin of one of its members) instead of
+// every member of every listed entry.
+template <int M, int G, bool MIX, bool GLOB, bool INTEG, bool N3, bool XR>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
-    const int4 *__restrict__ list, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
+    const int4 *__restrict__ list, const int32_t *__restrict__ xcount, const int32_t *__restrict__ xrow,
+    int32_t xstride, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
     SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd, Owned own) {
   static_assert(!(N3 && INTEG), "the fused update needs every pair's force first");
+  static_assert(!(N3 && XR), "Newton-3 lists stream whole entries");
   if (skip_launch(sk)) return;
   constexpr int LG = kWave / G;
   constexpr int NVAL = 3 * M + 1 + (GLOB ? 2 : 0);
@@ -270,14 +275,26 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
     int hits = 0;
     const int64_t beg = (int64_t)ic * row;
-    const int nj = lcount[ic] * M;  // >= M: every list holds its own cluster
-    // prefetch the first j of this lane (an empty list skips the loop and never loads)
+    // j number k of the stream as entry << 3 | member: from the member row, or entry k / M, member k % M
+    const int32_t *xr = XR ? xrow + (int64_t)ic * xstride : nullptr;
+    auto elem = [&](int k) -> int32_t {
+      if constexpr (XR) return xr[k];
+      else return (int32_t)((((uint32_t)k / M) << 3) | (uint32_t)(k & (M - 1)));
+    };
+    // >= M without member rows: every list holds its own cluster (a member row at least its real members)
+    const int nj = XR ? xcount[ic] : lcount[ic] * M;
+    // prefetch the first j of this lane (an empty list skips the loop and never loads), and the row
+    // element of the next one
     const int kk = l < nj ? l : nj - 1;
     int4 e = make_int4(0, 0, 0, 0);
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    int32_t xq = 0;
     if (nj > 0) {
-      e = list[beg + (uint32_t)kk / M];
-      p = pos32[(int64_t)(e.x & kClusterMask) * M + (kk & (M - 1))];
+      const int32_t x0 = elem(kk);
+      e = list[beg + (x0 >> 3)];
+      p = pos32[(int64_t)(e.x & kClusterMask) * M + (x0 & 7)];
+      const int k1 = l + LG < nj ? l + LG : nj - 1;
+      xq = elem(k1);
     }
     for (int base = 0; base < nj; base += LG) {
       const bool valid = base + l < nj;
@@ -295,12 +312,12 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         own_n3 = jc == ic && ((uint32_t)e.x >> kShiftBits) == 13u;
       }
       float fjx = 0.f, fjy = 0.f, fjz = 0.f;  // Newton-3: the reaction on j, summed over the i-members
-      // issue the next loads before the pair math
+      // issue the next loads before the pair math (and the row element of the one after)
       {
-        const int kn = base + LG + l;
-        const int kc = kn < nj ? kn : nj - 1;
-        e = list[beg + (uint32_t)kc / M];
-        p = pos32[(int64_t)(e.x & kClusterMask) * M + (kc & (M - 1))];
+        e = list[beg + (xq >> 3)];
+        p = pos32[(int64_t)(e.x & kClusterMask) * M + (xq & 7)];
+        const int k2 = base + 2 * LG + l;
+        xq = elem(k2 < nj ? k2 : nj - 1);
       }
       auto pair_term = [&](int q, float dx, float dy, float dz, float r2, bool m) {
         float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
@@ -347,8 +364,9 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         float jx = xj, jy = yj, jz = zj;
         asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz));
         const int kc = valid ? base + l : nj - 1;
-        const int4 ec = list[beg + (uint32_t)kc / M];
-        const int64_t sj = (int64_t)(ec.x & kClusterMask) * M + (kc & (M - 1));
+        const int32_t xc = elem(kc);
+        const int4 ec = list[beg + (xc >> 3)];
+        const int64_t sj = (int64_t)(ec.x & kClusterMask) * M + (xc & 7);
         const int sid = (int)((uint32_t)ec.x >> kShiftBits);
         const double4 pj = bd.posd[sj];
         double xjd = pj.x + shift_component(sid, 0) * Lx;
@@ -1012,12 +1030,24 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   bd.rc2 = a.rc2;
   bd.rc = (float)std::sqrt(a.rc2);
   bd.half_skin = (float)(0.5 * c.prm.skin);
+  // member rows when the build made them (every cluster-entry build but Newton-3 and whole rows)
+  const bool xr = c.xrows_valid && !n3;
+#define LJG_LAUNCH1(MIXV, GLOBV, INTV, N3V, XRV)                                                                     \
+  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV, N3V, XRV>), dim3(nb), dim3(kFThreads), 0,        \
+                     c.stream, items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, c.xcount.p, c.xrow.p,     \
+                     c.xstride, rc2, (float)a.eps24, (float)a.sig2, (float)a.shift6, a.ntypes, tab,                  \
+                     c.st[c.cur].fx.p, c.st[c.cur].fy.p, c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off,       \
+                     c.part_hits.p + off, c.hits_acc, c.skip, clg, ia, bd, owned_of(c))
 #define LJG_LAUNCH(MIXV, GLOBV, INTV, N3V)                                                                         \
-  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV, N3V>), dim3(nb), dim3(kFThreads), 0, c.stream, \
-                     items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, rc2, (float)a.eps24,             \
-                     (float)a.sig2, (float)a.shift6, a.ntypes, tab, c.st[c.cur].fx.p, c.st[c.cur].fy.p,           \
-                     c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off, c.part_hits.p + off, c.hits_acc, c.skip, \
-                     clg, ia, bd, owned_of(c))
+  do {                                                                                                             \
+    if constexpr (N3V) {                                                                                           \
+      LJG_LAUNCH1(MIXV, GLOBV, INTV, N3V, false);                                                                  \
+    } else if (xr) {                                                                                               \
+      LJG_LAUNCH1(MIXV, GLOBV, INTV, N3V, true);                                                                   \
+    } else {                                                                                                       \
+      LJG_LAUNCH1(MIXV, GLOBV, INTV, N3V, false);                                                                  \
+    }                                                                                                              \
+  } while (0)
   if (n3) {
     State &s = c.st[c.cur];
     hipLaunchKernelGGL(k_zero_forces, dim3(blocks_for(c.nslots, 256)), dim3(256), 0, c.stream, c.nslots, s.fx.p,
@@ -1035,6 +1065,7 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
     if (glob) LJG_LAUNCH(false, true, false, false); else LJG_LAUNCH(false, false, false, false);
   }
 #undef LJG_LAUNCH
+#undef LJG_LAUNCH1
 }
 
 template <int M>

@@ -321,6 +321,12 @@ struct Ctx {
   // List entry: x = j cluster | shift id << 27; y,z,w = FP32 bits of c0_j + shift - c0_i (the offset
   // that turns j's cluster-relative FP32 position into i's frame, formed in FP64 and rounded once).
   DevBuf<int4> list;
+  // Member rows of the FP32 cluster kernel: per i-cluster the (entry << 3 | member) of every listed
+  // j-cluster member within ilen of one of its members, at stride xstride = list_row * M (no row can
+  // outgrow it); xcount per i-cluster.
+  DevBuf<int32_t> xrow, xcount, xrow_tmp, xcount_tmp;
+  int32_t xstride = 0;
+  bool xrows_valid = false;     // the current cluster lists carry member rows
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
   DevBuf<float4> pos32_next;    // the fused integrator of the FP32 cluster kernel writes the next step's here
   // FP64 positions of the same step as pos32 (x, y, z; w unused), kept in step with pos32 and swapped with
@@ -542,6 +548,9 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
 // through the other state buffer; the other boxes' slots are not touched.  Nothing moves when *skip is
 // set on the device (skip may be null).  ljg_md.hip
 void permute_active(Ctx &c, const int32_t *src, const int32_t *active, const unsigned long long *skip = nullptr);
+// The member rows of the FP32 cluster kernel after a cluster-entry build (all clusters, or with `partial`
+// the marked boxes' of an ensemble).
+void vcl_member_rows(Ctx &c, bool partial, const int32_t *active);  // ljg_vcl.hip
 // Keep the boxes with new_base[b] >= 0 (their particles move to upload indices new_base[b] + local index,
 // with positions, velocities, forces, types and ids), then append `added` new particles from the host
 // after the `kept` ones (zero force).  Leaves the state in upload order in the other buffer, made current.

@@ -1085,6 +1085,133 @@ void vcl_pos32(Ctx &c, const int32_t *active) {
   LJG_HIP(hipGetLastError());
 }
 
+// Member rows of the FP32 cluster kernel.  VCLClusterFunctor evaluates every member of every listed
+// j-cluster against every i-member (VCLClusterFunctor.h:38-89); a listed j-cluster (bounding boxes within
+// ilen) holds members farther than ilen from all i-members, which can never come within rc before the
+// next build (each particle moves less than skin/2).  G i-clusters per wavefront stream their list's
+// (entry, member) pairs and keep those within ilen of a real i-member, compacted in stream order with a
+// ballot and a prefix popcount: xrow[ic * xstride + k] = entry << 3 | member, xcount[ic] of them.  The
+// test is FP32 on the cluster-relative positions (the kernel's own arithmetic) against ilen^2 (1 + 2^-16),
+// whose margin is far above the FP32 error of d^2, so a row holds every member within ilen in FP64 and
+// at most a few more.  The cluster's own members are kept (the force kernel drops the pair of a member
+// with itself).
+constexpr int kXrowThreads = 256;
+
+template <int M, int G>
+__global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, Segs sg,
+                                                                const float4 *__restrict__ pos32, int32_t row,
+                                                                const int32_t *__restrict__ lcount,
+                                                                const int4 *__restrict__ list, float thr,
+                                                                int32_t xstride, int32_t *__restrict__ xcount,
+                                                                int32_t *__restrict__ xrow) {
+  constexpr int LG = kWave / G;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int grp = lane / LG, l = lane % LG;
+  const int64_t item = ((int64_t)blockIdx.x * (kXrowThreads / kWave) + threadIdx.x / kWave) * G + grp;
+  const bool live = item < nitems;
+  const int64_t ic = seg_index(sg, live ? item : nitems - 1);
+  float xi[M], yi[M], zi[M];
+  uint32_t real = 0u;
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const float4 p = pos32[ic * M + q];
+    xi[q] = p.x;
+    yi[q] = p.y;
+    zi[q] = p.z;
+    real |= (p.x == kPadPos32 ? 0u : 1u) << q;
+  }
+  const int nj = live ? lcount[ic] * M : 0;
+  int nmax = nj;  // the loop runs to the wave's longest stream so that the ballots stay uniform
+#pragma unroll
+  for (int off = kWave / 2; off >= LG; off >>= 1) {
+    const int o = __shfl_xor(nmax, off, kWave);
+    nmax = o > nmax ? o : nmax;
+  }
+  const int4 *lr = list + ic * (int64_t)row;
+  int32_t *xr = xrow + ic * (int64_t)xstride;
+  const unsigned long long gmask = (LG == kWave ? ~0ull : ((1ull << LG) - 1ull)) << (grp * LG);
+  int32_t cnt = 0;
+  for (int base = 0; base < nmax; base += LG) {
+    const int k = base + l;
+    bool near = false;
+    if (k < nj) {
+      const int4 e = lr[k / M];
+      const float4 p = pos32[(int64_t)(e.x & kClusterMask) * M + (k % M)];
+      if (p.x != kPadPos32) {
+        const float jx = p.x + __int_as_float(e.y), jy = p.y + __int_as_float(e.z), jz = p.z + __int_as_float(e.w);
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
+          const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          near = near || (((real >> q) & 1u) && d2 <= thr);
+        }
+      }
+    }
+    const unsigned long long ball = __ballot(near) & gmask;
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
+    if (near) xr[cnt + below] = ((k / M) << 3) | (k % M);
+    cnt += __popcll(ball);
+  }
+  if (live && l == 0) xcount[ic] = cnt;
+}
+
+// A wavefront per cluster of an unmarked box copies its member row (xcount[c] entries) to stride newstride.
+__global__ __launch_bounds__(kXrowThreads) void k_restride_xrows(int64_t nclusters,
+                                                                 const int32_t *__restrict__ cluster_box,
+                                                                 const int32_t *__restrict__ box_active,
+                                                                 const int32_t *__restrict__ count,
+                                                                 const int32_t *__restrict__ in, int32_t oldstride,
+                                                                 int32_t *__restrict__ out, int32_t newstride) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = (int64_t)blockIdx.x * (kXrowThreads / kWave) + threadIdx.x / kWave;
+  if (c >= nclusters || box_active[cluster_box[c]] != 0) return;
+  const int32_t n = count[c] < oldstride ? count[c] : oldstride;
+  for (int32_t k = lane; k < n; k += kWave) out[c * (int64_t)newstride + k] = in[c * (int64_t)oldstride + k];
+}
+
+// The member rows after a cluster-entry build: every cluster's, or (partial ensemble build) the marked
+// boxes' clusters only, the other boxes' rows moved to the new stride when the list rows grew.
+void vcl_member_rows(Ctx &c, bool partial, const int32_t *active) {
+  const int M = c.M;
+  const int64_t ncl = c.nclusters;
+  const int32_t xs = c.list_row * M;
+  // A row holds the members within ilen at the box's last build: the unmarked boxes' rows must be kept
+  // (rows made from later positions could miss pairs that come within rc before the box's next build).
+  if (partial && !c.xrows_valid)
+    throw HipError{hipErrorInvalidValue, "a partial build needs the member rows of the boxes it keeps", -2};
+  const bool keep = partial;
+  if (keep && c.xstride != xs) {
+    c.xrow_tmp.ensure((size_t)ncl * xs + 1);
+    constexpr int per_block = kXrowThreads / kWave;
+    hipLaunchKernelGGL(k_restride_xrows, dim3(blocks_for(ncl, per_block)), dim3(kXrowThreads), 0, c.stream, ncl,
+                       c.cluster_box.p, active, c.xcount.p, c.xrow.p, c.xstride, c.xrow_tmp.p, xs);
+    LJG_HIP(hipGetLastError());
+    std::swap(c.xrow, c.xrow_tmp);
+  } else if (!keep) {
+    c.xrow.ensure((size_t)ncl * xs + 1);
+    c.xcount.ensure((size_t)ncl + 1);
+  }
+  c.xstride = xs;
+  const Segs none{nullptr, nullptr, 0};
+  const Segs sg = keep ? c.seg_clusters : none;
+  const int64_t ni = keep ? c.seg_nclusters : ncl;
+  const double ilen = c.prm.cutoff + c.prm.skin;
+  const float thr = (float)(ilen * ilen * (1.0 + std::ldexp(1.0, -16)));
+  if (ni > 0) {
+    constexpr int G = 4;
+    const int nb = blocks_for(ni, (kXrowThreads / kWave) * G);
+    switch (M) {
+      case 1: hipLaunchKernelGGL((k_cluster_xrows<1, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
+      case 2: hipLaunchKernelGGL((k_cluster_xrows<2, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
+      case 4: hipLaunchKernelGGL((k_cluster_xrows<4, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
+      case 8: hipLaunchKernelGGL((k_cluster_xrows<8, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
+      default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", __LINE__};
+    }
+    LJG_HIP(hipGetLastError());
+  }
+  c.xrows_valid = true;
+}
+
 __global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1417,6 +1544,8 @@ void vcl_build(Ctx &c) {
   c.built_M = M;
   c.layout = Layout::Clusters;
   if (!jrows) vcl_pos32(c, active);
+  if (!jrows && !c.lists_n3 && !c.prm.whole_cluster_rows) vcl_member_rows(c, partial, active);
+  else c.xrows_valid = false;
 }
 
 }  // namespace ljg

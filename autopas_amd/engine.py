@@ -62,6 +62,7 @@ class Params(ctypes.Structure):
         ("list_mode", _i32),
         ("shared_rebuilds", _i32),
         ("graphs", _i32),
+        ("whole_cluster_rows", _i32),
     ]
 
 
@@ -117,6 +118,7 @@ SIGNATURES = {
     "ljg_get_cells": (_i32, [_p, _p, _p]),
     "ljg_cluster_sizes": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
     "ljg_get_clusters": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
+    "ljg_get_member_rows": (_i32, [_p, _p, _p]),
     "ljg_get_verlet_lists": (_i32, [_p, _p, _p]),
     "ljg_generate_lattice": (_i32, [ctypes.c_uint64, _i64, _i64, _i64, _i64, _d, _p, _d, _d, _i32] + [_p] * 7),
     "ljg_generate_slab": (_i32, [ctypes.c_uint64, _p, _d, _d, _i64, _i64, _d, _i32, _d, _i64] + [_p] * 7),
@@ -238,7 +240,7 @@ class Engine:
 
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
-                 tuning_interval=0, list_mode=0, shared_rebuilds=False, graphs=False):
+                 tuning_interval=0, list_mode=0, shared_rebuilds=False, graphs=False, whole_cluster_rows=False):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -257,6 +259,7 @@ class Engine:
         p.list_mode = list_mode
         p.shared_rebuilds = int(bool(shared_rebuilds))
         p.graphs = int(bool(graphs))
+        p.whole_cluster_rows = int(bool(whole_cluster_rows))
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))
@@ -527,6 +530,17 @@ class Engine:
         out["list_shift"] = out["list_shift"][:ne]
         out.update(towers_per_dim=tpd, side=side, towers_per_il=int(r[0]))
         return out
+
+    def member_rows(self):
+        """Per i-cluster of a cluster-entry build: the j-cluster members the FP32 cluster kernel streams, as
+        (offsets, rows) with rows[offsets[c]:offsets[c + 1]] = entry << 3 | member."""
+        nc = np.zeros(1, dtype=np.int64)
+        self._check(self.lib.ljg_cluster_sizes(self.h, None, _ptr(nc), None, None, None, None), "ljg_cluster_sizes")
+        off = np.zeros(int(nc[0]) + 1, dtype=np.int64)
+        self._check(self.lib.ljg_get_member_rows(self.h, _ptr(off), None), "ljg_get_member_rows")
+        rows = np.empty(max(int(off[-1]), 1), dtype=np.int32)
+        self._check(self.lib.ljg_get_member_rows(self.h, _ptr(off), _ptr(rows)), "ljg_get_member_rows")
+        return off, rows[:int(off[-1])]
 
     def verlet_lists(self):
         """Neighbour lists (upload indices) after a Verlet-list build: (offsets[n+1], neighbors)."""

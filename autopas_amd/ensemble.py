@@ -75,11 +75,12 @@ def box_slices(counts):
     return [slice(int(e - n), int(e)) for n, e in zip(counts, ends)]
 
 
-def ensemble_engine(boxes, cluster_size=4, rebuild_frequency=10, dynamic_rebuild=True, tile_groups=0, arrays=None):
+def ensemble_engine(boxes, cluster_size=4, rebuild_frequency=10, dynamic_rebuild=True, tile_groups=0, arrays=None,
+                    whole_cluster_rows=False):
     """An engine context holding ``boxes`` as one ensemble, particles uploaded."""
     lo, hi, counts, st = arrays if arrays is not None else ensemble_arrays(boxes)
     e = engine.Engine(lo[0], hi[0], RC, SKIN, cluster_size=cluster_size, rebuild_frequency=rebuild_frequency,
-                      dynamic_rebuild=dynamic_rebuild, tile_groups=tile_groups)
+                      dynamic_rebuild=dynamic_rebuild, tile_groups=tile_groups, whole_cluster_rows=whole_cluster_rows)
     try:
         e.set_boxes(lo, hi, counts)
         e.upload(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"])

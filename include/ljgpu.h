@@ -82,6 +82,10 @@ typedef struct {
                             * of the previous chunk is updated in place when the launch sequence matches
                             * (hipGraphExecUpdate), else instantiated anew.  Per-launch timing events are
                             * then off (ljg_run_stats.force_ms stays 0). */
+  int32_t whole_cluster_rows; /* FP32 cluster kernel (LJG_KERNEL_CLUSTER_MIXED): 0 = each i-cluster streams
+                               * only the j-cluster members within rc + skin of one of its members (a row of
+                               * (entry, member) pairs filtered at every list build); 1 = every member of
+                               * every listed j-cluster, as VCLClusterFunctor does (VCLClusterFunctor.h:38-89) */
 } ljg_params;
 
 typedef struct {
@@ -257,6 +261,13 @@ ljg_status ljg_cluster_sizes(ljg_ctx *ctx, int64_t *num_slots, int64_t *num_clus
  * list_shift is -1 (minimum image). */
 ljg_status ljg_get_clusters(ljg_ctx *ctx, int64_t *slot_particle, int32_t *cluster_tower, int64_t *list_offsets,
                             int32_t *list_j, int32_t *list_shift, double *bbox);
+
+/* After a cluster-entry build with member rows (every one for LJG_KERNEL_CLUSTER_MIXED unless
+ * whole_cluster_rows is set): per i-cluster the j-cluster members the FP32 cluster kernel streams, as
+ * entry << 3 | member (entry: position in the cluster's entries of ljg_get_clusters).  offsets has
+ * num_clusters + 1 entries; rows may be NULL to query the sizes (offsets[num_clusters] is the total).
+ * Inspection entry point of this engine's layout (no AutoPas counterpart). */
+ljg_status ljg_get_member_rows(ljg_ctx *ctx, int64_t *offsets, int32_t *rows);
 
 /* Neighbour lists after a Verlet-list build, in upload indices: neighbours of particle p are
  * neighbors[offsets[p] .. offsets[p+1]).  offsets has n+1 entries; neighbors may be NULL to query sizes

@@ -178,6 +178,70 @@ def test_cluster_jrows_exact(oracle, M, n_side):
         assert set(row.tolist()) == expect, ci
 
 
+@pytest.mark.parametrize("M,n_side", [(4, 8), (8, 8), (8, 5)])
+def test_member_rows_are_the_ilen_members(M, n_side):
+    # The FP32 cluster kernel's member rows: per i-cluster exactly the members of its listed j-clusters
+    # that lie within ilen (FP64, with the entry's periodic image) of one of its real members, up to the
+    # FP32 test's slack (2^-16 relative on ilen^2).  n_side 5 (L = 5.5 < 2 ilen): entries of one j-cluster
+    # under several images.
+    announce(f"member rows M={M}: {n_side}^3 lattice rho 0.75")
+    s = lattice_box(n_side, 0.75, seed=21)
+    L = s["L"]
+    with make_engine(L, M=M) as e:
+        e.upload(s["x"], s["y"], s["z"])
+        e.force(ap.KERNEL_CLUSTER_MIXED, globals=False)
+        g = e.clusters()
+        off, rows = e.member_rows()
+        st = e.download()
+    P = np.stack([st["x"], st["y"], st["z"]], axis=1)
+    sp, loff, lj, ls = g["slot_particle"], g["list_offsets"], g["list_j"], g["list_shift"]
+    il2 = (RC + SKIN) ** 2
+    assert len(off) == len(loff)
+    for ci in range(len(loff) - 1):
+        mem = np.array([p for p in sp[ci * M:(ci + 1) * M] if p >= 0])
+        row = rows[off[ci]:off[ci + 1]].tolist()
+        assert len(row) == len(set(row)), ci
+        must, may = set(), set()
+        for k in range(int(loff[ci + 1] - loff[ci])):
+            J, sid = int(lj[loff[ci] + k]), int(ls[loff[ci] + k])
+            shift = np.array([sid % 3 - 1, (sid // 3) % 3 - 1, sid // 9 - 1]) * L
+            for m in range(M):
+                pj = int(sp[J * M + m])
+                if pj < 0:
+                    continue
+                d2 = float(np.min(np.sum((P[mem] - (P[pj] + shift)) ** 2, axis=1)))
+                if d2 <= il2:
+                    must.add((k << 3) | m)
+                if d2 <= il2 * (1 + 2.0 ** -15):
+                    may.add((k << 3) | m)
+        assert must <= set(row) <= may, ci
+
+
+def test_member_rows_forces_match_whole_rows():
+    # The member rows leave out only pairs beyond rc + skin: forces, energy, virial and the pair count of
+    # the FP32 cluster kernel are those of the whole entries, up to the order of the FP32 pair sums
+    announce("member rows vs whole cluster rows: 12^3 lattice rho 0.75, 20 steps")
+    s = lattice_box(12, 0.75, seed=5)
+    L = s["L"]
+    res = {}
+    for whole in (True, False):
+        with make_engine(L, M=8, tile_groups=4, whole_cluster_rows=whole) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            g = e.force(ap.KERNEL_CLUSTER_MIXED, globals=True)
+            f = e.download()
+            rs = e.step(20, 0.002, ap.KERNEL_CLUSTER_MIXED)
+            res[whole] = (g, f, rs, e.download())
+    (g1, f1, r1, t1), (g0, f0, r0, t0) = res[True], res[False]
+    assert g1.pair_count == g0.pair_count
+    assert rms_rel(f0, f1) <= 1e-6
+    assert abs(g0.potential_energy - g1.potential_energy) <= 1e-6 * abs(g1.potential_energy)
+    assert abs(g0.virial - g1.virial) <= 1e-6 * abs(g1.virial)
+    # (the two trajectories part by FP32 rounding: a pair crossing rc may be counted in one step only)
+    assert abs(r0.pair_interactions - r1.pair_interactions) <= 1e-6 * r1.pair_interactions
+    assert abs(r0.rebuilds - r1.rebuilds) <= 1
+    assert max(np.max(np.abs(t0[k] - t1[k])) for k in ("x", "y", "z")) <= 1e-6
+
+
 @pytest.mark.parametrize("kernel", ALL_KERNELS)
 @pytest.mark.parametrize("M", [4, 8])
 def test_forces_vs_direct_sum(oracle, kernel, M):
