@@ -1543,10 +1543,13 @@ ljg_status ljg_get_member_rows(ljg_ctx *h, int64_t *offsets, int32_t *rows) {
   offsets[0] = 0;
   for (size_t k = 0; k < ncl; ++k) offsets[k + 1] = offsets[k] + cnt[k];
   if (rows && ncl > 0) {
-    std::vector<int32_t> r(ncl * (size_t)c.xstride);
-    LJG_HIP(hipMemcpy(r.data(), c.xrow.p, r.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int2> r(ncl * (size_t)c.xstride);
+    LJG_HIP(hipMemcpy(r.data(), c.xrow.p, r.size() * sizeof(int2), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < ncl; ++k)
-      std::memcpy(rows + offsets[k], r.data() + k * (size_t)c.xstride, (size_t)cnt[k] * sizeof(int32_t));
+      for (int32_t i = 0; i < cnt[k]; ++i) {
+        const int2 v = r[k * (size_t)c.xstride + i];
+        rows[offsets[k] + i] = (v.y << 3) | (v.x % c.M);
+      }
   }
   return LJG_OK;
   LJG_API_END

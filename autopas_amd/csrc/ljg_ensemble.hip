@@ -341,14 +341,15 @@ __global__ __launch_bounds__(kMoveThreads) void k_move_clusters(
   }
 }
 
-// The same move for the member rows of the FP32 cluster kernel (entry << 3 | member: relative to the
-// cluster's own row, so copied as they are); an added box's clusters get empty rows until its build.
+// The same move for the member rows of the FP32 cluster kernel ({j slot, entry}: the slot shifted with the
+// region, the entry relative to the cluster's own row); an added box's clusters get empty rows until its
+// build.
 __global__ __launch_bounds__(kMoveThreads) void k_move_xrows(int64_t nclusters, const int32_t *__restrict__ cluster_box,
                                                              const int64_t *__restrict__ cbase,
-                                                             const int64_t *__restrict__ src_cbase,
+                                                             const int64_t *__restrict__ src_cbase, int32_t M,
                                                              const int32_t *__restrict__ count_old,
-                                                             const int32_t *__restrict__ xrow_old, int32_t stride,
-                                                             int32_t *__restrict__ count, int32_t *__restrict__ xrow) {
+                                                             const int2 *__restrict__ xrow_old, int32_t stride,
+                                                             int32_t *__restrict__ count, int2 *__restrict__ xrow) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t c = (int64_t)blockIdx.x * (kMoveThreads / kWave) + threadIdx.x / kWave;
   if (c >= nclusters) return;
@@ -359,9 +360,14 @@ __global__ __launch_bounds__(kMoveThreads) void k_move_xrows(int64_t nclusters, 
     return;
   }
   const int64_t f = c + (sc - cbase[k]);
+  const int32_t ds = (int32_t)(cbase[k] - sc) * M;
   const int32_t n = count_old[f] < stride ? count_old[f] : stride;
   if (lane == 0) count[c] = n;
-  for (int32_t e = lane; e < n; e += kWave) xrow[c * (int64_t)stride + e] = xrow_old[f * (int64_t)stride + e];
+  for (int32_t e = lane; e < n; e += kWave) {
+    int2 v = xrow_old[f * (int64_t)stride + e];
+    v.x += ds;
+    xrow[c * (int64_t)stride + e] = v;
+  }
 }
 
 // Added particle k (upload index first + k) into its box's region: box b by its upload index
@@ -454,7 +460,7 @@ void ensemble_edit_regions(Ctx &c, const std::vector<int32_t> &src_box, const st
     c.xrow_tmp.ensure((size_t)ncl * c.xstride + 1);
     c.xcount_tmp.ensure((size_t)ncl + 1);
     hipLaunchKernelGGL(k_move_xrows, dim3(blocks_for(ncl, per_block)), dim3(kMoveThreads), 0, c.stream, ncl,
-                       c.cluster_box.p, c.box_cbase.p, d_tab, c.xcount.p, c.xrow.p, c.xstride, c.xcount_tmp.p,
+                       c.cluster_box.p, c.box_cbase.p, d_tab, M, c.xcount.p, c.xrow.p, c.xstride, c.xcount_tmp.p,
                        c.xrow_tmp.p);
     LJG_HIP(hipGetLastError());
   }

@@ -205,13 +205,14 @@ struct MixedBand {
 // were cleared before the launch).  On the cluster's own entry (shift id 13) only member pairs b > q count,
 // as SoAFunctorSingle's triangle (LJFunctor.h:204-364).  Pair hits, energy and virial count twice, so that
 // the common halving of the full-list sums (LJFunctor::endTraversal) gives the pair values.
-// XR (member rows): the lanes stream the i-cluster's member row (xcount[ic] elements entry << 3 | member at
+// XR (member rows): the lanes stream the i-cluster's member row (xcount[ic] elements {j slot, entry} at
 // xrow[ic * xstride]: the listed j-cluster members within rc + skin of one of its members) instead of
-// every member of every listed entry.
+// every member of every listed entry; the element gives the j slot and its entry directly, so their two
+// loads are issued together one iteration ahead.
 template <int M, int G, bool MIX, bool GLOB, bool INTEG, bool N3, bool XR>
 __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
-    const int4 *__restrict__ list, const int32_t *__restrict__ xcount, const int32_t *__restrict__ xrow,
+    const int4 *__restrict__ list, const int32_t *__restrict__ xcount, const int2 *__restrict__ xrow,
     int32_t xstride, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
@@ -275,26 +276,25 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
     int hits = 0;
     const int64_t beg = (int64_t)ic * row;
-    // j number k of the stream as entry << 3 | member: from the member row, or entry k / M, member k % M
-    const int32_t *xr = XR ? xrow + (int64_t)ic * xstride : nullptr;
-    auto elem = [&](int k) -> int32_t {
-      if constexpr (XR) return xr[k];
-      else return (int32_t)((((uint32_t)k / M) << 3) | (uint32_t)(k & (M - 1)));
-    };
+    const int2 *xr = XR ? xrow + (int64_t)ic * xstride : nullptr;
     // >= M without member rows: every list holds its own cluster (a member row at least its real members)
     const int nj = XR ? xcount[ic] : lcount[ic] * M;
-    // prefetch the first j of this lane (an empty list skips the loop and never loads), and the row
-    // element of the next one
+    // prefetch the first j of this lane (an empty list skips the loop and never loads), and with member
+    // rows the element of the next one
     const int kk = l < nj ? l : nj - 1;
     int4 e = make_int4(0, 0, 0, 0);
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    int32_t xq = 0;
+    int2 xq = make_int2(0, 0);
     if (nj > 0) {
-      const int32_t x0 = elem(kk);
-      e = list[beg + (x0 >> 3)];
-      p = pos32[(int64_t)(e.x & kClusterMask) * M + (x0 & 7)];
-      const int k1 = l + LG < nj ? l + LG : nj - 1;
-      xq = elem(k1);
+      if constexpr (XR) {
+        const int2 x0 = xr[kk];
+        e = list[beg + x0.y];
+        p = pos32[x0.x];
+        xq = xr[l + LG < nj ? l + LG : nj - 1];
+      } else {
+        e = list[beg + (uint32_t)kk / M];
+        p = pos32[(int64_t)(e.x & kClusterMask) * M + (kk & (M - 1))];
+      }
     }
     for (int base = 0; base < nj; base += LG) {
       const bool valid = base + l < nj;
@@ -312,12 +312,17 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         own_n3 = jc == ic && ((uint32_t)e.x >> kShiftBits) == 13u;
       }
       float fjx = 0.f, fjy = 0.f, fjz = 0.f;  // Newton-3: the reaction on j, summed over the i-members
-      // issue the next loads before the pair math (and the row element of the one after)
-      {
-        e = list[beg + (xq >> 3)];
-        p = pos32[(int64_t)(e.x & kClusterMask) * M + (xq & 7)];
+      // issue the next loads before the pair math (with member rows, also the element of the one after)
+      if constexpr (XR) {
+        e = list[beg + xq.y];
+        p = pos32[xq.x];
         const int k2 = base + 2 * LG + l;
-        xq = elem(k2 < nj ? k2 : nj - 1);
+        xq = xr[k2 < nj ? k2 : nj - 1];
+      } else {
+        const int kn = base + LG + l;
+        const int kc = kn < nj ? kn : nj - 1;
+        e = list[beg + (uint32_t)kc / M];
+        p = pos32[(int64_t)(e.x & kClusterMask) * M + (kc & (M - 1))];
       }
       auto pair_term = [&](int q, float dx, float dy, float dz, float r2, bool m) {
         float eps24 = eps24_1, sig2 = sig2_1, shift6 = shift6_1;
@@ -364,9 +369,16 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
         float jx = xj, jy = yj, jz = zj;
         asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz));
         const int kc = valid ? base + l : nj - 1;
-        const int32_t xc = elem(kc);
-        const int4 ec = list[beg + (xc >> 3)];
-        const int64_t sj = (int64_t)(ec.x & kClusterMask) * M + (xc & 7);
+        int4 ec;
+        int64_t sj;
+        if constexpr (XR) {
+          const int2 xc = xr[kc];
+          ec = list[beg + xc.y];
+          sj = xc.x;
+        } else {
+          ec = list[beg + (uint32_t)kc / M];
+          sj = (int64_t)(ec.x & kClusterMask) * M + (kc & (M - 1));
+        }
         const int sid = (int)((uint32_t)ec.x >> kShiftBits);
         const double4 pj = bd.posd[sj];
         double xjd = pj.x + shift_component(sid, 0) * Lx;

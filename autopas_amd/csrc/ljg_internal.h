@@ -321,10 +321,11 @@ struct Ctx {
   // List entry: x = j cluster | shift id << 27; y,z,w = FP32 bits of c0_j + shift - c0_i (the offset
   // that turns j's cluster-relative FP32 position into i's frame, formed in FP64 and rounded once).
   DevBuf<int4> list;
-  // Member rows of the FP32 cluster kernel: per i-cluster the (entry << 3 | member) of every listed
-  // j-cluster member within ilen of one of its members, at stride xstride = list_row * M (no row can
-  // outgrow it); xcount per i-cluster.
-  DevBuf<int32_t> xrow, xcount, xrow_tmp, xcount_tmp;
+  // Member rows of the FP32 cluster kernel: per i-cluster {j slot, entry} of every listed j-cluster
+  // member within ilen of one of its members, at stride xstride = list_row * M (no row can outgrow it);
+  // xcount per i-cluster.
+  DevBuf<int2> xrow, xrow_tmp;
+  DevBuf<int32_t> xcount, xcount_tmp;
   int32_t xstride = 0;
   bool xrows_valid = false;     // the current cluster lists carry member rows
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w

@@ -1090,7 +1090,7 @@ void vcl_pos32(Ctx &c, const int32_t *active) {
 // ilen) holds members farther than ilen from all i-members, which can never come within rc before the
 // next build (each particle moves less than skin/2).  G i-clusters per wavefront stream their list's
 // (entry, member) pairs and keep those within ilen of a real i-member, compacted in stream order with a
-// ballot and a prefix popcount: xrow[ic * xstride + k] = entry << 3 | member, xcount[ic] of them.  The
+// ballot and a prefix popcount: xrow[ic * xstride + k] = {j slot, entry}, xcount[ic] of them.  The
 // test is FP32 on the cluster-relative positions (the kernel's own arithmetic) against ilen^2 (1 + 2^-16),
 // whose margin is far above the FP32 error of d^2, so a row holds every member within ilen in FP64 and
 // at most a few more.  The cluster's own members are kept (the force kernel drops the pair of a member
@@ -1103,7 +1103,7 @@ __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, 
                                                                 const int32_t *__restrict__ lcount,
                                                                 const int4 *__restrict__ list, float thr,
                                                                 int32_t xstride, int32_t *__restrict__ xcount,
-                                                                int32_t *__restrict__ xrow) {
+                                                                int2 *__restrict__ xrow) {
   constexpr int LG = kWave / G;
   const int lane = threadIdx.x & (kWave - 1);
   const int grp = lane / LG, l = lane % LG;
@@ -1111,14 +1111,12 @@ __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, 
   const bool live = item < nitems;
   const int64_t ic = seg_index(sg, live ? item : nitems - 1);
   float xi[M], yi[M], zi[M];
-  uint32_t real = 0u;
 #pragma unroll
   for (int q = 0; q < M; ++q) {
     const float4 p = pos32[ic * M + q];
     xi[q] = p.x;
     yi[q] = p.y;
     zi[q] = p.z;
-    real |= (p.x == kPadPos32 ? 0u : 1u) << q;
   }
   const int nj = live ? lcount[ic] * M : 0;
   int nmax = nj;  // the loop runs to the wave's longest stream so that the ballots stay uniform
@@ -1128,28 +1126,43 @@ __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, 
     nmax = o > nmax ? o : nmax;
   }
   const int4 *lr = list + ic * (int64_t)row;
-  int32_t *xr = xrow + ic * (int64_t)xstride;
+  int2 *xr = xrow + ic * (int64_t)xstride;
   const unsigned long long gmask = (LG == kWave ? ~0ull : ((1ull << LG) - 1ull)) << (grp * LG);
   int32_t cnt = 0;
+  // the entry and position of this lane's next candidate are loaded one iteration ahead
+  const int klast = nj > 0 ? nj - 1 : 0;
+  int4 en = make_int4(0, 0, 0, 0);
+  float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (nj > 0) {
+    const int k0 = l < nj ? l : klast;
+    en = lr[k0 / M];
+    pn = pos32[(int64_t)(en.x & kClusterMask) * M + (k0 % M)];
+  }
   for (int base = 0; base < nmax; base += LG) {
     const int k = base + l;
+    const int4 e = en;
+    const float4 p = pn;
+    if (nj > 0) {
+      const int kn = base + LG + l < nj ? base + LG + l : klast;
+      en = lr[kn / M];
+      pn = pos32[(int64_t)(en.x & kClusterMask) * M + (kn % M)];
+    }
     bool near = false;
+    const int32_t js = (e.x & kClusterMask) * M + (k % M);
     if (k < nj) {
-      const int4 e = lr[k / M];
-      const float4 p = pos32[(int64_t)(e.x & kClusterMask) * M + (k % M)];
       if (p.x != kPadPos32) {
         const float jx = p.x + __int_as_float(e.y), jy = p.y + __int_as_float(e.z), jz = p.z + __int_as_float(e.w);
 #pragma unroll
         for (int q = 0; q < M; ++q) {
           const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
           const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-          near = near || (((real >> q) & 1u) && d2 <= thr);
+          near = near || d2 <= thr;  // a padding i-member (at +3e30) is never near
         }
       }
     }
     const unsigned long long ball = __ballot(near) & gmask;
     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
-    if (near) xr[cnt + below] = ((k / M) << 3) | (k % M);
+    if (near) xr[cnt + below] = make_int2(js, k / M);
     cnt += __popcll(ball);
   }
   if (live && l == 0) xcount[ic] = cnt;
@@ -1160,8 +1173,8 @@ __global__ __launch_bounds__(kXrowThreads) void k_restride_xrows(int64_t ncluste
                                                                  const int32_t *__restrict__ cluster_box,
                                                                  const int32_t *__restrict__ box_active,
                                                                  const int32_t *__restrict__ count,
-                                                                 const int32_t *__restrict__ in, int32_t oldstride,
-                                                                 int32_t *__restrict__ out, int32_t newstride) {
+                                                                 const int2 *__restrict__ in, int32_t oldstride,
+                                                                 int2 *__restrict__ out, int32_t newstride) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t c = (int64_t)blockIdx.x * (kXrowThreads / kWave) + threadIdx.x / kWave;
   if (c >= nclusters || box_active[cluster_box[c]] != 0) return;
