@@ -1211,15 +1211,23 @@ void vcl_member_rows(Ctx &c, bool partial, const int32_t *active) {
   const double ilen = c.prm.cutoff + c.prm.skin;
   const float thr = (float)(ilen * ilen * (1.0 + std::ldexp(1.0, -16)));
   if (ni > 0) {
-    constexpr int G = 4;
-    const int nb = blocks_for(ni, (kXrowThreads / kWave) * G);
+    // a small build (few boxes of an ensemble) is bound by the latency of one wavefront's stream: a
+    // wavefront per i-cluster then; four per wavefront otherwise
+    const bool small = ni <= 32768;
+    const int nb = blocks_for(ni, (kXrowThreads / kWave) * (small ? 1 : 4));
+#define LJG_XROWS(MV, GV)                                                                                          \
+  hipLaunchKernelGGL((k_cluster_xrows<MV, GV>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p,       \
+                     c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p)
+#define LJG_XROWS_M(MV)                                                                                            \
+  case MV:                                                                                                         \
+    if (small) LJG_XROWS(MV, 1); else LJG_XROWS(MV, 4);                                                            \
+    break;
     switch (M) {
-      case 1: hipLaunchKernelGGL((k_cluster_xrows<1, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
-      case 2: hipLaunchKernelGGL((k_cluster_xrows<2, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
-      case 4: hipLaunchKernelGGL((k_cluster_xrows<4, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
-      case 8: hipLaunchKernelGGL((k_cluster_xrows<8, G>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, thr, xs, c.xcount.p, c.xrow.p); break;
+      LJG_XROWS_M(1) LJG_XROWS_M(2) LJG_XROWS_M(4) LJG_XROWS_M(8)
       default: throw HipError{hipErrorInvalidValue, "cluster size must be 1, 2, 4 or 8", __LINE__};
     }
+#undef LJG_XROWS_M
+#undef LJG_XROWS
     LJG_HIP(hipGetLastError());
   }
   c.xrows_valid = true;

@@ -195,6 +195,7 @@ class _Lane:
         self.active = []  # [job, steps left], in box order
         self.inbox = collections.deque()  # dispatched jobs, in arrival order
         self.phase = 0  # batch steps since the last phase boundary
+        self.groups = 1  # i-clusters per wavefront of its force launches
 
     def work_left(self):
         """Particle-steps still to run for the jobs running and dispatched here."""
@@ -246,6 +247,12 @@ class BatchedServer:
             raise ValueError("dispatch is 'work' or 'particles'")
         self.dispatch = _Lane.work_left if dispatch == "work" else _Lane.particles
         self.kernel = kernel
+        # "size": each lane's tile shape follows the particles it holds, as job_layout picks it for one job
+        # (results then equal each job alone up to the rounding of the FP32 pair sums)
+        self.size_tiles = tile_groups == "size"
+        if self.size_tiles:
+            tile_groups = 1
+        self.cluster_size = cluster_size
         self.epoch = epoch
         self.aligned = aligned
         self.frequency = rebuild_frequency
@@ -358,6 +365,11 @@ class BatchedServer:
                 # to the next phase boundary, or with no job waiting to a later one within the epoch
                 periods = 1 if lane.inbox else max(1, self.epoch // self.frequency)
                 steps = min(steps, self.frequency - lane.phase + (periods - 1) * self.frequency)
+            if self.size_tiles:
+                g = job_layout(sum(j.n for j, _ in active))[2]
+                if g != lane.groups:
+                    lane.engine.set_layout(self.cluster_size, g)
+                    lane.groups = g
             rs = lane.engine.step(steps, DT, self.kernel)
             lane.phase = (lane.phase + steps) % self.frequency
             with self._lock:

@@ -286,7 +286,8 @@ def serving(jobs, workers):
     10-1000 steps) arriving as a Poisson process at 80% of the worker pool's measured throughput, served
     (a) by `workers` reused contexts, each job with the cluster-tile kernel at a tile shape chosen from its
     size, and (b) with continuous batching in `workers` ensemble contexts that batch concurrently (and in
-    one), all on the same arrival stream; beside them the same jobs run one at a time with the plain FP64
+    one; with one rebuild schedule per lane; with each lane's tile shape following its particles), all on the
+    same arrival stream; beside them the same jobs run one at a time with the plain FP64
     kernel (closed loop, so its latency is the bare service time)."""
     import autopas_amd as ap
     from autopas_amd import serve
@@ -298,10 +299,11 @@ def serving(jobs, workers):
         wall = srv.run(base_jobs)
     base = serve.latency_summary(base_jobs, wall)
     service = np.array([j.finish - j.start for j in base_jobs]) * 1e3
-    for key, shared, lanes in (("batched", False, workers), ("batched_one_context", False, 1),
-                               ("batched_shared_schedule", True, workers)):
+    for key, shared, lanes, tiles in (("batched", False, workers, 1), ("batched_one_context", False, 1, 1),
+                                      ("batched_shared_schedule", True, workers, 1),
+                                      ("batched_size_tiles", False, workers, "size")):
         batched, _ = serve.serve_open_loop_batched(jobs, seed=5, rate=out["load"] * out["capacity_jobs_per_s"],
-                                                   shared_rebuilds=shared, lanes=lanes)
+                                                   shared_rebuilds=shared, lanes=lanes, tile_groups=tiles)
         out[key] = dict(
             p50_ms=batched["p50_ms"], p99_ms=batched["p99_ms"], mean_ms=batched["mean_ms"], jobs=batched["jobs"],
             pairs_per_s=batched["pairs_per_s"], jobs_per_s=batched["jobs_per_s"],
@@ -317,8 +319,11 @@ def serving(jobs, workers):
                  + "jobs admitted at the batch's rebuild-period boundaries under the HBM particle budget, finished "
                    "ones retired at once; "
                  + ("one rebuild schedule for the batch (results equal to each job alone up to FP32 rounding)"
-                    if shared else "each job keeps its own rebuild schedule (results bit-identical to the job "
-                                   "alone with the same tile shape)"))
+                    if shared else "each job keeps its own rebuild schedule, and each lane's tile shape follows "
+                                   "the particles it holds (results equal to each job alone up to FP32 "
+                                   "rounding)" if tiles == "size" else
+                    "each job keeps its own rebuild schedule (results bit-identical to the job alone with the "
+                    "same tile shape)"))
     out["fp64_one_at_a_time"] = dict(
         jobs=base["jobs"], service_p50_ms=float(np.percentile(service, 50)),
         service_p99_ms=float(np.percentile(service, 99)), pairs_per_s=base["pairs_per_s"],
@@ -483,7 +488,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-pair/fp64-accumulate",
+        "dtype": "fp32-pair/fp32-lane-partial/fp64-particle-sum",
         "data": "synthetic (seeded jittered lattice)",
         "config": {
             "workload": "uniform bulk LJ liquid, 1e6 particles (100^3 lattice, rho 0.75, T 0.9), rc 2.5, "

@@ -102,9 +102,13 @@ class _FakeEnsemble:
         self.kwargs = kwargs
         self.clock = 0  # steps run since the box list was last empty
         self.admit_clocks = []  # the clock at each edit that added boxes to a running batch
+        self.layouts = []  # (particles held, cluster size, tile groups) at each set_layout
 
     def memory_info(self):
         return 1 << 40, 1 << 41
+
+    def set_layout(self, cluster_size, tile_groups=0, build_groups=0):
+        self.layouts.append((sum(b[1] for b in self.boxes), cluster_size, tile_groups))
 
     def edit_boxes(self, keep, box_min=None, box_max=None, counts=None, x=None, **parts):
         assert len(set(keep)) == len(keep) and all(0 <= k < len(self.boxes) for k in keep)
@@ -144,8 +148,9 @@ class _FakeEnsemble:
         pass
 
 
-@pytest.mark.parametrize("max_jobs,aligned,lanes", [(None, True, 1), (3, True, 1), (None, False, 1), (None, True, 3)])
-def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs, aligned, lanes):
+@pytest.mark.parametrize("max_jobs,aligned,lanes,tiles", [(None, True, 1, 1), (3, True, 1, 1), (None, False, 1, 1),
+                                                          (None, True, 3, 1), (None, True, 2, "size")])
+def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs, aligned, lanes, tiles):
     # jobs arrive over ~20 ms; the budget holds a few at once; every job must run exactly its own steps,
     # start no earlier than it arrived, and be read back from its own place in its batch; aligned, jobs join
     # a running batch only at multiples of the rebuild frequency of its steps; with several lanes the jobs
@@ -154,7 +159,8 @@ def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs, aligned,
     jobs = serve.make_jobs(16, rate=800.0, seed=9, n_range=(256, 4000), steps_range=(3, 40))
     budget = 6000
     with serve.BatchedServer(particle_budget=budget, keep_results=True, epoch=10, max_jobs=max_jobs,
-                             shared_rebuilds=True, aligned=aligned, rebuild_frequency=10, lanes=lanes) as srv:
+                             shared_rebuilds=True, aligned=aligned, rebuild_frequency=10, lanes=lanes,
+                             tile_groups=tiles) as srv:
         engines = [lane.engine for lane in srv.lanes]
         assert len(engines) == lanes and srv.engine is engines[0]
         assert all(e.kwargs["shared_rebuilds"] is True and e.kwargs["rebuild_frequency"] == 10 for e in engines)
@@ -165,6 +171,12 @@ def test_batched_server_admits_steps_and_retires(monkeypatch, max_jobs, aligned,
         assert e.peak <= max(share, max(j.n for j in jobs))
         if max_jobs is not None:
             assert e.most <= max_jobs
+    # "size": a lane's tile shape follows the particles it holds (job_layout's rule), set only on a change
+    layouts = [lay for e in engines for lay in e.layouts]
+    if tiles == "size":
+        assert all(g == serve.job_layout(n)[2] and m == 8 for n, m, g in layouts), layouts
+    else:
+        assert not layouts
     clocks = [c for e in engines for c in e.admit_clocks]
     if aligned:
         assert all(c % 10 == 0 for c in clocks), clocks
