@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     int32_t xstride, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
     const double *__restrict__ tab64, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ pe, double *__restrict__ pv, int64_t *__restrict__ ph, unsigned long long *__restrict__ acc,
-    SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd, Owned own) {
+    SkipIf sk, double *__restrict__ clg, IntegArgs ia, MixedBand bd, Owned own, WorkQueue wq) {
   static_assert(!(N3 && INTEG), "the fused update needs every pair's force first");
   static_assert(!(N3 && XR), "Newton-3 lists stream whole entries");
   if (skip_launch(sk)) return;
@@ -231,10 +231,6 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
   }
   const int lane = threadIdx.x & (kWave - 1);
   const int grp = lane / LG, l = lane % LG;
-  // item -> i-cluster: every cluster, or the clusters of the segments sg (nclusters items)
-  const int item = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * G + grp;
-  int ic = item < nclusters ? (int)seg_index(sg, item) : -1;
-  if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
   // the FP64 band around rc^2 (uniform over the launch)
   uint32_t cut_lo, cut_hi;
   {
@@ -251,6 +247,37 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
   }
   double e_out = 0., v_out = 0.;
   int64_t h_out = 0;
+  // Tiles of this wavefront (G i-clusters each): its place in the grid, or (work queue) chunks of
+  // consecutive tiles taken from the regions' counters, its XCD's region first (wave-uniform values).
+  const int wave_tile = (int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
+  const int nchunks = wq.ctr ? (wq.ntiles + wq.chunk - 1) / wq.chunk : 0;
+  int region = (int)(blockIdx.x % kQueueRegions), tries = 0;
+  int tile = wq.ctr ? -1 : wave_tile, tile_end = tile + 1;
+  for (;;) {
+    if (wq.ctr) {
+      ++tile;
+      while (tile >= tile_end && tries < kQueueRegions) {
+        const int r0 = (int)((int64_t)nchunks * region / kQueueRegions);
+        const int r1 = (int)((int64_t)nchunks * (region + 1) / kQueueRegions);
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(wq.ctr + region, 1u);
+        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+        if ((int64_t)r0 + t < r1) {
+          tile = (r0 + (int)t) * wq.chunk;
+          tile_end = min(tile + wq.chunk, wq.ntiles);
+        } else {
+          region = (region + 1) % kQueueRegions;
+          ++tries;
+        }
+      }
+      if (tile >= tile_end) break;
+    } else if (tile >= tile_end) {
+      break;
+    }
+    const int item = tile * G + grp;
+    int ic = item < nclusters ? (int)seg_index(sg, item) : -1;
+    if constexpr (G == 1) ic = __builtin_amdgcn_readfirstlane(ic);
+    if (!wq.ctr) ++tile;
   if (ic >= 0) {
     float xi[M], yi[M], zi[M];
     int ti[M];
@@ -456,13 +483,13 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
             else *dst = val;
           }
         } else if (q == 3 * M) {
-          h_out = (int64_t)val;
+          h_out += (int64_t)val;
           if (GLOB && clg) clg[3 * (int64_t)ic + 2] = val;  // per-cluster sums for per-box globals
         } else if (GLOB && q == 3 * M + 1) {
-          e_out = val;
+          e_out += val;
           if (clg) clg[3 * (int64_t)ic] = val;
         } else if (GLOB && q == 3 * M + 2) {
-          v_out = val;
+          v_out += val;
           if (clg) clg[3 * (int64_t)ic + 1] = val;
         }
       }
@@ -517,6 +544,20 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
           ia.posd_next[i] = make_double4(x, y, z, 0.);
         }
       }
+      // the next tile's sums go to the same LDS slots
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  }
+  if (wq.ctr && lane == 0) {
+    // the last wavefront done clears the counters for the next launch (every other one has taken its
+    // last tile and failed on every region by then)
+    __threadfence();
+    if (atomicAdd(wq.ctr + kQueueRegions, 1u) == (uint32_t)wq.nwaves - 1u) {
+      for (int r = 0; r <= kQueueRegions; ++r) atomicExch(wq.ctr + r, 0u);
+      __threadfence();
     }
   }
   block_partials(e_out, v_out, h_out, pe, pv, ph, GLOB, acc);
@@ -1044,12 +1085,30 @@ static void launch_mixed(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   bd.half_skin = (float)(0.5 * c.prm.skin);
   // member rows when the build made them (every cluster-entry build but Newton-3 and whole rows)
   const bool xr = c.xrows_valid && !n3;
+  // Persistent launch (the step loop's full launches): a few wavefronts per SIMD take the tiles from the
+  // work queue, four consecutive tiles at a time; its counters are cleared once here and then by every
+  // launch's last wavefront.  Block partials then depend on which tiles a block took, so launches that
+  // sum globals keep a wavefront per tile.
+  WorkQueue wq{nullptr, 0, 0, 0};
+  const int64_t tiles = blocks_for(items, G);
+  const int persist_blocks = 256 * 4;  // 256 CUs, 4 workgroups (16 wavefronts) each
+  int launch_blocks = nb;
+  if (c.prm.persistent && integ && !sub && !glob && !n3 && nb > persist_blocks) {
+    if (!c.queue_ready) {
+      c.queue.ensure(kQueueRegions + 1);
+      fill_i32(c, reinterpret_cast<int32_t *>(c.queue.p), kQueueRegions + 1, 0);
+      c.queue_ready = true;
+    }
+    wq = WorkQueue{c.queue.p, (int32_t)tiles, 4, persist_blocks * kWavesPerBlock};
+    launch_blocks = persist_blocks;
+    c.part_items = launch_blocks;  // the block partials written
+  }
 #define LJG_LAUNCH1(MIXV, GLOBV, INTV, N3V, XRV)                                                                     \
-  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV, N3V, XRV>), dim3(nb), dim3(kFThreads), 0,        \
-                     c.stream, items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, c.xcount.p, c.xrow.p,     \
+  hipLaunchKernelGGL((k_force_cluster_mixed<M, G, MIXV, GLOBV, INTV, N3V, XRV>), dim3(launch_blocks), dim3(kFThreads), \
+                     0, c.stream, items, sg, c.pos32.p, c.list_row, c.list_count.p, c.list.p, c.xcount.p, c.xrow.p,  \
                      c.xstride, rc2, (float)a.eps24, (float)a.sig2, (float)a.shift6, a.ntypes, tab,                  \
                      c.st[c.cur].fx.p, c.st[c.cur].fy.p, c.st[c.cur].fz.p, c.part_e.p + off, c.part_v.p + off,       \
-                     c.part_hits.p + off, c.hits_acc, c.skip, clg, ia, bd, owned_of(c))
+                     c.part_hits.p + off, c.hits_acc, c.skip, clg, ia, bd, owned_of(c), wq)
 #define LJG_LAUNCH(MIXV, GLOBV, INTV, N3V)                                                                         \
   do {                                                                                                             \
     if constexpr (N3V) {                                                                                           \

@@ -328,6 +328,8 @@ struct Ctx {
   DevBuf<int32_t> xcount, xcount_tmp;
   int32_t xstride = 0;
   bool xrows_valid = false;     // the current cluster lists carry member rows
+  DevBuf<unsigned int> queue;   // work-queue counters of persistent force launches (kQueueRegions + 1)
+  bool queue_ready = false;     // the counters were cleared once after allocation
   DevBuf<float4> pos32;         // per slot: position - c0 of its cluster, type bits in w
   DevBuf<float4> pos32_next;    // the fused integrator of the FP32 cluster kernel writes the next step's here
   // FP64 positions of the same step as pos32 (x, y, z; w unused), kept in step with pos32 and swapped with
@@ -482,6 +484,18 @@ void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v);  // ljg_vcl.hip: p[0..n
 void vcl_pos32(Ctx &c, const int32_t *active = nullptr);  // ljg_vcl.hip (active: ensemble boxes to write)
 // A launch of the FP32 cluster kernel over some i-clusters only (the segments' nitems clusters), with
 // its block partials from part_offset (reserved by the caller).
+// Work queue of a persistent force launch: the tiles (G i-clusters of one wavefront) are taken in chunks
+// of `chunk` consecutive tiles from kQueueRegions counters, one per contiguous region of the tiles, each
+// wavefront starting at its XCD's region (workgroup b runs on XCD b % 8) and moving on to the next
+// regions when its own is used up.  ctr[kQueueRegions] counts the wavefronts that are done; the last one
+// clears every counter, so the next launch finds them at zero.  ctr null: one tile per wavefront, given
+// by the launch grid.
+constexpr int kQueueRegions = 8;
+struct WorkQueue {
+  unsigned int *ctr;
+  int32_t ntiles, chunk, nwaves;
+};
+
 struct ClusterSubset {
   Segs segs;
   int64_t nitems;

@@ -86,6 +86,9 @@ typedef struct {
                                * only the j-cluster members within rc + skin of one of its members (a row of
                                * (entry, member) pairs filtered at every list build); 1 = every member of
                                * every listed j-cluster, as VCLClusterFunctor does (VCLClusterFunctor.h:38-89) */
+  int32_t persistent;         /* FP32 cluster kernel in ljg_step: 1 = one persistent launch per step, whose
+                               * wavefronts (a few per SIMD) take tiles from a work queue in device memory,
+                               * each starting at its XCD's share; 0 = a wavefront per tile */
 } ljg_params;
 
 typedef struct {

@@ -83,6 +83,29 @@ def test_ensemble_graph_captured_steps_match_stream_launches():
         assert np.array_equal(outs[0][1][k], outs[1][1][k]), k
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_persistent_launch_matches_grid_launch(graphs):
+    # persistent force launches (wavefronts taking tiles from the work queue) end every box bit-identical to
+    # a wavefront per tile: 160 boxes of the ensemble workload (~2.8e5 particles, more tiles than the
+    # persistent grid has wavefronts), 45 steps with per-box rebuilds, the queue reused launch after launch
+    boxes = en.make_boxes(160, seed=4)
+    lo, hi, counts, st = en.ensemble_arrays(boxes)
+    announce(f"persistent vs grid launches: {len(boxes)} boxes, {int(counts.sum())} particles, graphs={graphs}")
+    outs = []
+    for persistent in (False, True):
+        e = ap.Engine(lo[0], hi[0], en.RC, en.SKIN, cluster_size=8, tile_groups=4, rebuild_frequency=10,
+                      dynamic_rebuild=True, graphs=graphs, persistent=persistent)
+        with e:
+            e.set_boxes(lo, hi, counts)
+            e.upload(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"])
+            rs = e.step(45, en.DT, ap.KERNEL_CLUSTER_MIXED)
+            outs.append((rs.pair_interactions, rs.rebuilds, e.download()))
+    assert int(counts.sum()) // 8 > 4 * 4 * 1024  # clusters: more tiles than 1024 workgroups of 4 wavefronts
+    assert outs[0][:2] == outs[1][:2]
+    for k in outs[0][2]:
+        assert np.array_equal(outs[0][2][k], outs[1][2][k]), k
+
+
 def test_ensemble_dynamic_rebuild_matches_boxes_alone():
     # every box keeps the rebuild schedule it has alone (frequency 20 and its own displacement trigger):
     # boxes at four temperatures move skin/2 at different steps, each is rebuilt in place at its own steps
