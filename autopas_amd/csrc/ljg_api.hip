@@ -1308,7 +1308,8 @@ static const TuneCandidate kTuneCandidates[] = {
     {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 8, 4}, {LJG_KERNEL_CLUSTER_JROWS_MIXED, 8, 4, 4},
     {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 8, 4}, {LJG_KERNEL_CLUSTER_JROWS_MIXED, 4, 16, 4},
     {LJG_KERNEL_CLUSTER_MIXED, 4, 8, 0},       {LJG_KERNEL_CLUSTER_MIXED, 8, 2, 0},
-    {LJG_KERNEL_CLUSTER_MIXED_N3, 8, 2, 0},    {LJG_KERNEL_VERLET_MIXED, 4, 0, 0},
+    {LJG_KERNEL_CLUSTER_MIXED, 8, 8, 0},       {LJG_KERNEL_CLUSTER_MIXED_N3, 8, 2, 0},
+    {LJG_KERNEL_VERLET_MIXED, 4, 0, 0},
 };
 constexpr int kNumTuneCandidates = (int)(sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]));
 static_assert(kNumTuneCandidates <= LJG_MAX_TUNE_CANDIDATES, "ljg_tuning holds every candidate");

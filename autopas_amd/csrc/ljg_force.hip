@@ -1248,7 +1248,7 @@ void cluster_force(Ctx &c, bool mixed, bool globals, const IntegArgs *integ, con
   } else {
 #define LJG_CASE(MV, GV) \
   if (c.M == MV && G == GV) { launch_mixed<MV, GV>(c, mix, globals, a, tab, integ, sub, n3); goto launched; }
-    LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4)
+    LJG_CASE(8, 1) LJG_CASE(8, 2) LJG_CASE(8, 4) LJG_CASE(8, 8)
     LJG_CASE(4, 1) LJG_CASE(4, 2) LJG_CASE(4, 4) LJG_CASE(4, 8)
     LJG_CASE(2, 1) LJG_CASE(2, 2) LJG_CASE(2, 4) LJG_CASE(2, 8) LJG_CASE(2, 16)
     LJG_CASE(1, 4) LJG_CASE(1, 8) LJG_CASE(1, 16) LJG_CASE(1, 32) LJG_CASE(1, 64)

@@ -332,7 +332,7 @@ def serving(jobs, workers):
     return out
 
 
-ENSEMBLE_CANDIDATES = ((8, 4), (8, 2), (4, 8))  # (cluster size, i-clusters per wavefront)
+ENSEMBLE_CANDIDATES = ((8, 8), (8, 4), (8, 2), (4, 8))  # (cluster size, i-clusters per wavefront)
 
 
 def ensemble_bench(nboxes, steps, warmup=10, sample=64):

@@ -382,7 +382,7 @@ struct Configuration {
     if (tileGroups == 0) return true;
     if (k == LJG_KERNEL_CLUSTER_FP64 || k == LJG_KERNEL_CLUSTER_FP64_N3) return false;  // one i-cluster per wave
     static const std::map<std::pair<int, int>, std::set<int>> tiles = {
-        {{LJG_KERNEL_CLUSTER_MIXED, 8}, {1, 2, 4}},        {{LJG_KERNEL_CLUSTER_MIXED, 4}, {1, 2, 4, 8}},
+        {{LJG_KERNEL_CLUSTER_MIXED, 8}, {1, 2, 4, 8}},     {{LJG_KERNEL_CLUSTER_MIXED, 4}, {1, 2, 4, 8}},
         {{LJG_KERNEL_CLUSTER_MIXED, 2}, {1, 2, 4, 8, 16}}, {{LJG_KERNEL_CLUSTER_MIXED, 1}, {4, 8, 16, 32, 64}},
         {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 8}, {2, 4, 8}},  {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 4}, {4, 8, 16}},
         {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 2}, {8, 16}},    {{LJG_KERNEL_CLUSTER_JROWS_MIXED, 1}, {16, 32}}};
