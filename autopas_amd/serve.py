@@ -95,9 +95,11 @@ def job_layout(n):
     """Layout for a job of n particles: (kernel, cluster size, i-clusters per wavefront).  Serving-sized boxes
     (up to 65,536 particles) are bound by launch and rebuild latency, where the cluster tiles (whose build
     has no expansion pass) with few i-clusters per wavefront were fastest on MI355X at every size from 512
-    to 64,000 particles (15-30 us per step)."""
+    to 64,000 particles: one i-cluster per wavefront up to 2048 clusters, four above (measured at density
+    0.75, us per step for 1 / 2 / 4 / 8 i-clusters per wavefront: 256 clusters 19 / 22 / 22 / 27,
+    2048 clusters 25 / 27 / 27 / 31, 4096 clusters 30 / 31 / 27 / 32, 8192 clusters 39 / 37 / 34 / 36)."""
     clusters = (n + 7) // 8
-    return engine.KERNEL_CLUSTER_MIXED, 8, (1 if clusters <= 2048 else 2 if clusters <= 4096 else 4)
+    return engine.KERNEL_CLUSTER_MIXED, 8, (1 if clusters <= 2048 else 4)
 
 
 def run_job(eng, job, kernel):
