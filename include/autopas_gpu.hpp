@@ -233,10 +233,12 @@ class LJFunctor {
   double getSigmaSquared() const { return _sigmaSquared; }
   ParticlePropertiesLibrary *getParticlePropertiesLibrary() const { return _ppl; }
 
+  /* LJFunctor.h:641-655: globals and FLOP counters start again. */
   void initTraversal() {
     _postProcessed = false;
     _potentialEnergy = 0.;
     _virial = 0.;
+    _pairsNoN3 = _pairsN3 = _distanceCalls = 0;
   }
 
   /* LJFunctor.h:691-720, same messages. */
@@ -257,19 +259,23 @@ class LJFunctor {
     return _virial;
   }
 
-  /* FLOP model of LJFunctor.h:722-789: 8 per distance, 15 per non-Newton-3 kernel call (the engine's lists
-   * are full, so every pair within rc is two kernel calls), 8 or 9 per non-Newton-3 globals calculation. */
+  /* FLOP model of LJFunctor.h:722-789: 8 per distance, 15 per non-Newton-3 kernel call and 18 per Newton-3
+   * one, 8 or 9 per non-Newton-3 globals calculation and 12 or 13 per Newton-3 one.  A full-list kernel
+   * calls the pair kernel twice per pair within rc (once from each side), a Newton-3 kernel once. */
   size_t getNumFLOPs() const {
     if constexpr (countFLOPs) {
-      const size_t calls = 2 * _pairs;
-      return 8 * _distanceCalls + 15 * calls + (calculateGlobals ? (applyShift ? 9 : 8) * calls : 0);
+      const size_t noN3 = 2 * _pairsNoN3, n3 = _pairsN3;
+      size_t flops = 8 * _distanceCalls + 15 * noN3 + 18 * n3;
+      if (calculateGlobals) flops += (applyShift ? 9 : 8) * noN3 + (applyShift ? 13 : 12) * n3;
+      return flops;
     } else {
       return std::numeric_limits<size_t>::max();
     }
   }
+  /* Kernel calls (pairs within rc) per distance calculation (LJFunctor.h:794-816). */
   double getHitRate() const {
     if constexpr (countFLOPs) {
-      return _distanceCalls ? (double)(2 * _pairs) / (double)_distanceCalls : 0.;
+      return _distanceCalls ? (double)(2 * _pairsNoN3 + _pairsN3) / (double)_distanceCalls : 0.;
     } else {
       return std::numeric_limits<double>::quiet_NaN();
     }
@@ -289,11 +295,12 @@ class LJFunctor {
     throw AutoPasException(getName() + "::SoAFunctorVerlet: not implemented");
   }
 
-  /* Called by AutoPas::computeInteractions with the engine's results (endTraversal's role). */
-  void setTraversalResults(const ljg_globals &g) {
+  /* Called by AutoPas::computeInteractions with the engine's results (endTraversal's role); newton3: the
+   * kernel evaluated each pair once. */
+  void setTraversalResults(const ljg_globals &g, bool newton3 = false) {
     _potentialEnergy = g.potential_energy;
     _virial = g.virial;
-    _pairs += (size_t)g.pair_count;
+    (newton3 ? _pairsN3 : _pairsNoN3) += (size_t)g.pair_count;
     _distanceCalls += g.candidate_evals > 0 ? (size_t)g.candidate_evals : 0;
     _postProcessed = true;
   }
@@ -305,7 +312,7 @@ class LJFunctor {
   double _sigmaSquared = 1.;
   bool _postProcessed = false;
   double _potentialEnergy = 0., _virial = 0.;
-  size_t _pairs = 0, _distanceCalls = 0;
+  size_t _pairsNoN3 = 0, _pairsN3 = 0, _distanceCalls = 0;
 };
 
 /* ---- the GPU configurations in AutoPas's tuning space -------------------------------------------------
@@ -685,6 +692,7 @@ class AutoPas {
     const bool autoTune = _kernel == LJG_KERNEL_AUTO;
     bool tuning = false;
     ljg_globals g{};
+    int32_t kernelUsed = LJG_KERNEL_CLUSTER_MIXED;
     for (;;) {
       int32_t kernel = forceKernel();
       bool configChanged = false;
@@ -725,6 +733,7 @@ class AutoPas {
       f->initTraversal();
       detail::check(_ctx, ljg_force(_ctx, kernel, Functor::kCalculateGlobals ? 1 : 0, &g),
                     "AutoPas::computeInteractions: ljg_force");
+      kernelUsed = kernel;
       const auto t2 = std::chrono::steady_clock::now();
       // LogicHandler.h:1309-1324: the iteration's rebuild and traversal times go to the tuner
       if (tuning)
@@ -732,7 +741,7 @@ class AutoPas {
                                std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count(), rebuilt);
       break;
     }
-    f->setTraversalResults(g);
+    f->setTraversalResults(g, kernelUsed == LJG_KERNEL_CLUSTER_FP64_N3 || kernelUsed == LJG_KERNEL_CLUSTER_MIXED_N3);
     pullState(true);
     ++_stepsSinceRebuild;
     ++_iterations;

@@ -393,6 +393,27 @@ int mode_tune(const char *path) {
 
 }  // namespace
 
+// FLOP model of the functor (LJFunctor.h:722-789) after a full-list and a Newton-3 kernel on the same
+// lattice: one line "flops <kernel> <hit rate> <FLOPs>" each.
+int mode_flops(const char *path) {
+  FILE *out = std::fopen(path, "w");
+  if (!out) return fail("cannot open output");
+  const Lattice s = lattice(8, 0.75, 3, 1);
+  for (int32_t kernel : {LJG_KERNEL_CLUSTER_FP64, LJG_KERNEL_CLUSTER_FP64_N3}) {
+    AutoPas<MoleculeLJ> ap;
+    setup(ap, s.L, kernel);
+    add_all(ap, s, false);
+    autopas_amd::LJFunctor<MoleculeLJ, true, false, FunctorN3Modes::Both, true, true> f(kCutoff);
+    f.setParticleProperties(24., 1.);
+    ap.computeInteractions(&f);
+    ap.computeInteractions(&f);  // the counters start again with each traversal
+    std::fprintf(out, "flops %d %.17g %zu\n", (int)kernel, f.getHitRate(), f.getNumFLOPs());
+  }
+  std::fclose(out);
+  std::printf("ok flops\n");
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) return fail("usage: autopas_api_test errors|nodevice|forces <file>|loop <file> [kernel]");
   const std::string mode = argv[1];
@@ -402,6 +423,7 @@ int main(int argc, char **argv) {
     if (mode == "rebuilds") return mode_rebuilds();
     if (mode == "tune" && argc >= 3) return mode_tune(argv[2]);
     if (mode == "forces" && argc >= 3) return mode_forces(argv[2]);
+    if (mode == "flops" && argc >= 3) return mode_flops(argv[2]);
     if (mode == "loop" && argc >= 3) return mode_loop(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
     if (mode == "halo" && argc >= 3) return mode_halo(argv[2], argc >= 4 ? std::atoi(argv[3]) : -1);
   } catch (const AutoPasException &e) {

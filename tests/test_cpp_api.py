@@ -100,6 +100,31 @@ def test_compute_interactions_matches_direct_sum(driver, oracle, tmp_path):
 
 
 @pytest.mark.gpu
+def test_functor_flop_model_full_and_newton3(driver, oracle, tmp_path):
+    # LJFunctor.h:722-789 with shift and globals: 8 per distance, 15 + 9 per kernel call without Newton-3
+    # (two calls per pair within rc on full lists), 18 + 13 per call with it (one per pair).  From each
+    # line's hit rate h (calls per distance) and FLOPs F the pair count follows; both equal the oracle's.
+    announce("C++ API functor FLOP model: full lists and Newton-3, 8^3 lattice")
+    path = str(tmp_path / "flops.txt")
+    run(driver, "flops", path)
+    rows = [ln.split() for ln in open(path).read().splitlines() if ln.startswith("flops")]
+    assert len(rows) == 2
+    L = 8 * 0.75 ** (-1.0 / 3.0)
+    import autopas_amd as ap
+
+    x, y, z, _, _, _, _ = ap.generate_lattice(3, 8 ** 3, (8,) * 3, 0.75 ** (-1.0 / 3.0), (0.0, 0.0, 0.0), 0.05, 0.9, 1)
+    pairs = oracle.direct_sum(x, y, z, RC, box_len=(L, L, L))["pairs"]
+    for _, kernel, h, flops in rows:
+        h, flops = float(h), int(flops)
+        per_call = 15 + 9 if int(kernel) == ap.KERNEL_CLUSTER_FP64 else 18 + 13
+        dist = flops / (8 + per_call * h)
+        calls = h * dist
+        assert abs(dist - round(dist)) < 1e-3 * dist
+        expect_calls = 2 * pairs if int(kernel) == ap.KERNEL_CLUSTER_FP64 else pairs
+        assert round(calls) == expect_calls, (kernel, calls, expect_calls)
+
+
+@pytest.mark.gpu
 def test_compute_interactions_rebuild_policy(driver):
     announce("C++ API computeInteractions: rebuild every 10 calls and on a skin/2 move")
     assert "ok rebuilds" in run(driver, "rebuilds")
