@@ -1096,6 +1096,9 @@ void vcl_pos32(Ctx &c, const int32_t *active) {
 // at most a few more.  The cluster's own members are kept (the force kernel drops the pair of a member
 // with itself).
 constexpr int kXrowThreads = 256;
+#ifndef LJG_XROWS_SMALL
+#define LJG_XROWS_SMALL 32768
+#endif
 
 template <int M, int G>
 __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, Segs sg,
@@ -1213,7 +1216,7 @@ void vcl_member_rows(Ctx &c, bool partial, const int32_t *active) {
   if (ni > 0) {
     // a small build (few boxes of an ensemble) is bound by the latency of one wavefront's stream: a
     // wavefront per i-cluster then; four per wavefront otherwise
-    const bool small = ni <= 32768;
+    const bool small = ni <= LJG_XROWS_SMALL;
     const int nb = blocks_for(ni, (kXrowThreads / kWave) * (small ? 1 : 4));
 #define LJG_XROWS(MV, GV)                                                                                          \
   hipLaunchKernelGGL((k_cluster_xrows<MV, GV>), dim3(nb), dim3(kXrowThreads), 0, c.stream, ni, sg, c.pos32.p,       \
