@@ -55,6 +55,10 @@ def parse():
     p.add_argument("--ensemble-steps", type=int, default=200)
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the batched-ensemble measurement, printed as one JSON object (for rocprof)")
+    p.add_argument("--ensemble-layout", default=None,
+                   help="tile shape of the batched ensemble, e.g. cluster_M8_G8 (default: a short tuning pass)")
+    p.add_argument("--ensemble-sample", type=int, default=64,
+                   help="boxes also run one at a time, each in its own context, beside the ensemble (0: skip)")
     p.add_argument("--profile-only", action="store_true", help="only the timed mixed-kernel run (for rocprof)")
     p.add_argument("--slab-steps", type=int, default=100,
                    help="timed steps of the two-phase four-type slab measurement (0: skip)")
@@ -63,7 +67,8 @@ def parse():
                    help="only the binning and list builds of the liquid (LinkedCells binning, cluster-pair build, "
                         "j-row build; 5 each from the upload order), then the calibration copy (for rocprof)")
     p.add_argument("--copy-calibration", action="store_true",
-                   help="with --profile-only: finish with a 256 MiB device-to-device copy, the known-size "
+                   help="with --profile-only or --ensemble-only: finish with a 256 MiB device-to-device copy, "
+                        "the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     return p.parse_args()
 
@@ -335,11 +340,12 @@ def serving(jobs, workers):
 ENSEMBLE_CANDIDATES = ((8, 8), (8, 4), (8, 2), (4, 8))  # (cluster size, i-clusters per wavefront)
 
 
-def ensemble_bench(nboxes, steps, warmup=10, sample=64):
+def ensemble_bench(nboxes, steps, warmup=10, sample=64, layout=None):
     """The batched ensemble: `nboxes` independent periodic boxes (N log-uniform in [64, 8192], rho in
     [0.3, 0.9], T in [0.7, 1.5], seeds 4000 + b) advanced together by one context with the FP32 cluster
-    kernel, `steps` steps at dt 0.002.  The tile shape is picked by a short pass over
-    ENSEMBLE_CANDIDATES.  Beside it, the first `sample` boxes one at a time, each in its own context."""
+    kernel, `steps` steps at dt 0.002.  The tile shape is `layout` ("cluster_M8_G8") or, by default, the
+    best of a short pass over ENSEMBLE_CANDIDATES.  Beside it, the first `sample` boxes one at a time, each in
+    its own context (none for sample 0)."""
     import autopas_amd as ap
     from autopas_amd import ensemble as en
 
@@ -348,14 +354,19 @@ def ensemble_bench(nboxes, steps, warmup=10, sample=64):
     lo, hi, counts, st = arrays
     kernel = ap.KERNEL_CLUSTER_MIXED
     table = {}
-    for M, G in ENSEMBLE_CANDIDATES:
-        with en.ensemble_engine(boxes, cluster_size=M, tile_groups=G, arrays=arrays) as e:
-            e.step(5, en.DT, kernel)
-            t0 = time.perf_counter()
-            rs = e.step(20, en.DT, kernel)
-            table[f"cluster_M{M}_G{G}"] = (rs.pair_interactions / (time.perf_counter() - t0), M, G)
-    best = max(table, key=lambda k: table[k][0])
-    _, M, G = table[best]
+    if layout is None:
+        for M, G in ENSEMBLE_CANDIDATES:
+            with en.ensemble_engine(boxes, cluster_size=M, tile_groups=G, arrays=arrays) as e:
+                e.step(5, en.DT, kernel)
+                t0 = time.perf_counter()
+                rs = e.step(20, en.DT, kernel)
+                table[f"cluster_M{M}_G{G}"] = (rs.pair_interactions / (time.perf_counter() - t0), M, G)
+        best = max(table, key=lambda k: table[k][0])
+        _, M, G = table[best]
+    else:
+        best = layout
+        _, m, g = layout.split("_")
+        M, G = int(m[1:]), int(g[1:])
     with en.ensemble_engine(boxes, cluster_size=M, tile_groups=G, arrays=arrays) as e:
         e.step(warmup, en.DT, kernel)
         t0 = time.perf_counter()
@@ -369,7 +380,7 @@ def ensemble_bench(nboxes, steps, warmup=10, sample=64):
             pairs += a.step(steps, en.DT, kernel).pair_interactions
     one_wall = time.perf_counter() - t0
     value = rs.pair_interactions / wall
-    one = pairs / one_wall
+    one = pairs / one_wall if sample else 0.0
     return {
         "workload": f"{nboxes} independent periodic LJ boxes, N log-uniform in [64, 8192], rho U(0.3, 0.9), "
                     f"T U(0.7, 1.5), rc 2.5, skin 0.3, {steps} steps at dt 0.002, one context for all boxes",
@@ -446,7 +457,11 @@ def main():
     import autopas_amd as ap
 
     if args.ensemble_only:
-        print(json.dumps({"ensemble": ensemble_bench(args.ensemble_boxes, args.ensemble_steps)}))
+        out = ensemble_bench(args.ensemble_boxes, args.ensemble_steps, sample=args.ensemble_sample,
+                             layout=args.ensemble_layout)
+        if args.copy_calibration:
+            copy_calibration()
+        print(json.dumps({"ensemble": out}))
         return
     if args.slab_only:
         print(json.dumps({"two_phase_slab": slab_bench(args.slab_steps)}))
