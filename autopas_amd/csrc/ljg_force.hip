@@ -645,15 +645,12 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
     }
   }
   const int jlast = nj > 0 ? nj - 1 : 0;
-  int4 p = make_int4(0, 0, 0, 0);
-  if (nj > 0) p = posfx[rowp[l < nj ? l : jlast]];
-  for (int base = 0; base < njmax; base += LG) {
+  // every cluster's row holds its own members, so entry jlast exists for every group (dead groups read the
+  // row of a real cluster): the loads need no guard
+  auto load_j = [&](int k) { return posfx[rowp[k < nj ? k : jlast]]; };
+  // one j per lane of the group against the M members: the j at base + l
+  auto j_step = [&](int base, const int4 &pj) {
     const bool valid = base + l < nj;
-    const int4 pj = p;
-    {
-      const int kn = base + LG + l;
-      if (nj > 0) p = posfx[rowp[kn < nj ? kn : jlast]];
-    }
     const int tj = pj.w < 0 ? 0 : pj.w;
     float xj = 0.f, yj = 0.f, zj = 0.f;
     if constexpr (FAST) {
@@ -749,6 +746,16 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_jrows(
                      "+v"(ay[q]), "+v"(az[q]));
       }
     }
+  };
+  // two j steps per trip, each one's positions loaded during the other's: the prefetched values stay in
+  // their registers across the back edge (a one-step loop copies them there every trip)
+  int4 pa = load_j(l);
+  for (int base = 0; base < njmax; base += 2 * LG) {
+    const int4 pb = load_j(base + LG + l);
+    j_step(base, pa);
+    if (base + LG >= njmax) break;
+    pa = load_j(base + 2 * LG + l);
+    j_step(base + LG, pb);
   }
   // the sums pass through an empty asm: otherwise the compiler folds them to 0 for lanes that skip the
   // loop and materialises those zeros before it, 2 NV registers live for the whole loop

@@ -484,7 +484,10 @@ def main():
     if args.profile_only:
         if args.copy_calibration:
             copy_calibration()
-        print(json.dumps({"profile_only": True, "pairs_per_s": mixed["pairs_per_s"], "layout": best}))
+        print(json.dumps({"profile_only": True, "pairs_per_s": mixed["pairs_per_s"], "layout": best,
+                          "ms_per_step": mixed["wall_s"] / args.steps * 1e3, "rebuilds": mixed["rebuilds"],
+                          "rebuild_ms_total": mixed["rebuild_ms"],
+                          "force_ms_per_launch": mixed["force_ms"] / max(mixed["force_launches"], 1)}))
         return
     plain = run(s, ap.KERNEL_LC_FP64, 4, args.plain_steps, min(args.warmup, 10))
     cl64 = run(s, ap.KERNEL_CLUSTER_FP64, 4, args.plain_steps, min(args.warmup, 10))
