@@ -210,7 +210,7 @@ struct MixedBand {
 // every member of every listed entry; the element gives the j slot and its entry directly, so their two
 // loads are issued together one iteration ahead.
 template <int M, int G, bool MIX, bool GLOB, bool INTEG, bool N3, bool XR>
-__global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
+__global__ __launch_bounds__(kFThreads, (XR && !MIX && !GLOB) ? 4 : 1) void k_force_cluster_mixed(
     int64_t nclusters, Segs sg, const float4 *__restrict__ pos32, int32_t row, const int32_t *__restrict__ lcount,
     const int4 *__restrict__ list, const int32_t *__restrict__ xcount, const int2 *__restrict__ xrow,
     int32_t xstride, float rc2, float eps24_1, float sig2_1, float shift6_1, int32_t ntypes,
@@ -293,11 +293,6 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
     uint32_t real = 0;  // bit q set for the i-cluster's real (non-padding) members
 #pragma unroll
     for (int q = 0; q < M; ++q) real |= (xi[q] == -kPadPos32 ? 0u : 1u) << q;
-    double Lx = bd.L[0], Ly = bd.L[1], Lz = bd.L[2];
-    if (bd.btg) {
-      const TowerGeom &g = bd.btg[bd.cluster_box[ic]];
-      Lx = g.L[0]; Ly = g.L[1]; Lz = g.L[2];
-    }
     float ax[M], ay[M], az[M], ae[M], av[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) { ax[q] = ay[q] = az[q] = ae[q] = av[q] = 0.f; }
@@ -407,6 +402,12 @@ __global__ __launch_bounds__(kFThreads) void k_force_cluster_mixed(
           sj = (int64_t)(ec.x & kClusterMask) * M + (kc & (M - 1));
         }
         const int sid = (int)((uint32_t)ec.x >> kShiftBits);
+        // the box lengths are read here, not held through the loop (six registers the common path needs)
+        double Lx = bd.L[0], Ly = bd.L[1], Lz = bd.L[2];
+        if (bd.btg) {
+          const TowerGeom &g = bd.btg[bd.cluster_box[ic]];
+          Lx = g.L[0]; Ly = g.L[1]; Lz = g.L[2];
+        }
         const double4 pj = bd.posd[sj];
         double xjd = pj.x + shift_component(sid, 0) * Lx;
         double yjd = pj.y + shift_component(sid, 1) * Ly;
