@@ -29,11 +29,11 @@ COPY_BYTES = 256 << 20    # calibration copy for the HBM counters
 PEAK_HBM_GBS = 8000.0     # MI355X published HBM3E bandwidth
 # HBM bytes per force launch from the FETCH_SIZE / WRITE_SIZE counter passes (scaled by the calibration
 # copy) recorded in profiles/cluster_jrows_m8g8_bulk_liquid_1M_hbm_traffic.txt, per layout.
-TRAFFIC_BYTES = {"jrows_M8_G8": 421.1e6}
+TRAFFIC_BYTES = {"jrows_M8_G8": 421.2e6}
 # Wavefront VALU instructions per force launch (SQ_INSTS_VALU) recorded in
 # profiles/cluster_jrows_m8g8_bulk_liquid_1M_valu_counters.txt, per layout; the issue capacity of a launch is
 # one wavefront instruction per CU per clock.
-VALU_INSTRUCTIONS = {"jrows_M8_G8": 6.688e7}
+VALU_INSTRUCTIONS = {"jrows_M8_G8": 6.461e7}
 MAX_CLOCK_GHZ = 2.4  # MI355X published peak engine clock
 
 
