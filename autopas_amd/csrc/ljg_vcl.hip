@@ -879,7 +879,10 @@ constexpr int kExpandThreads = 256;
 // in those units; tests whose FP32 d^2 lies in a band around it (cut_lo, cut_hi on the bit pattern; the
 // FP32 error of d^2 is ~5 roundings, the band 2^-18 relative) are decided again by the FP64 test above,
 // so the rows are the same.  Otherwise every test is the FP64 one.
-template <int M, int NS, int G, bool CUB>
+// STAGED: each group first copies its concatenated sub-rows into LDS (ccap int32 per group, dynamic
+// shared memory of the launch), so that entry e of the stream is one LDS read instead of a search over
+// the sub-row offsets.
+template <int M, int NS, int G, bool CUB, bool STAGED>
 __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
     int64_t nclusters, const int4 *__restrict__ posfx, const double4 *__restrict__ c0, int32_t ccap,
     const int32_t *__restrict__ ccount, const int32_t *__restrict__ jcl, FixGeom f, double ilen, uint32_t cut_lo,
@@ -940,8 +943,21 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
     }
   }
   const int32_t *crow = jcl + icl * (int64_t)ccap;
+  extern __shared__ int32_t s_entries[];
+  int32_t *srow = STAGED ? s_entries + (int64_t)(threadIdx.x / LG) * ccap : nullptr;
+  if constexpr (STAGED) {
+    // the group's row in LDS (also for a dead group, whose icl is a real cluster: its lanes' clamped
+    // prefetches below read real entries); a wavefront reads only its own groups' rows
+#pragma unroll
+    for (int w = 0; w < NS; ++w)
+      for (int k = l; k < pre[w + 1] - pre[w]; k += LG) srow[pre[w] + k] = crow[w * subcap + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   // entry e of the concatenated sub-rows
   auto centry = [&](int e) {
+    if constexpr (STAGED) return srow[e];
     int w = 0;
 #pragma unroll
     for (int k = 1; k < NS; ++k) w += e >= pre[k] ? 1 : 0;
@@ -953,11 +969,15 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
   int32_t *jrow = jrows + icl * (int64_t)cap;
   const unsigned long long gmask = (LG == kWave ? ~0ull : ((1ull << LG) - 1ull)) << (grp * LG);
   int32_t cnt = 0;
-  // prefetch the first entry of this lane
-  int32_t jsn = 0;
-  int4 xjn = make_int4(0, 0, 0, -1);
-  if (l < nflat) {
-    jsn = centry(l / M) * M + (l % M);
+  // Every row holds at least one entry (the cluster's own, or a full sub-row), so with the element
+  // index clamped to the stream's last one every lane, dead groups included, loads a real slot and the
+  // prefetches need no guard.
+  const int klast = pre[NS] * M - 1;
+  int32_t jsn;
+  int4 xjn;
+  {
+    const uint32_t k0 = (uint32_t)(l < klast ? l : klast);
+    jsn = centry((int)(k0 / M)) * M + (int)(k0 % M);
     xjn = posfx[jsn];
   }
   for (int base = 0; base < nmax; base += LG) {
@@ -966,10 +986,9 @@ __global__ __launch_bounds__(kExpandThreads) void k_jrows_expand(
     const bool inrow = base + l < nflat;
     {
       const int kn = base + LG + l;
-      if (kn < nflat) {
-        jsn = centry(kn / M) * M + (kn % M);
-        xjn = posfx[jsn];
-      }
+      const uint32_t kc = (uint32_t)(kn < klast ? kn : klast);
+      jsn = centry((int)(kc / M)) * M + (int)(kc % M);
+      xjn = posfx[jsn];
     }
     bool near = false;
     // the FP64 member test
@@ -1493,22 +1512,32 @@ void vcl_build(Ctx &c) {
       // FP32 member test of the cubic variant: ilen^2 in fixed-point units, band 2^-18 relative
       uint32_t cut_lo = 0, cut_hi = 0;
       cutoff_band((float)(ilen * ilen * c.fg.inv_scale[0] * c.fg.inv_scale[0]), std::ldexp(1.0, -18), cut_lo, cut_hi);
+// the staged form when the block's rows fit in 64 KiB of LDS (4 G rows of cap int32)
+#define LJG_EXPAND_LAUNCH(MV, GV, CUBV)                                                                           \
+  {                                                                                                               \
+    const size_t lds = (size_t)(kExpandThreads / (kWave / GV)) * (size_t)cap * sizeof(int32_t);                   \
+    if (lds <= 65536)                                                                                             \
+      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, CUBV, true>), dim3(nb), dim3(kExpandThreads),         \
+                         (unsigned)lds, c.stream, ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg,     \
+                         ilen, cut_lo, cut_hi, jcap, c.jcount.p, c.jlist.p, c.list_stats.p);                      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, CUBV, false>), dim3(nb), dim3(kExpandThreads), 0,     \
+                         c.stream, ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, cut_lo,      \
+                         cut_hi, jcap, c.jcount.p, c.jlist.p, c.list_stats.p);                                    \
+  }
 #define LJG_EXPAND(MV, GV)                                                                                        \
   if (M == MV && eg == GV) {                                                                                      \
     const int nb = blocks_for(ncl, (kExpandThreads / kWave) * GV);                                               \
     if (c.fg.cubic)                                                                                               \
-      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, true>), dim3(nb), dim3(kExpandThreads), 0, c.stream, \
-                         ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, cut_lo, cut_hi, jcap,  \
-                         c.jcount.p, c.jlist.p, c.list_stats.p);                                                 \
+      LJG_EXPAND_LAUNCH(MV, GV, true)                                                                             \
     else                                                                                                          \
-      hipLaunchKernelGGL((k_jrows_expand<MV, kJclSplit, GV, false>), dim3(nb), dim3(kExpandThreads), 0,          \
-                         c.stream, ncl, c.posfx.p, c.c0.p, cap, c.list_count.p, c.jcl.p, c.fg, ilen, cut_lo,      \
-                         cut_hi, jcap, c.jcount.p, c.jlist.p, c.list_stats.p);                                   \
+      LJG_EXPAND_LAUNCH(MV, GV, false)                                                                            \
     goto expanded;                                                                                                \
   }
       LJG_EXPAND(1, 4) LJG_EXPAND(1, 8) LJG_EXPAND(2, 2) LJG_EXPAND(2, 4) LJG_EXPAND(4, 1) LJG_EXPAND(4, 2)
       LJG_EXPAND(4, 4) LJG_EXPAND(8, 1) LJG_EXPAND(8, 2) LJG_EXPAND(8, 4)
 #undef LJG_EXPAND
+#undef LJG_EXPAND_LAUNCH
       throw HipError{hipErrorInvalidValue, "unsupported (cluster size, expansion groups) combination", -2};
     expanded:
       LJG_HIP(hipGetLastError());
