@@ -532,6 +532,12 @@ constexpr int kMarkFlagBoxes = 2 * (kMarkWords - 1);    // ensembles up to this 
 // memory), into out, with one marker kernel whose pinned word the host polls.
 void stream_wait(Ctx &c, const unsigned long long *src = nullptr, int n = 0, unsigned long long *out = nullptr,
                  const unsigned long long *src2 = nullptr, int n2 = 0);
+// The two halves of stream_wait, for work that can be queued between them (it runs while the host waits
+// for the marker): stream_mark queues the marker that copies n words of src and returns its sequence
+// number; stream_spin waits for that marker and reads the n words into out.  No other marker may be
+// queued in between.
+unsigned long long stream_mark(Ctx &c, const unsigned long long *src, int n);
+void stream_spin(Ctx &c, unsigned long long seq, int n, unsigned long long *out);
 // stats[0] += sum(cnt[0..n)), stats[1] = max(stats[1], max(cnt)); a few hundred atomics at most.
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats);  // ljg_md.hip
 // Ensemble (ljg_ensemble.hip): per-box tower grids and fixed regions for cluster size c.M, the tower

@@ -480,14 +480,19 @@ __global__ void k_mark(unsigned long long *__restrict__ dst, const unsigned long
   }
 }
 
-void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long long *out,
-                 const unsigned long long *src2, int n2) {
+static unsigned long long mark_launch(Ctx &c, const unsigned long long *src, int n, const unsigned long long *src2,
+                                      int n2) {
   if (n < 0 || n2 < 0 || n + n2 > kMarkWords)
     throw HipError{hipErrorInvalidValue, "stream_wait reads at most kMarkWords words", __LINE__};
   static_assert(kMarkWords < 64, "k_mark copies a word per lane of one wavefront");
   const unsigned long long seq = ++c.mark_seq;
   hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, c.stream, c.pin_mark_dev, src, (int32_t)n, src2, (int32_t)n2, seq);
   LJG_HIP(hipGetLastError());
+  return seq;
+}
+
+void stream_spin(Ctx &c, unsigned long long seq, int n, unsigned long long *out) {
+  if (seq != c.mark_seq) throw HipError{hipErrorInvalidValue, "another stream marker was queued", __LINE__};
   volatile unsigned long long *m = c.pin_mark;
   const auto t0 = std::chrono::steady_clock::now();
   for (uint64_t spins = 0; m[0] != seq; ++spins) {
@@ -497,7 +502,17 @@ void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long lon
       break;
     }
   }
-  for (int i = 0; i < n + n2; ++i) out[i] = m[1 + i];
+  for (int i = 0; i < n; ++i) out[i] = m[1 + i];
+}
+
+unsigned long long stream_mark(Ctx &c, const unsigned long long *src, int n) {
+  return mark_launch(c, src, n, nullptr, 0);
+}
+
+void stream_wait(Ctx &c, const unsigned long long *src, int n, unsigned long long *out,
+                 const unsigned long long *src2, int n2) {
+  const unsigned long long seq = mark_launch(c, src, n, src2, n2);
+  stream_spin(c, seq, n + n2, out);
 }
 
 void count_stats(Ctx &c, const int32_t *cnt, int64_t n, unsigned long long *stats) {

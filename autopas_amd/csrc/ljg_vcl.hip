@@ -1348,6 +1348,15 @@ void vcl_build(Ctx &c) {
   const Segs sg_slots = partial ? c.seg_slots : none, sg_towers = partial ? c.seg_towers : none;
   const Segs sg_clusters = partial ? c.seg_clusters : none;
   const int64_t n_slots = partial ? c.seg_nslots : ns, n_towers = partial ? c.seg_ntowers : ntow;
+  const bool guess = partial && !c.partial_tower_sync;
+  // The sizes' marker goes before the scatter and the clearing of the list statistics, which need no
+  // size: they run while the host waits for it, and the first launch after the wait is a long one.
+  const unsigned long long tmark = guess ? 0ull : stream_mark(c, c.tstats.p, 4);
+  // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
+  // pass, [3], [4] total and longest j-particle row, [5] sort tile exceeded; read back once at the end
+  c.list_stats.ensure(6);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)6, c.list_stats.p);
+  LJG_HIP(hipGetLastError());
   if (n_slots > 0) {
     hipLaunchKernelGGL(k_tower_scatter, dim3(blocks_for(n_slots, 256)), dim3(256), 0, c.stream, n_slots, sg_slots,
                        c.tower_of.p, c.rank.p, c.tower_start.p, s.z.p, s.id.p, c.tower_items.p, c.tower_z.p,
@@ -1360,11 +1369,10 @@ void vcl_build(Ctx &c) {
   // register sorts up to it: a longer tower (rare) is reported by the longest instance launched, the
   // permutation then skips, and the build runs again with the readback.
   unsigned long long ts[4] = {0, 0, 0, 0};
-  const bool guess = partial && !c.partial_tower_sync;
   if (guess) {
     ts[1] = (unsigned long long)(c.seg_max_count < 8 * kWave ? c.seg_max_count : 8 * kWave);
   } else {
-    stream_wait(c, c.tstats.p, 4, ts);
+    stream_spin(c, tmark, 4, ts);
   }
   const int64_t ncl = ens ? c.box_clusters : (int64_t)ts[2];
   if (ts[1] > (unsigned long long)kTowerSortCap)
@@ -1374,11 +1382,6 @@ void vcl_build(Ctx &c) {
   c.perm.ensure(new_slots);
   c.cluster_tower.ensure(ncl);
   if (ens) ensemble_region_prep(c, active);  // the regions' tails stay padding
-  // list_stats: [0], [1] total and longest row of the cluster-pair pass, [2] frame flag of the expansion
-  // pass, [3], [4] total and longest j-particle row, [5] sort tile exceeded; read back once at the end
-  c.list_stats.ensure(6);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int64_t)6, c.list_stats.p);
-  LJG_HIP(hipGetLastError());
   // towers of up to 512 particles sort in registers, a wavefront each (three instances by length, each
   // launched only if some tower is in its range); longer ones in LDS
   const bool wave_sort = ens || (int64_t)ts[0] <= (int64_t)kWaveSortBelow * ntow;
@@ -1453,6 +1456,7 @@ void vcl_build(Ctx &c) {
   int32_t cap = jrows ? (c.jccap > 0 ? c.jccap : kJclSplit * 24)
                       : partial ? c.list_row : (c.list_cap > 0 ? c.list_cap : 64);
   int32_t jcap = c.jcap > 0 ? c.jcap : 160;
+  bool pos32_queued = false;
   for (;;) {
     if (jrows) {
       c.jcl.ensure((size_t)ncl * cap + 1);
@@ -1543,6 +1547,13 @@ void vcl_build(Ctx &c) {
       LJG_HIP(hipGetLastError());
       count_stats(c, c.jcount.p, ncl, c.list_stats.p + 3);
     }
+    // the FP32 positions of the cluster kernels need no list: queued before the wait, they are made while
+    // the host waits (once, also when the lists are made again with larger rows; not in a partial build
+    // on guessed sizes, which may find that nothing was permuted and start again)
+    if (!jrows && !guess && !pos32_queued) {
+      vcl_pos32(c, active);
+      pos32_queued = true;
+    }
     unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
     stream_wait(c, c.list_stats.p, 6, st);
     if (st[5] && guess) {
@@ -1596,7 +1607,7 @@ void vcl_build(Ctx &c) {
   c.lists_n3 = c.want_n3 && !jrows;
   c.built_M = M;
   c.layout = Layout::Clusters;
-  if (!jrows) vcl_pos32(c, active);
+  if (!jrows && !pos32_queued) vcl_pos32(c, active);
   if (!jrows && !c.lists_n3 && !c.prm.whole_cluster_rows) vcl_member_rows(c, partial, active);
   else c.xrows_valid = false;
 }
