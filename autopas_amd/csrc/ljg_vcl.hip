@@ -353,59 +353,72 @@ __global__ __launch_bounds__(kSortThreads) void k_tower_sort(
 // of 257 to 512.
 constexpr int kSortWaveThreads = 256;
 
-// zid_less on values held in registers, without branches (padding sa < 0 sorts last).
-__device__ inline bool zid_less_v(double za, int32_t sa, int64_t ia, double zb, int32_t sb, int64_t ib) {
-  return (sa >= 0) & ((sb < 0) | (za < zb) | ((za == zb) & (ia < ib)));
+// The register sort carries two words per element: its z rounded to FP32 and its index k in the tower's
+// item list.  Rounding is monotone, so a smaller FP32 z means a smaller z; equal FP32 z (rare) are
+// ordered by the exact (z, id) read back from the item list, in a branch a wavefront takes only when one
+// of its lanes holds such a pair.  Indices k < 0 are padding and sort last.
+struct TowerItems {
+  const double *z;
+  const int64_t *id;
+};
+
+__device__ inline void zk_less2(float za, int32_t ka, float zb, int32_t kb, const TowerItems &ti, bool &ab, bool &ba) {
+  ab = (ka >= 0) & ((kb < 0) | (za < zb));
+  ba = (kb >= 0) & ((ka < 0) | (zb < za));
+  const bool tie = (ka >= 0) & (kb >= 0) & (za == zb);
+  if (__builtin_amdgcn_ballot_w64(tie) != 0ull) {
+    if (tie) {
+      const double da = ti.z[ka], db = ti.z[kb];
+      const int64_t ia = ti.id[ka], ib = ti.id[kb];
+      ab = (da < db) | ((da == db) & (ia < ib));
+      ba = !ab;
+    }
+  }
 }
 
-// One compare-exchange stage of the register bitonic network (size, stride known at compile time).
 template <int E, int SIZE, int STRIDE>
-__device__ inline void bitonic_stage(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
+__device__ inline void bitonic2_stage(float (&zv)[E], int32_t (&kv)[E], int lane, const TowerItems &ti) {
   if constexpr (STRIDE >= kWave) {
-    // both elements of a pair sit in this lane: slots e and e + STRIDE / 64
     constexpr int ks = STRIDE / kWave;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       if ((e & ks) != 0) continue;
       const int e2 = e + ks;
       const bool up = ((e * kWave + lane) & SIZE) == 0;
-      const bool swap = up ? zid_less_v(zv[e2], sv[e2], iv[e2], zv[e], sv[e], iv[e])
-                           : zid_less_v(zv[e], sv[e], iv[e], zv[e2], sv[e2], iv[e2]);
-      const double za = zv[e], zb = zv[e2];
-      const int32_t sa = sv[e], sb = sv[e2];
-      const int64_t ia = iv[e], ib = iv[e2];
+      bool lt, gt;  // zv[e2] before zv[e], and the reverse
+      zk_less2(zv[e2], kv[e2], zv[e], kv[e], ti, lt, gt);
+      const bool swap = up ? lt : gt;
+      const float za = zv[e], zb = zv[e2];
+      const int32_t ka = kv[e], kb = kv[e2];
       zv[e] = swap ? zb : za; zv[e2] = swap ? za : zb;
-      sv[e] = swap ? sb : sa; sv[e2] = swap ? sa : sb;
-      iv[e] = swap ? ib : ia; iv[e2] = swap ? ia : ib;
+      kv[e] = swap ? kb : ka; kv[e2] = swap ? ka : kb;
     }
   } else {
-    const bool lower = (lane & STRIDE) == 0;  // this lane holds the lower index of its pairs
+    const bool lower = (lane & STRIDE) == 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const double zo = __shfl_xor(zv[e], STRIDE, kWave);
-      const int32_t so = __shfl_xor(sv[e], STRIDE, kWave);
-      const int64_t io = __shfl_xor(iv[e], STRIDE, kWave);
-      const bool up = ((e * kWave + lane) & SIZE) == 0;  // this element's block sorts ascending
-      // the lower element of an ascending pair keeps the smaller key, and so on
-      const bool take = (lower == up) ? zid_less_v(zo, so, io, zv[e], sv[e], iv[e])
-                                      : zid_less_v(zv[e], sv[e], iv[e], zo, so, io);
+      const float zo = __shfl_xor(zv[e], STRIDE, kWave);
+      const int32_t ko = __shfl_xor(kv[e], STRIDE, kWave);
+      const bool up = ((e * kWave + lane) & SIZE) == 0;
+      bool o_first, me_first;
+      zk_less2(zo, ko, zv[e], kv[e], ti, o_first, me_first);
+      const bool take = (lower == up) ? o_first : me_first;
       zv[e] = take ? zo : zv[e];
-      sv[e] = take ? so : sv[e];
-      iv[e] = take ? io : iv[e];
+      kv[e] = take ? ko : kv[e];
     }
   }
 }
 
 template <int E, int SIZE, int STRIDE>
-__device__ inline void bitonic_strides(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
-  bitonic_stage<E, SIZE, STRIDE>(zv, sv, iv, lane);
-  if constexpr (STRIDE > 1) bitonic_strides<E, SIZE, STRIDE / 2>(zv, sv, iv, lane);
+__device__ inline void bitonic2_strides(float (&zv)[E], int32_t (&kv)[E], int lane, const TowerItems &ti) {
+  bitonic2_stage<E, SIZE, STRIDE>(zv, kv, lane, ti);
+  if constexpr (STRIDE > 1) bitonic2_strides<E, SIZE, STRIDE / 2>(zv, kv, lane, ti);
 }
 
 template <int E, int SIZE>
-__device__ inline void bitonic_sizes(double (&zv)[E], int32_t (&sv)[E], int64_t (&iv)[E], int lane) {
-  bitonic_strides<E, SIZE, SIZE / 2>(zv, sv, iv, lane);
-  if constexpr (SIZE < kWave * E) bitonic_sizes<E, SIZE * 2>(zv, sv, iv, lane);
+__device__ inline void bitonic2_sizes(float (&zv)[E], int32_t (&kv)[E], int lane, const TowerItems &ti) {
+  bitonic2_strides<E, SIZE, SIZE / 2>(zv, kv, lane, ti);
+  if constexpr (SIZE < kWave * E) bitonic2_sizes<E, SIZE * 2>(zv, kv, lane, ti);
 }
 
 template <int E>
@@ -426,23 +439,22 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
   // (over: the longest instance launched reports a tower beyond its reach, which no launched sort takes)
   if (n > P && over && lane == 0) over[0] = 1ull;
   if (n > P || n <= LO) return;  // another instance sorts this tower
-  int32_t sv[E];
-  double zv[E];
-  int64_t iv[E];
+  float zv[E];
+  int32_t kv[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = e * kWave + lane;
-    sv[e] = k < n ? items[b + k] : -1;
-    zv[e] = k < n ? items_z[b + k] : 0.;
-    iv[e] = k < n ? items_id[b + k] : 0;
+    kv[e] = k < n ? k : -1;
+    zv[e] = k < n ? (float)items_z[b + k] : 0.f;
   }
-  bitonic_sizes<E, 2>(zv, sv, iv, lane);
+  const TowerItems ti{items_z + b, items_id + b};
+  bitonic2_sizes<E, 2>(zv, kv, lane, ti);
   const int64_t c0 = cstart[t], c1 = c0 + ccount[t];
   const int nslot = (int)((c1 - c0) * M);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = e * kWave + lane;
-    if (k < nslot) src[c0 * M + k] = k < n ? sv[e] : -1;
+    if (k < nslot) src[c0 * M + k] = kv[e] >= 0 ? items[b + kv[e]] : -1;
   }
   if (P + lane < nslot) src[c0 * M + P + lane] = -1;
   for (int64_t c = c0 + lane; c < c1; c += kWave) cluster_tower[c] = (int32_t)t;

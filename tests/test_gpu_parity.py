@@ -148,6 +148,34 @@ def test_cluster_lists_bit_exact_long_towers(oracle, nz, sorter, list_mode):
         assert np.array_equal(g[k], r[k]), k
 
 
+@pytest.mark.parametrize("nz", [12, 40, 100])
+def test_tower_sort_ties_in_fp32_z(oracle, nz):
+    # The register tower sorts order by z rounded to FP32 and settle equal FP32 z by the exact (z, id).
+    # An unjittered lattice holds whole planes of equal z; half of its particles are then moved up by
+    # 2^-40 of their z (the same FP32 z, a different FP64 z), and the ids are shuffled, so that the id
+    # order disagrees with the z order inside every tie.  nz 12, 40, 100: towers for each register sort.
+    announce(f"tower sort ties in FP32 z, M=8: 20x20x{nz} unjittered lattice rho 0.75, shuffled ids")
+    a = 0.75 ** (-1.0 / 3.0)
+    dims = (20, 20, nz)
+    n = 20 * 20 * nz
+    x, y, z0, _, _, _, _ = ap.generate_lattice(5, n, dims, a, (0.0, 0.0, 0.0), 0.0, 0.9)
+    rng = np.random.default_rng(nz)
+    up = rng.random(n) < 0.5
+    z = np.where(up, z0 + z0 * 2.0 ** -40, z0)
+    moved = up & (z != z0)
+    assert np.count_nonzero(moved & (z.astype(np.float32) == z0.astype(np.float32))) > n // 4
+    box = tuple(d * a for d in dims)
+    ids = rng.permutation(n).astype(np.int64)
+    with ap.Engine((0.0, 0.0, 0.0), box, RC, SKIN, cluster_size=8) as e:
+        e.upload(x, y, z, id=ids)
+        e.build_pairs()
+        g = e.clusters()
+    wx, wy, wz = oracle.wrap(x, y, z, (0, 0, 0), box)
+    r = oracle.vcl_periodic(wx, wy, wz, ids, (0, 0, 0), box, RC + SKIN, 8)
+    for k in ("slot_particle", "cluster_tower", "bbox"):
+        assert np.array_equal(g[k], r[k]), k
+
+
 @pytest.mark.parametrize("M,n_side", [(1, 10), (2, 10), (4, 10), (8, 10), (4, 6), (8, 6)])
 def test_cluster_jrows_exact(oracle, M, n_side):
     # Rows of the j-particle cluster kernel = union of the members' fixed-point Verlet lists (members
