@@ -579,7 +579,7 @@ __device__ inline void wave_sync() {
 // MB (an ensemble of boxes): tower t belongs to box tbox[t], whose grid is tgs[box] and whose towers
 // start at tbase[box]; the traversal inside the box is the single-box one.
 template <bool JCL, int NS, bool MB>
-__global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
+__global__ __launch_bounds__(kListThreads * NS, JCL ? 4 : 1) void k_cluster_lists_tower(
     int64_t ntowers, Segs sg, TowerGeom g1, const TowerGeom *__restrict__ tgs, const int64_t *__restrict__ tbase,
     const int32_t *__restrict__ tbox, const int32_t *__restrict__ box_active, const int64_t *__restrict__ cstart,
     const int32_t *__restrict__ ccount, const double *__restrict__ bbox, const double4 *__restrict__ c0,
@@ -623,29 +623,70 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
     }
     const int64_t w0 = ci * (int64_t)cap + (int64_t)wv * subcap;
     int32_t cnt = 0;
-    for (int v = vbeg; v < vend; ++v) {
-      const int oy = v / side - g.R, ox = v % side - g.R;
-      const int aoy = oy < 0 ? -oy : oy;
-      const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
-      const int aox = ox < 0 ? -ox : ox;
-      const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
-      if (dtx * dtx + dty * dty > il2) continue;
-      int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
-      while (jx < 0) { jx += g.nt[0]; --sx; }
-      while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
-      while (jy < 0) { jy += g.nt[1]; --sy; }
-      while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
-      const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
-      const int64_t cb = cstart[tj], ce = cb + ccount[tj];
-      if (cb == ce) continue;
+    // The visits with work, in order: the neighbour tower of visit v within reach and not empty.
+    auto next_visit = [&](int v0, int &vo, int &sxo, int &syo, int64_t &cbo, int64_t &ceo) {
+      for (int v = v0; v < vend; ++v) {
+        const int oy = v / side - g.R, ox = v % side - g.R;
+        const int aoy = oy < 0 ? -oy : oy;
+        const double dty = (double)(aoy - 1 > 0 ? aoy - 1 : 0) * g.side[1];
+        const int aox = ox < 0 ? -ox : ox;
+        const double dtx = (double)(aox - 1 > 0 ? aox - 1 : 0) * g.side[0];
+        if (dtx * dtx + dty * dty > il2) continue;
+        int jx = tx + ox, jy = ty + oy, sx = 0, sy = 0;
+        while (jx < 0) { jx += g.nt[0]; --sx; }
+        while (jx >= g.nt[0]) { jx -= g.nt[0]; ++sx; }
+        while (jy < 0) { jy += g.nt[1]; --sy; }
+        while (jy >= g.nt[1]) { jy -= g.nt[1]; ++sy; }
+        const int64_t tj = tb + jx + (int64_t)jy * g.nt[0];
+        const int64_t cb = cstart[tj], ce = cb + ccount[tj];
+        if (cb == ce) continue;
+        vo = v; sxo = sx; syo = sy; cbo = cb; ceo = ce;
+        return true;
+      }
+      vo = vend;
+      return false;
+    };
+    // JCL: the boxes of the next visit's tower (one chunk) are loaded into registers while this visit runs
+    // from LDS, and stored to LDS when the next visit starts, so no visit waits for its tower's boxes
+    constexpr int kPf = JCL ? 6 * LDSN / kWave : 1;
+    double pf[kPf];
+    auto prefetch = [&](int64_t pcb, int64_t pce) {
+      const int pn = 6 * (int)(pce - pcb);
+#pragma unroll
+      for (int i = 0; i < kPf; ++i) {
+        const int k = lane + i * kWave;
+        pf[i] = k < pn ? bbox[6 * pcb + k] : 0.;
+      }
+    };
+    int v = vbeg, sx = 0, sy = 0;
+    int64_t cb = 0, ce = 0;
+    bool have = next_visit(vbeg, v, sx, sy, cb, ce);
+    if (JCL && have && ce - cb <= LDSN) prefetch(cb, ce);
+    while (have) {
+      int v2 = vend, sx2 = 0, sy2 = 0;
+      int64_t cb2 = 0, ce2 = 0;
+      const bool have2 = next_visit(v + 1, v2, sx2, sy2, cb2, ce2);
       const bool single = ce - cb <= LDSN;
+      if (JCL && single) {
+        const int pn = 6 * (int)(ce - cb);
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < kPf; ++i) {
+          const int k = lane + i * kWave;
+          if (k < pn) sb[k] = pf[i];
+        }
+        wave_sync();
+      }
+      // (after this visit's boxes left the registers; a visit of several chunks stages its own)
+      if (JCL && have2 && ce2 - cb2 <= LDSN) prefetch(cb2, ce2);
+      const int oy = v / side - g.R, ox = v % side - g.R;
       for (int sz = -1; sz <= 1; ++sz) {
         const double shx = sx * g.L[0], shy = sy * g.L[1], shz = sz * g.L[2];
         const double zmin = bi[2] - zwin - shz;  // window in the j tower's own frame
         const double zmax = bi[5] + zwin - shz;
         for (int64_t jb = cb; jb < ce; jb += LDSN) {
           const int nj = (int)(ce - jb < LDSN ? ce - jb : LDSN);
-          if (!single || (sz == -1 && jb == cb)) {
+          if (!single || (!JCL && sz == -1 && jb == cb)) {
             wave_sync();
             for (int k = lane; k < 6 * nj; k += kWave) sb[k] = bbox[6 * jb + k];
             if constexpr (!JCL)
@@ -688,6 +729,7 @@ __global__ __launch_bounds__(kListThreads * NS) void k_cluster_lists_tower(
           }
         }
       }
+      v = v2; sx = sx2; sy = sy2; cb = cb2; ce = ce2; have = have2;
     }
     if (act) count[ci * NS + wv] = cnt;
   }
