@@ -1103,7 +1103,23 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
     bool sum_read = false;
     if (rebuild_next) {
       const unsigned long long marks = c.mark_seq;
-      rebuild_now();
+      // a j-row build followed by its force leaves its list lengths to be read after that launch
+      c.defer_lists = force_next && jrows;
+      try {
+        rebuild_now();
+      } catch (...) {
+        c.defer_lists = false;
+        throw;
+      }
+      c.defer_lists = false;
+      // the force after the build, checked: a launch on rows that overflowed exited, so build again
+      auto force_after_build = [&]() {
+        force_step(s + k, true, stamp_of(s + k) - 1);
+        if (!vcl_finish_lists(c)) {
+          rebuild_now();
+          force_step(s + k, true, stamp_of(s + k) - 1);
+        }
+      };
       if (force_next && (!dyn || c.mark_seq != marks)) {
         // The build waited for the stream, so every drift queued before it has run and the host copy of
         // the flag is current.  Raised by no drift, or only by the one of step s + k (the rebuild just made
@@ -1115,7 +1131,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
             hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, c.flag.p, (int32_t *)nullptr);
             LJG_HIP(hipGetLastError());
           }
-          force_step(s + k, true, stamp_of(s + k) - 1);
+          force_after_build();
           c.steps_since_rebuild = 1;
           for (auto &v : c.box_since) v = 1;
           total_read = false;
@@ -1123,7 +1139,7 @@ static void run_steps(Ctx &c, int32_t kernel, int32_t nsteps, double dt, bool ti
           continue;
         }
       }
-      if (force_next) force_step(s + k, true, stamp_of(s + k) - 1);
+      if (force_next) force_after_build();
       stream_wait(c);
     } else if (dyn || due) {
       unsigned long long words[kMarkWords] = {};

@@ -1204,11 +1204,20 @@ static void launch_jrows(Ctx &c, bool mix, bool glob, const ForceArgs &a, const 
   cutoff_band(rc2, rel, cut_lo, cut_hi);
   State &s = c.st[c.cur];
   const IntegArgs ia = integ ? *integ : IntegArgs{};
+  // rows whose lengths the host has not read: the launch exits on the device if they overflowed
+  SkipIf sk = c.skip;
+  if (c.lists_pending) {
+    sk.lst = c.list_stats.p;
+    sk.cap1 = (uint32_t)(c.pend_cap / kJclSplit);
+    sk.cap2 = (uint32_t)c.pend_jcap;
+    sk.frame = fast ? 1 : 0;
+    c.pend_fast = fast;
+  }
 #define LJG_LAUNCH(CUB, MIXV, GLOBV, INTV)                                                                          \
   hipLaunchKernelGGL((k_force_cluster_jrows<M, G, CUB, MIXV, GLOBV, INTV>), dim3(nb), dim3(kFThreads), 0, c.stream, \
                      c.nclusters, c.posfx.p, c.c0.p, c.list_row, c.jcount.p, c.jlist.p, f, cut_lo, cut_hi, a.rc2, \
                      (float)a.eps24, sig2, (float)a.shift6, a.ntypes, tab, s.fx.p, s.fy.p, s.fz.p, c.part_e.p,    \
-                     c.part_v.p, c.part_hits.p, c.hits_acc, c.skip, ia, owned_of(c))
+                     c.part_v.p, c.part_hits.p, c.hits_acc, sk, ia, owned_of(c))
 #define LJG_LAUNCH2(CUB)                                                                                 \
   if (integ) {                                                                                           \
     if (mix) LJG_LAUNCH(CUB, true, false, true); else LJG_LAUNCH(CUB, false, false, true);              \

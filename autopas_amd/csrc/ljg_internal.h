@@ -134,11 +134,21 @@ struct FixGeom {
 // stays set until the host has rebuilt.  The step loop queues several steps before it reads the flag;
 // every launch of a step at or after the raising one exits at once, so the host resumes from that step
 // after the rebuild.  Writes of the flag by the launch itself carry a larger stamp and do not count.
+// lst (a j-row build whose list lengths the host has not read yet): the launch also exits when the
+// build's statistics show a row longer than its capacity (lst[1] > cap1 for the first pass's sub-rows,
+// lst[4] > cap2 for the j rows, lst[5] a sort tile exceeded), or, with frame set, a row that leaves the
+// first member's frame (lst[2]).
 struct SkipIf {
   const int32_t *flag;
   int32_t stamp;
+  const unsigned long long *lst = nullptr;
+  uint32_t cap1 = 0, cap2 = 0;
+  int32_t frame = 0;
 };
 __device__ inline bool skip_launch(const SkipIf &s) {
+  if (s.lst != nullptr &&
+      (s.lst[1] > s.cap1 || s.lst[4] > s.cap2 || s.lst[5] != 0ull || (s.frame != 0 && s.lst[2] != 0ull)))
+    return true;
   if (s.flag == nullptr) return false;
   const int32_t f = *s.flag;
   return f != 0 && f <= s.stamp;
@@ -352,6 +362,13 @@ struct Ctx {
   bool want_jrows = false;       // the next cluster build emits j-particle rows instead of cluster entries
   bool jrows_valid = false;      // the current cluster rows are j-particle rows
   bool jrows_frame_ok = false;   // every row pair within ilen has the same separation in the first member's frame
+  // A j-row build the step loop follows with its force at once may leave its list lengths unread
+  // (defer_lists, set by the step loop around that build): the force launch then carries a device-side
+  // veto on them (SkipIf::lst) and vcl_finish_lists reads them afterwards.
+  bool defer_lists = false;
+  bool lists_pending = false;    // the current j rows' lengths are not yet checked on the host
+  int32_t pend_cap = 0, pend_jcap = 0;
+  bool pend_fast = false;        // the vetoed launch used the first member's frame
   bool want_n3 = false;          // the next cluster build emits Newton-3 (forward-only) cluster-entry lists
   bool lists_n3 = false;         // the current cluster-entry lists are Newton-3 lists
   DevBuf<int32_t> vcount;        // neighbours per particle
@@ -480,6 +497,10 @@ void reduce_launch(hipStream_t st, int64_t nitems, const double *pe, const doubl
                    double *out, int64_t *out_h, bool globals);  // ljg_md.hip
 void lc_force(Ctx &c, bool globals);       // ljg_bin.hip (plain FP64 kernel)
 void vcl_build(Ctx &c);                    // ljg_vcl.hip
+// Reads and checks the list lengths of a j-row build that left them pending (true when none is pending):
+// false when a row overflowed or the vetoed force launch assumed a frame the rows leave, in which case
+// that launch exited without effect and the caller builds again and launches again.  ljg_vcl.hip
+bool vcl_finish_lists(Ctx &c);
 void fill_i32(Ctx &c, int32_t *p, int64_t n, int32_t v);  // ljg_vcl.hip: p[0..n) = v on the stream
 void vcl_pos32(Ctx &c, const int32_t *active = nullptr);  // ljg_vcl.hip (active: ensemble boxes to write)
 // A launch of the FP32 cluster kernel over some i-clusters only (the segments' nitems clusters), with

@@ -1331,7 +1331,35 @@ constexpr int kWaveSortBelow = 448;
 // twice, and a build this large fills the GPU with lanes anyway)
 constexpr int64_t kSplitListsBelow = 1 << 18;
 
+// The capacities a build keeps for the next one, with headroom for fluctuations (a melting lattice
+// lengthens its rows by ~10% between rebuilds early on: a quarter of headroom keeps the two passes from
+// running twice); a partial build keeps the stride the other boxes' rows are stored with.
+static int32_t keep_capacity(bool partial, int32_t used, int32_t had) {
+  if (partial) return had;
+  const int32_t want = used + used / 4 + 8;
+  return want > had ? want : (had > 2 * want ? want : had);
+}
+
+bool vcl_finish_lists(Ctx &c) {
+  if (!c.lists_pending) return true;
+  c.lists_pending = false;
+  unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+  stream_wait(c, c.list_stats.p, 6, st);
+  const int32_t maxc = (int32_t)st[1], maxj = (int32_t)st[4];
+  const bool overflow = st[5] != 0ull || maxc > c.pend_cap / kJclSplit || maxj > c.pend_jcap;
+  c.jrows_frame_ok = st[2] == 0ull;
+  if (overflow) {
+    c.jrows_valid = false;  // the rows are incomplete: the next use builds again
+    return false;
+  }
+  c.jccap = kJclSplit * keep_capacity(false, maxc, c.pend_cap / kJclSplit);
+  c.jcap = keep_capacity(false, maxj, c.pend_jcap);
+  c.nentries = (int64_t)st[3];
+  return !(c.pend_fast && !c.jrows_frame_ok);
+}
+
 void vcl_build(Ctx &c) {
+  c.lists_pending = false;
   const int M = c.M;
   const double ilen = c.prm.cutoff + c.prm.skin;
   // Tower grid: ClusterTowerBlock2D::estimateOptimalGridSideLength (ClusterTowerBlock2D.h:140-168) with
@@ -1601,6 +1629,21 @@ void vcl_build(Ctx &c) {
       LJG_HIP(hipGetLastError());
       count_stats(c, c.jcount.p, ncl, c.list_stats.p + 3);
     }
+    // A j-row build the step loop follows with its force at once: the lengths are read after that launch,
+    // which exits on the device if a row overflowed (vcl_finish_lists).  The first build of a context
+    // (capacities not yet measured) is checked here.
+    if (jrows && c.defer_lists && !partial && !ens && c.jccap > 0 && c.jcap > 0) {
+      c.lists_pending = true;
+      c.pend_cap = cap;
+      c.pend_jcap = jcap;
+      c.list_row = jcap;
+      c.jrows_valid = true;
+      c.lists_n3 = false;
+      c.built_M = M;
+      c.layout = Layout::Clusters;
+      c.xrows_valid = false;
+      return;
+    }
     // the FP32 positions of the cluster kernels need no list: queued before the wait, they are made while
     // the host waits (once, also when the lists are made again with larger rows; not in a partial build
     // on guessed sizes, which may find that nothing was permuted and start again)
@@ -1635,15 +1678,8 @@ void vcl_build(Ctx &c) {
     }
     if (jrows && ok1 && !ok2) jcap = maxj + maxj / 8 + 8;
     if (!(ok1 && ok2)) continue;
-    // keep the row capacities for the next build, with headroom for fluctuations
-    // (a melting lattice lengthens its rows by ~10% between rebuilds early on: a quarter of headroom keeps
-    // the two passes from running twice)
-    // (a partial build keeps the stride the other boxes' rows are stored with)
-    auto keep = [partial](int32_t used, int32_t had) {
-      if (partial) return had;
-      const int32_t want = used + used / 4 + 8;
-      return want > had ? want : (had > 2 * want ? want : had);
-    };
+    // keep the row capacities for the next build
+    auto keep = [partial](int32_t used, int32_t had) { return keep_capacity(partial, used, had); };
     if (jrows) {
       c.jccap = kJclSplit * keep(maxc, cap / kJclSplit);
       c.jcap = keep(maxj, jcap);
