@@ -290,6 +290,7 @@ ljg_status ljg_create(const ljg_params *p, ljg_ctx **out) {
   if (p->list_mode < 0 || p->list_mode > 3) return LJG_ERR_ARG;
   if (p->shared_rebuilds < 0 || p->shared_rebuilds > 1) return LJG_ERR_ARG;
   if (p->graphs < 0 || p->graphs > 1) return LJG_ERR_ARG;
+  if (p->row_headroom < -1 || p->row_headroom > 400) return LJG_ERR_ARG;
   ljg_ctx *h = new (std::nothrow) ljg_ctx;
   if (!h) return LJG_ERR_HIP;
   Ctx &c = h->c;

@@ -1332,11 +1332,13 @@ constexpr int kWaveSortBelow = 448;
 constexpr int64_t kSplitListsBelow = 1 << 18;
 
 // The capacities a build keeps for the next one, with headroom for fluctuations (a melting lattice
-// lengthens its rows by ~10% between rebuilds early on: a quarter of headroom keeps the two passes from
-// running twice); a partial build keeps the stride the other boxes' rows are stored with.
-static int32_t keep_capacity(bool partial, int32_t used, int32_t had) {
+// lengthens its rows by ~10% between rebuilds early on: by default a quarter of headroom keeps the two
+// passes from running twice; ljg_params.row_headroom); a partial build keeps the stride the other boxes'
+// rows are stored with.
+static int32_t keep_capacity(const Ctx &c, bool partial, int32_t used, int32_t had) {
   if (partial) return had;
-  const int32_t want = used + used / 4 + 8;
+  const int32_t pct = c.prm.row_headroom == 0 ? 25 : (c.prm.row_headroom < 0 ? 0 : c.prm.row_headroom);
+  const int32_t want = used + (int32_t)((int64_t)used * pct / 100) + 8;
   return want > had ? want : (had > 2 * want ? want : had);
 }
 
@@ -1352,8 +1354,8 @@ bool vcl_finish_lists(Ctx &c) {
     c.jrows_valid = false;  // the rows are incomplete: the next use builds again
     return false;
   }
-  c.jccap = kJclSplit * keep_capacity(false, maxc, c.pend_cap / kJclSplit);
-  c.jcap = keep_capacity(false, maxj, c.pend_jcap);
+  c.jccap = kJclSplit * keep_capacity(c, false, maxc, c.pend_cap / kJclSplit);
+  c.jcap = keep_capacity(c, false, maxj, c.pend_jcap);
   c.nentries = (int64_t)st[3];
   return !(c.pend_fast && !c.jrows_frame_ok);
 }
@@ -1679,7 +1681,7 @@ void vcl_build(Ctx &c) {
     if (jrows && ok1 && !ok2) jcap = maxj + maxj / 8 + 8;
     if (!(ok1 && ok2)) continue;
     // keep the row capacities for the next build
-    auto keep = [partial](int32_t used, int32_t had) { return keep_capacity(partial, used, had); };
+    auto keep = [&c, partial](int32_t used, int32_t had) { return keep_capacity(c, partial, used, had); };
     if (jrows) {
       c.jccap = kJclSplit * keep(maxc, cap / kJclSplit);
       c.jcap = keep(maxj, jcap);

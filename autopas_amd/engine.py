@@ -64,6 +64,7 @@ class Params(ctypes.Structure):
         ("graphs", _i32),
         ("whole_cluster_rows", _i32),
         ("persistent", _i32),
+        ("row_headroom", _i32),
     ]
 
 
@@ -242,7 +243,7 @@ class Engine:
     def __init__(self, box_min, box_max, cutoff, skin, cluster_size=4, rebuild_frequency=10,
                  dynamic_rebuild=True, apply_shift=True, device=0, tile_groups=0, build_groups=0, tuning_samples=0,
                  tuning_interval=0, list_mode=0, shared_rebuilds=False, graphs=False, whole_cluster_rows=False,
-                 persistent=False):
+                 persistent=False, row_headroom=0):
         self.lib = load_library()
         p = Params()
         p.box_min[:] = list(box_min)
@@ -263,6 +264,7 @@ class Engine:
         p.graphs = int(bool(graphs))
         p.whole_cluster_rows = int(bool(whole_cluster_rows))
         p.persistent = int(bool(persistent))
+        p.row_headroom = int(row_headroom)
         self.params = p
         h = _p()
         st = self.lib.ljg_create(ctypes.byref(p), ctypes.byref(h))

@@ -75,7 +75,7 @@ def test_params_struct_matches_header():
     body = text[text.index("typedef struct {"):text.index("} ljg_params;")]
     fields = re.findall(r"(double|int32_t)\s+(\w+)(\[\d+\])?;", body)
     assert [f[1] for f in fields] == [f[0] for f in engine.Params._fields_]
-    assert ctypes.sizeof(engine.Params) == 8 * 8 + 14 * 4  # 8 doubles, 14 int32
+    assert ctypes.sizeof(engine.Params) == 8 * 8 + 16 * 4  # 8 doubles, 15 int32 and the struct's padding
 
 
 def test_kernel_ids_match_header():

@@ -148,6 +148,29 @@ def test_cluster_lists_bit_exact_long_towers(oracle, nz, sorter, list_mode):
         assert np.array_equal(g[k], r[k]), k
 
 
+def test_jrow_rebuild_after_vetoed_force_is_bit_identical():
+    # The force after a j-row build is launched before the host reads the list lengths, with a device-side
+    # veto on rows longer than their capacity; on a veto the step loop builds again and launches again.
+    # With no spare row capacity (row_headroom -1) the rows of a melting lattice outgrow the lengths kept
+    # from the previous build, so the vetoed path runs (more builds are counted); the trajectory must
+    # equal the one with the default headroom bit for bit.
+    announce("j-row force after a vetoed launch: 16^3 lattice rho 0.75, 200 steps, row_headroom -1 vs 0")
+    s = lattice_box(16, 0.75, seed=31, jitter=0.0, temperature=1.5)
+    L = s["L"]
+    out = {}
+    for hr in (0, -1):
+        with make_engine(L, M=8, row_headroom=hr) as e:
+            e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+            rs = e.step(200, 0.002, ap.KERNEL_CLUSTER_JROWS_MIXED)
+            out[hr] = (e.download(), rs.rebuilds, rs.pair_interactions)
+    (a, ra, pa), (b, rb, pb) = out[0], out[-1]
+    print(f"builds: headroom 25% {ra}, none {rb}; pair interactions {pa} {pb}")
+    assert rb > ra
+    assert pa == pb
+    for k in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"):
+        assert np.array_equal(a[k], b[k]), k
+
+
 @pytest.mark.parametrize("nz", [12, 40, 100])
 def test_tower_sort_ties_in_fp32_z(oracle, nz):
     # The register tower sorts order by z rounded to FP32 and settle equal FP32 z by the exact (z, id).
