@@ -9,8 +9,8 @@
  *   AutoPas<Particle_T>::computeInteractions(Functor*)    src/autopas/AutoPasImpl.h:116-135
  *     -> LogicHandler::computeInteractions                src/autopas/LogicHandler.h:1066-1143
  *       -> container->rebuildNeighborLists / computeInteractions(traversal)
- * The entry points below are the calls a C++ container + traversal plugged in under that API (see
- * INTEGRATION.md) makes; each one cites the reference interface it stands in for.
+ * The entry points below are the calls a C++ container + traversal plugged in under that API makes;
+ * each one cites the reference interface it stands in for.
  */
 #ifndef LJGPU_H
 #define LJGPU_H
