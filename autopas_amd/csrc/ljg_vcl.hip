@@ -460,6 +460,75 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
   for (int64_t c = c0 + lane; c < c1; c += kWave) cluster_tower[c] = (int32_t)t;
 }
 
+// Towers of 257 to 512 particles: two wavefronts per tower, each sorting one half in registers as
+// k_tower_sort_wave<4> does, then a merge by rank: an element's place is its index in its half plus the
+// number of the other half's elements before it (a binary search in the other half, held in LDS; equal
+// FP32 z are ordered by the exact z and id).  Twice the wavefronts of a single-wave sort of 512 slots,
+// each with half the network.
+constexpr int kSortPairThreads = 2 * kWave;
+
+__global__ __launch_bounds__(kSortPairThreads) void k_tower_sort_pair(
+    int64_t ntowers, Segs sg, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
+    const int32_t *__restrict__ items, const double *__restrict__ items_z, const int64_t *__restrict__ items_id,
+    const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
+    int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ over) {
+  constexpr int E = 4, P = kWave * E;  // each half up to 256
+  __shared__ float s_z[2][P];
+  __shared__ int32_t s_k[2][P];
+  const int lane = threadIdx.x & (kWave - 1), half = (int)threadIdx.x / kWave;
+  if ((int64_t)blockIdx.x >= ntowers) return;  // the whole workgroup: one tower
+  const int64_t t = seg_index(sg, blockIdx.x);
+  const int64_t b = start[t];
+  const int n = count[t];
+  if (n > 2 * P && over && threadIdx.x == 0) over[0] = 1ull;
+  if (n > 2 * P || n <= P) return;  // another instance sorts this tower
+  const int h = (n + 1) / 2;
+  const int first = half ? h : 0, len = half ? n - h : h, olen = half ? h : n - h;
+  float zf[E];
+  int32_t kv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = e * kWave + lane;
+    kv[e] = k < len ? first + k : -1;
+    zf[e] = k < len ? (float)items_z[b + first + k] : 0.f;
+  }
+  const TowerItems ti{items_z + b, items_id + b};
+  bitonic2_sizes<E, 2>(zf, kv, lane, ti);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    s_z[half][e * kWave + lane] = zf[e];
+    s_k[half][e * kWave + lane] = kv[e];
+  }
+  __syncthreads();
+  const float *oz = s_z[1 - half];
+  const int32_t *ok = s_k[1 - half];
+  const int64_t c0 = cstart[t], c1 = c0 + ccount[t];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = e * kWave + lane;
+    if (idx >= len) continue;
+    const float z = zf[e];
+    const int32_t k = kv[e];
+    int lo = 0, hi = olen;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const float zo = oz[mid];
+      bool before = zo < z;
+      if (zo == z) {
+        const int32_t ko = ok[mid];
+        const double da = ti.z[ko], db = ti.z[k];
+        before = (da < db) | ((da == db) & (ti.id[ko] < ti.id[k]));
+      }
+      if (before) lo = mid + 1;
+      else hi = mid;
+    }
+    src[c0 * M + idx + lo] = items[b + k];
+  }
+  const int nslot = (int)((c1 - c0) * M);
+  for (int q = n + (int)threadIdx.x; q < nslot; q += kSortPairThreads) src[c0 * M + q] = -1;
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += kSortPairThreads) cluster_tower[c] = (int32_t)t;
+}
+
 // One lane per slot, the M lanes of a cluster reduce with shuffles (M divides the wavefront, and the
 // cluster's slots are consecutive, so a cluster never straddles two wavefronts).  The members are sorted
 // by z, so the first and last real members' z are the minimum and maximum.
@@ -1487,7 +1556,12 @@ void vcl_build(Ctx &c) {
     const unsigned long long shortest = ens ? 0ull : ts[3];
     if (shortest <= (unsigned long long)kWave) LJG_WAVE_SORT(1, top == 1 ? over : nullptr)
     if (top >= 4 && shortest <= (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(4, top == 4 ? over : nullptr)
-    if (top == 8) LJG_WAVE_SORT(8, over)
+    if (top == 8 && n_towers > 0) {
+      hipLaunchKernelGGL(k_tower_sort_pair, dim3((unsigned)n_towers), dim3(kSortPairThreads), 0, c.stream, n_towers,
+                         sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p, c.tower_z.p, c.tower_id.p,
+                         c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, over);
+      LJG_HIP(hipGetLastError());
+    }
 #undef LJG_WAVE_SORT
   }
   if (ts[1] > (unsigned long long)min_n && n_towers > 0) {
