@@ -460,19 +460,22 @@ __global__ __launch_bounds__(kSortWaveThreads) void k_tower_sort_wave(
   for (int64_t c = c0 + lane; c < c1; c += kWave) cluster_tower[c] = (int32_t)t;
 }
 
-// Towers of 257 to 512 particles: two wavefronts per tower, each sorting one half in registers as
-// k_tower_sort_wave<4> does, then a merge by rank: an element's place is its index in its half plus the
-// number of the other half's elements before it (a binary search in the other half, held in LDS; equal
-// FP32 z are ordered by the exact z and id).  Twice the wavefronts of a single-wave sort of 512 slots,
-// each with half the network.
+// Towers of 65 to 256 (E = 2) and 257 to 512 particles (E = 4): two wavefronts per tower, each sorting
+// one half of up to 64 E in registers as k_tower_sort_wave<E> does, then a merge by rank: an element's
+// place is its index in its half plus the number of the other half's elements before it (a binary
+// search in the other half, held in LDS; equal FP32 z are ordered by the exact z and id).  Twice the
+// wavefronts of a single-wave sort of the tower, each with half the network.
 constexpr int kSortPairThreads = 2 * kWave;
 
+template <int E>
 __global__ __launch_bounds__(kSortPairThreads) void k_tower_sort_pair(
     int64_t ntowers, Segs sg, const int64_t *__restrict__ start, const int32_t *__restrict__ count,
     const int32_t *__restrict__ items, const double *__restrict__ items_z, const int64_t *__restrict__ items_id,
     const int64_t *__restrict__ cstart, const int32_t *__restrict__ ccount, int32_t M, int32_t *__restrict__ src,
     int32_t *__restrict__ cluster_tower, unsigned long long *__restrict__ over) {
-  constexpr int E = 4, P = kWave * E;  // each half up to 256
+  static_assert(E == 2 || E == 4, "pairs cover 65-256 and 257-512 particles");
+  constexpr int P = kWave * E;             // each half up to 64 E
+  constexpr int LO = E == 2 ? kWave : 4 * kWave;  // the next shorter instance's reach
   __shared__ float s_z[2][P];
   __shared__ int32_t s_k[2][P];
   const int lane = threadIdx.x & (kWave - 1), half = (int)threadIdx.x / kWave;
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(kSortPairThreads) void k_tower_sort_pair(
   const int64_t b = start[t];
   const int n = count[t];
   if (n > 2 * P && over && threadIdx.x == 0) over[0] = 1ull;
-  if (n > 2 * P || n <= P) return;  // another instance sorts this tower
+  if (n > 2 * P || n <= LO) return;  // another instance sorts this tower
   const int h = (n + 1) / 2;
   const int first = half ? h : 0, len = half ? n - h : h, olen = half ? h : n - h;
   float zf[E];
@@ -1554,14 +1557,17 @@ void vcl_build(Ctx &c) {
     unsigned long long *over = guess ? c.list_stats.p + 5 : nullptr;
     const int top = ts[1] > (unsigned long long)(4 * kWave) ? 8 : ts[1] > (unsigned long long)kWave ? 4 : 1;
     const unsigned long long shortest = ens ? 0ull : ts[3];
+#define LJG_PAIR_SORT(E, OVER)                                                                                  \
+  if (n_towers > 0) {                                                                                           \
+    hipLaunchKernelGGL(k_tower_sort_pair<E>, dim3((unsigned)n_towers), dim3(kSortPairThreads), 0, c.stream,    \
+                       n_towers, sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p, c.tower_z.p,     \
+                       c.tower_id.p, c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, OVER);     \
+    LJG_HIP(hipGetLastError());                                                                                 \
+  }
     if (shortest <= (unsigned long long)kWave) LJG_WAVE_SORT(1, top == 1 ? over : nullptr)
-    if (top >= 4 && shortest <= (unsigned long long)(4 * kWave)) LJG_WAVE_SORT(4, top == 4 ? over : nullptr)
-    if (top == 8 && n_towers > 0) {
-      hipLaunchKernelGGL(k_tower_sort_pair, dim3((unsigned)n_towers), dim3(kSortPairThreads), 0, c.stream, n_towers,
-                         sg_towers, c.tower_start.p, c.tower_count.p, c.tower_items.p, c.tower_z.p, c.tower_id.p,
-                         c.ctow_start.p, c.ctow_count.p, M, c.perm.p, c.cluster_tower.p, over);
-      LJG_HIP(hipGetLastError());
-    }
+    if (top >= 4 && shortest <= (unsigned long long)(4 * kWave)) LJG_PAIR_SORT(2, top == 4 ? over : nullptr)
+    if (top == 8) LJG_PAIR_SORT(4, over)
+#undef LJG_PAIR_SORT
 #undef LJG_WAVE_SORT
   }
   if (ts[1] > (unsigned long long)min_n && n_towers > 0) {
