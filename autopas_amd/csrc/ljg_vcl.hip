@@ -1245,6 +1245,16 @@ constexpr int kXrowThreads = 256;
 #define LJG_XROWS_SMALL 32768
 #endif
 
+// A list entry and the FP32 position of one of its members, loaded together by the member-row pass.
+struct EntryPos {
+  int4 e;
+  float4 p;
+};
+__device__ inline EntryPos make_pair_ef(const int4 &e, const float4 &p) { return EntryPos{e, p}; }
+// The entry a lane of an empty row reads: cluster 0, member slots of a real cluster, never near anything
+// it is tested against (the lane is past its row).
+__device__ const int4 kZeroEntry = {0, 0, 0, 0};
+
 template <int M, int G>
 __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, Segs sg,
                                                                 const float4 *__restrict__ pos32, int32_t row,
@@ -1277,29 +1287,26 @@ __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, 
   int2 *xr = xrow + ic * (int64_t)xstride;
   const unsigned long long gmask = (LG == kWave ? ~0ull : ((1ull << LG) - 1ull)) << (grp * LG);
   int32_t cnt = 0;
-  // the entry and position of this lane's next candidate are loaded one iteration ahead
-  const int klast = nj > 0 ? nj - 1 : 0;
-  int4 en = make_int4(0, 0, 0, 0);
-  float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (nj > 0) {
-    const int k0 = l < nj ? l : klast;
-    en = lr[k0 / M];
-    pn = pos32[(int64_t)(en.x & kClusterMask) * M + (k0 % M)];
-  }
-  for (int base = 0; base < nmax; base += LG) {
+  // The entry and position of this lane's next candidate are loaded one step ahead, two steps per loop
+  // trip so that the loaded values stay in their registers across the back edge.  A lane whose cluster
+  // lists nothing (an empty cluster of a region's tail) reads a zero entry instead of its row, so every
+  // load is in bounds without a guard.
+  const uint32_t klast = nj > 0 ? (uint32_t)(nj - 1) : 0u;
+  auto load_entry = [&](uint32_t kk) {
+    const uint32_t kc = kk < klast ? kk : klast;
+    const int4 *ep = nj > 0 ? lr + kc / M : &kZeroEntry;
+    const int4 e = *ep;
+    return make_pair_ef(e, pos32[(int64_t)(e.x & kClusterMask) * M + (kc % M)]);
+  };
+  auto x_step = [&](int base, const EntryPos &ep) {
     const int k = base + l;
-    const int4 e = en;
-    const float4 p = pn;
-    if (nj > 0) {
-      const int kn = base + LG + l < nj ? base + LG + l : klast;
-      en = lr[kn / M];
-      pn = pos32[(int64_t)(en.x & kClusterMask) * M + (kn % M)];
-    }
     bool near = false;
-    const int32_t js = (e.x & kClusterMask) * M + (k % M);
+    const int32_t js = (ep.e.x & kClusterMask) * M + (int32_t)((uint32_t)k % M);
     if (k < nj) {
+      const float4 p = ep.p;
       if (p.x != kPadPos32) {
-        const float jx = p.x + __int_as_float(e.y), jy = p.y + __int_as_float(e.z), jz = p.z + __int_as_float(e.w);
+        const float jx = p.x + __int_as_float(ep.e.y), jy = p.y + __int_as_float(ep.e.z),
+                    jz = p.z + __int_as_float(ep.e.w);
 #pragma unroll
         for (int q = 0; q < M; ++q) {
           const float dx = xi[q] - jx, dy = yi[q] - jy, dz = zi[q] - jz;
@@ -1310,8 +1317,16 @@ __global__ __launch_bounds__(kXrowThreads) void k_cluster_xrows(int64_t nitems, 
     }
     const unsigned long long ball = __ballot(near) & gmask;
     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
-    if (near) xr[cnt + below] = make_int2(js, k / M);
+    if (near) xr[cnt + below] = make_int2(js, (int32_t)((uint32_t)k / M));
     cnt += __popcll(ball);
+  };
+  EntryPos pa = load_entry((uint32_t)l);
+  for (int base = 0; base < nmax; base += 2 * LG) {
+    const EntryPos pb = load_entry((uint32_t)(base + LG + l));
+    x_step(base, pa);
+    if (base + LG >= nmax) break;
+    pa = load_entry((uint32_t)(base + 2 * LG + l));
+    x_step(base + LG, pb);
   }
   if (live && l == 0) xcount[ic] = cnt;
 }
