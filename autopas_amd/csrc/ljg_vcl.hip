@@ -1424,7 +1424,8 @@ constexpr int64_t kSplitListsBelow = 1 << 18;
 // rows are stored with.
 static int32_t keep_capacity(const Ctx &c, bool partial, int32_t used, int32_t had) {
   if (partial) return had;
-  const int32_t pct = c.prm.row_headroom == 0 ? 25 : (c.prm.row_headroom < 0 ? 0 : c.prm.row_headroom);
+  if (c.prm.row_headroom < 0) return used > had ? used : (had > 2 * used ? used : had);  // none
+  const int32_t pct = c.prm.row_headroom == 0 ? 25 : c.prm.row_headroom;
   const int32_t want = used + (int32_t)((int64_t)used * pct / 100) + 8;
   return want > had ? want : (had > 2 * want ? want : had);
 }

@@ -90,9 +90,9 @@ typedef struct {
                                * wavefronts (a few per SIMD) take tiles from a work queue in device memory,
                                * each starting at its XCD's share; 0 = a wavefront per tile */
   int32_t row_headroom;       /* cluster list builds: spare row capacity kept for the next build, in percent
-                               * of the longest row plus 8 entries; 0 = 25 %, -1 = the 8 entries only.  A row
-                               * that outgrows its capacity makes the build (and a force launched on it)
-                               * run again with the lengths read back. */
+                               * of the longest row plus 8 entries; 0 = 25 %, -1 = none (the longest row's
+                               * length only).  A row that outgrows its capacity makes the build (and a force
+                               * launched on it) run again with the lengths read back. */
 } ljg_params;
 
 typedef struct {

@@ -154,6 +154,7 @@ def test_jrow_rebuild_after_vetoed_force_is_bit_identical():
     # With no spare row capacity (row_headroom -1) the rows of a melting lattice outgrow the lengths kept
     # from the previous build, so the vetoed path runs (more builds are counted); the trajectory must
     # equal the one with the default headroom bit for bit.
+    kernel = ap.KERNEL_CLUSTER_JROWS_MIXED
     announce("j-row force after a vetoed launch: 16^3 lattice rho 0.75, 200 steps, row_headroom -1 vs 0")
     s = lattice_box(16, 0.75, seed=31, jitter=0.0, temperature=1.5)
     L = s["L"]
@@ -161,7 +162,7 @@ def test_jrow_rebuild_after_vetoed_force_is_bit_identical():
     for hr in (0, -1):
         with make_engine(L, M=8, row_headroom=hr) as e:
             e.upload(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
-            rs = e.step(200, 0.002, ap.KERNEL_CLUSTER_JROWS_MIXED)
+            rs = e.step(200, 0.002, kernel)
             out[hr] = (e.download(), rs.rebuilds, rs.pair_interactions)
     (a, ra, pa), (b, rb, pb) = out[0], out[-1]
     print(f"builds: headroom 25% {ra}, none {rb}; pair interactions {pa} {pb}")
